@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Time-to-rtol of the assemble-and-solve path on one MI355X.
+
+Workload (default): the heterogeneous plate -- rectangle 2x1, 1413x1413 nodes (3,993,138
+DOFs), interior nodes jittered +-0.2h, per-element ln E ~ N(0,1), nu=0.3, plane strain,
+ux=0 on x=0, uy=0 on y=0, traction tx=1 on x=2, body force (0,-1).  One step = stiffness
+pattern + assembly + loads + Dirichlet elimination + Jacobi setup + solve to a true FP64
+relative residual below 1e-10, all on the device; the mesh and material are resident in
+HBM before the timed region starts.
+
+Prints one JSON line.  The roofline object covers the dominant kernel (the fused SpMV +
+p'Ap of every CG iteration), timed with HIP events on the library stream; cpu_baseline is
+the oracle (the CPU restatement of the reference's element loop and PCG, bitwise equal to
+the reference on uniform-E inputs) timed on a bounded sample on one host core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mpi-cst-fem-solver_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "time-to-rtol 1e-10 (s) learned/mixed FCG vs FP64 Jacobi-CG; SpMV HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n, nnz):
+    """Algorithmic bytes of one FP64 SpMV on 2x2 blocks: values, block column indices,
+    block row pointers, x read once, y written once."""
+    return 8 * nnz + nnz + 4 * (n // 2 + 1) + 16 * n
+
+
+def cpu_baseline(prob, iterations, sample_iters):
+    """Oracle on one core: full pattern + assembly + loads + Dirichlet, then
+    `sample_iters` PCG iterations; time-to-rtol estimated as setup time + per-iteration time
+    times the GPU FP64 iteration count (same recurrence)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    oc = oracle_lib.Oracle(oracle_lib.ORACLE_PATH, "oc")
+    m = prob.mesh
+    strain = 1 if prob.plane == 1 else 0
+    t0 = time.perf_counter()
+    rp, c = oc.build_pattern(m.num_nodes, m.tri)
+    v = oc.assemble(m.xy, m.tri, rp, c, E=prob.E, nu=prob.nu, strain=strain, t=prob.thickness,
+                    E_elem=prob.E_elem)
+    f = np.zeros(m.num_dofs)
+    for g, kw in prob.tractions:
+        oc.add_traction(m.xy, m.tri, m.groups[g], f, t=prob.thickness, **kw)
+    if prob.body is not None:
+        oc.add_body_force(m.xy, m.tri, f, prob.thickness, *prob.body)
+    d, val = prob.dirichlet_dofs()
+    K, b = oc.apply_dirichlet(rp, c, v, f, d, val)
+    t1 = time.perf_counter()
+    log(f"cpu baseline: assembly+bcs {t1 - t0:.2f} s")
+    _, st = oc.pcg_jacobi(rp, c, K, b, 1e-300, sample_iters)
+    t2 = time.perf_counter()
+    per_it = (t2 - t1) / max(1, st["iterations"])
+    est = (t1 - t0) + per_it * iterations
+    log(f"cpu baseline: {st['iterations']} iterations in {t2 - t1:.2f} s ({per_it * 1e3:.1f} ms/it)")
+    return dict(value=est, unit="s", cores=1, kind="port",
+                sample=(f"oracle on 1 core: full pattern+assembly+loads+Dirichlet "
+                        f"({t1 - t0:.2f} s) + {st['iterations']} Jacobi-PCG iterations "
+                        f"({per_it * 1e3:.2f} ms/it), extrapolated to the GPU FP64 count of "
+                        f"{iterations} iterations"),
+                assembly_s=t1 - t0, s_per_iteration=per_it)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--max-iter", type=int, default=200000)
+    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the whole plate sits in one MI355X's HBM and this "
+              f"build has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import cstfem_hip
+    from cstfem_hip import pipeline
+
+    t = time.perf_counter()
+    prob = pipeline.config_problem(args.config, args.scale)
+    m = prob.mesh
+    log(f"{prob.name}: {m.num_nodes} nodes, {m.num_elements} elements, {m.num_dofs} DOFs "
+        f"(generated in {time.perf_counter() - t:.1f} s)")
+    dev = cstfem_hip.Device(0)
+    name, cus = dev.info()
+    pipeline.load_problem(dev, prob)
+
+    def step():
+        times = pipeline.PhaseTimes()
+        pipeline.assemble_and_constrain(dev, prob, times)
+        _, st = pipeline.solve(dev, rtol=1e-10, max_iter=args.max_iter, times=times)
+        return st, times
+
+    for i in range(args.warmup):
+        st, times = step()
+        log(f"warmup {i}: {st.message}, true rel {st.true_rel_residual:.3e}")
+    step_times, stats = [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        st, times = step()
+        dt = time.perf_counter() - t0
+        step_times.append(dt)
+        stats.append((st, times))
+        log(f"step {i}: {dt:.3f} s, {st.message}, true rel {st.true_rel_residual:.3e}")
+    st, times = stats[-1]
+    if not st.converged or st.true_rel_residual >= 1e-10:
+        log(f"WARNING: not verified: {st.message}; true rel {st.true_rel_residual:.3e}")
+    total = sum(step_times)
+    per_step = total / len(step_times)
+
+    # dominant kernel: fused SpMV + p'Ap, average launch duration with HIP events
+    n, nnz = dev.n, dev.nnz
+    t_spmv = dev.profile_kernel(0, reps=50)
+    t_upd = dev.profile_kernel(2, reps=50)
+    t_dir = dev.profile_kernel(3, reps=50)
+    copy_bytes = 1 << 30
+    t_copy = dev.profile_kernel(4, reps=20, nbytes=copy_bytes)
+    b_spmv = spmv_bytes(n, nnz)
+    achieved = b_spmv / t_spmv / 1e9
+    copy_gbs = 2 * copy_bytes / t_copy / 1e9
+    vec_bytes_upd = 8 * n * 7     # reads x p r Ap invd, writes x r
+    vec_bytes_dir = 8 * n * 4     # reads r invd p, writes p
+    iter_bytes = b_spmv + 8 * n * 11
+    per_iter = st.breakdown["total"] / max(1, st.iterations)
+    log(f"spmv {t_spmv * 1e6:.1f} us ({achieved:.0f} GB/s), update {t_upd * 1e6:.1f} us "
+        f"({vec_bytes_upd / t_upd / 1e9:.0f} GB/s), direction {t_dir * 1e6:.1f} us "
+        f"({vec_bytes_dir / t_dir / 1e9:.0f} GB/s), copy {copy_gbs:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(prob, st.iterations, args.cpu_iters)
+
+    out = {
+        "metric": METRIC,
+        "value": per_step,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": per_step * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{prob.name}: {m.num_dofs} DOFs, {m.num_elements} CST elements, "
+                         f"FP64 Jacobi-CG to true rel residual 1e-10 (assembly+BCs+solve)"),
+            "dofs": m.num_dofs, "nnz": int(nnz), "elements": m.num_elements,
+            "method": "cg_jacobi_fp64", "device": name, "cus": cus,
+        },
+        "iterations": st.iterations,
+        "true_rel_residual": st.true_rel_residual,
+        "s_per_iteration": per_iter,
+        "phases_s": {"pattern": times.pattern, "assembly": times.assembly, "bcs": times.bcs,
+                     "solve": times.solve},
+        "dofs_assembled_per_s": m.num_dofs / times.assembly if times.assembly > 0 else None,
+        "spmv_gbs": achieved,
+        "spmv_gbs_scalar_csr_model": (12 * nnz + 4 * (n + 1) + 16 * n) / t_spmv / 1e9,
+        "copy_gbs": copy_gbs,
+        "iteration_gbs": iter_bytes / per_iter / 1e9,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_cg_spmv", "bytes_per_launch": b_spmv,
+                     "avg_launch_s": t_spmv},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

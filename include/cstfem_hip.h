@@ -1,0 +1,145 @@
+/*
+ * cstfem_hip.h -- C ABI of the MI355X (gfx950) assemble-and-solve path for 2D linear-elastic
+ * constant-strain triangles.
+ *
+ * Conventions (kept from the reference, include/cstfem headers):
+ *   - indices are 32-bit int, DOF = 2*node + component (assembly.hpp:10);
+ *   - the stiffness pattern is the reference's CSR (csr.hpp:14-32): row_ptr[n+1], sorted
+ *     cols[nnz], rows 2n and 2n+1 share one column list [2m, 2m+1 for m in adj(n)];
+ *   - the caller owns every host pointer; the context owns all device memory;
+ *   - every call returns a status: 0 ok, 2 not converged, negative = error class below;
+ *     cstfem_last_error() gives the message (thread-local), with the reference's message
+ *     prefixes ("cg_jacobi: non-positive diagonal at row", "apply_dirichlet: ...").
+ * One context per host thread; contexts share nothing.
+ */
+#ifndef CSTFEM_HIP_H
+#define CSTFEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CSTFEM_OK = 0,
+  CSTFEM_NOT_CONVERGED = 2,
+  CSTFEM_ERR_ARGUMENT = -1,
+  CSTFEM_ERR_DEVICE = -2,
+  CSTFEM_ERR_BAD_DIAGONAL = -3,   /* std::runtime_error in the reference */
+  CSTFEM_ERR_MISSING_ENTRY = -4,  /* std::logic_error in the reference */
+  CSTFEM_ERR_OUT_OF_RANGE = -5,   /* std::runtime_error */
+  CSTFEM_ERR_ALLOC = -6,
+  CSTFEM_ERR_TRACTION_EDGE = -7,  /* std::runtime_error */
+  CSTFEM_ERR_CONFLICT = -8,       /* std::runtime_error: conflicting Dirichlet values */
+  CSTFEM_ERR_ORDER = -9           /* std::logic_error: e.g. solve() before setup() */
+};
+
+enum { CSTFEM_PLANE_STRESS = 0, CSTFEM_PLANE_STRAIN = 1 };
+
+/* Solver precision / method of cstfem_solve(). */
+enum {
+  CSTFEM_METHOD_CG_FP64 = 0,      /* Jacobi-PCG in FP64, the reference recurrence */
+  CSTFEM_METHOD_IR_FP32 = 1,      /* FP32 Jacobi-PCG inner solves under FP64 refinement */
+  CSTFEM_METHOD_FCG_LEARNED = 2   /* flexible CG with the learned preconditioner (FP32 inner) */
+};
+
+typedef struct cstfem_ctx cstfem_ctx;
+
+/* Mirrors SolverOptions (solver.hpp:93-97) plus device-path knobs. */
+typedef struct {
+  double rtol;            /* default 1e-10 */
+  int max_iter;           /* default 20000 */
+  int record_history;     /* residual_history of ||r_k||/||b|| (k = 0..iterations) */
+  int instrument;         /* fill the breakdown fields (kernel-class times) */
+  int method;             /* CSTFEM_METHOD_* */
+  double inner_rtol;      /* IR / FCG: inner relative tolerance (0 = default 1e-4) */
+  int max_outer;          /* IR: max refinement steps (0 = default 60) */
+} cstfem_solver_options;
+
+/* Mirrors SolverStats (solver.hpp:99-107) and SolveBreakdown (solver.hpp:17-35). */
+typedef struct {
+  int iterations;          /* total inner iterations (FP64 CG: the reference's count) */
+  int converged;
+  int breakdown;           /* 1 if p'Ap <= 0 stopped CG */
+  int zero_rhs;
+  int outer_iterations;    /* IR / FCG outer steps; 0 for plain CG */
+  int fallback;            /* FCG: 1 if it fell back to Jacobi-CG on stagnation */
+  int warning;             /* 1 if converged but true residual > 10*rtol */
+  int pad_;
+  double rel_residual;     /* recursive ||r||/||b|| at exit */
+  double true_rel_residual;/* ||b - A x|| / ||b|| in FP64 */
+  double breakdown_pap;
+  /* breakdown (seconds; the reference's field names) */
+  double setup, spmv_local, dot_local, vector_ops, total;
+  long long spmv_calls, reductions;
+} cstfem_solver_stats;
+
+/* ---- context ------------------------------------------------------------------------ */
+const char* cstfem_last_error(void);
+int cstfem_ctx_create(int device, cstfem_ctx** out);
+int cstfem_ctx_destroy(cstfem_ctx* ctx);
+/* Device properties for reports: name (NUL-terminated, <= len), CU count. */
+int cstfem_device_info(cstfem_ctx* ctx, char* name, int len, int* cu_count);
+
+/* ---- mesh and material (replaces the Mesh argument of assembly.hpp:15,22) -------------- */
+/* xy[2*nn], tri[3*ne] counter-clockwise, E_elem[ne] or NULL (then E is uniform, as the
+ * reference's constitutive_matrix(E, nu, plane), element.cpp:7-15). */
+int cstfem_upload_mesh(cstfem_ctx* ctx, int nn, const double* xy, int64_t ne, const int* tri);
+int cstfem_set_material(cstfem_ctx* ctx, double E, double nu, int plane, double thickness,
+                        const double* E_elem);
+
+/* ---- pattern and assembly ------------------------------------------------------------ */
+/* Replaces build_stiffness_pattern (assembly.cpp:9-42): builds the pattern on the device,
+ * plus the element-to-slot map used by assembly.  *nnz_out = scalar nnz. */
+int cstfem_build_pattern(cstfem_ctx* ctx, int64_t* nnz_out);
+/* Copies the pattern to the host: row_ptr[2*nn+1], cols[nnz] (either may be NULL). */
+int cstfem_download_pattern(cstfem_ctx* ctx, int* row_ptr, int* cols);
+/* Replaces assemble_stiffness (assembly.cpp:54-74): zeroes and sums every element's Ke
+ * through the slot map, in element order per entry (no atomics). */
+int cstfem_assemble(cstfem_ctx* ctx);
+/* Copies the values in the reference's CSR order (vals[nnz]). */
+int cstfem_download_values(cstfem_ctx* ctx, double* vals);
+/* Generic CSR input (the CsrMatrix handed to CgJacobiSolver::setup, solver.cpp:133):
+ * n rows, sorted columns.  Replaces any device matrix.  Values are copied. */
+int cstfem_upload_csr(cstfem_ctx* ctx, int n, const int* row_ptr, const int* cols,
+                      const double* vals);
+
+/* ---- right-hand side and Dirichlet elimination ---------------------------------------- */
+int cstfem_set_rhs(cstfem_ctx* ctx, const double* b, int n);
+int cstfem_download_rhs(cstfem_ctx* ctx, double* b, int n);
+/* Adds the consistent nodal load of a uniform body force (bx, by) per unit volume,
+ * t*|A|/3 per node of every element, in element order (no reference counterpart). */
+int cstfem_add_body_force(cstfem_ctx* ctx, double bx, double by);
+/* Replaces add_traction (bc.cpp:190-253) for one line group: 2-point Gauss per edge,
+ * outward normal from the adjacent triangle.  mode 0: constant (p0, p1) = (tx, ty);
+ * mode 1: Kirsch traction with (p0, p1) = (sigma_inf, a) (kirsch.cpp:40-45).
+ * Computed on the host from the host mesh, added to the device RHS. */
+int cstfem_add_traction(cstfem_ctx* ctx, int nedge, const int* edges, int mode, double p0,
+                        double p1);
+/* Replaces apply_dirichlet (bc.cpp:137-166) on the device matrix and RHS.  dofs/values:
+ * m prescribed DOFs; a repeated DOF must repeat its value (DirichletDofs::add, bc.cpp:107). */
+int cstfem_apply_dirichlet(cstfem_ctx* ctx, int m, const int* dofs, const double* values);
+
+/* ---- solve (replaces CgJacobiSolver::setup + solve / pcg_jacobi, solver.cpp:109-157) --- */
+int cstfem_setup(cstfem_ctx* ctx);
+/* x_out[n] (may be NULL: the solution stays on the device); history: NULL or room for
+ * max_iter + 1 doubles (plain CG only). */
+int cstfem_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt, cstfem_solver_stats* st,
+                 double* x_out, double* history);
+int cstfem_download_solution(cstfem_ctx* ctx, double* x, int n);
+/* y = A x with the device matrix (FP64, per-row summation in column order). */
+int cstfem_spmv(cstfem_ctx* ctx, const double* x, double* y, int n);
+
+/* ---- measurement (no reference counterpart) ------------------------------------------ */
+/* Average seconds of `reps` back-to-back launches of one hot kernel on the library's
+ * stream, timed with HIP events; solver state is untouched.  kind: 0 fused SpMV + p'Ap,
+ * 1 plain SpMV, 2 CG update (axpys + dots), 3 CG direction, 4 copy of `bytes` bytes
+ * (achievable HBM bandwidth). */
+int cstfem_profile_kernel(cstfem_ctx* ctx, int kind, int reps, int64_t bytes,
+                          double* avg_seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

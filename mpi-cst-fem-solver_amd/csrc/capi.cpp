@@ -1,0 +1,604 @@
+// capi.cpp -- the extern "C" boundary (include/cstfem_hip.h): argument checks, context life
+// cycle, host<->device copies and the host-side pieces of the path (traction loads, the
+// generic CSR -> SELL conversion).  Every entry point converts exceptions into a status
+// code plus a thread-local message.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <unordered_map>
+
+#include "internal.hpp"
+
+namespace cstfem_hip {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& m) { g_error = m; }
+
+void hip_check(hipError_t e, const char* what, const char* file, int line) {
+  if (e != hipSuccess)
+    throw Error(CSTFEM_ERR_DEVICE, std::string("HIP error '") + hipGetErrorString(e) + "' in " +
+                                       what + " (" + file + ":" + std::to_string(line) + ")");
+}
+
+Context::~Context() {
+  if (h_scal) (void)hipHostFree(h_scal);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const Error& e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_error = "host allocation failed";
+    return CSTFEM_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return CSTFEM_ERR_ARGUMENT;
+  }
+}
+
+void need(bool ok, const std::string& msg) {
+  if (!ok) throw Error(CSTFEM_ERR_ARGUMENT, msg);
+}
+
+void use_device(Context* c) { HIP_CHECK(hipSetDevice(c->device)); }
+
+std::uint64_t edge_key(int a, int b) {
+  if (a > b) std::swap(a, b);
+  return (static_cast<std::uint64_t>(a) << 32) | static_cast<std::uint32_t>(b);
+}
+
+// kirsch.cpp:7-45: Kirsch stress at (x, y) and the traction sigma.n
+void kirsch_traction(double sigma_inf, double a, double x, double y, double nx, double ny,
+                     double t[2]) {
+  const double r = std::hypot(x, y), theta = std::atan2(y, x);
+  const double q2 = (a * a) / (r * r), q4 = q2 * q2;
+  const double c2 = std::cos(2.0 * theta), s2 = std::sin(2.0 * theta);
+  const double s = sigma_inf;
+  const double rr = 0.5 * s * (1.0 - q2) + 0.5 * s * (1.0 - 4.0 * q2 + 3.0 * q4) * c2;
+  const double tt = 0.5 * s * (1.0 + q2) - 0.5 * s * (1.0 + 3.0 * q4) * c2;
+  const double rt = -0.5 * s * (1.0 + 2.0 * q2 - 3.0 * q4) * s2;
+  const double c = std::cos(theta), n = std::sin(theta);
+  const double sxx = rr * c * c + tt * n * n - 2.0 * rt * n * c;
+  const double syy = rr * n * n + tt * c * c + 2.0 * rt * n * c;
+  const double txy = (rr - tt) * n * c + rt * (c * c - n * n);
+  t[0] = sxx * nx + txy * ny;
+  t[1] = txy * nx + syy * ny;
+}
+
+__global__ void k_gather(int m, const int* __restrict__ idx, const double* __restrict__ src,
+                         double* __restrict__ dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) dst[k] = src[idx[k]];
+}
+
+__global__ void k_scatter(int m, const int* __restrict__ idx, const double* __restrict__ src,
+                          double* __restrict__ dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) dst[idx[k]] = src[k];
+}
+
+}  // namespace
+}  // namespace cstfem_hip
+
+using namespace cstfem_hip;
+
+struct cstfem_ctx {
+  Context c;
+};
+
+extern "C" {
+
+const char* cstfem_last_error(void) { return cstfem_hip::g_error.c_str(); }
+
+int cstfem_ctx_create(int device, cstfem_ctx** out) {
+  return guarded([&] {
+    need(out != nullptr, "cstfem_ctx_create: out is NULL");
+    int count = 0;
+    HIP_CHECK(hipGetDeviceCount(&count));
+    need(device >= 0 && device < count, "cstfem_ctx_create: device " + std::to_string(device) +
+                                            " not present (" + std::to_string(count) + " visible)");
+    auto* x = new cstfem_ctx;
+    Context& c = x->c;
+    try {
+      c.device = device;
+      HIP_CHECK(hipSetDevice(device));
+      HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+      hipDeviceProp_t prop;
+      HIP_CHECK(hipGetDeviceProperties(&prop, device));
+      c.cu_count = prop.multiProcessorCount;
+      HIP_CHECK(hipHostMalloc((void**)&c.h_scal, 2 * sizeof(PcgScalars), hipHostMallocDefault));
+      c.scal.alloc(1);
+      c.err_flag.alloc(4);
+    } catch (...) {
+      delete x;
+      throw;
+    }
+    *out = x;
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_ctx_destroy(cstfem_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return CSTFEM_OK;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    delete ctx;
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_device_info(cstfem_ctx* ctx, char* name, int len, int* cu_count) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_device_info: NULL context");
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, ctx->c.device));
+    if (name && len > 0) {
+      std::snprintf(name, len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_upload_mesh(cstfem_ctx* ctx, int nn, const double* xy, int64_t ne, const int* tri) {
+  return guarded([&] {
+    need(ctx && xy && tri, "cstfem_upload_mesh: NULL argument");
+    need(nn > 0 && ne > 0, "cstfem_upload_mesh: empty mesh");
+    need((int64_t)2 * nn < INT32_MAX && 9 * ne < INT32_MAX, "cstfem_upload_mesh: mesh too large for 32-bit indices");
+    for (int64_t k = 0; k < 3 * ne; ++k)
+      need(tri[k] >= 0 && tri[k] < nn, "cstfem_upload_mesh: triangle " + std::to_string(k / 3) +
+                                           " references node " + std::to_string(tri[k]) + " out of range");
+    Context& c = ctx->c;
+    use_device(&c);
+    c.nn = nn;
+    c.ne = ne;
+    c.h_xy.assign(xy, xy + 2 * (size_t)nn);
+    c.h_tri.assign(tri, tri + 3 * (size_t)ne);
+    c.xy.alloc(2 * (size_t)nn);
+    c.tri.alloc(3 * (size_t)ne);
+    HIP_CHECK(hipMemcpyAsync(c.xy.get(), xy, sizeof(double) * 2 * nn, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(c.tri.get(), tri, sizeof(int) * 3 * ne, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    c.have_pattern = c.mesh_pattern = c.setup_done = c.vals32_valid = false;
+    c.has_E_elem = false;
+    c.ne_ptr.release();
+    c.ne_list.release();
+    c.b.release();
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_set_material(cstfem_ctx* ctx, double E, double nu, int plane, double thickness,
+                        const double* E_elem) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_set_material: NULL context");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.nn > 0, "cstfem_set_material: upload a mesh first");
+    need(E > 0, "E must be positive");
+    need(nu > -1.0 && nu < 0.5, "nu must be in (-1, 0.5)");
+    need(thickness > 0, "thickness must be positive");
+    need(plane == CSTFEM_PLANE_STRESS || plane == CSTFEM_PLANE_STRAIN,
+         "plane must be plane_stress or plane_strain");
+    c.E = E;
+    c.nu = nu;
+    c.thick = thickness;
+    c.plane_strain = plane;
+    c.has_E_elem = E_elem != nullptr;
+    if (E_elem) {
+      for (int64_t e = 0; e < c.ne; ++e)
+        need(E_elem[e] > 0, "element " + std::to_string(e) + ": E must be positive");
+      c.E_elem.alloc(c.ne);
+      HIP_CHECK(hipMemcpyAsync(c.E_elem.get(), E_elem, sizeof(double) * c.ne, hipMemcpyHostToDevice, c.stream));
+      HIP_CHECK(hipStreamSynchronize(c.stream));
+    }
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_build_pattern(cstfem_ctx* ctx, int64_t* nnz_out) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_build_pattern: NULL context");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.nn > 0, "cstfem_build_pattern: upload a mesh first");
+    build_pattern(c);
+    ensure_vectors(c);
+    if (nnz_out) *nnz_out = 4 * c.nblk;
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_download_pattern(cstfem_ctx* ctx, int* row_ptr, int* cols) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_download_pattern: NULL context");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.have_pattern && c.mesh_pattern, "cstfem_download_pattern: no mesh pattern");
+    std::vector<int> bp(c.nb + 1), bc(c.nblk);
+    HIP_CHECK(hipMemcpyAsync(bp.data(), c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(bc.data(), c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    // 2x expansion (assembly.cpp:21-39): rows 2n and 2n+1 share [2m, 2m+1 for m in adj(n)]
+    if (row_ptr) {
+      row_ptr[0] = 0;
+      for (int nd = 0; nd < c.nb; ++nd) {
+        const int len = 2 * (bp[nd + 1] - bp[nd]);
+        row_ptr[2 * nd + 1] = row_ptr[2 * nd] + len;
+        row_ptr[2 * nd + 2] = row_ptr[2 * nd + 1] + len;
+      }
+    }
+    if (cols) {
+      int64_t k = 0;
+      for (int nd = 0; nd < c.nb; ++nd)
+        for (int comp = 0; comp < 2; ++comp)
+          for (int q = bp[nd]; q < bp[nd + 1]; ++q) {
+            cols[k++] = 2 * bc[q];
+            cols[k++] = 2 * bc[q] + 1;
+          }
+    }
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_assemble(cstfem_ctx* ctx) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_assemble: NULL context");
+    use_device(&ctx->c);
+    assemble_values(ctx->c);
+    HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_download_values(cstfem_ctx* ctx, double* vals) {
+  return guarded([&] {
+    need(ctx && vals, "cstfem_download_values: NULL argument");
+    use_device(&ctx->c);
+    download_values(ctx->c, vals);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_upload_csr(cstfem_ctx* ctx, int n, const int* row_ptr, const int* cols,
+                      const double* vals) {
+  return guarded([&] {
+    need(ctx && row_ptr && (n == 0 || (cols && vals)), "cstfem_upload_csr: NULL argument");
+    need(n > 0, "cstfem_upload_csr: empty matrix");
+    Context& c = ctx->c;
+    use_device(&c);
+    const int nb = (n + 1) / 2;
+    // block rows: union of the block columns of rows 2I and 2I+1, ascending
+    std::vector<int> bp(nb + 1, 0), bc;
+    std::vector<int> bdiag(nb, -1);
+    std::vector<int> tmp;
+    for (int I = 0; I < nb; ++I) {
+      tmp.clear();
+      for (int rr = 2 * I; rr < std::min(2 * I + 2, n); ++rr) {
+        need(row_ptr[rr + 1] >= row_ptr[rr], "cstfem_upload_csr: row_ptr not monotone");
+        for (int k = row_ptr[rr]; k < row_ptr[rr + 1]; ++k) {
+          need(cols[k] >= 0 && cols[k] < n, "cstfem_upload_csr: column out of range");
+          if (k > row_ptr[rr]) need(cols[k] > cols[k - 1], "cstfem_upload_csr: columns not sorted");
+          tmp.push_back(cols[k] / 2);
+        }
+      }
+      std::sort(tmp.begin(), tmp.end());
+      tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+      bp[I + 1] = bp[I] + (int)tmp.size();
+      for (size_t q = 0; q < tmp.size(); ++q) {
+        if (tmp[q] == I) bdiag[I] = bp[I] + (int)q;
+        bc.push_back(tmp[q]);
+      }
+    }
+    const int64_t nblk = bc.size();
+    // SELL-64 layout
+    const int nslices = (nb + kSlice - 1) / kSlice;
+    std::vector<int64_t> off(nslices + 1, 0);
+    for (int s = 0; s < nslices; ++s) {
+      int w = 0;
+      for (int l = 0; l < kSlice && s * kSlice + l < nb; ++l) {
+        const int r = s * kSlice + l;
+        w = std::max(w, bp[r + 1] - bp[r]);
+      }
+      off[s + 1] = off[s] + (int64_t)w * kSlice;
+    }
+    const int64_t sb = off[nslices];
+    std::vector<int> scol(sb);
+    std::vector<double> sval(4 * sb, 0.0);
+    std::vector<int64_t> spos(std::max<int64_t>(nblk, 1));
+    for (int s = 0; s < nslices; ++s) {
+      const int w = (int)((off[s + 1] - off[s]) / kSlice);
+      for (int l = 0; l < kSlice; ++l) {
+        const int r = s * kSlice + l;
+        const int d = r < nb ? bp[r + 1] - bp[r] : 0;
+        for (int k = 0; k < w; ++k) {
+          const int64_t slot = off[s] + (int64_t)k * kSlice + l;
+          if (k < d) {
+            scol[slot] = bc[bp[r] + k];
+            spos[bp[r] + k] = slot;
+          } else {
+            scol[slot] = r < nb ? r : 0;
+          }
+        }
+      }
+    }
+    c.h_generic_map.assign(row_ptr[n], 0);
+    for (int rr = 0; rr < n; ++rr) {
+      const int I = rr / 2, ci = rr % 2;
+      int q = bp[I];
+      for (int k = row_ptr[rr]; k < row_ptr[rr + 1]; ++k) {
+        const int J = cols[k] / 2, cj = cols[k] % 2;
+        while (bc[q] != J) ++q;
+        const int64_t vi = 4 * spos[q] + 2 * ci + cj;
+        sval[vi] = vals[k];
+        c.h_generic_map[k] = vi;
+      }
+    }
+    c.nb = nb;
+    c.n = n;
+    c.nblk = nblk;
+    c.nslices = nslices;
+    c.sell_blocks = sb;
+    c.h_sell_off = off;
+    c.brow_ptr.alloc(nb + 1);
+    c.bcol.alloc(std::max<int64_t>(nblk, 1));
+    c.bdiag.alloc(nb);
+    c.sell_off.alloc(nslices + 1);
+    c.sell_col.alloc(std::max<int64_t>(sb, 1));
+    c.sell_val.alloc(std::max<int64_t>(4 * sb, 1));
+    c.sell_pos.alloc(spos.size());
+    HIP_CHECK(hipMemcpyAsync(c.brow_ptr.get(), bp.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, c.stream));
+    if (nblk)
+      HIP_CHECK(hipMemcpyAsync(c.bcol.get(), bc.data(), sizeof(int) * nblk, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(c.bdiag.get(), bdiag.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(c.sell_off.get(), off.data(), sizeof(int64_t) * (nslices + 1), hipMemcpyHostToDevice, c.stream));
+    if (sb) {
+      HIP_CHECK(hipMemcpyAsync(c.sell_col.get(), scol.data(), sizeof(int) * sb, hipMemcpyHostToDevice, c.stream));
+      HIP_CHECK(hipMemcpyAsync(c.sell_val.get(), sval.data(), sizeof(double) * 4 * sb, hipMemcpyHostToDevice, c.stream));
+    }
+    HIP_CHECK(hipMemcpyAsync(c.sell_pos.get(), spos.data(), sizeof(int64_t) * spos.size(), hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    c.have_pattern = true;
+    c.mesh_pattern = false;
+    c.generic = true;
+    c.setup_done = false;
+    c.vals32_valid = false;
+    c.b.release();
+    ensure_vectors(c);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_set_rhs(cstfem_ctx* ctx, const double* b, int n) {
+  return guarded([&] {
+    need(ctx && b, "cstfem_set_rhs: NULL argument");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.have_pattern, "cstfem_set_rhs: no matrix on the device");
+    need(n == c.n, "cstfem_set_rhs: size " + std::to_string(n) + " != " + std::to_string(c.n));
+    ensure_vectors(c);
+    HIP_CHECK(hipMemsetAsync(c.b.get(), 0, sizeof(double) * c.b.n, c.stream));
+    HIP_CHECK(hipMemcpyAsync(c.b.get(), b, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_download_rhs(cstfem_ctx* ctx, double* b, int n) {
+  return guarded([&] {
+    need(ctx && b, "cstfem_download_rhs: NULL argument");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.b.n > 0 && n == c.n, "cstfem_download_rhs: size mismatch or no RHS");
+    HIP_CHECK(hipMemcpyAsync(b, c.b.get(), sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_add_body_force(cstfem_ctx* ctx, double bx, double by) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_add_body_force: NULL context");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.nn > 0 && c.have_pattern && c.mesh_pattern, "cstfem_add_body_force: build the mesh pattern first");
+    ensure_vectors(c);
+    add_body_force_device(c, bx, by);
+    return CSTFEM_OK;
+  });
+}
+
+// bc.cpp:190-253: consistent nodal forces of a traction on one line group
+int cstfem_add_traction(cstfem_ctx* ctx, int nedge, const int* edges, int mode, double p0,
+                        double p1) {
+  return guarded([&] {
+    need(ctx && (nedge == 0 || edges), "cstfem_add_traction: NULL argument");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.nn > 0 && c.have_pattern && c.mesh_pattern, "cstfem_add_traction: build the mesh pattern first");
+    need(mode == 0 || mode == 1, "cstfem_add_traction: mode must be 0 (constant) or 1 (kirsch)");
+    ensure_vectors(c);
+    const std::vector<double>& xy = c.h_xy;
+    const std::vector<int>& tri = c.h_tri;
+    std::unordered_map<std::uint64_t, int> opposite, count;
+    for (int k = 0; k < nedge; ++k) {
+      need(edges[2 * k] >= 0 && edges[2 * k] < c.nn && edges[2 * k + 1] >= 0 && edges[2 * k + 1] < c.nn,
+           "cstfem_add_traction: edge node out of range");
+      opposite.emplace(edge_key(edges[2 * k], edges[2 * k + 1]), -1);
+      count.emplace(edge_key(edges[2 * k], edges[2 * k + 1]), 0);
+    }
+    for (int64_t t = 0; t < c.ne; ++t) {
+      const int* nd = &tri[3 * t];
+      for (int k = 0; k < 3; ++k) {
+        const auto it = opposite.find(edge_key(nd[k], nd[(k + 1) % 3]));
+        if (it != opposite.end()) {
+          it->second = nd[(k + 2) % 3];
+          ++count[it->first];
+        }
+      }
+    }
+    // touched DOFs in first-touch order; accumulation continues from the device values
+    std::vector<int> dofs;
+    std::unordered_map<int, int> slot;
+    auto touch = [&](int dof) {
+      if (slot.emplace(dof, (int)dofs.size()).second) dofs.push_back(dof);
+    };
+    for (int k = 0; k < nedge; ++k) {
+      touch(2 * edges[2 * k]);
+      touch(2 * edges[2 * k] + 1);
+      touch(2 * edges[2 * k + 1]);
+      touch(2 * edges[2 * k + 1] + 1);
+    }
+    const int m = (int)dofs.size();
+    if (m == 0) return CSTFEM_OK;
+    std::vector<double> f(m);
+    DBuf<int> d_idx;
+    DBuf<double> d_val;
+    d_idx.alloc(m);
+    d_val.alloc(m);
+    HIP_CHECK(hipMemcpyAsync(d_idx.get(), dofs.data(), sizeof(int) * m, hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_gather, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, d_idx.get(),
+                       (const double*)c.b.get(), d_val.get());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(f.data(), d_val.get(), sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    const double g = 1.0 / std::sqrt(3.0);
+    const double xi[2] = {-g, g};
+    for (int k = 0; k < nedge; ++k) {
+      const int a = edges[2 * k], b = edges[2 * k + 1];
+      const auto key = edge_key(a, b);
+      if (count[key] != 1)
+        throw Error(CSTFEM_ERR_TRACTION_EDGE,
+                    "traction group: edge (" + std::to_string(a) + ", " + std::to_string(b) +
+                        ") is adjacent to " + std::to_string(count[key]) + " triangles (expected 1)");
+      const double xa = xy[2 * a], ya = xy[2 * a + 1], xb = xy[2 * b], yb = xy[2 * b + 1];
+      const double len = std::hypot(xb - xa, yb - ya);
+      double nx = (yb - ya) / len, ny = -(xb - xa) / len;
+      const int cn = opposite[key];
+      if (nx * (xy[2 * cn] - xa) + ny * (xy[2 * cn + 1] - ya) > 0.0) {
+        nx = -nx;
+        ny = -ny;
+      }
+      for (double q : xi) {
+        const double Na = 0.5 * (1.0 - q), Nb = 0.5 * (1.0 + q);
+        const double x = Na * xa + Nb * xb, y = Na * ya + Nb * yb;
+        double tr[2] = {p0, p1};
+        if (mode == 1) kirsch_traction(p0, p1, x, y, nx, ny, tr);
+        const double w = c.thick * 0.5 * len;
+        f[slot[2 * a]] += w * Na * tr[0];
+        f[slot[2 * a + 1]] += w * Na * tr[1];
+        f[slot[2 * b]] += w * Nb * tr[0];
+        f[slot[2 * b + 1]] += w * Nb * tr[1];
+      }
+    }
+    HIP_CHECK(hipMemcpyAsync(d_val.get(), f.data(), sizeof(double) * m, hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_scatter, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, d_idx.get(),
+                       (const double*)d_val.get(), c.b.get());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_apply_dirichlet(cstfem_ctx* ctx, int m, const int* dofs, const double* values) {
+  return guarded([&] {
+    need(ctx && (m == 0 || (dofs && values)), "cstfem_apply_dirichlet: NULL argument");
+    use_device(&ctx->c);
+    apply_dirichlet_device(ctx->c, m, dofs, values);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_setup(cstfem_ctx* ctx) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_setup: NULL context");
+    use_device(&ctx->c);
+    setup_solver(ctx->c);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt, cstfem_solver_stats* st,
+                 double* x_out, double* history) {
+  return guarded([&] {
+    need(ctx && opt && st, "cstfem_solve: NULL argument");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(opt->max_iter >= 0, "cstfem_solve: max_iter must be >= 0");
+    need(opt->rtol > 0 && std::isfinite(opt->rtol), "cstfem_solve: rtol must be positive");
+    std::memset(st, 0, sizeof *st);
+    if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: solve() called before setup()");
+    int rc;
+    switch (opt->method) {
+      case CSTFEM_METHOD_CG_FP64:
+        rc = solve_cg_fp64(c, *opt, *st, history);
+        break;
+      case CSTFEM_METHOD_IR_FP32:
+        rc = solve_ir_fp32(c, *opt, *st);
+        break;
+      default:
+        throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_solve: unknown method " + std::to_string(opt->method));
+    }
+    if (x_out) {
+      HIP_CHECK(hipMemcpyAsync(x_out, c.x.get(), sizeof(double) * c.n, hipMemcpyDeviceToHost, c.stream));
+      HIP_CHECK(hipStreamSynchronize(c.stream));
+    }
+    return rc;
+  });
+}
+
+int cstfem_download_solution(cstfem_ctx* ctx, double* x, int n) {
+  return guarded([&] {
+    need(ctx && x, "cstfem_download_solution: NULL argument");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.x.n > 0 && n == c.n, "cstfem_download_solution: size mismatch or no solution");
+    HIP_CHECK(hipMemcpyAsync(x, c.x.get(), sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_spmv(cstfem_ctx* ctx, const double* x, double* y, int n) {
+  return guarded([&] {
+    need(ctx && x && y, "cstfem_spmv: NULL argument");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.have_pattern && n == c.n, "cstfem_spmv: size mismatch or no matrix");
+    DBuf<double> dx, dy;
+    dx.alloc(2 * (size_t)c.nb);
+    dy.alloc(2 * (size_t)c.nb);
+    HIP_CHECK(hipMemsetAsync(dx.get(), 0, sizeof(double) * dx.n, c.stream));
+    HIP_CHECK(hipMemcpyAsync(dx.get(), x, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    spmv_fp64(c, dx.get(), dy.get());
+    HIP_CHECK(hipMemcpyAsync(y, dy.get(), sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_profile_kernel(cstfem_ctx* ctx, int kind, int reps, int64_t bytes, double* avg_seconds) {
+  return guarded([&] {
+    need(ctx && avg_seconds, "cstfem_profile_kernel: NULL argument");
+    need(kind >= 0 && kind <= 4, "cstfem_profile_kernel: kind must be 0..4");
+    need(reps > 0, "cstfem_profile_kernel: reps must be positive");
+    need(kind != 4 || bytes >= 32, "cstfem_profile_kernel: copy needs bytes >= 32");
+    use_device(&ctx->c);
+    *avg_seconds = profile_kernel(ctx->c, kind, reps, bytes);
+    return CSTFEM_OK;
+  });
+}
+
+}  // extern "C"

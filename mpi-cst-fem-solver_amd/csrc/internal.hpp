@@ -1,0 +1,155 @@
+// internal.hpp -- shared state of the MI355X assemble-and-solve library.
+//
+// Device data layout (all resident in HBM for the life of a context):
+//   mesh:      xy[2nn] f64, tri[3ne] i32, E_elem[ne] f64 (optional)
+//   node->elements:  ne_ptr[nn+1], ne_list[3ne] (element ids ascending per node)
+//   pattern:   block CSR of 2x2 node blocks: brow_ptr[nb+1], bcol[nblk] (ascending per row),
+//              bdiag[nb] = position of the diagonal block; the reference's scalar CSR is the
+//              2x expansion (rows 2n, 2n+1 share [2m, 2m+1 for m in adj(n)]).
+//   slot map:  slot_ptr[nblk+1], slot_src[9ne]: contributions to each block sorted by
+//              element, encoded e*9 + a*3 + b (a, b = local node of the row / column).
+//   matrix:    SELL-64 of 2x2 blocks: slice s holds block rows 64s..64s+63; its w_s block
+//              columns are stored column-major with stride 64, so lane l of a wavefront
+//              reads block k of row 64s+l at  sell_off[s] + 64k + l  (32 contiguous bytes
+//              of values, 4 bytes of column index).  Padding blocks are zero with the row's
+//              own column.  sell_pos[q] maps CSR block q to its SELL slot.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cstfem_hip.h"
+
+namespace cstfem_hip {
+
+constexpr int kSlice = 64;  // block rows per SELL slice = one wavefront
+
+// Error carrying a C-ABI status code.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what, const char* file, int line);
+#define HIP_CHECK(x) ::cstfem_hip::hip_check((x), #x, __FILE__, __LINE__)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess)
+      throw Error(CSTFEM_ERR_ALLOC, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    n = count;
+  }
+  T* get() const { return p; }
+};
+
+// Scalars of one PCG solve, resident on the device.
+struct PcgScalars {
+  double rz, pAp, alpha, beta, rr, bnorm, rel, rtol;
+  double rz_new, pad0;
+  int k, status, max_iter, record;   // status: 0 running, 1 converged, 2 breakdown, 3 max_iter
+  unsigned int ticket[4];            // last-block-done counters, one per kernel class
+};
+
+enum { kRunning = 0, kConverged = 1, kBreakdown = 2, kMaxIter = 3 };
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+
+  // mesh (host copies kept for boundary loads)
+  int nn = 0;
+  int64_t ne = 0;
+  std::vector<double> h_xy;
+  std::vector<int> h_tri;
+  DBuf<double> xy, E_elem;
+  DBuf<int> tri;
+  bool has_E_elem = false;
+  double E = 1.0, nu = 0.3, thick = 1.0;
+  int plane_strain = 0;
+
+  DBuf<int> ne_ptr, ne_list;
+
+  // pattern (block CSR) and slot map
+  bool have_pattern = false;
+  bool mesh_pattern = false;  // pattern came from the mesh (slot map valid)
+  int nb = 0;                 // block rows
+  int n = 0;                  // scalar rows (== 2*nb for mesh matrices)
+  int64_t nblk = 0;           // real blocks
+  DBuf<int> brow_ptr, bcol, bdiag;
+  DBuf<int> slot_ptr, slot_src;
+
+  // SELL storage
+  int nslices = 0;
+  int64_t sell_blocks = 0;
+  std::vector<int64_t> h_sell_off;
+  DBuf<int64_t> sell_off;
+  DBuf<int> sell_col;
+  DBuf<double> sell_val;   // 4 per block, row-major 2x2 (v00 v01 v10 v11)
+  DBuf<int64_t> sell_pos;  // CSR block -> SELL slot
+  DBuf<float> sell_val32;  // FP32 copy for the mixed-precision path
+  bool vals32_valid = false;
+
+  // generic CSR upload: scalar entry -> SELL value index (host side, for download)
+  std::vector<int64_t> h_generic_map;
+  bool generic = false;
+
+  // vectors
+  DBuf<double> b, x, r, p, Ap, inv_diag, work;
+  DBuf<float> r32, p32, Ap32, x32, invd32;
+  bool setup_done = false;
+
+  // reduction scratch
+  DBuf<double> partials;   // 4 * max_blocks
+  DBuf<PcgScalars> scal;
+  PcgScalars* h_scal = nullptr;  // pinned
+  DBuf<int> err_flag;            // [0] = min bad index (INT_MAX = none)
+  int red_blocks = 0;            // grid used by reduction kernels
+
+  ~Context();
+};
+
+// thread-local error message of the C ABI
+void set_error(const std::string& m);
+
+// helpers implemented in the .hip files
+void exclusive_scan_int(Context& c, const int* in, int* out, int64_t count, int64_t* total);
+int grid_for(int64_t threads, int block, int max_grid);
+
+// kernels-level entry points (assembly.hip, pcg.hip, ...)
+void build_node_elements(Context& c);
+void build_pattern(Context& c);
+void build_sell(Context& c);
+void build_slot_map(Context& c);
+void assemble_values(Context& c);
+void download_values(Context& c, double* vals);
+void apply_dirichlet_device(Context& c, int m, const int* dofs, const double* values);
+void add_body_force_device(Context& c, double bx, double by);
+void setup_solver(Context& c);
+void spmv_fp64(Context& c, const double* x, double* y);
+int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
+                  double* history);
+int solve_ir_fp32(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st);
+void ensure_vectors(Context& c);
+double profile_kernel(Context& c, int kind, int reps, int64_t bytes);
+
+}  // namespace cstfem_hip

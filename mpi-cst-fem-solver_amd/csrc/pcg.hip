@@ -1,0 +1,664 @@
+// pcg.hip -- FP64 Jacobi-preconditioned CG on the device.
+//
+// Reference: src/solver.cpp:35-157 (pcg_iterate, pcg_jacobi, CgJacobiSolver::setup/solve).
+// The recurrence, the stopping test (rel < rtol), the breakdown test !(p'Ap > 0), the
+// zero-RHS exit and the true residual with its 10*rtol warning are the reference's.  Per
+// iteration three kernels run, each one HBM pass:
+//   A  Ap = A p (SELL-64 2x2 blocks) fused with p'Ap, then alpha = rz / p'Ap
+//   B  x += alpha p, r -= alpha Ap fused with r'r and r'(D^-1 r), then the convergence test,
+//      beta and rz
+//   C  p = D^-1 r + beta p
+// Scalars never leave the device: the last workgroup of each kernel (ticket counter) sums
+// the per-workgroup partials in a fixed order, so results are run-to-run reproducible.
+// Iterations are replayed as a captured hipGraph of `chunk` iterations; kernels return at
+// once after convergence, and the host reads the status once per chunk.
+// Element-wise arithmetic and each SpMV row sum follow the reference's operation order
+// (no FMA contraction); only the dot products are summed in a different order.
+#include <climits>
+#include <cmath>
+#include <cstdio>
+
+#include "internal.hpp"
+
+namespace cstfem_hip {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sums `v` over the workgroup; the result is valid in thread 0.
+__device__ inline double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += lds[i];
+  __syncthreads();
+  return s;
+}
+
+// Last-workgroup-done election: every workgroup calls this after writing its partial.
+__device__ inline bool elect_last(unsigned int* ticket) {
+  __shared__ bool last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int t =
+        __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
+// Fixed-order sum of `count` partials (stride `stride`) by one workgroup; valid in thread 0.
+__device__ inline double sum_partials(const double* part, int count, int stride, double* lds) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < count; i += blockDim.x)
+    s += __hip_atomic_load(part + (size_t)i * stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return block_sum(s, lds);
+}
+
+// Maps workgroup b to its share of `nslices` so that the 8 XCDs (which receive workgroups
+// round-robin) each sweep one contiguous range of slices -> neighbouring x entries stay in
+// one XCD's L2.
+struct SliceRange {
+  int first, end, step;
+};
+__device__ inline SliceRange slice_range(int nslices) {
+  const int G = gridDim.x;
+  const int wave = threadIdx.x >> 6;
+  if (G >= 64 && (G & 7) == 0) {
+    const int xcd = blockIdx.x & 7, idx = blockIdx.x >> 3, per = G >> 3;
+    const int lo = (int)((int64_t)nslices * xcd / 8), hi = (int)((int64_t)nslices * (xcd + 1) / 8);
+    return {lo + idx * kWaves + wave, hi, per * kWaves};
+  }
+  return {(int)blockIdx.x * kWaves + wave, nslices, G * kWaves};
+}
+
+// y rows 2r, 2r+1 of the SELL matrix times x; sequential per-row sums in column order.
+__device__ inline void sell_row(const int64_t* __restrict__ off, const int* __restrict__ col,
+                                const double2* __restrict__ val, const double2* __restrict__ x,
+                                int s, int lane, double& y0, double& y1) {
+  const int64_t o = off[s];
+  const int w = (int)((off[s + 1] - o) >> 6);
+  const int* c = col + o + lane;
+  const v2d* v = reinterpret_cast<const v2d*>(val) + 2 * (o + lane);
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll 4
+  for (int k = 0; k < w; ++k) {
+    const int m = __builtin_nontemporal_load(c + (size_t)k * 64);
+    const v2d va = __builtin_nontemporal_load(v + (size_t)k * 128);
+    const v2d vb = __builtin_nontemporal_load(v + (size_t)k * 128 + 1);
+    const double2 xm = x[m];
+    a0 = a0 + va.x * xm.x;
+    a0 = a0 + va.y * xm.y;
+    a1 = a1 + vb.x * xm.x;
+    a1 = a1 + vb.y * xm.y;
+  }
+  y0 = a0;
+  y1 = a1;
+}
+
+struct MatArgs {
+  const int64_t* off;
+  const int* col;
+  const double2* val;
+  int nslices, nb;
+};
+
+__global__ __launch_bounds__(kThreads) void k_spmv(MatArgs A, const double2* __restrict__ x,
+                                                   double2* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const SliceRange sr = slice_range(A.nslices);
+  for (int s = sr.first; s < sr.end; s += sr.step) {
+    const int r = s * 64 + lane;
+    double y0, y1;
+    sell_row(A.off, A.col, A.val, x, s, lane, y0, y1);
+    if (r < A.nb) y[r] = make_double2(y0, y1);
+  }
+}
+
+// A: Ap = A p, p'Ap, alpha
+__global__ __launch_bounds__(kThreads) void k_cg_spmv(MatArgs A, const double2* __restrict__ p,
+                                                      double2* __restrict__ Ap,
+                                                      double* __restrict__ part,
+                                                      PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kWaves];
+  if (sc->status != kRunning) return;
+  const int lane = threadIdx.x & 63;
+  const SliceRange sr = slice_range(A.nslices);
+  double local = 0.0;
+  for (int s = sr.first; s < sr.end; s += sr.step) {
+    const int r = s * 64 + lane;
+    double y0, y1;
+    sell_row(A.off, A.col, A.val, p, s, lane, y0, y1);
+    if (r < A.nb) {
+      Ap[r] = make_double2(y0, y1);
+      const double2 pr = p[r];
+      local += pr.x * y0 + pr.y * y1;
+    }
+  }
+  const double bs = block_sum(local, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+  if (!elect_last(&sc->ticket[0])) return;
+  const double pAp = sum_partials(part, gridDim.x, 1, lds);
+  if (threadIdx.x == 0) {
+    sc->ticket[0] = 0;
+    sc->pAp = pAp;
+    if (!(pAp > 0.0)) {
+      sc->status = kBreakdown;
+    } else {
+      sc->alpha = sc->rz / pAp;
+    }
+  }
+}
+
+// B: x += alpha p; r -= alpha Ap; r'r, r'z; convergence test; beta
+__global__ __launch_bounds__(kThreads) void k_cg_update(
+    int nvec2, double2* __restrict__ x, double2* __restrict__ r, const double2* __restrict__ p,
+    const double2* __restrict__ Ap, const double2* __restrict__ invd, double* __restrict__ part,
+    PcgScalars* __restrict__ sc, double* __restrict__ history) {
+  __shared__ double lds[kWaves];
+  if (sc->status != kRunning) return;
+  const double alpha = sc->alpha;
+  double rr = 0.0, rz = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const double2 pi = p[i], api = Ap[i], di = invd[i];
+    double2 xi = x[i], ri = r[i];
+    xi.x = xi.x + alpha * pi.x;
+    xi.y = xi.y + alpha * pi.y;
+    ri.x = ri.x - alpha * api.x;
+    ri.y = ri.y - alpha * api.y;
+    x[i] = xi;
+    r[i] = ri;
+    rr += ri.x * ri.x + ri.y * ri.y;
+    rz += ri.x * (di.x * ri.x) + ri.y * (di.y * ri.y);
+  }
+  const double b_rr = block_sum(rr, lds);
+  const double b_rz = block_sum(rz, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = b_rr;
+    part[2 * blockIdx.x + 1] = b_rz;
+  }
+  if (!elect_last(&sc->ticket[1])) return;
+  const double t_rr = sum_partials(part, gridDim.x, 2, lds);
+  const double t_rz = sum_partials(part + 1, gridDim.x, 2, lds);
+  if (threadIdx.x == 0) {
+    sc->ticket[1] = 0;
+    const int k = sc->k + 1;
+    sc->k = k;
+    const double rel = sqrt(t_rr) / sc->bnorm;
+    sc->rel = rel;
+    sc->rr = t_rr;
+    if (sc->record) history[k] = rel;
+    if (rel < sc->rtol) {
+      sc->status = kConverged;
+    } else if (k >= sc->max_iter) {
+      sc->status = kMaxIter;
+    } else {
+      sc->beta = t_rz / sc->rz;
+      sc->rz = t_rz;
+    }
+  }
+}
+
+// C: p = D^-1 r + beta p
+__global__ __launch_bounds__(kThreads) void k_cg_direction(int nvec2, const double2* __restrict__ r,
+                                                           const double2* __restrict__ invd,
+                                                           double2* __restrict__ p,
+                                                           const PcgScalars* __restrict__ sc) {
+  if (sc->status != kRunning) return;
+  const double beta = sc->beta;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const double2 ri = r[i], di = invd[i];
+    double2 pi = p[i];
+    const double z0 = di.x * ri.x, z1 = di.y * ri.y;
+    pi.x = z0 + beta * pi.x;
+    pi.y = z1 + beta * pi.y;
+    p[i] = pi;
+  }
+}
+
+// x = 0, r = b, p = D^-1 b; b'b and r'z; bnorm, zero-RHS exit
+__global__ __launch_bounds__(kThreads) void k_cg_init(int nvec2, const double2* __restrict__ b,
+                                                      double2* __restrict__ x,
+                                                      double2* __restrict__ r,
+                                                      double2* __restrict__ p,
+                                                      const double2* __restrict__ invd,
+                                                      double* __restrict__ part,
+                                                      PcgScalars* __restrict__ sc,
+                                                      double* __restrict__ history) {
+  __shared__ double lds[kWaves];
+  double bb = 0.0, rz = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const double2 bi = b[i], di = invd[i];
+    x[i] = make_double2(0.0, 0.0);
+    r[i] = bi;
+    const double z0 = di.x * bi.x, z1 = di.y * bi.y;
+    p[i] = make_double2(z0, z1);
+    bb += bi.x * bi.x + bi.y * bi.y;
+    rz += bi.x * z0 + bi.y * z1;
+  }
+  const double b_bb = block_sum(bb, lds);
+  const double b_rz = block_sum(rz, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = b_bb;
+    part[2 * blockIdx.x + 1] = b_rz;
+  }
+  if (!elect_last(&sc->ticket[2])) return;
+  const double t_bb = sum_partials(part, gridDim.x, 2, lds);
+  const double t_rz = sum_partials(part + 1, gridDim.x, 2, lds);
+  if (threadIdx.x == 0) {
+    sc->ticket[2] = 0;
+    sc->bnorm = sqrt(t_bb);
+    sc->rz = t_rz;
+    sc->k = 0;
+    sc->rel = 1.0;
+    if (sc->bnorm == 0.0) {
+      sc->status = kConverged;
+      if (sc->record) history[0] = 0.0;
+    } else {
+      sc->status = kRunning;
+      if (sc->record) history[0] = 1.0;
+    }
+  }
+}
+
+// r = b - y; returns ||r||^2 into sc->rr (true residual of pcg_jacobi, solver.cpp:117-122)
+__global__ __launch_bounds__(kThreads) void k_residual(int nvec2, const double2* __restrict__ b,
+                                                       const double2* __restrict__ y,
+                                                       double2* __restrict__ r,
+                                                       double* __restrict__ part,
+                                                       PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kWaves];
+  double rr = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const double2 bi = b[i], yi = y[i];
+    const double2 ri = make_double2(bi.x - yi.x, bi.y - yi.y);
+    r[i] = ri;
+    rr += ri.x * ri.x + ri.y * ri.y;
+  }
+  const double b_rr = block_sum(rr, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = b_rr;
+  if (!elect_last(&sc->ticket[3])) return;
+  const double t = sum_partials(part, gridDim.x, 1, lds);
+  if (threadIdx.x == 0) {
+    sc->ticket[3] = 0;
+    sc->rr = t;
+  }
+}
+
+// inv_diag = 1 / K_ii (CgJacobiSolver::setup, solver.cpp:133-148)
+__global__ void k_inv_diag(int nb, int n, const int* __restrict__ bdiag,
+                           const int64_t* __restrict__ spos, const double* __restrict__ val,
+                           double* __restrict__ invd, int* __restrict__ err) {
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= nb) return;
+  const int q = bdiag[nd];
+  for (int c = 0; c < 2; ++c) {
+    const int i = 2 * nd + c;
+    if (i >= n) {
+      invd[i] = 0.0;  // padding row of an odd-sized generic matrix
+      continue;
+    }
+    const double d = q >= 0 ? val[4 * spos[q] + 3 * c] : 0.0;
+    if (!(d > 0.0)) {
+      atomicMin(err, i);
+      invd[i] = 0.0;
+    } else {
+      invd[i] = 1.0 / d;
+    }
+  }
+}
+
+MatArgs mat_args(Context& c) {
+  return MatArgs{c.sell_off.get(), c.sell_col.get(), (const double2*)c.sell_val.get(), c.nslices,
+                 c.nb};
+}
+
+int spmv_grid(Context& c) {
+  const int want = (c.nslices + kWaves - 1) / kWaves;
+  const int cap = c.cu_count > 0 ? 8 * c.cu_count : 2048;
+  int g = want < cap ? want : cap;
+  if (g >= 64) g &= ~7;
+  return g < 1 ? 1 : g;
+}
+
+int vec_grid(Context& c) {
+  const int nvec2 = c.nb;
+  const int cap = c.cu_count > 0 ? 4 * c.cu_count : 1024;
+  return grid_for(nvec2, kThreads, cap);
+}
+
+}  // namespace
+
+void ensure_vectors(Context& c) {
+  const size_t nv = 2 * (size_t)c.nb;
+  c.x.alloc(nv);
+  c.r.alloc(nv);
+  c.p.alloc(nv);
+  c.Ap.alloc(nv);
+  c.inv_diag.alloc(nv);
+  if (c.b.n != nv) {
+    c.b.alloc(nv);
+    HIP_CHECK(hipMemsetAsync(c.b.get(), 0, sizeof(double) * nv, c.stream));
+  }
+  const int parts = 2 * (c.cu_count > 0 ? 8 * c.cu_count : 2048) + 64;
+  c.partials.alloc(parts);
+}
+
+void setup_solver(Context& c) {
+  if (!c.have_pattern) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: no matrix on the device");
+  ensure_vectors(c);
+  const int big = INT_MAX;
+  HIP_CHECK(hipMemcpyAsync(c.err_flag.get(), &big, sizeof(int), hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(c.nb, 256, 0)), dim3(256), 0, c.stream, c.nb, c.n,
+                     c.bdiag.get(), c.sell_pos.get(), c.sell_val.get(), c.inv_diag.get(),
+                     c.err_flag.get());
+  HIP_CHECK(hipGetLastError());
+  int bad = INT_MAX;
+  HIP_CHECK(hipMemcpyAsync(&bad, c.err_flag.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (bad != INT_MAX)
+    throw Error(CSTFEM_ERR_BAD_DIAGONAL, "cg_jacobi: non-positive diagonal at row " + std::to_string(bad));
+  c.setup_done = true;
+}
+
+void spmv_fp64(Context& c, const double* x, double* y) {
+  hipLaunchKernelGGL(k_spmv, dim3(spmv_grid(c)), dim3(kThreads), 0, c.stream, mat_args(c),
+                     (const double2*)x, (double2*)y);
+  HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+
+struct Graph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  ~Graph() {
+    if (e) (void)hipGraphExecDestroy(e);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+void enqueue_iteration(Context& c, int gs, int gv, double* history) {
+  const int nvec2 = c.nb;
+  hipLaunchKernelGGL(k_cg_spmv, dim3(gs), dim3(kThreads), 0, c.stream, mat_args(c),
+                     (const double2*)c.p.get(), (double2*)c.Ap.get(), c.partials.get(), c.scal.get());
+  hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
+                     (double2*)c.x.get(), (double2*)c.r.get(), (const double2*)c.p.get(),
+                     (const double2*)c.Ap.get(), (const double2*)c.inv_diag.get(),
+                     c.partials.get(), c.scal.get(), history);
+  hipLaunchKernelGGL(k_cg_direction, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
+                     (const double2*)c.r.get(), (const double2*)c.inv_diag.get(),
+                     (double2*)c.p.get(), (const PcgScalars*)c.scal.get());
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+  return ms * 1e-3;
+}
+
+}  // namespace
+
+int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
+                  double* history) {
+  if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: solve() called before setup()");
+  const int nvec2 = c.nb;
+  const int gs = spmv_grid(c), gv = vec_grid(c);
+  DBuf<double> d_hist;
+  if (o.record_history) d_hist.alloc((size_t)o.max_iter + 1);
+  double* hist = o.record_history ? d_hist.get() : nullptr;
+
+  PcgScalars init{};
+  init.rtol = o.rtol;
+  init.max_iter = o.max_iter;
+  init.record = o.record_history ? 1 : 0;
+  HIP_CHECK(hipMemcpyAsync(c.scal.get(), &init, sizeof init, hipMemcpyHostToDevice, c.stream));
+
+  hipEvent_t ev[4];
+  for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+  HIP_CHECK(hipEventRecord(ev[0], c.stream));
+  hipLaunchKernelGGL(k_cg_init, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
+                     (const double2*)c.b.get(), (double2*)c.x.get(), (double2*)c.r.get(),
+                     (double2*)c.p.get(), (const double2*)c.inv_diag.get(), c.partials.get(),
+                     c.scal.get(), hist);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(c.h_scal, c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  const bool zero_rhs = c.h_scal->bnorm == 0.0;
+
+  double t_spmv = 0.0, t_vec = 0.0;
+  if (!zero_rhs && o.max_iter > 0) {
+    if (o.instrument) {
+      // eager launches with events around each kernel class (same kernels, same results)
+      hipEvent_t e0, e1, e2, e3;
+      HIP_CHECK(hipEventCreate(&e0));
+      HIP_CHECK(hipEventCreate(&e1));
+      HIP_CHECK(hipEventCreate(&e2));
+      HIP_CHECK(hipEventCreate(&e3));
+      for (;;) {
+        HIP_CHECK(hipEventRecord(e0, c.stream));
+        hipLaunchKernelGGL(k_cg_spmv, dim3(gs), dim3(kThreads), 0, c.stream, mat_args(c),
+                           (const double2*)c.p.get(), (double2*)c.Ap.get(), c.partials.get(),
+                           c.scal.get());
+        HIP_CHECK(hipEventRecord(e1, c.stream));
+        hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
+                           (double2*)c.x.get(), (double2*)c.r.get(), (const double2*)c.p.get(),
+                           (const double2*)c.Ap.get(), (const double2*)c.inv_diag.get(),
+                           c.partials.get(), c.scal.get(), hist);
+        hipLaunchKernelGGL(k_cg_direction, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
+                           (const double2*)c.r.get(), (const double2*)c.inv_diag.get(),
+                           (double2*)c.p.get(), (const PcgScalars*)c.scal.get());
+        HIP_CHECK(hipEventRecord(e2, c.stream));
+        HIP_CHECK(hipMemcpyAsync(c.h_scal, c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost,
+                                 c.stream));
+        HIP_CHECK(hipEventRecord(e3, c.stream));
+        HIP_CHECK(hipEventSynchronize(e3));
+        t_spmv += elapsed(e0, e1);
+        t_vec += elapsed(e1, e2);
+        if (c.h_scal->status != kRunning) break;
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      (void)hipEventDestroy(e2);
+      (void)hipEventDestroy(e3);
+    } else {
+      // chunk of iterations captured once, replayed until the device status leaves kRunning;
+      // one chunk stays in flight while the previous chunk's status is read.
+      const int chunk = c.n > 200000 ? 16 : 64;
+      Graph gr;
+      HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < chunk; ++i) enqueue_iteration(c, gs, gv, hist);
+      HIP_CHECK(hipStreamEndCapture(c.stream, &gr.g));
+      HIP_CHECK(hipGraphInstantiate(&gr.e, gr.g, nullptr, nullptr, 0));
+      PcgScalars* slot[2] = {c.h_scal, c.h_scal + 1};
+      hipEvent_t done[2];
+      HIP_CHECK(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+      HIP_CHECK(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+      const int64_t max_chunks = ((int64_t)o.max_iter + chunk - 1) / chunk + 1;
+      int cur = 0;
+      HIP_CHECK(hipGraphLaunch(gr.e, c.stream));
+      HIP_CHECK(hipMemcpyAsync(slot[cur], c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
+      HIP_CHECK(hipEventRecord(done[cur], c.stream));
+      for (int64_t ch = 1; ch <= max_chunks; ++ch) {
+        const int nxt = cur ^ 1;
+        HIP_CHECK(hipGraphLaunch(gr.e, c.stream));
+        HIP_CHECK(hipMemcpyAsync(slot[nxt], c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
+        HIP_CHECK(hipEventRecord(done[nxt], c.stream));
+        HIP_CHECK(hipEventSynchronize(done[cur]));
+        if (slot[cur]->status != kRunning) break;
+        cur = nxt;
+      }
+      HIP_CHECK(hipStreamSynchronize(c.stream));
+      (void)hipEventDestroy(done[0]);
+      (void)hipEventDestroy(done[1]);
+    }
+  }
+  HIP_CHECK(hipMemcpyAsync(c.h_scal, c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  const PcgScalars s = *c.h_scal;
+
+  // true residual b - A x (pcg_jacobi, solver.cpp:113-129)
+  double true_rel = 0.0;
+  if (!zero_rhs) {
+    hipEvent_t t0, t1;
+    HIP_CHECK(hipEventCreate(&t0));
+    HIP_CHECK(hipEventCreate(&t1));
+    HIP_CHECK(hipEventRecord(t0, c.stream));
+    spmv_fp64(c, c.x.get(), c.Ap.get());
+    HIP_CHECK(hipEventRecord(t1, c.stream));
+    hipLaunchKernelGGL(k_residual, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
+                       (const double2*)c.b.get(), (const double2*)c.Ap.get(), (double2*)c.r.get(),
+                       c.partials.get(), c.scal.get());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(c.h_scal, c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    true_rel = std::sqrt(c.h_scal->rr) / s.bnorm;
+    t_spmv += elapsed(t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+  }
+  HIP_CHECK(hipEventRecord(ev[1], c.stream));
+  HIP_CHECK(hipEventSynchronize(ev[1]));
+  const double total = elapsed(ev[0], ev[1]);
+  for (auto& e : ev) (void)hipEventDestroy(e);
+
+  if (history && o.record_history) {
+    const int cnt = zero_rhs ? 1 : s.k + 1;
+    HIP_CHECK(hipMemcpy(history, hist, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+  }
+  st.iterations = zero_rhs ? 0 : s.k;
+  st.converged = (zero_rhs || s.status == kConverged) ? 1 : 0;
+  st.breakdown = s.status == kBreakdown ? 1 : 0;
+  st.zero_rhs = zero_rhs ? 1 : 0;
+  st.rel_residual = zero_rhs ? 0.0 : s.rel;
+  st.true_rel_residual = true_rel;
+  st.breakdown_pap = s.status == kBreakdown ? s.pAp : 0.0;
+  st.warning = (st.converged && !zero_rhs && true_rel > 10.0 * o.rtol) ? 1 : 0;
+  st.outer_iterations = 0;
+  st.fallback = 0;
+  st.spmv_calls = zero_rhs ? 0 : (long long)st.iterations + (st.breakdown ? 1 : 0) + 1;
+  // the reference's reduction count: norm(b), r'z, then per iteration p'Ap, ||r|| (+ r'z
+  // unless it stopped), then pcg_jacobi's norm(b) and ||b - Ax||
+  long long red = 2;
+  if (!zero_rhs) {
+    const long long k = st.iterations;
+    red += 2 * k + (st.converged ? k - 1 : k) + (st.breakdown ? 1 : 0) + 2;
+  }
+  st.reductions = red;
+  if (o.instrument) {
+    st.spmv_local = t_spmv;
+    st.vector_ops = t_vec;
+    st.dot_local = 0.0;  // fused into the SpMV and update kernels
+    st.total = total;
+  } else {
+    st.spmv_local = st.vector_ops = st.dot_local = 0.0;
+    st.total = total;
+  }
+  if (zero_rhs || s.status == kConverged) return CSTFEM_OK;
+  return CSTFEM_NOT_CONVERGED;
+}
+
+namespace {
+
+__global__ __launch_bounds__(kThreads) void k_copy(int64_t n2, const double2* __restrict__ a,
+                                                   double2* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n2;
+       i += (int64_t)gridDim.x * kThreads)
+    reinterpret_cast<v2d*>(b)[i] = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(a) + i);
+}
+
+}  // namespace
+
+// Average duration (seconds) of `reps` back-to-back launches of one hot kernel on the
+// library stream, between two HIP events.  kind: 0 fused SpMV + p'Ap (k_cg_spmv),
+// 1 plain SpMV, 2 update (k_cg_update), 3 direction (k_cg_direction), 4 copy of
+// `bytes` bytes (achievable HBM bandwidth).  Solver state is not modified (private
+// scalars and vectors).
+double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
+  if (kind != 4 && !c.have_pattern) throw Error(CSTFEM_ERR_ORDER, "profile: no matrix on the device");
+  if (kind != 4) ensure_vectors(c);
+  DBuf<PcgScalars> sc;
+  sc.alloc(1);
+  PcgScalars init{};
+  init.rz = 1.0;
+  init.bnorm = 1.0;
+  init.rtol = 0.0;
+  init.max_iter = INT_MAX;
+  HIP_CHECK(hipMemcpyAsync(sc.get(), &init, sizeof init, hipMemcpyHostToDevice, c.stream));
+  DBuf<double> a, b, w;
+  const size_t nv = 2 * (size_t)c.nb;
+  if (kind == 4) {
+    const int64_t n2 = bytes / 32;
+    a.alloc(2 * n2);
+    b.alloc(2 * n2);
+    HIP_CHECK(hipMemsetAsync(a.get(), 0, sizeof(double) * 2 * n2, c.stream));
+  } else {
+    a.alloc(nv);
+    b.alloc(nv);
+    w.alloc(nv);
+    HIP_CHECK(hipMemsetAsync(a.get(), 0, sizeof(double) * nv, c.stream));
+    HIP_CHECK(hipMemsetAsync(w.get(), 0, sizeof(double) * nv, c.stream));
+    HIP_CHECK(hipMemcpyAsync(b.get(), c.inv_diag.get(), sizeof(double) * nv, hipMemcpyDeviceToDevice, c.stream));
+  }
+  const int gs = spmv_grid(c), gv = vec_grid(c);
+  auto launch = [&]() {
+    switch (kind) {
+      case 0:
+        hipLaunchKernelGGL(k_cg_spmv, dim3(gs), dim3(kThreads), 0, c.stream, mat_args(c),
+                           (const double2*)c.inv_diag.get(), (double2*)a.get(), c.partials.get(), sc.get());
+        break;
+      case 1:
+        hipLaunchKernelGGL(k_spmv, dim3(gs), dim3(kThreads), 0, c.stream, mat_args(c),
+                           (const double2*)c.inv_diag.get(), (double2*)a.get());
+        break;
+      case 2:
+        hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                           (double2*)a.get(), (double2*)w.get(), (const double2*)b.get(),
+                           (const double2*)b.get(), (const double2*)c.inv_diag.get(),
+                           c.partials.get(), sc.get(), (double*)nullptr);
+        break;
+      case 3:
+        hipLaunchKernelGGL(k_cg_direction, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                           (const double2*)b.get(), (const double2*)c.inv_diag.get(),
+                           (double2*)a.get(), (const PcgScalars*)sc.get());
+        break;
+      default: {
+        const int64_t n2 = bytes / 32;
+        const int g = c.cu_count > 0 ? 8 * c.cu_count : 2048;
+        hipLaunchKernelGGL(k_copy, dim3(g), dim3(kThreads), 0, c.stream, n2,
+                           (const double2*)a.get(), (double2*)b.get());
+      }
+    }
+  };
+  launch();  // warm-up
+  HIP_CHECK(hipGetLastError());
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  HIP_CHECK(hipEventRecord(e0, c.stream));
+  for (int i = 0; i < reps; ++i) launch();
+  HIP_CHECK(hipEventRecord(e1, c.stream));
+  HIP_CHECK(hipEventSynchronize(e1));
+  const double t = elapsed(e0, e1) / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return t;
+}
+
+}  // namespace cstfem_hip

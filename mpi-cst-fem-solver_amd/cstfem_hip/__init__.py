@@ -1,0 +1,310 @@
+"""Python bindings of the C ABI in include/cstfem_hip.h (ctypes over libcstfem_hip.so).
+
+The names follow the reference's C++ interface (src/assembly.cpp, src/bc.cpp,
+src/solver.cpp): ``build_stiffness_pattern``, ``assemble_stiffness``, ``apply_dirichlet``,
+``setup`` / ``solve`` of a Jacobi-CG solver.  Errors raise :class:`CstfemError` (the
+reference's ``std::runtime_error``) or :class:`CstfemLogicError` (``std::logic_error``)
+with the reference's message prefixes.  There is no CPU fallback: when the HIP library is
+missing or no GPU is visible, every call fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import meshes  # noqa: F401
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libcstfem_hip.so")
+
+OK, NOT_CONVERGED = 0, 2
+ERR_ARGUMENT, ERR_DEVICE, ERR_BAD_DIAGONAL, ERR_MISSING_ENTRY = -1, -2, -3, -4
+ERR_OUT_OF_RANGE, ERR_ALLOC, ERR_TRACTION_EDGE, ERR_CONFLICT, ERR_ORDER = -5, -6, -7, -8, -9
+PLANE_STRESS, PLANE_STRAIN = 0, 1
+METHOD_CG_FP64, METHOD_IR_FP32, METHOD_FCG_LEARNED = 0, 1, 2
+
+
+class CstfemError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class CstfemLogicError(CstfemError):
+    pass
+
+
+class SolverOptions(ctypes.Structure):
+    _fields_ = [("rtol", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("record_history", ctypes.c_int), ("instrument", ctypes.c_int),
+                ("method", ctypes.c_int), ("inner_rtol", ctypes.c_double),
+                ("max_outer", ctypes.c_int)]
+
+
+class SolverStatsC(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("breakdown", ctypes.c_int), ("zero_rhs", ctypes.c_int),
+                ("outer_iterations", ctypes.c_int), ("fallback", ctypes.c_int),
+                ("warning", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("rel_residual", ctypes.c_double), ("true_rel_residual", ctypes.c_double),
+                ("breakdown_pap", ctypes.c_double), ("setup", ctypes.c_double),
+                ("spmv_local", ctypes.c_double), ("dot_local", ctypes.c_double),
+                ("vector_ops", ctypes.c_double), ("total", ctypes.c_double),
+                ("spmv_calls", ctypes.c_longlong), ("reductions", ctypes.c_longlong)]
+
+
+@dataclass
+class SolverStats:
+    """Mirror of SolverStats (solver.hpp:99-107) with the reference's message strings."""
+    iterations: int = 0
+    rel_residual: float = 0.0
+    true_rel_residual: float = 0.0
+    converged: bool = False
+    message: str = ""
+    warning: str = ""
+    residual_history: list = field(default_factory=list)
+    outer_iterations: int = 0
+    fallback: bool = False
+    breakdown: dict = field(default_factory=dict)
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+
+SIGNATURES = {
+    "cstfem_last_error": (ctypes.c_char_p, []),
+    "cstfem_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+    "cstfem_ctx_destroy": (ctypes.c_int, [_P]),
+    "cstfem_device_info": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, _I]),
+    "cstfem_upload_mesh": (ctypes.c_int, [_P, ctypes.c_int, _D, ctypes.c_int64, _I]),
+    "cstfem_set_material": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_double, _D]),
+    "cstfem_build_pattern": (ctypes.c_int, [_P, _I64]),
+    "cstfem_download_pattern": (ctypes.c_int, [_P, _I, _I]),
+    "cstfem_assemble": (ctypes.c_int, [_P]),
+    "cstfem_download_values": (ctypes.c_int, [_P, _D]),
+    "cstfem_upload_csr": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _D]),
+    "cstfem_set_rhs": (ctypes.c_int, [_P, _D, ctypes.c_int]),
+    "cstfem_download_rhs": (ctypes.c_int, [_P, _D, ctypes.c_int]),
+    "cstfem_add_body_force": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double]),
+    "cstfem_add_traction": (ctypes.c_int, [_P, ctypes.c_int, _I, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double]),
+    "cstfem_apply_dirichlet": (ctypes.c_int, [_P, ctypes.c_int, _I, _D]),
+    "cstfem_setup": (ctypes.c_int, [_P]),
+    "cstfem_solve": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions),
+                                    ctypes.POINTER(SolverStatsC), _D, _D]),
+    "cstfem_download_solution": (ctypes.c_int, [_P, _D, ctypes.c_int]),
+    "cstfem_spmv": (ctypes.c_int, [_P, _D, _D, ctypes.c_int]),
+    "cstfem_profile_kernel": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _D]),
+}
+
+
+def lib():
+    """Loads libcstfem_hip.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype)) if a is not None else None
+
+
+def _check(rc: int):
+    if rc in (OK, NOT_CONVERGED):
+        return rc
+    msg = lib().cstfem_last_error().decode()
+    if rc in (ERR_MISSING_ENTRY, ERR_ORDER):
+        raise CstfemLogicError(rc, msg)
+    raise CstfemError(rc, msg)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Device:
+    """One context on one GPU: mesh, matrix, right-hand side and solver state in HBM."""
+
+    def __init__(self, device: int = 0):
+        self._h = _P()
+        _check(lib().cstfem_ctx_create(device, ctypes.byref(self._h)))
+        self.n = 0
+        self.nnz = 0
+        self.mesh = None
+
+    def close(self):
+        if self._h:
+            lib().cstfem_ctx_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self):
+        buf = ctypes.create_string_buffer(256)
+        cu = ctypes.c_int()
+        _check(lib().cstfem_device_info(self._h, buf, 256, ctypes.byref(cu)))
+        return buf.value.decode(), cu.value
+
+    # ---- mesh and material
+    def upload_mesh(self, mesh):
+        self.mesh = mesh
+        xy, tri = _f64(mesh.xy), _i32(mesh.tri)
+        _check(lib().cstfem_upload_mesh(self._h, int(xy.shape[0]), _ptr(xy, ctypes.c_double),
+                                        int(tri.shape[0]), _ptr(tri, ctypes.c_int)))
+        self.n = 2 * int(xy.shape[0])
+
+    def set_material(self, E=1.0, nu=0.3, plane=PLANE_STRESS, thickness=1.0, E_elem=None):
+        Ee = None if E_elem is None else _f64(E_elem)
+        _check(lib().cstfem_set_material(self._h, float(E), float(nu), int(plane),
+                                         float(thickness), _ptr(Ee, ctypes.c_double)))
+
+    # ---- pattern and assembly (assembly.cpp:9-74)
+    def build_stiffness_pattern(self):
+        nnz = ctypes.c_int64()
+        _check(lib().cstfem_build_pattern(self._h, ctypes.byref(nnz)))
+        self.nnz = nnz.value
+        return self.nnz
+
+    def download_pattern(self):
+        row_ptr = np.empty(self.n + 1, dtype=np.int32)
+        cols = np.empty(self.nnz, dtype=np.int32)
+        _check(lib().cstfem_download_pattern(self._h, _ptr(row_ptr, ctypes.c_int),
+                                             _ptr(cols, ctypes.c_int)))
+        return row_ptr, cols
+
+    def assemble_stiffness(self):
+        _check(lib().cstfem_assemble(self._h))
+
+    def download_values(self):
+        v = np.empty(self.nnz, dtype=np.float64)
+        _check(lib().cstfem_download_values(self._h, _ptr(v, ctypes.c_double)))
+        return v
+
+    def upload_csr(self, row_ptr, cols, vals):
+        rp, c, v = _i32(row_ptr), _i32(cols), _f64(vals)
+        n = int(rp.shape[0]) - 1
+        _check(lib().cstfem_upload_csr(self._h, n, _ptr(rp, ctypes.c_int), _ptr(c, ctypes.c_int),
+                                       _ptr(v, ctypes.c_double)))
+        self.n, self.nnz = n, int(rp[-1])
+
+    # ---- loads and Dirichlet (bc.cpp:137-253)
+    def set_rhs(self, b):
+        b = _f64(b)
+        _check(lib().cstfem_set_rhs(self._h, _ptr(b, ctypes.c_double), int(b.shape[0])))
+
+    def download_rhs(self):
+        b = np.empty(self.n, dtype=np.float64)
+        _check(lib().cstfem_download_rhs(self._h, _ptr(b, ctypes.c_double), self.n))
+        return b
+
+    def add_traction(self, edges, tx=0.0, ty=0.0, kirsch=None):
+        e = _i32(edges).reshape(-1)
+        if kirsch is not None:
+            mode, p0, p1 = 1, float(kirsch[0]), float(kirsch[1])
+        else:
+            mode, p0, p1 = 0, float(tx), float(ty)
+        _check(lib().cstfem_add_traction(self._h, int(e.size // 2), _ptr(e, ctypes.c_int), mode,
+                                         p0, p1))
+
+    def add_body_force(self, bx, by):
+        _check(lib().cstfem_add_body_force(self._h, float(bx), float(by)))
+
+    def apply_dirichlet(self, dofs, values):
+        d, v = _i32(dofs), _f64(values)
+        _check(lib().cstfem_apply_dirichlet(self._h, int(d.size), _ptr(d, ctypes.c_int),
+                                            _ptr(v, ctypes.c_double)))
+
+    # ---- solve (solver.cpp:109-157)
+    def setup(self):
+        _check(lib().cstfem_setup(self._h))
+
+    def solve(self, rtol=1e-10, max_iter=20000, record_history=False, instrument=False,
+              method=METHOD_CG_FP64, inner_rtol=0.0, max_outer=0, download=True):
+        o = SolverOptions(rtol, max_iter, int(record_history), int(instrument), method,
+                          inner_rtol, max_outer)
+        s = SolverStatsC()
+        x = np.empty(self.n, dtype=np.float64) if download else None
+        hist = np.zeros(max_iter + 1, dtype=np.float64) if record_history else None
+        rc = _check(lib().cstfem_solve(self._h, ctypes.byref(o), ctypes.byref(s),
+                                       _ptr(x, ctypes.c_double), _ptr(hist, ctypes.c_double)))
+        st = _stats_from(s, o, rc, hist)
+        return x, st
+
+    def download_solution(self):
+        x = np.empty(self.n, dtype=np.float64)
+        _check(lib().cstfem_download_solution(self._h, _ptr(x, ctypes.c_double), self.n))
+        return x
+
+    def profile_kernel(self, kind, reps=20, nbytes=0):
+        """Average seconds per launch of a hot kernel (0 fused SpMV, 1 SpMV, 2 update,
+        3 direction, 4 copy of nbytes)."""
+        t = ctypes.c_double()
+        _check(lib().cstfem_profile_kernel(self._h, int(kind), int(reps), int(nbytes),
+                                           ctypes.byref(t)))
+        return t.value
+
+    def spmv(self, x):
+        x = _f64(x)
+        y = np.empty(self.n, dtype=np.float64)
+        _check(lib().cstfem_spmv(self._h, _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double),
+                                 self.n))
+        return y
+
+
+def _stats_from(s: SolverStatsC, o: SolverOptions, rc: int, hist) -> SolverStats:
+    """Builds the reference's SolverStats, messages included (solver.cpp:46-130)."""
+    st = SolverStats(iterations=s.iterations, rel_residual=s.rel_residual,
+                     true_rel_residual=s.true_rel_residual, converged=bool(s.converged),
+                     outer_iterations=s.outer_iterations, fallback=bool(s.fallback))
+    if s.zero_rhs:
+        st.message = "zero right-hand side, x = 0"
+    elif s.breakdown:
+        st.message = (f"CG breakdown at iteration {s.iterations + 1}: p'Ap = "
+                      f"{s.breakdown_pap:g} (matrix not SPD?)")
+    elif s.converged:
+        st.message = (f"converged in {s.iterations} iterations, relative residual "
+                      f"{s.rel_residual:g}")
+    else:
+        st.message = (f"NOT converged: reached max_iter = {o.max_iter}, relative residual "
+                      f"{s.rel_residual:g} (rtol {o.rtol:g})")
+    if s.warning:
+        st.warning = (f"WARNING: CG recursive residual converged ({s.rel_residual:g}) but the "
+                      f"true residual ||b - Ax|| / ||b|| = {s.true_rel_residual:g} exceeds "
+                      f"10 * rtol = {10 * o.rtol:g}")
+    if hist is not None:
+        cnt = 1 if s.zero_rhs else s.iterations + 1
+        st.residual_history = list(hist[:cnt])
+    st.breakdown = dict(setup=s.setup, spmv_local=s.spmv_local, dot_local=s.dot_local,
+                        vector_ops=s.vector_ops, total=s.total, spmv_calls=s.spmv_calls,
+                        reductions=s.reductions)
+    return st
