@@ -604,7 +604,7 @@ double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
   DBuf<double> a, b, w;
   const size_t nv = 2 * (size_t)c.nb;
   if (kind == 4) {
-    const int64_t n2 = bytes / 32;
+    const int64_t n2 = bytes / 16;  // double2 elements in `bytes`
     a.alloc(2 * n2);
     b.alloc(2 * n2);
     HIP_CHECK(hipMemsetAsync(a.get(), 0, sizeof(double) * 2 * n2, c.stream));
@@ -639,7 +639,7 @@ double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
                            (double2*)a.get(), (const PcgScalars*)sc.get());
         break;
       default: {
-        const int64_t n2 = bytes / 32;
+        const int64_t n2 = bytes / 16;  // double2 elements in `bytes`
         const int g = c.cu_count > 0 ? 8 * c.cu_count : 2048;
         hipLaunchKernelGGL(k_copy, dim3(g), dim3(kThreads), 0, c.stream, n2,
                            (const double2*)a.get(), (double2*)b.get());
