@@ -39,9 +39,14 @@ enum { CSTFEM_PLANE_STRESS = 0, CSTFEM_PLANE_STRAIN = 1 };
 
 /* Solver precision / method of cstfem_solve(). */
 enum {
-  CSTFEM_METHOD_CG_FP64 = 0,      /* Jacobi-PCG in FP64, the reference recurrence */
-  CSTFEM_METHOD_IR_FP32 = 1,      /* FP32 Jacobi-PCG inner solves under FP64 refinement */
-  CSTFEM_METHOD_FCG_LEARNED = 2   /* flexible CG with the learned preconditioner (FP32 inner) */
+  CSTFEM_METHOD_CG_FP64 = 0,        /* Jacobi-PCG in FP64, the reference recurrence; stops on
+                                       the recursive residual (pcg_jacobi, solver.cpp:109) */
+  CSTFEM_METHOD_IR_FP32 = 1,        /* FP32 Jacobi-PCG (FP32 matrix and vectors, FP64 dots,
+                                       FP64 solution) refined by FP64 true residuals; stops when
+                                       the FP64 true residual is below rtol */
+  CSTFEM_METHOD_FCG_LEARNED = 2,    /* flexible CG with the learned preconditioner */
+  CSTFEM_METHOD_CG_FP64_REFINED = 3 /* the FP64 recurrence plus FP64 true-residual replacement
+                                       until the true residual is below rtol */
 };
 
 typedef struct cstfem_ctx cstfem_ctx;
@@ -53,8 +58,10 @@ typedef struct {
   int record_history;     /* residual_history of ||r_k||/||b|| (k = 0..iterations) */
   int instrument;         /* fill the breakdown fields (kernel-class times) */
   int method;             /* CSTFEM_METHOD_* */
-  double inner_rtol;      /* IR / FCG: inner relative tolerance (0 = default 1e-4) */
-  int max_outer;          /* IR: max refinement steps (0 = default 60) */
+  double inner_rtol;      /* refined methods: refresh r from the FP64 true residual when the
+                             recursive residual drops by this factor since the last refresh
+                             (0 = default: 1e-2 in FP32, only at rtol in FP64) */
+  int max_outer;          /* refined methods: max refreshes (0 = default 200) */
 } cstfem_solver_options;
 
 /* Mirrors SolverStats (solver.hpp:99-107) and SolveBreakdown (solver.hpp:17-35). */
@@ -88,6 +95,9 @@ int cstfem_device_info(cstfem_ctx* ctx, char* name, int len, int* cu_count);
 int cstfem_upload_mesh(cstfem_ctx* ctx, int nn, const double* xy, int64_t ne, const int* tri);
 int cstfem_set_material(cstfem_ctx* ctx, double E, double nu, int plane, double thickness,
                         const double* E_elem);
+/* The reference's assemble_stiffness(mesh, D, thickness, K) form: a uniform row-major 3x3
+ * constitutive matrix D[9] (assembly.hpp:22); clears any per-element modulus. */
+int cstfem_set_constitutive(cstfem_ctx* ctx, const double* D, double thickness);
 
 /* ---- pattern and assembly ------------------------------------------------------------ */
 /* Replaces build_stiffness_pattern (assembly.cpp:9-42): builds the pattern on the device,

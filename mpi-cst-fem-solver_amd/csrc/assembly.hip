@@ -469,7 +469,7 @@ __global__ void k_assemble(int nb, int nslices, const int64_t* __restrict__ off,
                            const int* __restrict__ brow_ptr, const int* __restrict__ slot_ptr,
                            const int* __restrict__ slot_src, const int* __restrict__ tri,
                            const double* __restrict__ xy, const double* __restrict__ E_elem,
-                           double E, double nu, int strain, double t, double* __restrict__ val) {
+                           Mat3 Dm, double nu, int strain, double t, double* __restrict__ val) {
   const int lane = threadIdx.x & (kSlice - 1);
   const int s = blockIdx.x * (blockDim.x / kSlice) + threadIdx.x / kSlice;
   if (s >= nslices) return;
@@ -477,8 +477,7 @@ __global__ void k_assemble(int nb, int nslices, const int64_t* __restrict__ off,
   if (r >= nb) return;
   const int64_t o = off[s];
   const int b0 = brow_ptr[r], d = brow_ptr[r + 1] - b0;
-  double Du[9];
-  constitutive(E, nu, strain, Du);
+  const double* Du = Dm.d;
   for (int k = 0; k < d; ++k) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const int q = b0 + k;
@@ -536,7 +535,7 @@ void assemble_values(Context& c) {
   hipLaunchKernelGGL(k_assemble, dim3(grid_for(c.nslices, wpb, 0)), dim3(wpb * kSlice), 0,
                      c.stream, c.nb, c.nslices, c.sell_off.get(), c.brow_ptr.get(),
                      c.slot_ptr.get(), c.slot_src.get(), c.tri.get(), c.xy.get(),
-                     c.has_E_elem ? c.E_elem.get() : (const double*)nullptr, c.E, c.nu,
+                     c.has_E_elem ? c.E_elem.get() : (const double*)nullptr, c.D, c.nu,
                      c.plane_strain, c.thick, c.sell_val.get());
   HIP_CHECK(hipGetLastError());
   c.setup_done = false;

@@ -196,6 +196,7 @@ int cstfem_set_material(cstfem_ctx* ctx, double E, double nu, int plane, double 
     c.nu = nu;
     c.thick = thickness;
     c.plane_strain = plane;
+    c.D = constitutive_host(E, nu, plane);
     c.has_E_elem = E_elem != nullptr;
     if (E_elem) {
       for (int64_t e = 0; e < c.ne; ++e)
@@ -204,6 +205,21 @@ int cstfem_set_material(cstfem_ctx* ctx, double E, double nu, int plane, double 
       HIP_CHECK(hipMemcpyAsync(c.E_elem.get(), E_elem, sizeof(double) * c.ne, hipMemcpyHostToDevice, c.stream));
       HIP_CHECK(hipStreamSynchronize(c.stream));
     }
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_set_constitutive(cstfem_ctx* ctx, const double* D, double thickness) {
+  return guarded([&] {
+    need(ctx && D, "cstfem_set_constitutive: NULL argument");
+    need(thickness > 0, "thickness must be positive");
+    Context& c = ctx->c;
+    for (int i = 0; i < 9; ++i) {
+      need(std::isfinite(D[i]), "cstfem_set_constitutive: D must be finite");
+      c.D.d[i] = D[i];
+    }
+    c.thick = thickness;
+    c.has_E_elem = false;
     return CSTFEM_OK;
   });
 }
@@ -546,7 +562,10 @@ int cstfem_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt, cstfem_solve
         rc = solve_cg_fp64(c, *opt, *st, history);
         break;
       case CSTFEM_METHOD_IR_FP32:
-        rc = solve_ir_fp32(c, *opt, *st);
+        rc = solve_refined(c, *opt, *st, true);
+        break;
+      case CSTFEM_METHOD_CG_FP64_REFINED:
+        rc = solve_refined(c, *opt, *st, false);
         break;
       default:
         throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_solve: unknown method " + std::to_string(opt->method));

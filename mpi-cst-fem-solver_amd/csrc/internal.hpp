@@ -28,6 +28,27 @@ namespace cstfem_hip {
 
 constexpr int kSlice = 64;  // block rows per SELL slice = one wavefront
 
+// 3x3 row-major constitutive matrix, passed to kernels by value
+struct Mat3 {
+  double d[9];
+};
+
+// element.cpp:7-15, evaluated on the host with the same operations as the device code
+inline Mat3 constitutive_host(double E, double nu, int strain) {
+  Mat3 m{};
+  if (!strain) {
+    const double c = E / (1.0 - nu * nu);
+    const double v[9] = {c, c * nu, 0.0, c * nu, c, 0.0, 0.0, 0.0, c * (1.0 - nu) / 2.0};
+    for (int i = 0; i < 9; ++i) m.d[i] = v[i];
+  } else {
+    const double c = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double v[9] = {c * (1.0 - nu), c * nu, 0.0, c * nu, c * (1.0 - nu), 0.0,
+                         0.0, 0.0, c * (1.0 - 2.0 * nu) / 2.0};
+    for (int i = 0; i < 9; ++i) m.d[i] = v[i];
+  }
+  return m;
+}
+
 // Error carrying a C-ABI status code.
 struct Error : std::runtime_error {
   int code;
@@ -62,14 +83,20 @@ struct DBuf {
 };
 
 // Scalars of one PCG solve, resident on the device.
+// Written only by workgroup 0 of a kernel, or by the host between kernels.
 struct PcgScalars {
-  double rz, pAp, alpha, beta, rr, bnorm, rel, rtol;
-  double rz_new, pad0;
-  int k, status, max_iter, record;   // status: 0 running, 1 converged, 2 breakdown, 3 max_iter
-  unsigned int ticket[4];            // last-block-done counters, one per kernel class
+  double pAp;      // p'Ap of a breakdown
+  double rr;       // ||b - A x||^2 of the last true-residual evaluation
+  double bnorm, rel, rtol;
+  double delta;    // FP32 refinement: refresh r when ||r|| drops by this factor
+  double rel_ref;  // FP32 refinement: ||r||/||b|| at the last refresh
+  double pad0;
+  int k_in[2];     // iterations completed, written by iterations of parity 0 / 1
+  int status, max_iter, record, refine_mode;
 };
 
-enum { kRunning = 0, kConverged = 1, kBreakdown = 2, kMaxIter = 3 };
+// kRefine: the FP32 iteration asks the host for an FP64 true-residual refresh.
+enum { kRunning = 0, kConverged = 1, kBreakdown = 2, kMaxIter = 3, kRefine = 4 };
 
 struct Context {
   int device = 0;
@@ -86,6 +113,7 @@ struct Context {
   bool has_E_elem = false;
   double E = 1.0, nu = 0.3, thick = 1.0;
   int plane_strain = 0;
+  Mat3 D{};  // uniform constitutive matrix (used when there is no per-element modulus)
 
   DBuf<int> ne_ptr, ne_list;
 
@@ -148,7 +176,7 @@ void setup_solver(Context& c);
 void spmv_fp64(Context& c, const double* x, double* y);
 int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
                   double* history);
-int solve_ir_fp32(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st);
+int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st, bool fp32);
 void ensure_vectors(Context& c);
 double profile_kernel(Context& c, int kind, int reps, int64_t bytes);
 

@@ -1,19 +1,23 @@
-// pcg.hip -- FP64 Jacobi-preconditioned CG on the device.
+// pcg.hip -- Jacobi-preconditioned CG on the device: the reference's FP64 recurrence and an
+// FP32 variant under FP64 refinement.
 //
 // Reference: src/solver.cpp:35-157 (pcg_iterate, pcg_jacobi, CgJacobiSolver::setup/solve).
 // The recurrence, the stopping test (rel < rtol), the breakdown test !(p'Ap > 0), the
-// zero-RHS exit and the true residual with its 10*rtol warning are the reference's.  Per
-// iteration three kernels run, each one HBM pass:
-//   A  Ap = A p (SELL-64 2x2 blocks) fused with p'Ap, then alpha = rz / p'Ap
-//   B  x += alpha p, r -= alpha Ap fused with r'r and r'(D^-1 r), then the convergence test,
-//      beta and rz
-//   C  p = D^-1 r + beta p
-// Scalars never leave the device: the last workgroup of each kernel (ticket counter) sums
-// the per-workgroup partials in a fixed order, so results are run-to-run reproducible.
-// Iterations are replayed as a captured hipGraph of `chunk` iterations; kernels return at
-// once after convergence, and the host reads the status once per chunk.
-// Element-wise arithmetic and each SpMV row sum follow the reference's operation order
-// (no FMA contraction); only the dot products are summed in a different order.
+// zero-RHS exit and the true residual with its 10*rtol warning are the reference's.
+//
+// One iteration = three kernels, each one pass over HBM:
+//   A  Ap = A p (SELL-64 2x2 blocks), per-workgroup partials of p'Ap
+//   B  every workgroup reduces A's partials (fixed order) -> alpha = rz/p'Ap, breakdown test;
+//      x += alpha p, r -= alpha Ap; per-workgroup partials of r'r and r'(D^-1 r)
+//   C  every workgroup reduces B's partials -> convergence test, beta; p = D^-1 r + beta p
+// No kernel needs a cross-workgroup fence (an agent-scope release writes back the whole L2
+// of an XCD on gfx950 and costs more than the HBM traffic): scalars are recomputed
+// redundantly, bit-identically, by each workgroup of the consumer kernel, and the kernel
+// boundary provides visibility.  B's partials live in two slots selected by iteration
+// parity, so C can read rz(k-1) and rz(k) without a race; graphs hold an even number of
+// iterations.  Element-wise arithmetic and each SpMV row sum follow the reference's operation
+// order (no FMA contraction); only dot products are summed in a different (fixed) order, so
+// results are run-to-run reproducible.
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -26,8 +30,10 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
+constexpr int kMaxGrid = 1024;  // partial slots per kernel
 
 typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -35,45 +41,30 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Sums `v` over the workgroup; the result is valid in thread 0.
-__device__ inline double block_sum(double v, double* lds) {
+// Workgroup sum, broadcast to every thread.
+__device__ inline double block_sum_all(double v, double* lds) {
   v = wave_sum(v);
   const int w = threadIdx.x >> 6;
+  __syncthreads();
   if ((threadIdx.x & 63) == 0) lds[w] = v;
   __syncthreads();
   double s = 0.0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += lds[i];
-  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) s += lds[i];
   return s;
 }
 
-// Last-workgroup-done election: every workgroup calls this after writing its partial.
-__device__ inline bool elect_last(unsigned int* ticket) {
-  __shared__ bool last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int t =
-        __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-
-// Fixed-order sum of `count` partials (stride `stride`) by one workgroup; valid in thread 0.
-__device__ inline double sum_partials(const double* part, int count, int stride, double* lds) {
+// Fixed-order sum of `count` partials (stride `stride`); identical in every workgroup.
+__device__ inline double reduce_partials(const double* __restrict__ part, int count, int stride,
+                                         double* lds) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < count; i += blockDim.x)
-    s += __hip_atomic_load(part + (size_t)i * stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return block_sum(s, lds);
+  for (int i = threadIdx.x; i < count; i += kThreads) s += part[(size_t)i * stride];
+  return block_sum_all(s, lds);
 }
 
 // Maps workgroup b to its share of `nslices` so that the 8 XCDs (which receive workgroups
-// round-robin) each sweep one contiguous range of slices -> neighbouring x entries stay in
-// one XCD's L2.
+// round-robin) each sweep one contiguous range of slices: neighbouring x entries then stay
+// in one XCD's L2.
 struct SliceRange {
   int first, end, step;
 };
@@ -88,9 +79,20 @@ __device__ inline SliceRange slice_range(int nslices) {
   return {(int)blockIdx.x * kWaves + wave, nslices, G * kWaves};
 }
 
-// y rows 2r, 2r+1 of the SELL matrix times x; sequential per-row sums in column order.
+// ---- precision traits ------------------------------------------------------------------
+
+struct F64 {
+  using V = double2;   // 2 DOFs of one node
+  using S = double;
+};
+struct F32 {
+  using V = float2;
+  using S = float;
+};
+
+// Rows 2r, 2r+1 of a SELL slice times x: sequential per-row sums in column order.
 __device__ inline void sell_row(const int64_t* __restrict__ off, const int* __restrict__ col,
-                                const double2* __restrict__ val, const double2* __restrict__ x,
+                                const double* __restrict__ val, const double2* __restrict__ x,
                                 int s, int lane, double& y0, double& y1) {
   const int64_t o = off[s];
   const int w = (int)((off[s + 1] - o) >> 6);
@@ -99,7 +101,7 @@ __device__ inline void sell_row(const int64_t* __restrict__ off, const int* __re
   double a0 = 0.0, a1 = 0.0;
 #pragma unroll 4
   for (int k = 0; k < w; ++k) {
-    const int m = __builtin_nontemporal_load(c + (size_t)k * 64);
+    const int m = c[(size_t)k * 64];
     const v2d va = __builtin_nontemporal_load(v + (size_t)k * 128);
     const v2d vb = __builtin_nontemporal_load(v + (size_t)k * 128 + 1);
     const double2 xm = x[m];
@@ -112,30 +114,57 @@ __device__ inline void sell_row(const int64_t* __restrict__ off, const int* __re
   y1 = a1;
 }
 
-struct MatArgs {
+__device__ inline void sell_row(const int64_t* __restrict__ off, const int* __restrict__ col,
+                                const float* __restrict__ val, const float2* __restrict__ x,
+                                int s, int lane, float& y0, float& y1) {
+  const int64_t o = off[s];
+  const int w = (int)((off[s + 1] - o) >> 6);
+  const int* c = col + o + lane;
+  const v4f* v = reinterpret_cast<const v4f*>(val) + (o + lane);
+  float a0 = 0.f, a1 = 0.f;
+#pragma unroll 4
+  for (int k = 0; k < w; ++k) {
+    const int m = c[(size_t)k * 64];
+    const v4f vv = __builtin_nontemporal_load(v + (size_t)k * 64);
+    const float2 xm = x[m];
+    a0 = a0 + vv.x * xm.x;
+    a0 = a0 + vv.y * xm.y;
+    a1 = a1 + vv.z * xm.x;
+    a1 = a1 + vv.w * xm.y;
+  }
+  y0 = a0;
+  y1 = a1;
+}
+
+template <class P>
+struct Mat {
   const int64_t* off;
   const int* col;
-  const double2* val;
+  const typename P::S* val;
   int nslices, nb;
 };
 
-__global__ __launch_bounds__(kThreads) void k_spmv(MatArgs A, const double2* __restrict__ x,
-                                                   double2* __restrict__ y) {
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_spmv(Mat<P> A, const typename P::V* __restrict__ x,
+                                                   typename P::V* __restrict__ y) {
+  using S = typename P::S;
   const int lane = threadIdx.x & 63;
   const SliceRange sr = slice_range(A.nslices);
   for (int s = sr.first; s < sr.end; s += sr.step) {
     const int r = s * 64 + lane;
-    double y0, y1;
+    S y0, y1;
     sell_row(A.off, A.col, A.val, x, s, lane, y0, y1);
-    if (r < A.nb) y[r] = make_double2(y0, y1);
+    if (r < A.nb) y[r] = {y0, y1};
   }
 }
 
-// A: Ap = A p, p'Ap, alpha
-__global__ __launch_bounds__(kThreads) void k_cg_spmv(MatArgs A, const double2* __restrict__ p,
-                                                      double2* __restrict__ Ap,
-                                                      double* __restrict__ part,
-                                                      PcgScalars* __restrict__ sc) {
+// A: Ap = A p; partials of p'Ap (accumulated in FP64)
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_cg_spmv(Mat<P> A, const typename P::V* __restrict__ p,
+                                                      typename P::V* __restrict__ Ap,
+                                                      double* __restrict__ partA,
+                                                      const PcgScalars* __restrict__ sc) {
+  using S = typename P::S;
   __shared__ double lds[kWaves];
   if (sc->status != kRunning) return;
   const int lane = threadIdx.x & 63;
@@ -143,38 +172,37 @@ __global__ __launch_bounds__(kThreads) void k_cg_spmv(MatArgs A, const double2* 
   double local = 0.0;
   for (int s = sr.first; s < sr.end; s += sr.step) {
     const int r = s * 64 + lane;
-    double y0, y1;
+    S y0, y1;
     sell_row(A.off, A.col, A.val, p, s, lane, y0, y1);
     if (r < A.nb) {
-      Ap[r] = make_double2(y0, y1);
-      const double2 pr = p[r];
-      local += pr.x * y0 + pr.y * y1;
+      Ap[r] = {y0, y1};
+      const typename P::V pr = p[r];
+      local += (double)pr.x * (double)y0 + (double)pr.y * (double)y1;
     }
   }
-  const double bs = block_sum(local, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  if (!elect_last(&sc->ticket[0])) return;
-  const double pAp = sum_partials(part, gridDim.x, 1, lds);
-  if (threadIdx.x == 0) {
-    sc->ticket[0] = 0;
-    sc->pAp = pAp;
-    if (!(pAp > 0.0)) {
-      sc->status = kBreakdown;
-    } else {
-      sc->alpha = sc->rz / pAp;
-    }
-  }
+  const double bs = block_sum_all(local, lds);
+  if (threadIdx.x == 0) partA[blockIdx.x] = bs;
 }
 
-// B: x += alpha p; r -= alpha Ap; r'r, r'z; convergence test; beta
+// B (FP64): alpha, breakdown; x += alpha p, r -= alpha Ap; partials (r'r, r'z)
 __global__ __launch_bounds__(kThreads) void k_cg_update(
     int nvec2, double2* __restrict__ x, double2* __restrict__ r, const double2* __restrict__ p,
-    const double2* __restrict__ Ap, const double2* __restrict__ invd, double* __restrict__ part,
-    PcgScalars* __restrict__ sc, double* __restrict__ history) {
+    const double2* __restrict__ Ap, const double2* __restrict__ invd,
+    const double* __restrict__ partA, int nA, const double* __restrict__ partPrev,
+    double* __restrict__ partCur, int nB, PcgScalars* __restrict__ sc) {
   __shared__ double lds[kWaves];
   if (sc->status != kRunning) return;
-  const double alpha = sc->alpha;
-  double rr = 0.0, rz = 0.0;
+  const double pAp = reduce_partials(partA, nA, 1, lds);
+  const double rz = reduce_partials(partPrev + 1, nB, 2, lds);
+  if (!(pAp > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->pAp = pAp;
+      sc->status = kBreakdown;
+    }
+    return;
+  }
+  const double alpha = rz / pAp;
+  double rr = 0.0, rzn = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
     const double2 pi = p[i], api = Ap[i], di = invd[i];
     double2 xi = x[i], ri = r[i];
@@ -185,62 +213,108 @@ __global__ __launch_bounds__(kThreads) void k_cg_update(
     x[i] = xi;
     r[i] = ri;
     rr += ri.x * ri.x + ri.y * ri.y;
-    rz += ri.x * (di.x * ri.x) + ri.y * (di.y * ri.y);
+    rzn += ri.x * (di.x * ri.x) + ri.y * (di.y * ri.y);
   }
-  const double b_rr = block_sum(rr, lds);
-  const double b_rz = block_sum(rz, lds);
+  const double b_rr = block_sum_all(rr, lds);
+  const double b_rz = block_sum_all(rzn, lds);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = b_rr;
-    part[2 * blockIdx.x + 1] = b_rz;
-  }
-  if (!elect_last(&sc->ticket[1])) return;
-  const double t_rr = sum_partials(part, gridDim.x, 2, lds);
-  const double t_rz = sum_partials(part + 1, gridDim.x, 2, lds);
-  if (threadIdx.x == 0) {
-    sc->ticket[1] = 0;
-    const int k = sc->k + 1;
-    sc->k = k;
-    const double rel = sqrt(t_rr) / sc->bnorm;
-    sc->rel = rel;
-    sc->rr = t_rr;
-    if (sc->record) history[k] = rel;
-    if (rel < sc->rtol) {
-      sc->status = kConverged;
-    } else if (k >= sc->max_iter) {
-      sc->status = kMaxIter;
-    } else {
-      sc->beta = t_rz / sc->rz;
-      sc->rz = t_rz;
-    }
+    partCur[2 * blockIdx.x] = b_rr;
+    partCur[2 * blockIdx.x + 1] = b_rz;
   }
 }
 
-// C: p = D^-1 r + beta p
-__global__ __launch_bounds__(kThreads) void k_cg_direction(int nvec2, const double2* __restrict__ r,
-                                                           const double2* __restrict__ invd,
-                                                           double2* __restrict__ p,
-                                                           const PcgScalars* __restrict__ sc) {
+// B (FP32 inner, FP64 solution): x64 += alpha p32; r32 -= alpha Ap32 (rounded once)
+__global__ __launch_bounds__(kThreads) void k_cg_update32(
+    int nvec2, double2* __restrict__ x, float2* __restrict__ r, const float2* __restrict__ p,
+    const float2* __restrict__ Ap, const float2* __restrict__ invd,
+    const double* __restrict__ partA, int nA, const double* __restrict__ partPrev,
+    double* __restrict__ partCur, int nB, PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kWaves];
   if (sc->status != kRunning) return;
-  const double beta = sc->beta;
+  const double pAp = reduce_partials(partA, nA, 1, lds);
+  const double rz = reduce_partials(partPrev + 1, nB, 2, lds);
+  if (!(pAp > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->pAp = pAp;
+      sc->status = kBreakdown;
+    }
+    return;
+  }
+  const double alpha = rz / pAp;
+  double rr = 0.0, rzn = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
-    const double2 ri = r[i], di = invd[i];
-    double2 pi = p[i];
-    const double z0 = di.x * ri.x, z1 = di.y * ri.y;
-    pi.x = z0 + beta * pi.x;
-    pi.y = z1 + beta * pi.y;
+    const float2 pi = p[i], api = Ap[i], di = invd[i];
+    double2 xi = x[i];
+    float2 ri = r[i];
+    xi.x = xi.x + alpha * (double)pi.x;
+    xi.y = xi.y + alpha * (double)pi.y;
+    ri.x = (float)((double)ri.x - alpha * (double)api.x);
+    ri.y = (float)((double)ri.y - alpha * (double)api.y);
+    x[i] = xi;
+    r[i] = ri;
+    const double r0 = ri.x, r1 = ri.y;
+    rr += r0 * r0 + r1 * r1;
+    rzn += r0 * ((double)di.x * r0) + r1 * ((double)di.y * r1);
+  }
+  const double b_rr = block_sum_all(rr, lds);
+  const double b_rz = block_sum_all(rzn, lds);
+  if (threadIdx.x == 0) {
+    partCur[2 * blockIdx.x] = b_rr;
+    partCur[2 * blockIdx.x + 1] = b_rz;
+  }
+}
+
+// C: convergence test, iteration count, beta; p = D^-1 r + beta p.  `cur` = iteration parity.
+// FP32 mode: reaching rtol (or dropping by `sc->delta` since the last refinement) stops the
+// chunk with kRefine so that the host refreshes r from the FP64 true residual.
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_cg_direction(
+    int nvec2, const typename P::V* __restrict__ r, const typename P::V* __restrict__ invd,
+    typename P::V* __restrict__ p, const double* __restrict__ partPrev,
+    const double* __restrict__ partCur, int nB, int cur, PcgScalars* __restrict__ sc,
+    double* __restrict__ history, int test) {
+  using S = typename P::S;
+  __shared__ double lds[kWaves];
+  if (sc->status != kRunning) return;
+  const double rr = reduce_partials(partCur, nB, 2, lds);
+  const double rz_new = reduce_partials(partCur + 1, nB, 2, lds);
+  const double rz_old = reduce_partials(partPrev + 1, nB, 2, lds);
+  const int k = sc->k_in[cur ^ 1] + 1;
+  const double rel = sqrt(rr) / sc->bnorm;
+  int status = kRunning;
+  if (!test)
+    status = kRunning;  // direction update after a refinement: the test already ran
+  else if (rel < sc->rtol)
+    status = sc->refine_mode ? kRefine : kConverged;
+  else if (sc->refine_mode && rel < sc->delta * sc->rel_ref)
+    status = kRefine;
+  else if (k >= sc->max_iter)
+    status = kMaxIter;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->k_in[cur] = k;
+    sc->rel = rel;
+    if (sc->record) history[k] = rel;
+    if (status != kRunning) sc->status = status;
+  }
+  if (status != kRunning) return;
+  const double beta = rz_new / rz_old;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const typename P::V ri = r[i], di = invd[i];
+    typename P::V pi = p[i];
+    const S z0 = di.x * ri.x, z1 = di.y * ri.y;
+    pi.x = z0 + (S)beta * pi.x;
+    pi.y = z1 + (S)beta * pi.y;
     p[i] = pi;
   }
 }
 
-// x = 0, r = b, p = D^-1 b; b'b and r'z; bnorm, zero-RHS exit
+// x = 0, r = b, p = D^-1 b; partials (b'b, r'z) into slot 1 (the "previous" slot of k = 0)
 __global__ __launch_bounds__(kThreads) void k_cg_init(int nvec2, const double2* __restrict__ b,
                                                       double2* __restrict__ x,
                                                       double2* __restrict__ r,
                                                       double2* __restrict__ p,
                                                       const double2* __restrict__ invd,
-                                                      double* __restrict__ part,
-                                                      PcgScalars* __restrict__ sc,
-                                                      double* __restrict__ history) {
+                                                      double* __restrict__ part) {
   __shared__ double lds[kWaves];
   double bb = 0.0, rz = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
@@ -252,20 +326,23 @@ __global__ __launch_bounds__(kThreads) void k_cg_init(int nvec2, const double2* 
     bb += bi.x * bi.x + bi.y * bi.y;
     rz += bi.x * z0 + bi.y * z1;
   }
-  const double b_bb = block_sum(bb, lds);
-  const double b_rz = block_sum(rz, lds);
+  const double s0 = block_sum_all(bb, lds);
+  const double s1 = block_sum_all(rz, lds);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = b_bb;
-    part[2 * blockIdx.x + 1] = b_rz;
+    part[2 * blockIdx.x] = s0;
+    part[2 * blockIdx.x + 1] = s1;
   }
-  if (!elect_last(&sc->ticket[2])) return;
-  const double t_bb = sum_partials(part, gridDim.x, 2, lds);
-  const double t_rz = sum_partials(part + 1, gridDim.x, 2, lds);
+}
+
+// one workgroup: bnorm, zero-RHS exit, counters
+__global__ __launch_bounds__(kThreads) void k_cg_init_finish(const double* __restrict__ part,
+                                                             int nB, PcgScalars* __restrict__ sc,
+                                                             double* __restrict__ history) {
+  __shared__ double lds[kWaves];
+  const double bb = reduce_partials(part, nB, 2, lds);
   if (threadIdx.x == 0) {
-    sc->ticket[2] = 0;
-    sc->bnorm = sqrt(t_bb);
-    sc->rz = t_rz;
-    sc->k = 0;
+    sc->bnorm = sqrt(bb);
+    sc->k_in[0] = sc->k_in[1] = 0;
     sc->rel = 1.0;
     if (sc->bnorm == 0.0) {
       sc->status = kConverged;
@@ -277,12 +354,11 @@ __global__ __launch_bounds__(kThreads) void k_cg_init(int nvec2, const double2* 
   }
 }
 
-// r = b - y; returns ||r||^2 into sc->rr (true residual of pcg_jacobi, solver.cpp:117-122)
+// r = b - y and partials of r'r (true residual of pcg_jacobi, solver.cpp:117-122)
 __global__ __launch_bounds__(kThreads) void k_residual(int nvec2, const double2* __restrict__ b,
                                                        const double2* __restrict__ y,
                                                        double2* __restrict__ r,
-                                                       double* __restrict__ part,
-                                                       PcgScalars* __restrict__ sc) {
+                                                       double* __restrict__ part) {
   __shared__ double lds[kWaves];
   double rr = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
@@ -291,99 +367,143 @@ __global__ __launch_bounds__(kThreads) void k_residual(int nvec2, const double2*
     r[i] = ri;
     rr += ri.x * ri.x + ri.y * ri.y;
   }
-  const double b_rr = block_sum(rr, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = b_rr;
-  if (!elect_last(&sc->ticket[3])) return;
-  const double t = sum_partials(part, gridDim.x, 1, lds);
+  const double s = block_sum_all(rr, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_sum_into(const double* __restrict__ part, int n,
+                                                       double* __restrict__ out) {
+  __shared__ double lds[kWaves];
+  const double s = reduce_partials(part, n, 1, lds);
+  if (threadIdx.x == 0) *out = s;
+}
+
+// FP32 refinement step: r32 = r64 (rounded), p32 = D^-1 r32 restarts nothing: the search
+// direction is kept; partials (r'r, r'z) of the refreshed residual into slot `slot`.
+__global__ __launch_bounds__(kThreads) void k_refresh32(int nvec2, const double2* __restrict__ r64,
+                                                        float2* __restrict__ r32,
+                                                        const float2* __restrict__ invd,
+                                                        double* __restrict__ part) {
+  __shared__ double lds[kWaves];
+  double rr = 0.0, rz = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const double2 a = r64[i];
+    const float2 ri = make_float2((float)a.x, (float)a.y), di = invd[i];
+    r32[i] = ri;
+    const double r0 = ri.x, r1 = ri.y;
+    rr += r0 * r0 + r1 * r1;
+    rz += r0 * ((double)di.x * r0) + r1 * ((double)di.y * r1);
+  }
+  const double s0 = block_sum_all(rr, lds);
+  const double s1 = block_sum_all(rz, lds);
   if (threadIdx.x == 0) {
-    sc->ticket[3] = 0;
-    sc->rr = t;
+    part[2 * blockIdx.x] = s0;
+    part[2 * blockIdx.x + 1] = s1;
   }
 }
 
-// inv_diag = 1 / K_ii (CgJacobiSolver::setup, solver.cpp:133-148)
+// FP64 refinement step: partials (r'r, r'z) of the refreshed residual r = b - A x
+__global__ __launch_bounds__(kThreads) void k_refresh64(int nvec2, const double2* __restrict__ r,
+                                                        const double2* __restrict__ invd,
+                                                        double* __restrict__ part) {
+  __shared__ double lds[kWaves];
+  double rr = 0.0, rz = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const double2 ri = r[i], di = invd[i];
+    rr += ri.x * ri.x + ri.y * ri.y;
+    rz += ri.x * (di.x * ri.x) + ri.y * (di.y * ri.y);
+  }
+  const double s0 = block_sum_all(rr, lds);
+  const double s1 = block_sum_all(rz, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s0;
+    part[2 * blockIdx.x + 1] = s1;
+  }
+}
+
+// FP32 start: r32 = b, p32 = D^-1 r32, x64 = 0; partials (b'b as FP64, r32'z32) into `part`
+__global__ __launch_bounds__(kThreads) void k_init32(int nvec2, const double2* __restrict__ b,
+                                                     double2* __restrict__ x,
+                                                     float2* __restrict__ r32,
+                                                     float2* __restrict__ p32,
+                                                     const float2* __restrict__ invd,
+                                                     double* __restrict__ part) {
+  __shared__ double lds[kWaves];
+  double bb = 0.0, rz = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const double2 bi = b[i];
+    const float2 ri = make_float2((float)bi.x, (float)bi.y), di = invd[i];
+    x[i] = make_double2(0.0, 0.0);
+    r32[i] = ri;
+    p32[i] = make_float2(di.x * ri.x, di.y * ri.y);
+    bb += bi.x * bi.x + bi.y * bi.y;
+    rz += (double)ri.x * ((double)di.x * ri.x) + (double)ri.y * ((double)di.y * ri.y);
+  }
+  const double s0 = block_sum_all(bb, lds);
+  const double s1 = block_sum_all(rz, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s0;
+    part[2 * blockIdx.x + 1] = s1;
+  }
+}
+
+// inv_diag = 1 / K_ii (CgJacobiSolver::setup, solver.cpp:133-148), plus the FP32 copies
 __global__ void k_inv_diag(int nb, int n, const int* __restrict__ bdiag,
                            const int64_t* __restrict__ spos, const double* __restrict__ val,
-                           double* __restrict__ invd, int* __restrict__ err) {
+                           double* __restrict__ invd, float* __restrict__ invd32,
+                           int* __restrict__ err) {
   const int nd = blockIdx.x * blockDim.x + threadIdx.x;
   if (nd >= nb) return;
   const int q = bdiag[nd];
   for (int c = 0; c < 2; ++c) {
     const int i = 2 * nd + c;
-    if (i >= n) {
-      invd[i] = 0.0;  // padding row of an odd-sized generic matrix
-      continue;
+    double v = 0.0;
+    if (i < n) {
+      const double d = q >= 0 ? val[4 * spos[q] + 3 * c] : 0.0;
+      if (!(d > 0.0))
+        atomicMin(err, i);
+      else
+        v = 1.0 / d;
     }
-    const double d = q >= 0 ? val[4 * spos[q] + 3 * c] : 0.0;
-    if (!(d > 0.0)) {
-      atomicMin(err, i);
-      invd[i] = 0.0;
-    } else {
-      invd[i] = 1.0 / d;
-    }
+    invd[i] = v;  // 0 on the padding row of an odd-sized generic matrix
+    invd32[i] = (float)v;
   }
 }
 
-MatArgs mat_args(Context& c) {
-  return MatArgs{c.sell_off.get(), c.sell_col.get(), (const double2*)c.sell_val.get(), c.nslices,
-                 c.nb};
+__global__ void k_round_values(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+__global__ __launch_bounds__(kThreads) void k_copy(int64_t n2, const double2* __restrict__ a,
+                                                   double2* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n2;
+       i += (int64_t)gridDim.x * kThreads)
+    reinterpret_cast<v2d*>(b)[i] = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(a) + i);
+}
+
+Mat<F64> mat64(Context& c) {
+  return Mat<F64>{c.sell_off.get(), c.sell_col.get(), c.sell_val.get(), c.nslices, c.nb};
+}
+Mat<F32> mat32(Context& c) {
+  return Mat<F32>{c.sell_off.get(), c.sell_col.get(), c.sell_val32.get(), c.nslices, c.nb};
 }
 
 int spmv_grid(Context& c) {
   const int want = (c.nslices + kWaves - 1) / kWaves;
-  const int cap = c.cu_count > 0 ? 8 * c.cu_count : 2048;
-  int g = want < cap ? want : cap;
+  int g = want < kMaxGrid ? want : kMaxGrid;
   if (g >= 64) g &= ~7;
   return g < 1 ? 1 : g;
 }
 
-int vec_grid(Context& c) {
-  const int nvec2 = c.nb;
-  const int cap = c.cu_count > 0 ? 4 * c.cu_count : 1024;
-  return grid_for(nvec2, kThreads, cap);
+int vec_grid(Context& c) { return grid_for(c.nb, kThreads, kMaxGrid); }
+
+double elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+  return ms * 1e-3;
 }
-
-}  // namespace
-
-void ensure_vectors(Context& c) {
-  const size_t nv = 2 * (size_t)c.nb;
-  c.x.alloc(nv);
-  c.r.alloc(nv);
-  c.p.alloc(nv);
-  c.Ap.alloc(nv);
-  c.inv_diag.alloc(nv);
-  if (c.b.n != nv) {
-    c.b.alloc(nv);
-    HIP_CHECK(hipMemsetAsync(c.b.get(), 0, sizeof(double) * nv, c.stream));
-  }
-  const int parts = 2 * (c.cu_count > 0 ? 8 * c.cu_count : 2048) + 64;
-  c.partials.alloc(parts);
-}
-
-void setup_solver(Context& c) {
-  if (!c.have_pattern) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: no matrix on the device");
-  ensure_vectors(c);
-  const int big = INT_MAX;
-  HIP_CHECK(hipMemcpyAsync(c.err_flag.get(), &big, sizeof(int), hipMemcpyHostToDevice, c.stream));
-  hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(c.nb, 256, 0)), dim3(256), 0, c.stream, c.nb, c.n,
-                     c.bdiag.get(), c.sell_pos.get(), c.sell_val.get(), c.inv_diag.get(),
-                     c.err_flag.get());
-  HIP_CHECK(hipGetLastError());
-  int bad = INT_MAX;
-  HIP_CHECK(hipMemcpyAsync(&bad, c.err_flag.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
-  if (bad != INT_MAX)
-    throw Error(CSTFEM_ERR_BAD_DIAGONAL, "cg_jacobi: non-positive diagonal at row " + std::to_string(bad));
-  c.setup_done = true;
-}
-
-void spmv_fp64(Context& c, const double* x, double* y) {
-  hipLaunchKernelGGL(k_spmv, dim3(spmv_grid(c)), dim3(kThreads), 0, c.stream, mat_args(c),
-                     (const double2*)x, (double2*)y);
-  HIP_CHECK(hipGetLastError());
-}
-
-namespace {
 
 struct Graph {
   hipGraph_t g = nullptr;
@@ -394,23 +514,134 @@ struct Graph {
   }
 };
 
-void enqueue_iteration(Context& c, int gs, int gv, double* history) {
-  const int nvec2 = c.nb;
-  hipLaunchKernelGGL(k_cg_spmv, dim3(gs), dim3(kThreads), 0, c.stream, mat_args(c),
-                     (const double2*)c.p.get(), (double2*)c.Ap.get(), c.partials.get(), c.scal.get());
-  hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
-                     (double2*)c.x.get(), (double2*)c.r.get(), (const double2*)c.p.get(),
-                     (const double2*)c.Ap.get(), (const double2*)c.inv_diag.get(),
-                     c.partials.get(), c.scal.get(), history);
-  hipLaunchKernelGGL(k_cg_direction, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
-                     (const double2*)c.r.get(), (const double2*)c.inv_diag.get(),
-                     (double2*)c.p.get(), (const PcgScalars*)c.scal.get());
+// Partial slots: A at [0, kMaxGrid), B slot s at kMaxGrid + 2*kMaxGrid*s, scratch after.
+double* partA(Context& c) { return c.partials.get(); }
+double* partB(Context& c, int s) { return c.partials.get() + kMaxGrid + 2 * kMaxGrid * s; }
+double* partX(Context& c) { return c.partials.get() + 5 * kMaxGrid; }
+
+}  // namespace
+
+void ensure_vectors(Context& c) {
+  const size_t nv = 2 * (size_t)c.nb;
+  c.x.alloc(nv);
+  c.r.alloc(nv);
+  c.p.alloc(nv);
+  c.Ap.alloc(nv);
+  c.inv_diag.alloc(nv);
+  c.invd32.alloc(nv);
+  if (c.b.n != nv) {
+    c.b.alloc(nv);
+    HIP_CHECK(hipMemsetAsync(c.b.get(), 0, sizeof(double) * nv, c.stream));
+  }
+  c.partials.alloc(6 * kMaxGrid + 64);
 }
 
-double elapsed(hipEvent_t a, hipEvent_t b) {
-  float ms = 0.f;
-  HIP_CHECK(hipEventElapsedTime(&ms, a, b));
-  return ms * 1e-3;
+void setup_solver(Context& c) {
+  if (!c.have_pattern) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: no matrix on the device");
+  ensure_vectors(c);
+  const int big = INT_MAX;
+  HIP_CHECK(hipMemcpyAsync(c.err_flag.get(), &big, sizeof(int), hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(c.nb, 256, 0)), dim3(256), 0, c.stream, c.nb, c.n,
+                     c.bdiag.get(), c.sell_pos.get(), c.sell_val.get(), c.inv_diag.get(),
+                     c.invd32.get(), c.err_flag.get());
+  HIP_CHECK(hipGetLastError());
+  int bad = INT_MAX;
+  HIP_CHECK(hipMemcpyAsync(&bad, c.err_flag.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (bad != INT_MAX)
+    throw Error(CSTFEM_ERR_BAD_DIAGONAL, "cg_jacobi: non-positive diagonal at row " + std::to_string(bad));
+  c.setup_done = true;
+  c.vals32_valid = false;
+}
+
+void spmv_fp64(Context& c, const double* x, double* y) {
+  hipLaunchKernelGGL(k_spmv<F64>, dim3(spmv_grid(c)), dim3(kThreads), 0, c.stream, mat64(c),
+                     (const double2*)x, (double2*)y);
+  HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+
+// ||b - A x||^2 into sc->rr (leaves r = b - A x in FP64 in c.r)
+void true_residual(Context& c, double* Ax_scratch) {
+  const int gv = vec_grid(c);
+  spmv_fp64(c, c.x.get(), Ax_scratch);
+  hipLaunchKernelGGL(k_residual, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                     (const double2*)c.b.get(), (const double2*)Ax_scratch, (double2*)c.r.get(),
+                     partX(c));
+  hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(kThreads), 0, c.stream, partX(c), gv,
+                     &c.scal.get()->rr);
+  HIP_CHECK(hipGetLastError());
+}
+
+void enqueue_iteration64(Context& c, int cur, double* hist) {
+  const int gs = spmv_grid(c), gv = vec_grid(c);
+  hipLaunchKernelGGL(k_cg_spmv<F64>, dim3(gs), dim3(kThreads), 0, c.stream, mat64(c),
+                     (const double2*)c.p.get(), (double2*)c.Ap.get(), partA(c),
+                     (const PcgScalars*)c.scal.get());
+  hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                     (double2*)c.x.get(), (double2*)c.r.get(), (const double2*)c.p.get(),
+                     (const double2*)c.Ap.get(), (const double2*)c.inv_diag.get(),
+                     (const double*)partA(c), gs, (const double*)partB(c, cur ^ 1), partB(c, cur),
+                     gv, c.scal.get());
+  hipLaunchKernelGGL(k_cg_direction<F64>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                     (const double2*)c.r.get(), (const double2*)c.inv_diag.get(),
+                     (double2*)c.p.get(), (const double*)partB(c, cur ^ 1),
+                     (const double*)partB(c, cur), gv, cur, c.scal.get(), hist, 1);
+}
+
+// FP32 direction update only (after a refinement refreshed the residual of iteration k-1)
+void enqueue_direction32(Context& c, int cur, int test) {
+  const int gv = vec_grid(c);
+  hipLaunchKernelGGL(k_cg_direction<F32>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                     (const float2*)c.r32.get(), (const float2*)c.invd32.get(),
+                     (float2*)c.p32.get(), (const double*)partB(c, cur ^ 1),
+                     (const double*)partB(c, cur), gv, cur, c.scal.get(), (double*)nullptr, test);
+}
+
+void enqueue_iteration32(Context& c, int cur) {
+  const int gs = spmv_grid(c), gv = vec_grid(c);
+  hipLaunchKernelGGL(k_cg_spmv<F32>, dim3(gs), dim3(kThreads), 0, c.stream, mat32(c),
+                     (const float2*)c.p32.get(), (float2*)c.Ap32.get(), partA(c),
+                     (const PcgScalars*)c.scal.get());
+  hipLaunchKernelGGL(k_cg_update32, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                     (double2*)c.x.get(), (float2*)c.r32.get(), (const float2*)c.p32.get(),
+                     (const float2*)c.Ap32.get(), (const float2*)c.invd32.get(),
+                     (const double*)partA(c), gs, (const double*)partB(c, cur ^ 1), partB(c, cur),
+                     gv, c.scal.get());
+  enqueue_direction32(c, cur, 1);
+}
+
+// Replays `graph` (an even number of iterations) until the device status leaves kRunning;
+// one chunk stays in flight while the previous chunk's status is read.
+void run_until_stopped(Context& c, hipGraphExec_t graph, int chunk, int64_t max_iter) {
+  PcgScalars* slot[2] = {c.h_scal, c.h_scal + 1};
+  hipEvent_t done[2];
+  HIP_CHECK(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+  const int64_t max_chunks = (max_iter + chunk - 1) / chunk + 2;
+  int cur = 0;
+  HIP_CHECK(hipGraphLaunch(graph, c.stream));
+  HIP_CHECK(hipMemcpyAsync(slot[cur], c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipEventRecord(done[cur], c.stream));
+  for (int64_t ch = 1; ch <= max_chunks; ++ch) {
+    const int nxt = cur ^ 1;
+    HIP_CHECK(hipGraphLaunch(graph, c.stream));
+    HIP_CHECK(hipMemcpyAsync(slot[nxt], c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipEventRecord(done[nxt], c.stream));
+    HIP_CHECK(hipEventSynchronize(done[cur]));
+    if (slot[cur]->status != kRunning) break;
+    cur = nxt;
+  }
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  (void)hipEventDestroy(done[0]);
+  (void)hipEventDestroy(done[1]);
+}
+
+PcgScalars read_scalars(Context& c) {
+  HIP_CHECK(hipMemcpyAsync(c.h_scal, c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  return *c.h_scal;
 }
 
 }  // namespace
@@ -418,8 +649,7 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
 int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
                   double* history) {
   if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: solve() called before setup()");
-  const int nvec2 = c.nb;
-  const int gs = spmv_grid(c), gv = vec_grid(c);
+  const int gv = vec_grid(c);
   DBuf<double> d_hist;
   if (o.record_history) d_hist.alloc((size_t)o.max_iter + 1);
   double* hist = o.record_history ? d_hist.get() : nullptr;
@@ -430,88 +660,64 @@ int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
   init.record = o.record_history ? 1 : 0;
   HIP_CHECK(hipMemcpyAsync(c.scal.get(), &init, sizeof init, hipMemcpyHostToDevice, c.stream));
 
-  hipEvent_t ev[4];
-  for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-  HIP_CHECK(hipEventRecord(ev[0], c.stream));
-  hipLaunchKernelGGL(k_cg_init, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
+  hipEvent_t ev0, ev1;
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  HIP_CHECK(hipEventRecord(ev0, c.stream));
+  hipLaunchKernelGGL(k_cg_init, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
                      (const double2*)c.b.get(), (double2*)c.x.get(), (double2*)c.r.get(),
-                     (double2*)c.p.get(), (const double2*)c.inv_diag.get(), c.partials.get(),
-                     c.scal.get(), hist);
+                     (double2*)c.p.get(), (const double2*)c.inv_diag.get(), partB(c, 1));
+  hipLaunchKernelGGL(k_cg_init_finish, dim3(1), dim3(kThreads), 0, c.stream,
+                     (const double*)partB(c, 1), gv, c.scal.get(), hist);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipMemcpyAsync(c.h_scal, c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
-  const bool zero_rhs = c.h_scal->bnorm == 0.0;
+  const bool zero_rhs = read_scalars(c).bnorm == 0.0;
 
   double t_spmv = 0.0, t_vec = 0.0;
   if (!zero_rhs && o.max_iter > 0) {
     if (o.instrument) {
       // eager launches with events around each kernel class (same kernels, same results)
-      hipEvent_t e0, e1, e2, e3;
+      hipEvent_t e0, e1, e2;
       HIP_CHECK(hipEventCreate(&e0));
       HIP_CHECK(hipEventCreate(&e1));
       HIP_CHECK(hipEventCreate(&e2));
-      HIP_CHECK(hipEventCreate(&e3));
-      for (;;) {
+      const int gs = spmv_grid(c);
+      for (int it = 0;; ++it) {
+        const int cur = it & 1;
         HIP_CHECK(hipEventRecord(e0, c.stream));
-        hipLaunchKernelGGL(k_cg_spmv, dim3(gs), dim3(kThreads), 0, c.stream, mat_args(c),
-                           (const double2*)c.p.get(), (double2*)c.Ap.get(), c.partials.get(),
-                           c.scal.get());
+        hipLaunchKernelGGL(k_cg_spmv<F64>, dim3(gs), dim3(kThreads), 0, c.stream, mat64(c),
+                           (const double2*)c.p.get(), (double2*)c.Ap.get(), partA(c),
+                           (const PcgScalars*)c.scal.get());
         HIP_CHECK(hipEventRecord(e1, c.stream));
-        hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
+        hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
                            (double2*)c.x.get(), (double2*)c.r.get(), (const double2*)c.p.get(),
                            (const double2*)c.Ap.get(), (const double2*)c.inv_diag.get(),
-                           c.partials.get(), c.scal.get(), hist);
-        hipLaunchKernelGGL(k_cg_direction, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
+                           (const double*)partA(c), gs, (const double*)partB(c, cur ^ 1),
+                           partB(c, cur), gv, c.scal.get());
+        hipLaunchKernelGGL(k_cg_direction<F64>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
                            (const double2*)c.r.get(), (const double2*)c.inv_diag.get(),
-                           (double2*)c.p.get(), (const PcgScalars*)c.scal.get());
+                           (double2*)c.p.get(), (const double*)partB(c, cur ^ 1),
+                           (const double*)partB(c, cur), gv, cur, c.scal.get(), hist, 1);
         HIP_CHECK(hipEventRecord(e2, c.stream));
-        HIP_CHECK(hipMemcpyAsync(c.h_scal, c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost,
-                                 c.stream));
-        HIP_CHECK(hipEventRecord(e3, c.stream));
-        HIP_CHECK(hipEventSynchronize(e3));
+        const PcgScalars s = read_scalars(c);
         t_spmv += elapsed(e0, e1);
         t_vec += elapsed(e1, e2);
-        if (c.h_scal->status != kRunning) break;
+        if (s.status != kRunning) break;
       }
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       (void)hipEventDestroy(e2);
-      (void)hipEventDestroy(e3);
     } else {
-      // chunk of iterations captured once, replayed until the device status leaves kRunning;
-      // one chunk stays in flight while the previous chunk's status is read.
-      const int chunk = c.n > 200000 ? 16 : 64;
+      const int chunk = c.n > 200000 ? 16 : 64;  // even
       Graph gr;
       HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-      for (int i = 0; i < chunk; ++i) enqueue_iteration(c, gs, gv, hist);
+      for (int i = 0; i < chunk; ++i) enqueue_iteration64(c, i & 1, hist);
       HIP_CHECK(hipStreamEndCapture(c.stream, &gr.g));
       HIP_CHECK(hipGraphInstantiate(&gr.e, gr.g, nullptr, nullptr, 0));
-      PcgScalars* slot[2] = {c.h_scal, c.h_scal + 1};
-      hipEvent_t done[2];
-      HIP_CHECK(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
-      HIP_CHECK(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
-      const int64_t max_chunks = ((int64_t)o.max_iter + chunk - 1) / chunk + 1;
-      int cur = 0;
-      HIP_CHECK(hipGraphLaunch(gr.e, c.stream));
-      HIP_CHECK(hipMemcpyAsync(slot[cur], c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
-      HIP_CHECK(hipEventRecord(done[cur], c.stream));
-      for (int64_t ch = 1; ch <= max_chunks; ++ch) {
-        const int nxt = cur ^ 1;
-        HIP_CHECK(hipGraphLaunch(gr.e, c.stream));
-        HIP_CHECK(hipMemcpyAsync(slot[nxt], c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
-        HIP_CHECK(hipEventRecord(done[nxt], c.stream));
-        HIP_CHECK(hipEventSynchronize(done[cur]));
-        if (slot[cur]->status != kRunning) break;
-        cur = nxt;
-      }
-      HIP_CHECK(hipStreamSynchronize(c.stream));
-      (void)hipEventDestroy(done[0]);
-      (void)hipEventDestroy(done[1]);
+      run_until_stopped(c, gr.e, chunk, o.max_iter);
     }
   }
-  HIP_CHECK(hipMemcpyAsync(c.h_scal, c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
-  const PcgScalars s = *c.h_scal;
+  PcgScalars s = read_scalars(c);
+  const int iters = s.k_in[0] > s.k_in[1] ? s.k_in[0] : s.k_in[1];
 
   // true residual b - A x (pcg_jacobi, solver.cpp:113-129)
   double true_rel = 0.0;
@@ -520,29 +726,25 @@ int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
     HIP_CHECK(hipEventCreate(&t0));
     HIP_CHECK(hipEventCreate(&t1));
     HIP_CHECK(hipEventRecord(t0, c.stream));
-    spmv_fp64(c, c.x.get(), c.Ap.get());
+    true_residual(c, c.Ap.get());
     HIP_CHECK(hipEventRecord(t1, c.stream));
-    hipLaunchKernelGGL(k_residual, dim3(gv), dim3(kThreads), 0, c.stream, nvec2,
-                       (const double2*)c.b.get(), (const double2*)c.Ap.get(), (double2*)c.r.get(),
-                       c.partials.get(), c.scal.get());
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(c.h_scal, c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
-    HIP_CHECK(hipStreamSynchronize(c.stream));
-    true_rel = std::sqrt(c.h_scal->rr) / s.bnorm;
+    const PcgScalars s2 = read_scalars(c);
+    true_rel = std::sqrt(s2.rr) / s.bnorm;
     t_spmv += elapsed(t0, t1);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
   }
-  HIP_CHECK(hipEventRecord(ev[1], c.stream));
-  HIP_CHECK(hipEventSynchronize(ev[1]));
-  const double total = elapsed(ev[0], ev[1]);
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  HIP_CHECK(hipEventRecord(ev1, c.stream));
+  HIP_CHECK(hipEventSynchronize(ev1));
+  const double total = elapsed(ev0, ev1);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
 
   if (history && o.record_history) {
-    const int cnt = zero_rhs ? 1 : s.k + 1;
+    const int cnt = zero_rhs ? 1 : iters + 1;
     HIP_CHECK(hipMemcpy(history, hist, sizeof(double) * cnt, hipMemcpyDeviceToHost));
   }
-  st.iterations = zero_rhs ? 0 : s.k;
+  st.iterations = zero_rhs ? 0 : iters;
   st.converged = (zero_rhs || s.status == kConverged) ? 1 : 0;
   st.breakdown = s.status == kBreakdown ? 1 : 0;
   st.zero_rhs = zero_rhs ? 1 : 0;
@@ -550,8 +752,6 @@ int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
   st.true_rel_residual = true_rel;
   st.breakdown_pap = s.status == kBreakdown ? s.pAp : 0.0;
   st.warning = (st.converged && !zero_rhs && true_rel > 10.0 * o.rtol) ? 1 : 0;
-  st.outer_iterations = 0;
-  st.fallback = 0;
   st.spmv_calls = zero_rhs ? 0 : (long long)st.iterations + (st.breakdown ? 1 : 0) + 1;
   // the reference's reduction count: norm(b), r'z, then per iteration p'Ap, ||r|| (+ r'z
   // unless it stopped), then pcg_jacobi's norm(b) and ||b - Ax||
@@ -561,48 +761,176 @@ int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
     red += 2 * k + (st.converged ? k - 1 : k) + (st.breakdown ? 1 : 0) + 2;
   }
   st.reductions = red;
-  if (o.instrument) {
-    st.spmv_local = t_spmv;
-    st.vector_ops = t_vec;
-    st.dot_local = 0.0;  // fused into the SpMV and update kernels
-    st.total = total;
-  } else {
-    st.spmv_local = st.vector_ops = st.dot_local = 0.0;
-    st.total = total;
-  }
+  st.spmv_local = o.instrument ? t_spmv : 0.0;
+  st.vector_ops = o.instrument ? t_vec : 0.0;
+  st.dot_local = 0.0;  // fused into the SpMV, update and direction kernels
+  st.total = total;
   if (zero_rhs || s.status == kConverged) return CSTFEM_OK;
   return CSTFEM_NOT_CONVERGED;
 }
 
-namespace {
+// Jacobi-PCG refined by FP64 true residuals.  FP32 mode: FP32 matrix values and vectors,
+// FP64-accumulated dots and an FP64 solution; FP64 mode: the reference recurrence.  Whenever
+// the recursive residual reaches rtol (or, FP32 mode, drops by `delta` since the last
+// refinement), r is replaced by b - A x computed in FP64 -- the search direction is kept, so
+// the Krylov information survives -- and the solve stops only when the FP64 true residual is
+// below rtol.
+int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st, bool fp32) {
+  if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: solve() called before setup()");
+  const size_t nv = 2 * (size_t)c.nb;
+  if (fp32) {
+    c.r32.alloc(nv);
+    c.p32.alloc(nv);
+    c.Ap32.alloc(nv);
+    if (!c.vals32_valid) {
+      c.sell_val32.alloc(4 * c.sell_blocks);
+      hipLaunchKernelGGL(k_round_values, dim3(2048), dim3(256), 0, c.stream,
+                         (int64_t)(4 * c.sell_blocks), (const double*)c.sell_val.get(),
+                         c.sell_val32.get());
+      HIP_CHECK(hipGetLastError());
+      c.vals32_valid = true;
+    }
+  }
+  const int gv = vec_grid(c);
+  const double delta = o.inner_rtol > 0 ? o.inner_rtol : (fp32 ? 1e-2 : 0.0);
+  const int max_outer = o.max_outer > 0 ? o.max_outer : 200;
+  auto iteration = [&](int cur) {
+    if (fp32)
+      enqueue_iteration32(c, cur);
+    else
+      enqueue_iteration64(c, cur, nullptr);
+  };
+  auto direction = [&](int cur) {
+    if (fp32) {
+      enqueue_direction32(c, cur, 0);
+    } else {
+      hipLaunchKernelGGL(k_cg_direction<F64>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                         (const double2*)c.r.get(), (const double2*)c.inv_diag.get(),
+                         (double2*)c.p.get(), (const double*)partB(c, cur ^ 1),
+                         (const double*)partB(c, cur), gv, cur, c.scal.get(), (double*)nullptr, 0);
+    }
+  };
 
-__global__ __launch_bounds__(kThreads) void k_copy(int64_t n2, const double2* __restrict__ a,
-                                                   double2* __restrict__ b) {
-  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n2;
-       i += (int64_t)gridDim.x * kThreads)
-    reinterpret_cast<v2d*>(b)[i] = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(a) + i);
+  PcgScalars init{};
+  init.rtol = o.rtol;
+  init.max_iter = o.max_iter;
+  init.refine_mode = 1;
+  init.delta = delta;
+  init.rel_ref = 1.0;
+  HIP_CHECK(hipMemcpyAsync(c.scal.get(), &init, sizeof init, hipMemcpyHostToDevice, c.stream));
+  hipEvent_t ev0, ev1;
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  HIP_CHECK(hipEventRecord(ev0, c.stream));
+  if (fp32)
+    hipLaunchKernelGGL(k_init32, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                       (const double2*)c.b.get(), (double2*)c.x.get(), (float2*)c.r32.get(),
+                       (float2*)c.p32.get(), (const float2*)c.invd32.get(), partB(c, 1));
+  else
+    hipLaunchKernelGGL(k_cg_init, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                       (const double2*)c.b.get(), (double2*)c.x.get(), (double2*)c.r.get(),
+                       (double2*)c.p.get(), (const double2*)c.inv_diag.get(), partB(c, 1));
+  hipLaunchKernelGGL(k_cg_init_finish, dim3(1), dim3(kThreads), 0, c.stream,
+                     (const double*)partB(c, 1), gv, c.scal.get(), (double*)nullptr);
+  HIP_CHECK(hipGetLastError());
+  PcgScalars s = read_scalars(c);
+  const double bnorm = s.bnorm;
+  const bool zero_rhs = bnorm == 0.0;
+  int outer = 0;
+  double true_rel = zero_rhs ? 0.0 : 1.0;
+  bool converged = zero_rhs;
+  int last_status = kRunning;
+  if (!zero_rhs && o.max_iter > 0) {
+    const int chunk = c.n > 200000 ? 16 : 64;  // even
+    Graph gr;
+    HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < chunk; ++i) iteration(i & 1);
+    HIP_CHECK(hipStreamEndCapture(c.stream, &gr.g));
+    HIP_CHECK(hipGraphInstantiate(&gr.e, gr.g, nullptr, nullptr, 0));
+    double prev_true = 1.0;
+    int stagnant = 0;
+    for (;;) {
+      run_until_stopped(c, gr.e, chunk, o.max_iter);
+      s = read_scalars(c);
+      last_status = s.status;
+      // iteration k-1 (parity (k-1)&1) stopped before its direction update
+      const int k = s.k_in[0] > s.k_in[1] ? s.k_in[0] : s.k_in[1];
+      if (s.status != kRefine) break;  // breakdown or max_iter
+      // refinement: FP64 true residual r = b - A x; the residual of iteration k-1 and its
+      // partials (r'r, r'z) are replaced by the refreshed ones
+      const int cur = (k - 1) & 1;
+      true_residual(c, c.Ap.get());
+      if (fp32)
+        hipLaunchKernelGGL(k_refresh32, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                           (const double2*)c.r.get(), (float2*)c.r32.get(),
+                           (const float2*)c.invd32.get(), partB(c, cur));
+      else
+        hipLaunchKernelGGL(k_refresh64, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                           (const double2*)c.r.get(), (const double2*)c.inv_diag.get(),
+                           partB(c, cur));
+      HIP_CHECK(hipGetLastError());
+      const PcgScalars s2 = read_scalars(c);
+      true_rel = std::sqrt(s2.rr) / bnorm;
+      ++outer;
+      if (true_rel < o.rtol) {
+        converged = true;
+        break;
+      }
+      stagnant = true_rel > 0.5 * prev_true ? stagnant + 1 : 0;
+      prev_true = true_rel;
+      if (outer >= max_outer || k >= o.max_iter || stagnant >= 4) break;
+      PcgScalars upd = s2;
+      upd.status = kRunning;
+      upd.rel_ref = true_rel;
+      HIP_CHECK(hipMemcpyAsync(c.scal.get(), &upd, sizeof upd, hipMemcpyHostToDevice, c.stream));
+      // the skipped direction update of iteration k-1, with the refreshed residual
+      direction(cur);
+      // the graph starts at parity 0: an odd next iteration index k runs once eagerly
+      if (k & 1) iteration(1);
+      HIP_CHECK(hipGetLastError());
+      last_status = kRunning;
+    }
+  }
+  s = read_scalars(c);
+  HIP_CHECK(hipEventRecord(ev1, c.stream));
+  HIP_CHECK(hipEventSynchronize(ev1));
+  const int iters = s.k_in[0] > s.k_in[1] ? s.k_in[0] : s.k_in[1];
+  st.iterations = zero_rhs ? 0 : iters;
+  st.outer_iterations = outer;
+  st.converged = converged ? 1 : 0;
+  st.breakdown = last_status == kBreakdown ? 1 : 0;
+  st.zero_rhs = zero_rhs ? 1 : 0;
+  st.rel_residual = zero_rhs ? 0.0 : s.rel;
+  st.true_rel_residual = true_rel;
+  st.breakdown_pap = last_status == kBreakdown ? s.pAp : 0.0;
+  st.warning = 0;
+  st.spmv_calls = zero_rhs ? 0 : (long long)iters + outer;
+  st.reductions = 0;
+  st.total = elapsed(ev0, ev1);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  return converged ? CSTFEM_OK : CSTFEM_NOT_CONVERGED;
 }
 
-}  // namespace
-
 // Average duration (seconds) of `reps` back-to-back launches of one hot kernel on the
-// library stream, between two HIP events.  kind: 0 fused SpMV + p'Ap (k_cg_spmv),
-// 1 plain SpMV, 2 update (k_cg_update), 3 direction (k_cg_direction), 4 copy of
-// `bytes` bytes (achievable HBM bandwidth).  Solver state is not modified (private
-// scalars and vectors).
+// library stream, between two HIP events.  kind: 0 SpMV + p'Ap partials (k_cg_spmv, FP64),
+// 1 plain FP64 SpMV, 2 update (k_cg_update), 3 direction (k_cg_direction), 4 copy of
+// `bytes` bytes (achievable HBM bandwidth), 5 FP32 SpMV + partials.  Solver state is not
+// modified (private scalars and vectors).
 double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
   if (kind != 4 && !c.have_pattern) throw Error(CSTFEM_ERR_ORDER, "profile: no matrix on the device");
   if (kind != 4) ensure_vectors(c);
   DBuf<PcgScalars> sc;
   sc.alloc(1);
   PcgScalars init{};
-  init.rz = 1.0;
   init.bnorm = 1.0;
   init.rtol = 0.0;
   init.max_iter = INT_MAX;
   HIP_CHECK(hipMemcpyAsync(sc.get(), &init, sizeof init, hipMemcpyHostToDevice, c.stream));
-  DBuf<double> a, b, w;
+  DBuf<double> a, b, w, part;
+  DBuf<float> a32, v32;
   const size_t nv = 2 * (size_t)c.nb;
+  part.alloc(6 * kMaxGrid);
   if (kind == 4) {
     const int64_t n2 = bytes / 16;  // double2 elements in `bytes`
     a.alloc(2 * n2);
@@ -614,34 +942,52 @@ double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
     w.alloc(nv);
     HIP_CHECK(hipMemsetAsync(a.get(), 0, sizeof(double) * nv, c.stream));
     HIP_CHECK(hipMemsetAsync(w.get(), 0, sizeof(double) * nv, c.stream));
+    HIP_CHECK(hipMemsetAsync(part.get(), 0, sizeof(double) * 6 * kMaxGrid, c.stream));
     HIP_CHECK(hipMemcpyAsync(b.get(), c.inv_diag.get(), sizeof(double) * nv, hipMemcpyDeviceToDevice, c.stream));
+    if (kind == 5) {
+      a32.alloc(nv);
+      v32.alloc(4 * c.sell_blocks);
+      hipLaunchKernelGGL(k_round_values, dim3(2048), dim3(256), 0, c.stream,
+                         (int64_t)(4 * c.sell_blocks), (const double*)c.sell_val.get(), v32.get());
+      HIP_CHECK(hipMemsetAsync(a32.get(), 0, sizeof(float) * nv, c.stream));
+    }
   }
   const int gs = spmv_grid(c), gv = vec_grid(c);
   auto launch = [&]() {
     switch (kind) {
       case 0:
-        hipLaunchKernelGGL(k_cg_spmv, dim3(gs), dim3(kThreads), 0, c.stream, mat_args(c),
-                           (const double2*)c.inv_diag.get(), (double2*)a.get(), c.partials.get(), sc.get());
+        hipLaunchKernelGGL(k_cg_spmv<F64>, dim3(gs), dim3(kThreads), 0, c.stream, mat64(c),
+                           (const double2*)c.inv_diag.get(), (double2*)a.get(), part.get(),
+                           (const PcgScalars*)sc.get());
         break;
       case 1:
-        hipLaunchKernelGGL(k_spmv, dim3(gs), dim3(kThreads), 0, c.stream, mat_args(c),
+        hipLaunchKernelGGL(k_spmv<F64>, dim3(gs), dim3(kThreads), 0, c.stream, mat64(c),
                            (const double2*)c.inv_diag.get(), (double2*)a.get());
         break;
       case 2:
         hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
                            (double2*)a.get(), (double2*)w.get(), (const double2*)b.get(),
                            (const double2*)b.get(), (const double2*)c.inv_diag.get(),
-                           c.partials.get(), sc.get(), (double*)nullptr);
+                           (const double*)part.get(), gs, (const double*)(part.get() + kMaxGrid),
+                           part.get() + 3 * kMaxGrid, gv, sc.get());
         break;
       case 3:
-        hipLaunchKernelGGL(k_cg_direction, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+        hipLaunchKernelGGL(k_cg_direction<F64>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
                            (const double2*)b.get(), (const double2*)c.inv_diag.get(),
-                           (double2*)a.get(), (const PcgScalars*)sc.get());
+                           (double2*)a.get(), (const double*)(part.get() + kMaxGrid),
+                           (const double*)(part.get() + 3 * kMaxGrid), gv, 0, sc.get(),
+                           (double*)nullptr, 1);
         break;
+      case 5: {
+        Mat<F32> m32{c.sell_off.get(), c.sell_col.get(), v32.get(), c.nslices, c.nb};
+        hipLaunchKernelGGL(k_cg_spmv<F32>, dim3(gs), dim3(kThreads), 0, c.stream, m32,
+                           (const float2*)c.invd32.get(), (float2*)a32.get(), part.get(),
+                           (const PcgScalars*)sc.get());
+        break;
+      }
       default: {
-        const int64_t n2 = bytes / 16;  // double2 elements in `bytes`
-        const int g = c.cu_count > 0 ? 8 * c.cu_count : 2048;
-        hipLaunchKernelGGL(k_copy, dim3(g), dim3(kThreads), 0, c.stream, n2,
+        const int64_t n2 = bytes / 16;
+        hipLaunchKernelGGL(k_copy, dim3(kMaxGrid), dim3(kThreads), 0, c.stream, n2,
                            (const double2*)a.get(), (double2*)b.get());
       }
     }

@@ -24,7 +24,7 @@ OK, NOT_CONVERGED = 0, 2
 ERR_ARGUMENT, ERR_DEVICE, ERR_BAD_DIAGONAL, ERR_MISSING_ENTRY = -1, -2, -3, -4
 ERR_OUT_OF_RANGE, ERR_ALLOC, ERR_TRACTION_EDGE, ERR_CONFLICT, ERR_ORDER = -5, -6, -7, -8, -9
 PLANE_STRESS, PLANE_STRAIN = 0, 1
-METHOD_CG_FP64, METHOD_IR_FP32, METHOD_FCG_LEARNED = 0, 1, 2
+METHOD_CG_FP64, METHOD_IR_FP32, METHOD_FCG_LEARNED, METHOD_CG_FP64_REFINED = 0, 1, 2, 3
 
 
 class CstfemError(RuntimeError):
@@ -86,6 +86,7 @@ SIGNATURES = {
     "cstfem_upload_mesh": (ctypes.c_int, [_P, ctypes.c_int, _D, ctypes.c_int64, _I]),
     "cstfem_set_material": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                            ctypes.c_double, _D]),
+    "cstfem_set_constitutive": (ctypes.c_int, [_P, _D, ctypes.c_double]),
     "cstfem_build_pattern": (ctypes.c_int, [_P, _I64]),
     "cstfem_download_pattern": (ctypes.c_int, [_P, _I, _I]),
     "cstfem_assemble": (ctypes.c_int, [_P]),
@@ -187,6 +188,11 @@ class Device:
         Ee = None if E_elem is None else _f64(E_elem)
         _check(lib().cstfem_set_material(self._h, float(E), float(nu), int(plane),
                                          float(thickness), _ptr(Ee, ctypes.c_double)))
+
+    def set_constitutive(self, D, thickness=1.0):
+        """Uniform 3x3 constitutive matrix, as assemble_stiffness(mesh, D, t, K) takes it."""
+        D = _f64(D).reshape(-1)
+        _check(lib().cstfem_set_constitutive(self._h, _ptr(D, ctypes.c_double), float(thickness)))
 
     # ---- pattern and assembly (assembly.cpp:9-74)
     def build_stiffness_pattern(self):

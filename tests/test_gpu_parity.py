@@ -254,3 +254,41 @@ def test_reproducible(dev):
     x0, s0 = dev.solve()
     x1, s1 = dev.solve()
     assert s0.iterations == s1.iterations and np.array_equal(x0, x1)
+
+
+@pytest.mark.parametrize("method", [cstfem_hip.METHOD_CG_FP64_REFINED, cstfem_hip.METHOD_IR_FP32])
+def test_refined_methods_reach_true_rtol(dev, oracle, method):
+    """FP64-refined and FP32+FP64-refinement solves stop on the FP64 true residual; their
+    solutions agree with the oracle's FP64 Jacobi-CG solution (rtol 1e-12) to 1e-8."""
+    prob = pipeline.plate_problem(80, 40, Lx=2.0, Ly=1.0, jitter=0.2, seed=12,
+                                  plane=cstfem_hip.PLANE_STRAIN, E_sigma=1.0, E_seed=3,
+                                  body=(0.0, -1.0))
+    o = run_oracle(oracle, prob, rtol=1e-13, max_iter=100000)
+    pipeline.load_problem(dev, prob)
+    pipeline.assemble_and_constrain(dev, prob)
+    dev.setup()
+    x, st = dev.solve(rtol=1e-10, max_iter=100000, method=method)
+    b = dev.download_rhs()
+    r = b - dev.spmv(x)
+    true_rel = np.linalg.norm(r) / np.linalg.norm(b)
+    print(f"method {method}: {st.iterations} iterations, {st.outer_iterations} refinements, "
+          f"true rel {st.true_rel_residual:.3e} (host check {true_rel:.3e}), "
+          f"oracle {o['st']['iterations']} iterations; "
+          f"max rel diff vs oracle {rel_err(x, o['x']):.2e}")
+    assert st.converged and st.true_rel_residual < 1e-10
+    assert true_rel < 1.5e-10
+    assert rel_err(x, o["x"]) < 1e-7
+
+
+def test_ir_fp32_zero_rhs_and_maxiter(dev):
+    prob = pipeline.plate_problem(16, 16, seed=2)
+    pipeline.load_problem(dev, prob)
+    pipeline.assemble_and_constrain(dev, prob)
+    dev.set_rhs(np.zeros(dev.n))
+    dev.setup()
+    x, st = dev.solve(method=cstfem_hip.METHOD_IR_FP32)
+    assert st.converged and st.iterations == 0 and not np.any(x)
+    pipeline.assemble_and_constrain(dev, prob)
+    dev.setup()
+    x, st = dev.solve(method=cstfem_hip.METHOD_IR_FP32, max_iter=4)
+    assert not st.converged and st.iterations <= 4
