@@ -83,6 +83,9 @@ def main():
     ap.add_argument("--max-iter", type=int, default=200000)
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time only the hot kernels (for hardware-counter profiles)")
+    ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--methods", default="fp64,ir32",
                     help="comma list of fp64 (FP64 Jacobi-CG, FP64 true-residual refinement) "
                          "and ir32 (FP32 Jacobi-CG under FP64 refinement)")
@@ -103,6 +106,21 @@ def main():
     dev = cstfem_hip.Device(0)
     name, cus = dev.info()
     pipeline.load_problem(dev, prob)
+
+    if args.kernels_only:
+        # hot kernels only (for hardware-counter runs): one assembly, then `reps` launches of
+        # each kernel class on the library stream
+        pipeline.assemble_and_constrain(dev, prob)
+        dev.setup()
+        n, nnz = dev.n, dev.nnz
+        out = {}
+        for kind, name_k in ((0, "k_cg_spmv<F64>"), (5, "k_cg_spmv<F32>"), (2, "k_cg_update"),
+                             (3, "k_cg_direction<F64>")):
+            out[name_k] = dev.profile_kernel(kind, reps=args.reps)
+        out["k_copy(256MiB)"] = dev.profile_kernel(4, reps=args.reps, nbytes=256 << 20)
+        out["spmv_bytes"] = spmv_bytes(n, nnz)
+        print(json.dumps(out), flush=True)
+        return
 
     methods = {"fp64": cstfem_hip.METHOD_CG_FP64_REFINED, "ir32": cstfem_hip.METHOD_IR_FP32}
     chosen = [m for m in args.methods.split(",") if m]

@@ -60,8 +60,11 @@ typedef struct {
   int method;             /* CSTFEM_METHOD_* */
   double inner_rtol;      /* refined methods: refresh r from the FP64 true residual when the
                              recursive residual drops by this factor since the last refresh
-                             (0 = default: 1e-2 in FP32, only at rtol in FP64) */
+                             (0 = default: 1e-3 in FP32, 1e-2 in FP64) */
   int max_outer;          /* refined methods: max refreshes (0 = default 200) */
+  int refresh;            /* refined methods, after r := b - A x:  0 restart (p := D^-1 r, the
+                             default), 1 keep the search direction after a corrective line
+                             search that restores r'p = 0, 2 keep the search direction */
 } cstfem_solver_options;
 
 /* Mirrors SolverStats (solver.hpp:99-107) and SolveBreakdown (solver.hpp:17-35). */
@@ -144,8 +147,8 @@ int cstfem_spmv(cstfem_ctx* ctx, const double* x, double* y, int n);
 /* ---- measurement (no reference counterpart) ------------------------------------------ */
 /* Average seconds of `reps` back-to-back launches of one hot kernel on the library's
  * stream, timed with HIP events; solver state is untouched.  kind: 0 fused SpMV + p'Ap,
- * 1 plain SpMV, 2 CG update (axpys + dots), 3 CG direction, 4 copy of `bytes` bytes
- * (achievable HBM bandwidth). */
+ * 1 plain SpMV, 2 CG update (axpys + dots), 3 CG direction, 4 copy of `bytes` bytes,
+ * (achievable HBM bandwidth), 5 FP32 SpMV + p'Ap. */
 int cstfem_profile_kernel(cstfem_ctx* ctx, int kind, int reps, int64_t bytes,
                           double* avg_seconds);
 

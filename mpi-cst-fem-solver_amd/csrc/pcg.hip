@@ -21,6 +21,8 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <functional>
 
 #include "internal.hpp"
 
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_direction(
   const int k = sc->k_in[cur ^ 1] + 1;
   const double rel = sqrt(rr) / sc->bnorm;
   int status = kRunning;
-  if (!test)
+  if (test != 1)
     status = kRunning;  // direction update after a refinement: the test already ran
   else if (rel < sc->rtol)
     status = sc->refine_mode ? kRefine : kConverged;
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_direction(
     if (status != kRunning) sc->status = status;
   }
   if (status != kRunning) return;
-  const double beta = rz_new / rz_old;
+  const double beta = test == 2 ? 0.0 : rz_new / rz_old;  // 2: restart after a refinement
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
     const typename P::V ri = r[i], di = invd[i];
     typename P::V pi = p[i];
@@ -378,40 +380,63 @@ __global__ __launch_bounds__(kThreads) void k_sum_into(const double* __restrict_
   if (threadIdx.x == 0) *out = s;
 }
 
-// FP32 refinement step: r32 = r64 (rounded), p32 = D^-1 r32 restarts nothing: the search
-// direction is kept; partials (r'r, r'z) of the refreshed residual into slot `slot`.
-__global__ __launch_bounds__(kThreads) void k_refresh32(int nvec2, const double2* __restrict__ r64,
-                                                        float2* __restrict__ r32,
-                                                        const float2* __restrict__ invd,
-                                                        double* __restrict__ part) {
+// Partials of r64'p (the corrective step of a refinement needs it).
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_dot_rp(int nvec2, const double2* __restrict__ r,
+                                                     const typename P::V* __restrict__ p,
+                                                     double* __restrict__ part) {
   __shared__ double lds[kWaves];
-  double rr = 0.0, rz = 0.0;
+  double s = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
-    const double2 a = r64[i];
-    const float2 ri = make_float2((float)a.x, (float)a.y), di = invd[i];
-    r32[i] = ri;
-    const double r0 = ri.x, r1 = ri.y;
-    rr += r0 * r0 + r1 * r1;
-    rz += r0 * ((double)di.x * r0) + r1 * ((double)di.y * r1);
+    const double2 ri = r[i];
+    const typename P::V pi = p[i];
+    s += ri.x * (double)pi.x + ri.y * (double)pi.y;
   }
-  const double s0 = block_sum_all(rr, lds);
-  const double s1 = block_sum_all(rz, lds);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = s0;
-    part[2 * blockIdx.x + 1] = s1;
-  }
+  const double t = block_sum_all(s, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// FP64 refinement step: partials (r'r, r'z) of the refreshed residual r = b - A x
-__global__ __launch_bounds__(kThreads) void k_refresh64(int nvec2, const double2* __restrict__ r,
-                                                        const double2* __restrict__ invd,
-                                                        double* __restrict__ part) {
+// Refinement with r64 = b - A x (FP64).  correct = 1: corrective line search along the kept
+// direction p, gamma = (r'p)/(p'Ap), x += gamma p, r -= gamma Ap, which restores r'p = 0
+// (broken by the replacement).  Then the working residual (FP32 copy in FP32 mode) and the
+// partials (r'r, r'z) of the refreshed residual into `part`.
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_refresh(
+    int nvec2, double2* __restrict__ x, double2* __restrict__ r64,
+    const typename P::V* __restrict__ p, const typename P::V* __restrict__ Ap,
+    const typename P::V* __restrict__ invd, float2* __restrict__ r32,
+    const double* __restrict__ partA, int nA, const double* __restrict__ partRP, int nRP,
+    int correct, double* __restrict__ part) {
   __shared__ double lds[kWaves];
+  double gamma = 0.0;
+  if (correct) {
+    const double pAp = reduce_partials(partA, nA, 1, lds);
+    const double rp = reduce_partials(partRP, nRP, 1, lds);
+    gamma = pAp > 0.0 ? rp / pAp : 0.0;
+  }
   double rr = 0.0, rz = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
-    const double2 ri = r[i], di = invd[i];
-    rr += ri.x * ri.x + ri.y * ri.y;
-    rz += ri.x * (di.x * ri.x) + ri.y * (di.y * ri.y);
+    double2 ri = r64[i];
+    const typename P::V di = invd[i];
+    if (correct) {
+      const typename P::V pi = p[i], api = Ap[i];
+      double2 xi = x[i];
+      xi.x = xi.x + gamma * (double)pi.x;
+      xi.y = xi.y + gamma * (double)pi.y;
+      x[i] = xi;
+      ri.x = ri.x - gamma * (double)api.x;
+      ri.y = ri.y - gamma * (double)api.y;
+      r64[i] = ri;
+    }
+    double r0 = ri.x, r1 = ri.y;
+    if (r32) {
+      const float2 f = make_float2((float)ri.x, (float)ri.y);
+      r32[i] = f;
+      r0 = f.x;
+      r1 = f.y;
+    }
+    rr += r0 * r0 + r1 * r1;
+    rz += r0 * ((double)di.x * r0) + r1 * ((double)di.y * r1);
   }
   const double s0 = block_sum_all(rr, lds);
   const double s1 = block_sum_all(rz, lds);
@@ -524,6 +549,7 @@ double* partX(Context& c) { return c.partials.get() + 5 * kMaxGrid; }
 void ensure_vectors(Context& c) {
   const size_t nv = 2 * (size_t)c.nb;
   c.x.alloc(nv);
+  c.work.alloc(nv);
   c.r.alloc(nv);
   c.p.alloc(nv);
   c.Ap.alloc(nv);
@@ -614,19 +640,46 @@ void enqueue_iteration32(Context& c, int cur) {
 
 // Replays `graph` (an even number of iterations) until the device status leaves kRunning;
 // one chunk stays in flight while the previous chunk's status is read.
-void run_until_stopped(Context& c, hipGraphExec_t graph, int chunk, int64_t max_iter) {
+// `chunk` iterations as one unit of work: a captured hipGraph, or -- when the environment
+// sets CSTFEM_GRAPH=0, for tools that cannot follow graph launches -- the same kernels
+// launched directly.
+struct Chunk {
+  Graph gr;
+  std::function<void(int)> iteration;
+  int n = 0;
+  bool graph = true;
+  Chunk(Context& c, int count, std::function<void(int)> it) : iteration(std::move(it)), n(count) {
+    const char* env = std::getenv("CSTFEM_GRAPH");
+    graph = !(env && env[0] == '0');
+    if (!graph) return;
+    HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < n; ++i) iteration(i & 1);
+    HIP_CHECK(hipStreamEndCapture(c.stream, &gr.g));
+    HIP_CHECK(hipGraphInstantiate(&gr.e, gr.g, nullptr, nullptr, 0));
+  }
+  void launch(hipStream_t s) {
+    if (graph) {
+      HIP_CHECK(hipGraphLaunch(gr.e, s));
+    } else {
+      for (int i = 0; i < n; ++i) iteration(i & 1);
+      HIP_CHECK(hipGetLastError());
+    }
+  }
+};
+
+void run_until_stopped(Context& c, Chunk& graph, int chunk, int64_t max_iter) {
   PcgScalars* slot[2] = {c.h_scal, c.h_scal + 1};
   hipEvent_t done[2];
   HIP_CHECK(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
   HIP_CHECK(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
   const int64_t max_chunks = (max_iter + chunk - 1) / chunk + 2;
   int cur = 0;
-  HIP_CHECK(hipGraphLaunch(graph, c.stream));
+  graph.launch(c.stream);
   HIP_CHECK(hipMemcpyAsync(slot[cur], c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipEventRecord(done[cur], c.stream));
   for (int64_t ch = 1; ch <= max_chunks; ++ch) {
     const int nxt = cur ^ 1;
-    HIP_CHECK(hipGraphLaunch(graph, c.stream));
+    graph.launch(c.stream);
     HIP_CHECK(hipMemcpyAsync(slot[nxt], c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
     HIP_CHECK(hipEventRecord(done[nxt], c.stream));
     HIP_CHECK(hipEventSynchronize(done[cur]));
@@ -708,12 +761,8 @@ int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
       (void)hipEventDestroy(e2);
     } else {
       const int chunk = c.n > 200000 ? 16 : 64;  // even
-      Graph gr;
-      HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-      for (int i = 0; i < chunk; ++i) enqueue_iteration64(c, i & 1, hist);
-      HIP_CHECK(hipStreamEndCapture(c.stream, &gr.g));
-      HIP_CHECK(hipGraphInstantiate(&gr.e, gr.g, nullptr, nullptr, 0));
-      run_until_stopped(c, gr.e, chunk, o.max_iter);
+      Chunk gr(c, chunk, [&](int cur) { enqueue_iteration64(c, cur, hist); });
+      run_until_stopped(c, gr, chunk, o.max_iter);
     }
   }
   PcgScalars s = read_scalars(c);
@@ -726,7 +775,7 @@ int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
     HIP_CHECK(hipEventCreate(&t0));
     HIP_CHECK(hipEventCreate(&t1));
     HIP_CHECK(hipEventRecord(t0, c.stream));
-    true_residual(c, c.Ap.get());
+    true_residual(c, c.work.get());
     HIP_CHECK(hipEventRecord(t1, c.stream));
     const PcgScalars s2 = read_scalars(c);
     true_rel = std::sqrt(s2.rr) / s.bnorm;
@@ -792,7 +841,7 @@ int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
     }
   }
   const int gv = vec_grid(c);
-  const double delta = o.inner_rtol > 0 ? o.inner_rtol : (fp32 ? 1e-2 : 0.0);
+  const double delta = o.inner_rtol > 0 ? o.inner_rtol : (fp32 ? 1e-3 : 1e-2);
   const int max_outer = o.max_outer > 0 ? o.max_outer : 200;
   auto iteration = [&](int cur) {
     if (fp32)
@@ -800,14 +849,17 @@ int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
     else
       enqueue_iteration64(c, cur, nullptr);
   };
+  const int refresh = o.refresh;                // 0 restart, 1 corrective step, 2 keep p
+  const int dir_mode = refresh == 0 ? 2 : 0;
   auto direction = [&](int cur) {
     if (fp32) {
-      enqueue_direction32(c, cur, 0);
+      enqueue_direction32(c, cur, dir_mode);
     } else {
       hipLaunchKernelGGL(k_cg_direction<F64>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
                          (const double2*)c.r.get(), (const double2*)c.inv_diag.get(),
                          (double2*)c.p.get(), (const double*)partB(c, cur ^ 1),
-                         (const double*)partB(c, cur), gv, cur, c.scal.get(), (double*)nullptr, 0);
+                         (const double*)partB(c, cur), gv, cur, c.scal.get(), (double*)nullptr,
+                         dir_mode);
     }
   };
 
@@ -842,15 +894,11 @@ int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
   int last_status = kRunning;
   if (!zero_rhs && o.max_iter > 0) {
     const int chunk = c.n > 200000 ? 16 : 64;  // even
-    Graph gr;
-    HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < chunk; ++i) iteration(i & 1);
-    HIP_CHECK(hipStreamEndCapture(c.stream, &gr.g));
-    HIP_CHECK(hipGraphInstantiate(&gr.e, gr.g, nullptr, nullptr, 0));
+    Chunk gr(c, chunk, iteration);
     double prev_true = 1.0;
     int stagnant = 0;
     for (;;) {
-      run_until_stopped(c, gr.e, chunk, o.max_iter);
+      run_until_stopped(c, gr, chunk, o.max_iter);
       s = read_scalars(c);
       last_status = s.status;
       // iteration k-1 (parity (k-1)&1) stopped before its direction update
@@ -859,15 +907,27 @@ int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
       // refinement: FP64 true residual r = b - A x; the residual of iteration k-1 and its
       // partials (r'r, r'z) are replaced by the refreshed ones
       const int cur = (k - 1) & 1;
-      true_residual(c, c.Ap.get());
-      if (fp32)
-        hipLaunchKernelGGL(k_refresh32, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
-                           (const double2*)c.r.get(), (float2*)c.r32.get(),
-                           (const float2*)c.invd32.get(), partB(c, cur));
-      else
-        hipLaunchKernelGGL(k_refresh64, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
-                           (const double2*)c.r.get(), (const double2*)c.inv_diag.get(),
-                           partB(c, cur));
+      true_residual(c, c.work.get());
+      const int correct = refresh == 1 ? 1 : 0;
+      if (fp32) {
+        if (correct)
+          hipLaunchKernelGGL(k_dot_rp<F32>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                             (const double2*)c.r.get(), (const float2*)c.p32.get(), partX(c));
+        hipLaunchKernelGGL(k_refresh<F32>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                           (double2*)c.x.get(), (double2*)c.r.get(), (const float2*)c.p32.get(),
+                           (const float2*)c.Ap32.get(), (const float2*)c.invd32.get(),
+                           (float2*)c.r32.get(), (const double*)partA(c), spmv_grid(c),
+                           (const double*)partX(c), gv, correct, partB(c, cur));
+      } else {
+        if (correct)
+          hipLaunchKernelGGL(k_dot_rp<F64>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                             (const double2*)c.r.get(), (const double2*)c.p.get(), partX(c));
+        hipLaunchKernelGGL(k_refresh<F64>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                           (double2*)c.x.get(), (double2*)c.r.get(), (const double2*)c.p.get(),
+                           (const double2*)c.Ap.get(), (const double2*)c.inv_diag.get(),
+                           (float2*)nullptr, (const double*)partA(c), spmv_grid(c),
+                           (const double*)partX(c), gv, correct, partB(c, cur));
+      }
       HIP_CHECK(hipGetLastError());
       const PcgScalars s2 = read_scalars(c);
       true_rel = std::sqrt(s2.rr) / bnorm;
@@ -942,7 +1002,11 @@ double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
     w.alloc(nv);
     HIP_CHECK(hipMemsetAsync(a.get(), 0, sizeof(double) * nv, c.stream));
     HIP_CHECK(hipMemsetAsync(w.get(), 0, sizeof(double) * nv, c.stream));
-    HIP_CHECK(hipMemsetAsync(part.get(), 0, sizeof(double) * 6 * kMaxGrid, c.stream));
+    // positive partials: p'Ap > 0 and rz > 0, so the update kernel takes its full path
+    const std::vector<double> ones(6 * kMaxGrid, 1.0);
+    HIP_CHECK(hipMemcpyAsync(part.get(), ones.data(), sizeof(double) * ones.size(),
+                             hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
     HIP_CHECK(hipMemcpyAsync(b.get(), c.inv_diag.get(), sizeof(double) * nv, hipMemcpyDeviceToDevice, c.stream));
     if (kind == 5) {
       a32.alloc(nv);

@@ -41,7 +41,7 @@ class SolverOptions(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("record_history", ctypes.c_int), ("instrument", ctypes.c_int),
                 ("method", ctypes.c_int), ("inner_rtol", ctypes.c_double),
-                ("max_outer", ctypes.c_int)]
+                ("max_outer", ctypes.c_int), ("refresh", ctypes.c_int)]
 
 
 class SolverStatsC(ctypes.Structure):
@@ -255,9 +255,9 @@ class Device:
         _check(lib().cstfem_setup(self._h))
 
     def solve(self, rtol=1e-10, max_iter=20000, record_history=False, instrument=False,
-              method=METHOD_CG_FP64, inner_rtol=0.0, max_outer=0, download=True):
+              method=METHOD_CG_FP64, inner_rtol=0.0, max_outer=0, refresh=0, download=True):
         o = SolverOptions(rtol, max_iter, int(record_history), int(instrument), method,
-                          inner_rtol, max_outer)
+                          inner_rtol, max_outer, refresh)
         s = SolverStatsC()
         x = np.empty(self.n, dtype=np.float64) if download else None
         hist = np.zeros(max_iter + 1, dtype=np.float64) if record_history else None
