@@ -60,7 +60,8 @@ typedef struct {
   int method;             /* CSTFEM_METHOD_* */
   double inner_rtol;      /* refined methods: refresh r from the FP64 true residual when the
                              recursive residual drops by this factor since the last refresh
-                             (0 = default: 1e-3 in FP32, 1e-2 in FP64) */
+                             (0 = default: 1e-3 in FP32; in FP64 only when the recursive
+                             residual reaches rtol, where the reference stops) */
   int max_outer;          /* refined methods: max refreshes (0 = default 200) */
   int refresh;            /* refined methods, after r := b - A x:  0 restart (p := D^-1 r, the
                              default), 1 keep the search direction after a corrective line
@@ -143,6 +144,43 @@ int cstfem_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt, cstfem_solve
 int cstfem_download_solution(cstfem_ctx* ctx, double* x, int n);
 /* y = A x with the device matrix (FP64, per-row summation in column order). */
 int cstfem_spmv(cstfem_ctx* ctx, const double* x, double* y, int n);
+
+/* ---- batches of independent plates (no reference counterpart) ------------------------- */
+/* Per-element modulus and Poisson ratio (plates of a batch differ in both); D_e is
+ * constitutive_matrix(E_e, nu_e, plane) (element.cpp:7-15). */
+int cstfem_set_material_fields(cstfem_ctx* ctx, const double* E_elem, const double* nu_elem,
+                               int plane, double thickness);
+/* The uploaded mesh is the concatenation of `count` plates, plate k owning nodes
+ * [node_off[k], node_off[k+1]); no element may join two plates.  Call after
+ * cstfem_build_pattern and before cstfem_assemble (the matrix layout is re-aligned to the
+ * plates). */
+int cstfem_batch_define(cstfem_ctx* ctx, int count, const int* node_off);
+/* Launch shape of the persistent batched solve: wavefronts per workgroup (4, 8, 16) and
+ * resident workgroups per CU (1..4).  Results do not depend on it. */
+typedef struct {
+  int waves_per_group;
+  int groups_per_cu;
+} cstfem_batch_config;
+typedef struct {
+  int iterations;
+  int converged;            /* FP64 true residual below rtol */
+  int refinements;          /* FP64 true-residual refreshes */
+  int status;               /* 1 converged, 2 breakdown, 3 max_iter / no progress */
+  double rel_residual;      /* recursive ||r||/||b|| at exit */
+  double true_rel_residual; /* FP64 ||b - A x|| / ||b|| at the last refresh */
+} cstfem_batch_result;
+/* Solves every plate in one persistent launch (method CSTFEM_METHOD_CG_FP64[_REFINED] or
+ * CSTFEM_METHOD_IR_FP32; every plate is verified by its FP64 true residual).  cfg NULL = the
+ * default shape (8 waves, 1 group per CU).  results[count]; x_out[n] may be NULL; *seconds =
+ * device time of the launch.  Returns 2 if any plate did not converge. */
+int cstfem_batch_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt,
+                       const cstfem_batch_config* cfg, cstfem_batch_result* results,
+                       double* x_out, double* seconds);
+/* Times every launch shape on the current batch, checks that all give identical results,
+ * returns the fastest; cached per device/batch shape/method in cache_path (NULL = no cache). */
+int cstfem_batch_autotune(cstfem_ctx* ctx, const cstfem_solver_options* opt,
+                          const char* cache_path, cstfem_batch_config* best,
+                          double* best_seconds);
 
 /* ---- measurement (no reference counterpart) ------------------------------------------ */
 /* Average seconds of `reps` back-to-back launches of one hot kernel on the library's

@@ -270,31 +270,34 @@ void build_pattern(Context& c) {
 
 namespace {
 
-__global__ void k_slice_width(int nb, int nslices, const int* __restrict__ brow_ptr,
+__global__ void k_slice_width(int nslices, const int* __restrict__ row0,
+                              const int* __restrict__ nrow, const int* __restrict__ brow_ptr,
                               int* __restrict__ width) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslices) return;
   int w = 0;
-  for (int l = 0; l < kSlice; ++l) {
-    const int r = s * kSlice + l;
-    if (r < nb) w = max(w, brow_ptr[r + 1] - brow_ptr[r]);
+  for (int l = 0; l < nrow[s]; ++l) {
+    const int r = row0[s] + l;
+    w = max(w, brow_ptr[r + 1] - brow_ptr[r]);
   }
   width[s] = w;
 }
 
 // one wavefront per slice; lane = row within the slice
-__global__ void k_sell_fill(int nb, int nslices, const int* __restrict__ brow_ptr,
-                            const int* __restrict__ bcol, const int64_t* __restrict__ off,
-                            int* __restrict__ scol, int64_t* __restrict__ spos) {
+__global__ void k_sell_fill(int nslices, const int* __restrict__ row0, const int* __restrict__ nrow,
+                            const int* __restrict__ brow_ptr, const int* __restrict__ bcol,
+                            const int64_t* __restrict__ off, int* __restrict__ scol,
+                            int64_t* __restrict__ spos) {
   const int lane = threadIdx.x & (kSlice - 1);
   const int s = blockIdx.x * (blockDim.x / kSlice) + threadIdx.x / kSlice;
   if (s >= nslices) return;
-  const int r = s * kSlice + lane;
+  const bool live = lane < nrow[s];
+  const int r = row0[s] + lane;
   const int64_t o = off[s];
   const int w = (int)((off[s + 1] - o) / kSlice);
-  const int b = r < nb ? brow_ptr[r] : 0;
-  const int d = r < nb ? brow_ptr[r + 1] - b : 0;
-  const int self = r < nb ? r : 0;
+  const int b = live ? brow_ptr[r] : 0;
+  const int d = live ? brow_ptr[r + 1] - b : 0;
+  const int self = live ? r : row0[s];
   for (int k = 0; k < w; ++k) {
     const int64_t slot = o + (int64_t)k * kSlice + lane;
     if (k < d) {
@@ -308,13 +311,33 @@ __global__ void k_sell_fill(int nb, int nslices, const int* __restrict__ brow_pt
 
 }  // namespace
 
+// Slices of at most 64 block rows that never cross a segment boundary (c.h_segments: ascending
+// block-row boundaries 0 = s0 < s1 < ... = nb; empty = one segment).  Rows of slice s are
+// row0[s] .. row0[s] + nrow[s] - 1.
 void build_sell(Context& c) {
   const int nb = c.nb;
-  const int nslices = (nb + kSlice - 1) / kSlice;
+  std::vector<int> seg = c.h_segments;
+  if (seg.empty()) seg = {0, nb};
+  std::vector<int> row0, nrow;
+  c.h_seg_slice.assign(1, 0);
+  for (size_t k = 0; k + 1 < seg.size(); ++k) {
+    for (int r = seg[k]; r < seg[k + 1]; r += kSlice) {
+      row0.push_back(r);
+      nrow.push_back(std::min(kSlice, seg[k + 1] - r));
+    }
+    c.h_seg_slice.push_back(static_cast<int>(row0.size()));
+  }
+  const int nslices = static_cast<int>(row0.size());
+  c.sell_row0.alloc(nslices + 1);
+  c.sell_nrow.alloc(nslices + 1);
+  HIP_CHECK(hipMemcpyAsync(c.sell_row0.get(), row0.data(), sizeof(int) * nslices,
+                           hipMemcpyHostToDevice, c.stream));
+  HIP_CHECK(hipMemcpyAsync(c.sell_nrow.get(), nrow.data(), sizeof(int) * nslices,
+                           hipMemcpyHostToDevice, c.stream));
   DBuf<int> width;
   width.alloc(nslices);
-  hipLaunchKernelGGL(k_slice_width, dim3(grid_for(nslices, 256, 0)), dim3(256), 0, c.stream, nb,
-                     nslices, c.brow_ptr.get(), width.get());
+  hipLaunchKernelGGL(k_slice_width, dim3(grid_for(nslices, 256, 0)), dim3(256), 0, c.stream,
+                     nslices, c.sell_row0.get(), c.sell_nrow.get(), c.brow_ptr.get(), width.get());
   HIP_CHECK(hipGetLastError());
   std::vector<int> w(nslices);
   HIP_CHECK(hipMemcpyAsync(w.data(), width.get(), sizeof(int) * nslices, hipMemcpyDeviceToHost,
@@ -333,9 +356,10 @@ void build_sell(Context& c) {
   HIP_CHECK(hipMemsetAsync(c.sell_val.get(), 0, sizeof(double) * 4 * c.sell_blocks, c.stream));
   const int wpb = 4;  // wavefronts per block
   hipLaunchKernelGGL(k_sell_fill, dim3(grid_for(nslices, wpb, 0)), dim3(wpb * kSlice), 0,
-                     c.stream, nb, nslices, c.brow_ptr.get(), c.bcol.get(), c.sell_off.get(),
-                     c.sell_col.get(), c.sell_pos.get());
+                     c.stream, nslices, c.sell_row0.get(), c.sell_nrow.get(), c.brow_ptr.get(),
+                     c.bcol.get(), c.sell_off.get(), c.sell_col.get(), c.sell_pos.get());
   HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c.stream));
   c.vals32_valid = false;
 }
 
@@ -465,16 +489,17 @@ __device__ inline void ke_block(const double xy[6], const double D[9], double t,
 }
 
 // one wavefront per SELL slice, lane = block row; writes are coalesced 32-byte blocks
-__global__ void k_assemble(int nb, int nslices, const int64_t* __restrict__ off,
+__global__ void k_assemble(const int* __restrict__ row0, const int* __restrict__ nrow,
+                           int nslices, const int64_t* __restrict__ off,
                            const int* __restrict__ brow_ptr, const int* __restrict__ slot_ptr,
                            const int* __restrict__ slot_src, const int* __restrict__ tri,
                            const double* __restrict__ xy, const double* __restrict__ E_elem,
-                           Mat3 Dm, double nu, int strain, double t, double* __restrict__ val) {
+                           const double* __restrict__ nu_elem, Mat3 Dm, double nu, int strain, double t, double* __restrict__ val) {
   const int lane = threadIdx.x & (kSlice - 1);
   const int s = blockIdx.x * (blockDim.x / kSlice) + threadIdx.x / kSlice;
   if (s >= nslices) return;
-  const int r = s * kSlice + lane;
-  if (r >= nb) return;
+  if (lane >= nrow[s]) return;
+  const int r = row0[s] + lane;
   const int64_t o = off[s];
   const int b0 = brow_ptr[r], d = brow_ptr[r + 1] - b0;
   const double* Du = Dm.d;
@@ -490,7 +515,7 @@ __global__ void k_assemble(int nb, int nslices, const int64_t* __restrict__ off,
                              xy[2 * n1 + 1], xy[2 * n2], xy[2 * n2 + 1]};
       double De[9];
       if (E_elem) {
-        constitutive(E_elem[e], nu, strain, De);
+        constitutive(E_elem[e], nu_elem ? nu_elem[e] : nu, strain, De);
       } else {
 #pragma unroll
         for (int i = 0; i < 9; ++i) De[i] = Du[i];
@@ -533,9 +558,10 @@ void assemble_values(Context& c) {
   const int wpb = 4;
   HIP_CHECK(hipMemsetAsync(c.sell_val.get(), 0, sizeof(double) * 4 * c.sell_blocks, c.stream));
   hipLaunchKernelGGL(k_assemble, dim3(grid_for(c.nslices, wpb, 0)), dim3(wpb * kSlice), 0,
-                     c.stream, c.nb, c.nslices, c.sell_off.get(), c.brow_ptr.get(),
-                     c.slot_ptr.get(), c.slot_src.get(), c.tri.get(), c.xy.get(),
-                     c.has_E_elem ? c.E_elem.get() : (const double*)nullptr, c.D, c.nu,
+                     c.stream, c.sell_row0.get(), c.sell_nrow.get(), c.nslices, c.sell_off.get(),
+                     c.brow_ptr.get(), c.slot_ptr.get(), c.slot_src.get(), c.tri.get(), c.xy.get(),
+                     c.has_E_elem ? c.E_elem.get() : (const double*)nullptr,
+                     c.has_nu_elem ? c.nu_elem.get() : (const double*)nullptr, c.D, c.nu,
                      c.plane_strain, c.thick, c.sell_val.get());
   HIP_CHECK(hipGetLastError());
   c.setup_done = false;

@@ -1,0 +1,553 @@
+// batched.hip -- many independent plates solved by one persistent Jacobi-CG launch.
+//
+// The plates are one block-diagonal system: their meshes are concatenated (node ranges
+// [node_off[k], node_off[k+1])), so the pattern, the slot map, assembly, loads and Dirichlet
+// elimination are the single-system kernels unchanged; only the SELL slices are aligned to the
+// plate boundaries (build_sell with segments).  The solve is one launch of a persistent
+// kernel: each workgroup takes the next plate from a device work queue (plates ordered by
+// decreasing estimated cost n^1.5) and runs that plate's whole CG with workgroup barriers
+// only, so no workgroup ever waits for another and every workgroup exits when the queue is
+// empty.  Per-iteration reductions are per-slice partial sums (a fixed 64-lane butterfly)
+// combined by a fixed pairwise tree over the slice index: the result does not depend on the
+// workgroup size or the number of resident workgroups, which is what lets the autotuner pick
+// those for speed alone.
+//
+// Every plate runs the reference's Jacobi-PCG recurrence (solver.cpp:35-105) and is verified
+// by an FP64 true residual b - A x; when the recursive residual reaches rtol (or drops by
+// delta since the last refresh) but the true one has not, the plate restarts from the true
+// residual.  FP32 mode keeps values and CG vectors in FP32 (dots in FP64, x in FP64).
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+
+#include "internal.hpp"
+
+namespace cstfem_hip {
+
+namespace {
+
+constexpr int kMaxSlicesPerPlate = 1024;  // 65,536 nodes per plate
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct BatchArgs {
+  const int64_t* off;
+  const int* col;
+  const double* val64;
+  const float* val32;
+  const int* row0;
+  const int* nrow;
+  const double2* b;
+  const double2* invd;
+  const float2* invd32;
+  double2* x;
+  double2* r;    // FP64 residual (FP64 mode) / FP64 true residual (FP32 mode)
+  double2* p;
+  double2* Ap;
+  float2* r32;
+  float2* p32;
+  float2* Ap32;
+  const int* order;
+  const int* seg_slice;
+  int count;
+  int* queue;
+  cstfem_batch_result* out;
+  double rtol, delta;
+  int max_iter, max_refresh;
+};
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Fixed pairwise tree over a[0..n): the same additions in the same order for any workgroup
+// size.  Every thread returns the total.
+__device__ inline double tree_sum(double* a, int n) {
+  __syncthreads();
+  for (int stride = 1; stride < n; stride <<= 1) {
+    for (int i = threadIdx.x * 2 * stride; i < n; i += blockDim.x * 2 * stride)
+      if (i + stride < n) a[i] += a[i + stride];
+    __syncthreads();
+  }
+  const double t = a[0];
+  __syncthreads();
+  return t;
+}
+
+__device__ inline void row_spmv(const BatchArgs& A, const double2* __restrict__ x, int s, int lane,
+                                double& y0, double& y1) {
+  const int64_t o = A.off[s];
+  const int w = (int)((A.off[s + 1] - o) >> 6);
+  const int* c = A.col + o + lane;
+  const v2d* v = reinterpret_cast<const v2d*>(A.val64) + 2 * (o + lane);
+  double a0 = 0.0, a1 = 0.0;
+  for (int k = 0; k < w; ++k) {
+    const int m = c[(size_t)k * 64];
+    const v2d va = v[(size_t)k * 128];
+    const v2d vb = v[(size_t)k * 128 + 1];
+    const double2 xm = x[m];
+    a0 = a0 + va.x * xm.x;
+    a0 = a0 + va.y * xm.y;
+    a1 = a1 + vb.x * xm.x;
+    a1 = a1 + vb.y * xm.y;
+  }
+  y0 = a0;
+  y1 = a1;
+}
+
+__device__ inline void row_spmv(const BatchArgs& A, const float2* __restrict__ x, int s, int lane,
+                                float& y0, float& y1) {
+  const int64_t o = A.off[s];
+  const int w = (int)((A.off[s + 1] - o) >> 6);
+  const int* c = A.col + o + lane;
+  const v4f* v = reinterpret_cast<const v4f*>(A.val32) + (o + lane);
+  float a0 = 0.f, a1 = 0.f;
+  for (int k = 0; k < w; ++k) {
+    const int m = c[(size_t)k * 64];
+    const v4f vv = v[(size_t)k * 64];
+    const float2 xm = x[m];
+    a0 = a0 + vv.x * xm.x;
+    a0 = a0 + vv.y * xm.y;
+    a1 = a1 + vv.z * xm.x;
+    a1 = a1 + vv.w * xm.y;
+  }
+  y0 = a0;
+  y1 = a1;
+}
+
+template <bool kF32>
+struct Vec;
+template <>
+struct Vec<false> {
+  using V = double2;
+  using S = double;
+};
+template <>
+struct Vec<true> {
+  using V = float2;
+  using S = float;
+};
+
+// One plate, whole solve, one workgroup.  Slices s0..s1-1; wave w owns slices s0+w, s0+w+W, ...
+// in every phase, so a row's values are produced and consumed by the same lane except across
+// the barriers that separate the SpMV (which reads neighbours) from the updates.
+template <bool kF32>
+__device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* part1,
+                            double* part2) {
+  using V = typename Vec<kF32>::V;
+  using S = typename Vec<kF32>::S;
+  V* r = kF32 ? reinterpret_cast<V*>(A.r32) : reinterpret_cast<V*>(A.r);
+  V* p = kF32 ? reinterpret_cast<V*>(A.p32) : reinterpret_cast<V*>(A.p);
+  V* Ap = kF32 ? reinterpret_cast<V*>(A.Ap32) : reinterpret_cast<V*>(A.Ap);
+  const V* invd = kF32 ? reinterpret_cast<const V*>(A.invd32) : reinterpret_cast<const V*>(A.invd);
+  const int s0 = A.seg_slice[k], s1 = A.seg_slice[k + 1], ns = s1 - s0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+
+  // x = 0, r = b, p = D^-1 r; b'b (FP64) and r'z
+  for (int s = s0 + wave; s < s1; s += W) {
+    double bb = 0.0, rz = 0.0;
+    if (lane < A.nrow[s]) {
+      const int row = A.row0[s] + lane;
+      const double2 bi = A.b[row];
+      const V di = invd[row];
+      V ri;
+      ri.x = (S)bi.x;
+      ri.y = (S)bi.y;
+      A.x[row] = make_double2(0.0, 0.0);
+      r[row] = ri;
+      const S z0 = di.x * ri.x, z1 = di.y * ri.y;
+      V pi;
+      pi.x = z0;
+      pi.y = z1;
+      p[row] = pi;
+      bb = bi.x * bi.x + bi.y * bi.y;
+      rz = (double)ri.x * (double)z0 + (double)ri.y * (double)z1;
+    }
+    bb = wave_sum(bb);
+    rz = wave_sum(rz);
+    if (lane == 0) {
+      part1[s - s0] = bb;
+      part2[s - s0] = rz;
+    }
+  }
+  const double bnorm = sqrt(tree_sum(part1, ns));
+  double rz = tree_sum(part2, ns);
+  int it = 0, refreshes = 0, status = kRunning;
+  double rel = 1.0, true_rel = 0.0, rel_ref = 1.0, prev_true = 1.0;
+  int stagnant = 0;
+  if (bnorm == 0.0) {
+    status = kConverged;
+    rel = 0.0;
+  }
+  while (status == kRunning) {
+    // Ap = A p, p'Ap
+    for (int s = s0 + wave; s < s1; s += W) {
+      S y0, y1;
+      row_spmv(A, (const V*)p, s, lane, y0, y1);
+      double t = 0.0;
+      if (lane < A.nrow[s]) {
+        const int row = A.row0[s] + lane;
+        V a;
+        a.x = y0;
+        a.y = y1;
+        Ap[row] = a;
+        const V pr = p[row];
+        t = (double)pr.x * (double)y0 + (double)pr.y * (double)y1;
+      }
+      t = wave_sum(t);
+      if (lane == 0) part0[s - s0] = t;
+    }
+    const double pAp = tree_sum(part0, ns);
+    if (!(pAp > 0.0)) {
+      status = kBreakdown;
+      break;
+    }
+    const double alpha = rz / pAp;
+    // x += alpha p, r -= alpha Ap; r'r, r'z
+    for (int s = s0 + wave; s < s1; s += W) {
+      double rr_s = 0.0, rz_s = 0.0;
+      if (lane < A.nrow[s]) {
+        const int row = A.row0[s] + lane;
+        const V pi = p[row], api = Ap[row], di = invd[row];
+        double2 xi = A.x[row];
+        xi.x = xi.x + alpha * (double)pi.x;
+        xi.y = xi.y + alpha * (double)pi.y;
+        A.x[row] = xi;
+        V ri = r[row];
+        if (kF32) {
+          ri.x = (S)((double)ri.x - alpha * (double)api.x);
+          ri.y = (S)((double)ri.y - alpha * (double)api.y);
+        } else {
+          ri.x = ri.x - (S)alpha * api.x;
+          ri.y = ri.y - (S)alpha * api.y;
+        }
+        r[row] = ri;
+        const double r0 = ri.x, r1 = ri.y;
+        rr_s = r0 * r0 + r1 * r1;
+        rz_s = r0 * ((double)di.x * r0) + r1 * ((double)di.y * r1);
+      }
+      rr_s = wave_sum(rr_s);
+      rz_s = wave_sum(rz_s);
+      if (lane == 0) {
+        part1[s - s0] = rr_s;
+        part2[s - s0] = rz_s;
+      }
+    }
+    const double rr = tree_sum(part1, ns);
+    double rz_new = tree_sum(part2, ns);
+    ++it;
+    rel = sqrt(rr) / bnorm;
+    bool restart = false;
+    if (rel < A.rtol || rel < A.delta * rel_ref) {
+      // refresh: FP64 true residual r64 = b - A x (x of every row is final after the barrier)
+      __syncthreads();
+      for (int s = s0 + wave; s < s1; s += W) {
+        double y0, y1;
+        row_spmv(A, (const double2*)A.x, s, lane, y0, y1);
+        double t_rr = 0.0, t_rz = 0.0;
+        if (lane < A.nrow[s]) {
+          const int row = A.row0[s] + lane;
+          const double2 bi = A.b[row];
+          const double2 ri = make_double2(bi.x - y0, bi.y - y1);
+          const V di = invd[row];
+          V rw;
+          rw.x = (S)ri.x;
+          rw.y = (S)ri.y;
+          r[row] = rw;  // the working residual restarts from the true one
+          t_rr = ri.x * ri.x + ri.y * ri.y;
+          const double w0 = rw.x, w1 = rw.y;
+          t_rz = w0 * ((double)di.x * w0) + w1 * ((double)di.y * w1);
+        }
+        t_rr = wave_sum(t_rr);
+        t_rz = wave_sum(t_rz);
+        if (lane == 0) {
+          part1[s - s0] = t_rr;
+          part2[s - s0] = t_rz;
+        }
+      }
+      true_rel = sqrt(tree_sum(part1, ns)) / bnorm;
+      rz_new = tree_sum(part2, ns);
+      ++refreshes;
+      if (true_rel < A.rtol) {
+        status = kConverged;
+        break;
+      }
+      stagnant = true_rel > 0.5 * prev_true ? stagnant + 1 : 0;
+      prev_true = true_rel;
+      if (refreshes >= A.max_refresh || stagnant >= 4) {
+        status = kMaxIter;
+        break;
+      }
+      rel_ref = true_rel;
+      restart = true;
+    }
+    if (it >= A.max_iter) {
+      status = kMaxIter;
+      break;
+    }
+    const double beta = restart ? 0.0 : rz_new / rz;
+    rz = rz_new;
+    for (int s = s0 + wave; s < s1; s += W) {
+      if (lane < A.nrow[s]) {
+        const int row = A.row0[s] + lane;
+        const V ri = r[row], di = invd[row];
+        V pi = p[row];
+        const S z0 = di.x * ri.x, z1 = di.y * ri.y;
+        pi.x = z0 + (S)beta * pi.x;
+        pi.y = z1 + (S)beta * pi.y;
+        p[row] = pi;
+      }
+    }
+    __syncthreads();  // p complete before the next SpMV reads neighbours
+  }
+  if (threadIdx.x == 0) {
+    cstfem_batch_result res{};
+    res.iterations = it;
+    res.converged = status == kConverged ? 1 : 0;
+    res.refinements = refreshes;
+    res.status = status;
+    res.rel_residual = rel;
+    res.true_rel_residual = true_rel;
+    A.out[k] = res;
+  }
+  __syncthreads();
+}
+
+template <bool kF32>
+__global__ void k_batch_cg(BatchArgs A) {
+  __shared__ double part0[kMaxSlicesPerPlate], part1[kMaxSlicesPerPlate],
+      part2[kMaxSlicesPerPlate];
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(A.queue, 1);
+    __syncthreads();
+    const int q = next;
+    __syncthreads();
+    if (q >= A.count) return;  // the queue only grows: every workgroup reaches this exit
+    solve_plate<kF32>(A, A.order[q], part0, part1, part2);
+  }
+}
+
+__global__ void k_reset_queue(int* q) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
+}
+
+__global__ void k_round(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+}  // namespace
+
+void batch_define(Context& c, int count, const int* node_off) {
+  if (!c.have_pattern || !c.mesh_pattern)
+    throw Error(CSTFEM_ERR_ORDER, "cstfem_batch_define: build the pattern of the batch mesh first");
+  if (count <= 0 || node_off[0] != 0 || node_off[count] != c.nn)
+    throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_define: node offsets must run from 0 to the node count");
+  for (int k = 0; k < count; ++k) {
+    if (node_off[k + 1] <= node_off[k])
+      throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_define: empty or decreasing plate " + std::to_string(k));
+    if (node_off[k + 1] - node_off[k] > kMaxSlicesPerPlate * kSlice)
+      throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_define: plate " + std::to_string(k) +
+                                           " has more than " +
+                                           std::to_string(kMaxSlicesPerPlate * kSlice) + " nodes");
+  }
+  // plates must not couple: every block of a row of plate k has its column in plate k
+  std::vector<int> bp(c.nb + 1), bc(c.nblk);
+  HIP_CHECK(hipMemcpyAsync(bp.data(), c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipMemcpyAsync(bc.data(), c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  for (int k = 0; k < count; ++k)
+    for (int r = node_off[k]; r < node_off[k + 1]; ++r)
+      for (int q = bp[r]; q < bp[r + 1]; ++q)
+        if (bc[q] < node_off[k] || bc[q] >= node_off[k + 1])
+          throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_define: plate " + std::to_string(k) +
+                                               " shares an element with another plate");
+  c.h_segments.assign(node_off, node_off + count + 1);
+  build_sell(c);  // slices aligned to the plates; values are re-assembled by the caller
+  c.batch_count = count;
+  c.batch_seg_slice.alloc(count + 1);
+  HIP_CHECK(hipMemcpyAsync(c.batch_seg_slice.get(), c.h_seg_slice.data(), sizeof(int) * (count + 1),
+                           hipMemcpyHostToDevice, c.stream));
+  // work order: decreasing n^1.5 (CG iterations grow like sqrt(n))
+  std::vector<int> order(count);
+  for (int k = 0; k < count; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return node_off[a + 1] - node_off[a] > node_off[b + 1] - node_off[b];
+  });
+  c.batch_order.alloc(count);
+  HIP_CHECK(hipMemcpyAsync(c.batch_order.get(), order.data(), sizeof(int) * count, hipMemcpyHostToDevice, c.stream));
+  c.batch_results.alloc(count);
+  c.batch_queue.alloc(1);
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  c.setup_done = false;
+}
+
+namespace {
+
+BatchArgs batch_args(Context& c, const cstfem_solver_options& o) {
+  BatchArgs a{};
+  a.off = c.sell_off.get();
+  a.col = c.sell_col.get();
+  a.val64 = c.sell_val.get();
+  a.val32 = c.sell_val32.get();
+  a.row0 = c.sell_row0.get();
+  a.nrow = c.sell_nrow.get();
+  a.b = (const double2*)c.b.get();
+  a.invd = (const double2*)c.inv_diag.get();
+  a.invd32 = (const float2*)c.invd32.get();
+  a.x = (double2*)c.x.get();
+  a.r = (double2*)c.r.get();
+  a.p = (double2*)c.p.get();
+  a.Ap = (double2*)c.Ap.get();
+  a.r32 = (float2*)c.r32.get();
+  a.p32 = (float2*)c.p32.get();
+  a.Ap32 = (float2*)c.Ap32.get();
+  a.order = c.batch_order.get();
+  a.seg_slice = c.batch_seg_slice.get();
+  a.count = c.batch_count;
+  a.queue = c.batch_queue.get();
+  a.out = c.batch_results.get();
+  a.rtol = o.rtol;
+  const bool fp32 = o.method == CSTFEM_METHOD_IR_FP32;
+  a.delta = o.inner_rtol > 0 ? o.inner_rtol : (fp32 ? 1e-3 : 0.0);
+  a.max_iter = o.max_iter;
+  a.max_refresh = o.max_outer > 0 ? o.max_outer : 200;
+  return a;
+}
+
+}  // namespace
+
+double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batch_config& cfg,
+                   cstfem_batch_result* results) {
+  if (c.batch_count <= 0) throw Error(CSTFEM_ERR_ORDER, "cstfem_batch_solve: define the batch first");
+  if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: solve() called before setup()");
+  if (o.method != CSTFEM_METHOD_CG_FP64_REFINED && o.method != CSTFEM_METHOD_CG_FP64 &&
+      o.method != CSTFEM_METHOD_IR_FP32)
+    throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_solve: method must be FP64 CG or FP32 refinement");
+  const int waves = cfg.waves_per_group;
+  if (waves != 4 && waves != 8 && waves != 16)
+    throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_solve: waves_per_group must be 4, 8 or 16");
+  if (cfg.groups_per_cu < 1 || cfg.groups_per_cu > 4)
+    throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_solve: groups_per_cu must be 1..4");
+  const bool fp32 = o.method == CSTFEM_METHOD_IR_FP32;
+  const size_t nv = 2 * (size_t)c.nb;
+  if (fp32) {
+    c.r32.alloc(nv);
+    c.p32.alloc(nv);
+    c.Ap32.alloc(nv);
+    if (!c.vals32_valid) {
+      c.sell_val32.alloc(4 * c.sell_blocks);
+      hipLaunchKernelGGL(k_round, dim3(2048), dim3(256), 0, c.stream, (int64_t)(4 * c.sell_blocks),
+                         (const double*)c.sell_val.get(), c.sell_val32.get());
+      HIP_CHECK(hipGetLastError());
+      c.vals32_valid = true;
+    }
+  }
+  const BatchArgs args = batch_args(c, o);
+  int grid = cfg.groups_per_cu * (c.cu_count > 0 ? c.cu_count : 256);
+  if (grid > c.batch_count) grid = c.batch_count;
+  // one captured graph: queue reset (a kernel, not a memset node) + the persistent solve
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, c.stream, c.batch_queue.get());
+  if (fp32)
+    hipLaunchKernelGGL(k_batch_cg<true>, dim3(grid), dim3(64 * waves), 0, c.stream, args);
+  else
+    hipLaunchKernelGGL(k_batch_cg<false>, dim3(grid), dim3(64 * waves), 0, c.stream, args);
+  HIP_CHECK(hipStreamEndCapture(c.stream, &g));
+  HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  HIP_CHECK(hipEventRecord(e0, c.stream));
+  HIP_CHECK(hipGraphLaunch(ge, c.stream));
+  HIP_CHECK(hipEventRecord(e1, c.stream));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipGraphExecDestroy(ge);
+  (void)hipGraphDestroy(g);
+  if (results)
+    HIP_CHECK(hipMemcpy(results, c.batch_results.get(), sizeof(cstfem_batch_result) * c.batch_count,
+                        hipMemcpyDeviceToHost));
+  return ms * 1e-3;
+}
+
+// Autotuner: times every candidate launch shape on the current batch and keeps the fastest;
+// results are identical across shapes by construction (and checked here), only speed differs.
+// The choice is cached in `cache_path` (one line per device, batch shape and precision).
+cstfem_batch_config batch_autotune(Context& c, const cstfem_solver_options& o,
+                                   const char* cache_path, double* best_seconds) {
+  char name[256] = {0};
+  hipDeviceProp_t prop;
+  HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
+  std::snprintf(name, sizeof name, "%s", prop.gcnArchName);
+  std::ostringstream key;
+  key << name << " cus=" << c.cu_count << " plates=" << c.batch_count << " rows=" << c.nb
+      << " slices=" << c.nslices << " method=" << o.method;
+  if (cache_path && *cache_path) {
+    std::ifstream in(cache_path);
+    std::string line;
+    while (std::getline(in, line)) {
+      const auto bar = line.find('|');
+      if (bar == std::string::npos || line.substr(0, bar) != key.str()) continue;
+      std::istringstream vals(line.substr(bar + 1));
+      cstfem_batch_config cfg{};
+      double t = 0.0;
+      if (vals >> cfg.waves_per_group >> cfg.groups_per_cu >> t) {
+        if (best_seconds) *best_seconds = t;
+        return cfg;
+      }
+    }
+  }
+  const int waves[] = {4, 8, 16};
+  const int groups[] = {1, 2, 4};
+  cstfem_batch_config best{8, 1};
+  double best_t = 1e300;
+  std::vector<cstfem_batch_result> ref(c.batch_count), cur(c.batch_count);
+  std::vector<double> x_ref(c.n), x_cur(c.n);
+  bool first = true;
+  for (int w : waves)
+    for (int gpc : groups) {
+      const cstfem_batch_config cfg{w, gpc};
+      const double t = batch_solve(c, o, cfg, cur.data());
+      HIP_CHECK(hipMemcpy(x_cur.data(), c.x.get(), sizeof(double) * c.n, hipMemcpyDeviceToHost));
+      if (first) {
+        ref = cur;
+        x_ref = x_cur;
+        first = false;
+      } else {
+        for (int k = 0; k < c.batch_count; ++k)
+          if (cur[k].iterations != ref[k].iterations)
+            throw Error(CSTFEM_ERR_DEVICE, "batch autotune: launch shape changed plate " +
+                                               std::to_string(k) + "'s iteration count");
+        if (x_cur != x_ref)
+          throw Error(CSTFEM_ERR_DEVICE, "batch autotune: launch shape changed the solution");
+      }
+      if (t < best_t) {
+        best_t = t;
+        best = cfg;
+      }
+    }
+  if (cache_path && *cache_path) {
+    std::ofstream out(cache_path, std::ios::app);
+    out << key.str() << "|" << best.waves_per_group << " " << best.groups_per_cu << " " << best_t
+        << "\n";
+  }
+  if (best_seconds) *best_seconds = best_t;
+  return best;
+}
+
+}  // namespace cstfem_hip

@@ -172,7 +172,11 @@ int cstfem_upload_mesh(cstfem_ctx* ctx, int nn, const double* xy, int64_t ne, co
     HIP_CHECK(hipMemcpyAsync(c.tri.get(), tri, sizeof(int) * 3 * ne, hipMemcpyHostToDevice, c.stream));
     HIP_CHECK(hipStreamSynchronize(c.stream));
     c.have_pattern = c.mesh_pattern = c.setup_done = c.vals32_valid = false;
-    c.has_E_elem = false;
+    c.has_E_elem = c.has_nu_elem = false;
+    c.h_segments.clear();
+    c.h_ne_ptr.clear();
+    c.h_ne_list.clear();
+    c.batch_count = 0;
     c.ne_ptr.release();
     c.ne_list.release();
     c.b.release();
@@ -198,6 +202,7 @@ int cstfem_set_material(cstfem_ctx* ctx, double E, double nu, int plane, double 
     c.plane_strain = plane;
     c.D = constitutive_host(E, nu, plane);
     c.has_E_elem = E_elem != nullptr;
+    c.has_nu_elem = false;
     if (E_elem) {
       for (int64_t e = 0; e < c.ne; ++e)
         need(E_elem[e] > 0, "element " + std::to_string(e) + ": E must be positive");
@@ -219,7 +224,7 @@ int cstfem_set_constitutive(cstfem_ctx* ctx, const double* D, double thickness) 
       c.D.d[i] = D[i];
     }
     c.thick = thickness;
-    c.has_E_elem = false;
+    c.has_E_elem = c.has_nu_elem = false;
     return CSTFEM_OK;
   });
 }
@@ -385,7 +390,18 @@ int cstfem_upload_csr(cstfem_ctx* ctx, int n, const int* row_ptr, const int* col
       HIP_CHECK(hipMemcpyAsync(c.sell_val.get(), sval.data(), sizeof(double) * 4 * sb, hipMemcpyHostToDevice, c.stream));
     }
     HIP_CHECK(hipMemcpyAsync(c.sell_pos.get(), spos.data(), sizeof(int64_t) * spos.size(), hipMemcpyHostToDevice, c.stream));
+    std::vector<int> row0(nslices), nrow(nslices);
+    for (int s = 0; s < nslices; ++s) {
+      row0[s] = s * kSlice;
+      nrow[s] = std::min(kSlice, nb - s * kSlice);
+    }
+    c.sell_row0.alloc(nslices + 1);
+    c.sell_nrow.alloc(nslices + 1);
+    HIP_CHECK(hipMemcpyAsync(c.sell_row0.get(), row0.data(), sizeof(int) * nslices, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(c.sell_nrow.get(), nrow.data(), sizeof(int) * nslices, hipMemcpyHostToDevice, c.stream));
     HIP_CHECK(hipStreamSynchronize(c.stream));
+    c.h_segments.clear();
+    c.h_seg_slice = {0, nslices};
     c.have_pattern = true;
     c.mesh_pattern = false;
     c.generic = true;
@@ -448,22 +464,35 @@ int cstfem_add_traction(cstfem_ctx* ctx, int nedge, const int* edges, int mode, 
     ensure_vectors(c);
     const std::vector<double>& xy = c.h_xy;
     const std::vector<int>& tri = c.h_tri;
+    // triangles adjacent to each group edge (bc.cpp:194-211): the triangles holding both
+    // nodes, found through a host node -> elements map built once per mesh; the opposite node
+    // is taken from the last such triangle in element order, as the reference's scan does
+    if (c.h_ne_ptr.empty()) {
+      c.h_ne_ptr.assign(c.nn + 1, 0);
+      for (int64_t k = 0; k < 3 * c.ne; ++k) ++c.h_ne_ptr[tri[k] + 1];
+      for (int i = 0; i < c.nn; ++i) c.h_ne_ptr[i + 1] += c.h_ne_ptr[i];
+      c.h_ne_list.assign(3 * c.ne, 0);
+      std::vector<int> fill(c.h_ne_ptr.begin(), c.h_ne_ptr.end() - 1);
+      for (int64_t e = 0; e < c.ne; ++e)
+        for (int a = 0; a < 3; ++a) c.h_ne_list[fill[tri[3 * e + a]]++] = (int)e;
+    }
     std::unordered_map<std::uint64_t, int> opposite, count;
     for (int k = 0; k < nedge; ++k) {
-      need(edges[2 * k] >= 0 && edges[2 * k] < c.nn && edges[2 * k + 1] >= 0 && edges[2 * k + 1] < c.nn,
-           "cstfem_add_traction: edge node out of range");
-      opposite.emplace(edge_key(edges[2 * k], edges[2 * k + 1]), -1);
-      count.emplace(edge_key(edges[2 * k], edges[2 * k + 1]), 0);
-    }
-    for (int64_t t = 0; t < c.ne; ++t) {
-      const int* nd = &tri[3 * t];
-      for (int k = 0; k < 3; ++k) {
-        const auto it = opposite.find(edge_key(nd[k], nd[(k + 1) % 3]));
-        if (it != opposite.end()) {
-          it->second = nd[(k + 2) % 3];
-          ++count[it->first];
-        }
+      const int a = edges[2 * k], b = edges[2 * k + 1];
+      need(a >= 0 && a < c.nn && b >= 0 && b < c.nn, "cstfem_add_traction: edge node out of range");
+      const auto key = edge_key(a, b);
+      if (count.count(key)) continue;
+      int cnt = 0, opp = -1;
+      for (int q = c.h_ne_ptr[a]; q < c.h_ne_ptr[a + 1]; ++q) {
+        const int* nd = &tri[3 * (int64_t)c.h_ne_list[q]];
+        if (nd[0] != b && nd[1] != b && nd[2] != b) continue;
+        if (a == b) continue;
+        ++cnt;
+        for (int i = 0; i < 3; ++i)
+          if (nd[i] != a && nd[i] != b) opp = nd[i];
       }
+      opposite[key] = opp;
+      count[key] = cnt;
     }
     // touched DOFs in first-touch order; accumulation continues from the device values
     std::vector<int> dofs;
@@ -604,6 +633,76 @@ int cstfem_spmv(cstfem_ctx* ctx, const double* x, double* y, int n) {
     spmv_fp64(c, dx.get(), dy.get());
     HIP_CHECK(hipMemcpyAsync(y, dy.get(), sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
     HIP_CHECK(hipStreamSynchronize(c.stream));
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_set_material_fields(cstfem_ctx* ctx, const double* E_elem, const double* nu_elem,
+                               int plane, double thickness) {
+  return guarded([&] {
+    need(ctx && E_elem && nu_elem, "cstfem_set_material_fields: NULL argument");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.nn > 0, "cstfem_set_material_fields: upload a mesh first");
+    need(thickness > 0, "thickness must be positive");
+    need(plane == CSTFEM_PLANE_STRESS || plane == CSTFEM_PLANE_STRAIN,
+         "plane must be plane_stress or plane_strain");
+    for (int64_t e = 0; e < c.ne; ++e) {
+      need(E_elem[e] > 0, "element " + std::to_string(e) + ": E must be positive");
+      need(nu_elem[e] > -1.0 && nu_elem[e] < 0.5,
+           "element " + std::to_string(e) + ": nu must be in (-1, 0.5)");
+    }
+    c.E_elem.alloc(c.ne);
+    c.nu_elem.alloc(c.ne);
+    HIP_CHECK(hipMemcpyAsync(c.E_elem.get(), E_elem, sizeof(double) * c.ne, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(c.nu_elem.get(), nu_elem, sizeof(double) * c.ne, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    c.has_E_elem = c.has_nu_elem = true;
+    c.thick = thickness;
+    c.plane_strain = plane;
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_batch_define(cstfem_ctx* ctx, int count, const int* node_off) {
+  return guarded([&] {
+    need(ctx && node_off, "cstfem_batch_define: NULL argument");
+    use_device(&ctx->c);
+    batch_define(ctx->c, count, node_off);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_batch_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt,
+                       const cstfem_batch_config* cfg, cstfem_batch_result* results,
+                       double* x_out, double* seconds) {
+  return guarded([&] {
+    need(ctx && opt, "cstfem_batch_solve: NULL argument");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(opt->rtol > 0 && opt->max_iter >= 0, "cstfem_batch_solve: bad rtol or max_iter");
+    const cstfem_batch_config def{8, 1};
+    std::vector<cstfem_batch_result> res(c.batch_count > 0 ? c.batch_count : 1);
+    const double t = batch_solve(c, *opt, cfg ? *cfg : def, res.data());
+    if (results) std::copy(res.begin(), res.begin() + c.batch_count, results);
+    if (seconds) *seconds = t;
+    if (x_out) {
+      HIP_CHECK(hipMemcpyAsync(x_out, c.x.get(), sizeof(double) * c.n, hipMemcpyDeviceToHost, c.stream));
+      HIP_CHECK(hipStreamSynchronize(c.stream));
+    }
+    for (int k = 0; k < c.batch_count; ++k)
+      if (!res[k].converged) return CSTFEM_NOT_CONVERGED;
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_batch_autotune(cstfem_ctx* ctx, const cstfem_solver_options* opt,
+                          const char* cache_path, cstfem_batch_config* best,
+                          double* best_seconds) {
+  return guarded([&] {
+    need(ctx && opt && best, "cstfem_batch_autotune: NULL argument");
+    use_device(&ctx->c);
+    *best = batch_autotune(ctx->c, *opt, cache_path, best_seconds);
     return CSTFEM_OK;
   });
 }

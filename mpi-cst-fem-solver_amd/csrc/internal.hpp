@@ -108,14 +108,16 @@ struct Context {
   int64_t ne = 0;
   std::vector<double> h_xy;
   std::vector<int> h_tri;
-  DBuf<double> xy, E_elem;
+  DBuf<double> xy, E_elem, nu_elem;
   DBuf<int> tri;
   bool has_E_elem = false;
+  bool has_nu_elem = false;  // per-element Poisson ratio (only with a per-element modulus)
   double E = 1.0, nu = 0.3, thick = 1.0;
   int plane_strain = 0;
   Mat3 D{};  // uniform constitutive matrix (used when there is no per-element modulus)
 
   DBuf<int> ne_ptr, ne_list;
+  std::vector<int> h_ne_ptr, h_ne_list;  // host node -> elements (boundary loads)
 
   // pattern (block CSR) and slot map
   bool have_pattern = false;
@@ -127,6 +129,9 @@ struct Context {
   DBuf<int> slot_ptr, slot_src;
 
   // SELL storage
+  std::vector<int> h_segments;   // block-row boundaries slices may not cross (batches)
+  std::vector<int> h_seg_slice;  // first slice of each segment (+ end)
+  DBuf<int> sell_row0, sell_nrow;
   int nslices = 0;
   int64_t sell_blocks = 0;
   std::vector<int64_t> h_sell_off;
@@ -145,6 +150,11 @@ struct Context {
   DBuf<double> b, x, r, p, Ap, inv_diag, work;
   DBuf<float> r32, p32, Ap32, x32, invd32;
   bool setup_done = false;
+
+  // batch of independent plates (batched.hip)
+  int batch_count = 0;
+  DBuf<int> batch_seg_slice, batch_order, batch_queue;
+  DBuf<cstfem_batch_result> batch_results;
 
   // reduction scratch
   DBuf<double> partials;   // 4 * max_blocks
@@ -179,5 +189,10 @@ int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
 int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st, bool fp32);
 void ensure_vectors(Context& c);
 double profile_kernel(Context& c, int kind, int reps, int64_t bytes);
+void batch_define(Context& c, int count, const int* node_off);
+double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batch_config& cfg,
+                   cstfem_batch_result* results);
+cstfem_batch_config batch_autotune(Context& c, const cstfem_solver_options& o,
+                                   const char* cache_path, double* best_seconds);
 
 }  // namespace cstfem_hip

@@ -56,6 +56,16 @@ class SolverStatsC(ctypes.Structure):
                 ("spmv_calls", ctypes.c_longlong), ("reductions", ctypes.c_longlong)]
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("refinements", ctypes.c_int), ("status", ctypes.c_int),
+                ("rel_residual", ctypes.c_double), ("true_rel_residual", ctypes.c_double)]
+
+
 @dataclass
 class SolverStats:
     """Mirror of SolverStats (solver.hpp:99-107) with the reference's message strings."""
@@ -104,6 +114,13 @@ SIGNATURES = {
     "cstfem_download_solution": (ctypes.c_int, [_P, _D, ctypes.c_int]),
     "cstfem_spmv": (ctypes.c_int, [_P, _D, _D, ctypes.c_int]),
     "cstfem_profile_kernel": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _D]),
+    "cstfem_set_material_fields": (ctypes.c_int, [_P, _D, _D, ctypes.c_int, ctypes.c_double]),
+    "cstfem_batch_define": (ctypes.c_int, [_P, ctypes.c_int, _I]),
+    "cstfem_batch_solve": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions),
+                                          ctypes.POINTER(BatchConfig),
+                                          ctypes.POINTER(BatchResult), _D, _D]),
+    "cstfem_batch_autotune": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), ctypes.c_char_p,
+                                             ctypes.POINTER(BatchConfig), _D]),
 }
 
 
@@ -270,6 +287,41 @@ class Device:
         x = np.empty(self.n, dtype=np.float64)
         _check(lib().cstfem_download_solution(self._h, _ptr(x, ctypes.c_double), self.n))
         return x
+
+    # ---- batches of independent plates
+    def set_material_fields(self, E_elem, nu_elem, plane=PLANE_STRESS, thickness=1.0):
+        E, nu = _f64(E_elem), _f64(nu_elem)
+        _check(lib().cstfem_set_material_fields(self._h, _ptr(E, ctypes.c_double),
+                                                _ptr(nu, ctypes.c_double), int(plane),
+                                                float(thickness)))
+
+    def batch_define(self, node_off):
+        off = _i32(node_off)
+        self.batch_count = int(off.size - 1)
+        _check(lib().cstfem_batch_define(self._h, self.batch_count, _ptr(off, ctypes.c_int)))
+
+    def batch_solve(self, rtol=1e-10, max_iter=200000, method=METHOD_CG_FP64_REFINED,
+                    config=None, inner_rtol=0.0, download=True):
+        """Returns (x, [BatchResult], device seconds)."""
+        o = SolverOptions(rtol, max_iter, 0, 0, method, inner_rtol, 0, 0)
+        cfg = None if config is None else BatchConfig(*config)
+        res = (BatchResult * self.batch_count)()
+        x = np.empty(self.n, dtype=np.float64) if download else None
+        t = ctypes.c_double()
+        _check(lib().cstfem_batch_solve(self._h, ctypes.byref(o),
+                                        ctypes.byref(cfg) if cfg is not None else None, res,
+                                        _ptr(x, ctypes.c_double), ctypes.byref(t)))
+        return x, list(res), t.value
+
+    def batch_autotune(self, rtol=1e-10, max_iter=200000, method=METHOD_CG_FP64_REFINED,
+                       cache_path=None):
+        o = SolverOptions(rtol, max_iter, 0, 0, method, 0.0, 0, 0)
+        best = BatchConfig()
+        t = ctypes.c_double()
+        path = cache_path.encode() if cache_path else None
+        _check(lib().cstfem_batch_autotune(self._h, ctypes.byref(o), path, ctypes.byref(best),
+                                           ctypes.byref(t)))
+        return (best.waves_per_group, best.groups_per_cu), t.value
 
     def profile_kernel(self, kind, reps=20, nbytes=0):
         """Average seconds per launch of a hot kernel (0 fused SpMV, 1 SpMV, 2 update,

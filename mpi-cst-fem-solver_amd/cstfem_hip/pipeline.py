@@ -80,6 +80,75 @@ def config_problem(index: int, scale: float = 1.0) -> Problem:
 
 
 @dataclasses.dataclass
+class BatchProblem:
+    """Independent plates concatenated into one mesh (plate k owns nodes
+    node_off[k]..node_off[k+1]-1 and its own E, nu and traction)."""
+    mesh: meshes.Mesh
+    node_off: np.ndarray
+    E_elem: np.ndarray
+    nu_elem: np.ndarray
+    plates: list          # per-plate Problem (local numbering)
+    plane: int = PLANE_STRESS
+    thickness: float = 1.0
+
+    @property
+    def count(self):
+        return len(self.plates)
+
+
+def batch_problem(count=512, seed=3, nmin=16, nmax=160) -> BatchProblem:
+    """The batched workload: plates from meshes.batch_plates, ux=0 on x=0, uy=0 on y=0,
+    traction (t, 0) on x = max."""
+    plates = []
+    xy, tri, E, nu = [], [], [], []
+    node_off = [0]
+    for m, prm in meshes.batch_plates(count, seed, nmin, nmax):
+        p = Problem(mesh=m, E=prm["E"], nu=prm["nu"], plane=PLANE_STRESS, thickness=prm["t"],
+                    tractions=[("right", dict(tx=prm["tx"], ty=0.0))],
+                    dirichlet=[("left", 0, 0.0), ("bottom", 1, 0.0)], name="plate")
+        plates.append(p)
+        xy.append(m.xy)
+        tri.append(m.tri + node_off[-1])
+        E.append(np.full(m.num_elements, prm["E"]))
+        nu.append(np.full(m.num_elements, prm["nu"]))
+        node_off.append(node_off[-1] + m.num_nodes)
+    big = meshes.Mesh(xy=np.concatenate(xy), tri=np.concatenate(tri).astype(np.int32), groups={},
+                      meta=dict(kind="batch", count=count, seed=seed))
+    return BatchProblem(mesh=big, node_off=np.array(node_off, np.int32), E_elem=np.concatenate(E),
+                        nu_elem=np.concatenate(nu), plates=plates)
+
+
+def load_batch(dev: Device, bp: BatchProblem, times: "PhaseTimes | None" = None):
+    """Uploads the concatenated plates and assembles/constrains them on the device."""
+    t0 = time.perf_counter()
+    dev.upload_mesh(bp.mesh)
+    dev.set_material_fields(bp.E_elem, bp.nu_elem, bp.plane, bp.thickness)
+    t1 = time.perf_counter()
+    dev.build_stiffness_pattern()
+    dev.batch_define(bp.node_off)
+    t2 = time.perf_counter()
+    dev.assemble_stiffness()
+    t3 = time.perf_counter()
+    dev.set_rhs(np.zeros(dev.n))
+    dofs, vals = [], []
+    for k, p in enumerate(bp.plates):
+        off = int(bp.node_off[k])
+        for g, kw in p.tractions:
+            dev.add_traction(p.mesh.groups[g] + off, **kw)
+        d, v = p.dirichlet_dofs()
+        dofs.append(d + 2 * off)
+        vals.append(v)
+    dev.apply_dirichlet(np.concatenate(dofs), np.concatenate(vals))
+    dev.setup()
+    t4 = time.perf_counter()
+    if times is not None:
+        times.upload += t1 - t0
+        times.pattern += t2 - t1
+        times.assembly += t3 - t2
+        times.bcs += t4 - t3
+
+
+@dataclasses.dataclass
 class PhaseTimes:
     upload: float = 0.0
     pattern: float = 0.0
