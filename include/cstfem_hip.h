@@ -44,9 +44,15 @@ enum {
   CSTFEM_METHOD_IR_FP32 = 1,        /* FP32 Jacobi-PCG (FP32 matrix and vectors, FP64 dots,
                                        FP64 solution) refined by FP64 true residuals; stops when
                                        the FP64 true residual is below rtol */
-  CSTFEM_METHOD_FCG_LEARNED = 2,    /* flexible CG with the learned preconditioner */
-  CSTFEM_METHOD_CG_FP64_REFINED = 3 /* the FP64 recurrence plus FP64 true-residual replacement
+  CSTFEM_METHOD_FCG_LEARNED = 2,    /* flexible CG (one kept direction) with the learned
+                                       preconditioner: FP64 matrix and vectors, M in FP32;
+                                       falls back to Jacobi-CG on breakdown or stall; stops on
+                                       the FP64 true residual */
+  CSTFEM_METHOD_CG_FP64_REFINED = 3,/* the FP64 recurrence plus FP64 true-residual replacement
                                        until the true residual is below rtol */
+  CSTFEM_METHOD_FCG_LEARNED_FP32 = 4 /* the flexible CG with FP32 matrix and vectors (FP64 dots
+                                       and solution), FP64 true-residual refreshes that keep the
+                                       search direction */
 };
 
 typedef struct cstfem_ctx cstfem_ctx;
@@ -181,6 +187,36 @@ int cstfem_batch_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt,
 int cstfem_batch_autotune(cstfem_ctx* ctx, const cstfem_solver_options* opt,
                           const char* cache_path, cstfem_batch_config* best,
                           double* best_seconds);
+
+/* ---- learned preconditioner (no reference counterpart) ------------------------------- */
+/* A message-passing network over the mesh graph predicts the 2x2 blocks of a symmetric
+ * sparse approximate inverse on the stiffness pattern,
+ * M_ij = S_i [[a, b], [b, d]] S_j + c D_i^-1 K_ij D_j^-1 (S = D^-1/2, D = diag(K));
+ * CSTFEM_METHOD_FCG_LEARNED runs flexible CG with M and falls back to Jacobi-CG from the
+ * current iterate when it stagnates or breaks down.  Parameters are FP32 (a flat vector of
+ * cstfem_mpnn_num_params() floats); the network computes in BF16 with FP32 accumulation.
+ * All calls below need an assembled, constrained mesh system and cstfem_setup. */
+int cstfem_mpnn_num_params(void);
+/* Random-normal hidden layers (He scaling) from `seed`, decoder = Jacobi (M = D^-1). */
+int cstfem_mpnn_init(cstfem_ctx* ctx, int seed);
+int cstfem_mpnn_set_params(cstfem_ctx* ctx, const float* theta, int n);
+int cstfem_mpnn_get_params(cstfem_ctx* ctx, float* theta, int n);
+/* Node features, network forward and M for the current matrix (before a FCG solve). */
+int cstfem_mpnn_build(cstfem_ctx* ctx);
+/* Training pairs from the current system: x* by FP64 CG to 1e-12, then e = x* - x_k and
+ * r = K e after plain FP64 Jacobi-CG iterations k = k_first, k_first + stride, ... <= k_last.
+ * *npairs = number of pairs kept on the device. */
+int cstfem_mpnn_harvest(cstfem_ctx* ctx, int k_first, int k_last, int stride, int* npairs);
+/* Caller-supplied pairs: R, E are K rows of n floats (either may be NULL = zeros). */
+int cstfem_mpnn_set_pairs(cstfem_ctx* ctx, int K, const float* R, const float* E);
+/* Forward, loss (1/K) sum_k |M r_k - e_k|_K^2 / |e_k|_K^2 and its gradient w.r.t. the
+ * parameters (kept on the device, see cstfem_mpnn_download). */
+int cstfem_mpnn_loss_grad(cstfem_ctx* ctx, double* loss);
+/* cstfem_mpnn_loss_grad followed by one Adam step (beta 0.9 / 0.999, eps 1e-8). */
+int cstfem_mpnn_train_step(cstfem_ctx* ctx, float lr, double* loss);
+/* what: 0 node features (nb x 16), 1 final hidden state (nb x 64), 2 decoder outputs
+ * [a, b, d, c] per stored block (nblk x 4), 3 gradient (num_params).  count must match. */
+int cstfem_mpnn_download(cstfem_ctx* ctx, int what, float* out, int64_t count);
 
 /* ---- measurement (no reference counterpart) ------------------------------------------ */
 /* Average seconds of `reps` back-to-back launches of one hot kernel on the library's

@@ -566,6 +566,8 @@ void assemble_values(Context& c) {
   HIP_CHECK(hipGetLastError());
   c.setup_done = false;
   c.vals32_valid = false;
+  c.fixed.release();  // a fresh matrix has no constrained DOFs
+  c.mpnn.built = false;
 }
 
 void download_values(Context& c, double* vals) {

@@ -177,6 +177,8 @@ int cstfem_upload_mesh(cstfem_ctx* ctx, int nn, const double* xy, int64_t ne, co
     c.h_ne_ptr.clear();
     c.h_ne_list.clear();
     c.batch_count = 0;
+    c.fixed.release();
+    c.mpnn.built = false;
     c.ne_ptr.release();
     c.ne_list.release();
     c.b.release();
@@ -596,6 +598,12 @@ int cstfem_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt, cstfem_solve
       case CSTFEM_METHOD_CG_FP64_REFINED:
         rc = solve_refined(c, *opt, *st, false);
         break;
+      case CSTFEM_METHOD_FCG_LEARNED:
+        rc = solve_fcg_learned(c, *opt, *st, false);
+        break;
+      case CSTFEM_METHOD_FCG_LEARNED_FP32:
+        rc = solve_fcg_learned(c, *opt, *st, true);
+        break;
       default:
         throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_solve: unknown method " + std::to_string(opt->method));
     }
@@ -703,6 +711,93 @@ int cstfem_batch_autotune(cstfem_ctx* ctx, const cstfem_solver_options* opt,
     need(ctx && opt && best, "cstfem_batch_autotune: NULL argument");
     use_device(&ctx->c);
     *best = batch_autotune(ctx->c, *opt, cache_path, best_seconds);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mpnn_num_params(void) { return mpnn_num_params(); }
+
+int cstfem_mpnn_init(cstfem_ctx* ctx, int seed) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_mpnn_init: NULL context");
+    use_device(&ctx->c);
+    mpnn_init(ctx->c, seed);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mpnn_set_params(cstfem_ctx* ctx, const float* theta, int n) {
+  return guarded([&] {
+    need(ctx && theta, "cstfem_mpnn_set_params: NULL argument");
+    use_device(&ctx->c);
+    mpnn_set_params(ctx->c, theta, n);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mpnn_get_params(cstfem_ctx* ctx, float* theta, int n) {
+  return guarded([&] {
+    need(ctx && theta, "cstfem_mpnn_get_params: NULL argument");
+    use_device(&ctx->c);
+    mpnn_get_params(ctx->c, theta, n);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mpnn_build(cstfem_ctx* ctx) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_mpnn_build: NULL context");
+    use_device(&ctx->c);
+    mpnn_forward(ctx->c);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mpnn_harvest(cstfem_ctx* ctx, int k_first, int k_last, int stride, int* npairs) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_mpnn_harvest: NULL context");
+    use_device(&ctx->c);
+    const int got = mpnn_harvest(ctx->c, k_first, k_last, stride);
+    if (npairs) *npairs = got;
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mpnn_set_pairs(cstfem_ctx* ctx, int K, const float* R, const float* E) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_mpnn_set_pairs: NULL context");
+    need(K > 0, "cstfem_mpnn_set_pairs: K must be positive");
+    use_device(&ctx->c);
+    mpnn_set_pairs(ctx->c, K, R, E);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mpnn_loss_grad(cstfem_ctx* ctx, double* loss) {
+  return guarded([&] {
+    need(ctx && loss, "cstfem_mpnn_loss_grad: NULL argument");
+    use_device(&ctx->c);
+    *loss = mpnn_loss_grad(ctx->c);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mpnn_train_step(cstfem_ctx* ctx, float lr, double* loss) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_mpnn_train_step: NULL context");
+    need(lr > 0.f && std::isfinite(lr), "cstfem_mpnn_train_step: lr must be positive");
+    use_device(&ctx->c);
+    const double l = mpnn_train_step(ctx->c, lr);
+    if (loss) *loss = l;
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mpnn_download(cstfem_ctx* ctx, int what, float* out, int64_t count) {
+  return guarded([&] {
+    need(ctx && out, "cstfem_mpnn_download: NULL argument");
+    use_device(&ctx->c);
+    mpnn_download(ctx->c, what, out, count);
     return CSTFEM_OK;
   });
 }

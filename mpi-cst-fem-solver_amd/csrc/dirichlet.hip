@@ -14,10 +14,12 @@ namespace cstfem_hip {
 namespace {
 
 __global__ void k_mark_fixed(int m, const int* __restrict__ dofs, const double* __restrict__ val,
-                             unsigned char* __restrict__ fixed, double* __restrict__ ubar) {
+                             unsigned char* __restrict__ fixed, double* __restrict__ ubar,
+                             unsigned char* __restrict__ all_fixed) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   fixed[dofs[k]] = 1;
+  all_fixed[dofs[k]] = 1;
   ubar[dofs[k]] = val[k];
 }
 
@@ -120,13 +122,17 @@ void apply_dirichlet_device(Context& c, int m, const int* dofs, const double* va
   ubar.alloc(n);
   HIP_CHECK(hipMemsetAsync(fixed.get(), 0, n, c.stream));
   HIP_CHECK(hipMemsetAsync(ubar.get(), 0, sizeof(double) * n, c.stream));
+  if (c.fixed.n != (size_t)n) {  // union of all constrained DOFs since the last assembly
+    c.fixed.alloc(n);
+    HIP_CHECK(hipMemsetAsync(c.fixed.get(), 0, n, c.stream));
+  }
   if (m > 0) {
     d_dofs.alloc(m);
     d_vals.alloc(m);
     HIP_CHECK(hipMemcpyAsync(d_dofs.get(), dofs, sizeof(int) * m, hipMemcpyHostToDevice, c.stream));
     HIP_CHECK(hipMemcpyAsync(d_vals.get(), values, sizeof(double) * m, hipMemcpyHostToDevice, c.stream));
     hipLaunchKernelGGL(k_mark_fixed, dim3(grid_for(m, 256, 0)), dim3(256), 0, c.stream, m,
-                       d_dofs.get(), d_vals.get(), fixed.get(), ubar.get());
+                       d_dofs.get(), d_vals.get(), fixed.get(), ubar.get(), c.fixed.get());
     HIP_CHECK(hipGetLastError());
   }
   const int big = INT_MAX;

@@ -91,12 +91,36 @@ struct PcgScalars {
   double delta;    // FP32 refinement: refresh r when ||r|| drops by this factor
   double rel_ref;  // FP32 refinement: ||r||/||b|| at the last refresh
   double pad0;
+  double best_rel[2];  // flexible CG: smallest ||r||/||b|| so far, per iteration parity
   int k_in[2];     // iterations completed, written by iterations of parity 0 / 1
   int status, max_iter, record, refine_mode;
+  int best_k[2];   // flexible CG: iteration of best_rel, per parity
+  int stall_window;
+  int pad1;
 };
 
 // kRefine: the FP32 iteration asks the host for an FP64 true-residual refresh.
-enum { kRunning = 0, kConverged = 1, kBreakdown = 2, kMaxIter = 3, kRefine = 4 };
+// kStalled: flexible CG made no progress for stall_window iterations.
+enum { kRunning = 0, kConverged = 1, kBreakdown = 2, kMaxIter = 3, kRefine = 4, kStalled = 5 };
+
+constexpr int kMpnnRounds = 3;  // message-passing rounds of the learned preconditioner
+
+// Learned preconditioner state (mpnn.hip): FP32 master parameters with Adam moments, BF16
+// weight copies, per-node activations, the decoded blocks and M's SELL values, and the
+// training pairs (r_k, e_k) as FP32 rows of length n.
+struct Mpnn {
+  bool initialized = false;
+  bool built = false;  // gsell holds M for the current parameters and matrix
+  int step = 0;        // Adam steps taken
+  int npairs = 0;
+  DBuf<float> theta, grad, am, av;
+  DBuf<uint16_t> wt[kMpnnRounds + 1], wn[kMpnnRounds + 1];  // [out x in], [in x out]
+  DBuf<uint16_t> feat;
+  DBuf<uint16_t> H[kMpnnRounds + 1], X[kMpnnRounds + 1];
+  DBuf<float> ghat, dghat, gsell;
+  DBuf<int> btrans;
+  DBuf<float> R, E;
+};
 
 struct Context {
   int device = 0;
@@ -156,6 +180,10 @@ struct Context {
   DBuf<int> batch_seg_slice, batch_order, batch_queue;
   DBuf<cstfem_batch_result> batch_results;
 
+  // constrained DOFs (union over apply_dirichlet calls since the last assembly)
+  DBuf<unsigned char> fixed;
+  Mpnn mpnn;
+
   // reduction scratch
   DBuf<double> partials;   // 4 * max_blocks
   DBuf<PcgScalars> scal;
@@ -186,8 +214,22 @@ void setup_solver(Context& c);
 void spmv_fp64(Context& c, const double* x, double* y);
 int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
                   double* history);
-int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st, bool fp32);
+int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st, bool fp32,
+                  bool warm = false);
+int solve_fcg_learned(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
+                      bool fp32);
 void ensure_vectors(Context& c);
+void ensure_vals32(Context& c);
+int mpnn_num_params();
+void mpnn_init(Context& c, int seed);
+void mpnn_set_params(Context& c, const float* theta, int n);
+void mpnn_get_params(Context& c, float* theta, int n);
+void mpnn_forward(Context& c);
+void mpnn_set_pairs(Context& c, int K, const float* R, const float* E);
+int mpnn_harvest(Context& c, int k_first, int k_last, int stride);
+double mpnn_loss_grad(Context& c);
+double mpnn_train_step(Context& c, float lr);
+void mpnn_download(Context& c, int what, float* out, int64_t count);
 double profile_kernel(Context& c, int kind, int reps, int64_t bytes);
 void batch_define(Context& c, int count, const int* node_off);
 double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batch_config& cfg,

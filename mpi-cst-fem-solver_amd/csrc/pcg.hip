@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
+#include <type_traits>
 
 #include "internal.hpp"
 
@@ -448,19 +449,22 @@ __global__ __launch_bounds__(kThreads) void k_refresh(
   }
 }
 
-// FP32 start: r32 = b, p32 = D^-1 r32, x64 = 0; partials (b'b as FP64, r32'z32) into `part`
+// FP32 start: r32 = b, p32 = D^-1 r32, x64 = 0; partials (b'b as FP64, r32'z32) into `part`.
+// Warm start (r64 != NULL): r32 = r64 = b - A x for the given x, which is kept.
 __global__ __launch_bounds__(kThreads) void k_init32(int nvec2, const double2* __restrict__ b,
                                                      double2* __restrict__ x,
                                                      float2* __restrict__ r32,
                                                      float2* __restrict__ p32,
                                                      const float2* __restrict__ invd,
-                                                     double* __restrict__ part) {
+                                                     double* __restrict__ part,
+                                                     const double2* __restrict__ r64 = nullptr) {
   __shared__ double lds[kWaves];
   double bb = 0.0, rz = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
     const double2 bi = b[i];
-    const float2 ri = make_float2((float)bi.x, (float)bi.y), di = invd[i];
-    x[i] = make_double2(0.0, 0.0);
+    const double2 rs = r64 ? r64[i] : bi;
+    const float2 ri = make_float2((float)rs.x, (float)rs.y), di = invd[i];
+    if (!r64) x[i] = make_double2(0.0, 0.0);
     r32[i] = ri;
     p32[i] = make_float2(di.x * ri.x, di.y * ri.y);
     bb += bi.x * bi.x + bi.y * bi.y;
@@ -582,6 +586,18 @@ void setup_solver(Context& c) {
     throw Error(CSTFEM_ERR_BAD_DIAGONAL, "cg_jacobi: non-positive diagonal at row " + std::to_string(bad));
   c.setup_done = true;
   c.vals32_valid = false;
+  c.mpnn.built = false;  // M depends on the values and the Jacobi scaling
+}
+
+// FP32 copy of the matrix values for the mixed-precision paths (rounded once per matrix)
+void ensure_vals32(Context& c) {
+  if (c.vals32_valid) return;
+  c.sell_val32.alloc(4 * c.sell_blocks);
+  hipLaunchKernelGGL(k_round_values, dim3(2048), dim3(256), 0, c.stream,
+                     (int64_t)(4 * c.sell_blocks), (const double*)c.sell_val.get(),
+                     c.sell_val32.get());
+  HIP_CHECK(hipGetLastError());
+  c.vals32_valid = true;
 }
 
 void spmv_fp64(Context& c, const double* x, double* y) {
@@ -828,21 +844,16 @@ int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
 // refinement), r is replaced by b - A x computed in FP64 -- the search direction is kept, so
 // the Krylov information survives -- and the solve stops only when the FP64 true residual is
 // below rtol.
-int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st, bool fp32) {
+int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st, bool fp32,
+                  bool warm) {
   if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: solve() called before setup()");
+  if (warm && !fp32) throw Error(CSTFEM_ERR_ARGUMENT, "solve_refined: warm start needs FP32 mode");
   const size_t nv = 2 * (size_t)c.nb;
   if (fp32) {
     c.r32.alloc(nv);
     c.p32.alloc(nv);
     c.Ap32.alloc(nv);
-    if (!c.vals32_valid) {
-      c.sell_val32.alloc(4 * c.sell_blocks);
-      hipLaunchKernelGGL(k_round_values, dim3(2048), dim3(256), 0, c.stream,
-                         (int64_t)(4 * c.sell_blocks), (const double*)c.sell_val.get(),
-                         c.sell_val32.get());
-      HIP_CHECK(hipGetLastError());
-      c.vals32_valid = true;
-    }
+    ensure_vals32(c);
   }
   const int gv = vec_grid(c);
   // FP64: refresh only where the reference stops (rtol); FP32 drifts faster
@@ -879,10 +890,12 @@ int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
   HIP_CHECK(hipEventCreate(&ev0));
   HIP_CHECK(hipEventCreate(&ev1));
   HIP_CHECK(hipEventRecord(ev0, c.stream));
+  if (warm) true_residual(c, c.work.get());  // r = b - A x for the given x
   if (fp32)
     hipLaunchKernelGGL(k_init32, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
                        (const double2*)c.b.get(), (double2*)c.x.get(), (float2*)c.r32.get(),
-                       (float2*)c.p32.get(), (const float2*)c.invd32.get(), partB(c, 1));
+                       (float2*)c.p32.get(), (const float2*)c.invd32.get(), partB(c, 1),
+                       warm ? (const double2*)c.r.get() : (const double2*)nullptr);
   else
     hipLaunchKernelGGL(k_cg_init, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
                        (const double2*)c.b.get(), (double2*)c.x.get(), (double2*)c.r.get(),
@@ -895,12 +908,18 @@ int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
   const bool zero_rhs = bnorm == 0.0;
   int outer = 0;
   double true_rel = zero_rhs ? 0.0 : 1.0;
-  bool converged = zero_rhs;
+  if (warm && !zero_rhs) {  // refinement thresholds relative to the starting residual
+    true_rel = std::sqrt(s.rr) / bnorm;
+    s.rel_ref = true_rel;
+    HIP_CHECK(hipMemcpyAsync(c.scal.get(), &s, sizeof s, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+  }
+  bool converged = zero_rhs || (warm && true_rel < o.rtol);
   int last_status = kRunning;
-  if (!zero_rhs && o.max_iter > 0) {
+  if (!converged && o.max_iter > 0) {
     const int chunk = c.n > 200000 ? 16 : 64;  // even
     Chunk gr(c, chunk, iteration);
-    double prev_true = 1.0;
+    double prev_true = true_rel;
     int stagnant = 0;
     for (;;) {
       run_until_stopped(c, gr, chunk, o.max_iter);
@@ -975,6 +994,394 @@ int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stat
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   return converged ? CSTFEM_OK : CSTFEM_NOT_CONVERGED;
+}
+
+// ---------------------------------------------------------------------------------------
+// Flexible CG (Notay's FCG with one kept direction) with the learned preconditioner M (FP32
+// values in the matrix's SELL layout).  P = F64: the reference's FP64 matrix and vectors, M
+// applied with FP64 accumulation; P = F32: FP32 matrix and vectors, FP64 dots and solution.
+// Both stop on the FP64 true residual.
+//   A  q = A p; partials p'q, p'r
+//   B  alpha = p'r / p'q (breakdown if !(p'q > 0)); x += alpha p; r -= alpha q; partial r'r
+//   C  convergence / refinement / stall test; z = M r; partial z'q
+//   D  beta = -z'q / p'q (0 on a restart); p = z + beta p
+
+namespace {
+
+// rows of a SELL slice of M (FP32 values) times an FP64 vector, FP64 accumulation
+__device__ inline void sell_row(const int64_t* __restrict__ off, const int* __restrict__ col,
+                                const float* __restrict__ val, const double2* __restrict__ x,
+                                int s, int lane, double& y0, double& y1) {
+  const int64_t o = off[s];
+  const int w = (int)((off[s + 1] - o) >> 6);
+  const int* c = col + o + lane;
+  const v4f* v = reinterpret_cast<const v4f*>(val) + (o + lane);
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll 4
+  for (int k = 0; k < w; ++k) {
+    const int m = c[(size_t)k * 64];
+    const v4f vv = __builtin_nontemporal_load(v + (size_t)k * 64);
+    const double2 xm = x[m];
+    a0 = a0 + (double)vv.x * xm.x;
+    a0 = a0 + (double)vv.y * xm.y;
+    a1 = a1 + (double)vv.z * xm.x;
+    a1 = a1 + (double)vv.w * xm.y;
+  }
+  y0 = a0;
+  y1 = a1;
+}
+
+template <class P>
+__device__ inline typename P::V make_v(double a, double b) {
+  if constexpr (std::is_same<typename P::V, double2>::value)
+    return make_double2(a, b);
+  else
+    return make_float2((float)a, (float)b);
+}
+
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_fcg_spmv(Mat<P> A, const typename P::V* __restrict__ p,
+                                                       const typename P::V* __restrict__ r,
+                                                       typename P::V* __restrict__ q,
+                                                       double* __restrict__ part,
+                                                       const PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kWaves];
+  if (sc->status != kRunning) return;
+  const int lane = threadIdx.x & 63;
+  const SliceRange sr = slice_range(A.nslices);
+  double pq = 0.0, pr = 0.0;
+  for (int s = sr.first; s < sr.end; s += sr.step) {
+    const int row = A.row0[s] + lane;
+    typename P::S y0, y1;
+    sell_row(A.off, A.col, A.val, p, s, lane, y0, y1);
+    if (lane < A.nrow[s]) {
+      q[row] = make_v<P>(y0, y1);
+      const typename P::V pi = p[row], ri = r[row];
+      pq += (double)pi.x * (double)y0 + (double)pi.y * (double)y1;
+      pr += (double)pi.x * (double)ri.x + (double)pi.y * (double)ri.y;
+    }
+  }
+  const double a = block_sum_all(pq, lds);
+  const double b = block_sum_all(pr, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[kMaxGrid + blockIdx.x] = b;
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_fcg_update(
+    int nvec2, double2* __restrict__ x, typename P::V* __restrict__ r,
+    const typename P::V* __restrict__ p, const typename P::V* __restrict__ q,
+    const double* __restrict__ partA, int nA, double* __restrict__ partB,
+    PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kWaves];
+  if (sc->status != kRunning) return;
+  const double pq = reduce_partials(partA, nA, 1, lds);
+  const double pr = reduce_partials(partA + kMaxGrid, nA, 1, lds);
+  if (!(pq > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->pAp = pq;
+      sc->status = kBreakdown;
+    }
+    return;
+  }
+  const double alpha = pr / pq;
+  double rr = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const typename P::V pi = p[i], qi = q[i];
+    double2 xi = x[i];
+    const typename P::V r0 = r[i];
+    xi.x = xi.x + alpha * (double)pi.x;
+    xi.y = xi.y + alpha * (double)pi.y;
+    const typename P::V ri = make_v<P>((double)r0.x - alpha * (double)qi.x,
+                                       (double)r0.y - alpha * (double)qi.y);
+    x[i] = xi;
+    r[i] = ri;
+    rr += (double)ri.x * (double)ri.x + (double)ri.y * (double)ri.y;
+  }
+  const double t = block_sum_all(rr, lds);
+  if (threadIdx.x == 0) partB[blockIdx.x] = t;
+}
+
+// test: 1 = convergence / refinement / stall test first; 0 = preconditioner application only
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_fcg_precond(
+    Mat<F32> G, const typename P::V* __restrict__ r, const typename P::V* __restrict__ q,
+    typename P::V* __restrict__ z, const double* __restrict__ partB, int nB,
+    double* __restrict__ partC, int cur, PcgScalars* __restrict__ sc, int test) {
+  __shared__ double lds[kWaves];
+  if (sc->status != kRunning) return;
+  if (test) {
+    const double rr = reduce_partials(partB, nB, 1, lds);
+    const int k = sc->k_in[cur ^ 1] + 1;
+    const double rel = sqrt(rr) / sc->bnorm;
+    // progress record of the previous iteration (other parity slot: stable during this kernel)
+    const bool better = rel < sc->best_rel[cur ^ 1];
+    const double best = better ? rel : sc->best_rel[cur ^ 1];
+    const int best_k = better ? k : sc->best_k[cur ^ 1];
+    int status = kRunning;
+    if (rel < sc->rtol || rel < sc->delta * sc->rel_ref)
+      status = kRefine;
+    else if (k >= sc->max_iter)
+      status = kMaxIter;
+    else if (!(rel == rel) || k - best_k > sc->stall_window)
+      status = kStalled;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->k_in[cur] = k;
+      sc->rel = rel;
+      sc->best_rel[cur] = best;
+      sc->best_k[cur] = best_k;
+      if (status != kRunning) sc->status = status;
+    }
+    if (status != kRunning) return;
+  }
+  const int lane = threadIdx.x & 63;
+  const SliceRange sr = slice_range(G.nslices);
+  double zq = 0.0;
+  for (int s = sr.first; s < sr.end; s += sr.step) {
+    const int row = G.row0[s] + lane;
+    typename P::S y0, y1;
+    sell_row(G.off, G.col, G.val, r, s, lane, y0, y1);
+    if (lane < G.nrow[s]) {
+      z[row] = make_v<P>(y0, y1);
+      const typename P::V qi = q[row];
+      zq += (double)y0 * (double)qi.x + (double)y1 * (double)qi.y;
+    }
+  }
+  const double t = block_sum_all(zq, lds);
+  if (threadIdx.x == 0) partC[blockIdx.x] = t;
+}
+
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_fcg_direction(
+    int nvec2, const typename P::V* __restrict__ z, typename P::V* __restrict__ p,
+    const double* __restrict__ partA, int nA, const double* __restrict__ partC, int nC,
+    const PcgScalars* __restrict__ sc, int restart) {
+  __shared__ double lds[kWaves];
+  if (sc->status != kRunning) return;
+  double beta = 0.0;
+  if (!restart) {
+    const double pq = reduce_partials(partA, nA, 1, lds);
+    const double zq = reduce_partials(partC, nC, 1, lds);
+    beta = -zq / pq;
+  }
+  const typename P::S b = (typename P::S)beta;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const typename P::V zi = z[i];
+    typename P::V pi = p[i];
+    pi.x = zi.x + b * pi.x;
+    pi.y = zi.y + b * pi.y;
+    p[i] = pi;
+  }
+}
+
+// r = b (x = 0) or r = r64 (warm); bnorm partials as k_init32 does
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_fcg_init(int nvec2, const double2* __restrict__ b,
+                                                       double2* __restrict__ x,
+                                                       typename P::V* __restrict__ r,
+                                                       typename P::V* __restrict__ q,
+                                                       double* __restrict__ part) {
+  __shared__ double lds[kWaves];
+  double bb = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const double2 bi = b[i];
+    x[i] = make_double2(0.0, 0.0);
+    r[i] = make_v<P>(bi.x, bi.y);
+    q[i] = make_v<P>(0.0, 0.0);
+    bb += bi.x * bi.x + bi.y * bi.y;
+  }
+  const double s0 = block_sum_all(bb, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s0;
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+// after a refinement: the working residual from r64 = b - A x (FP64)
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_fcg_set_r(int nvec2, const double2* __restrict__ r64,
+                                                        typename P::V* __restrict__ r) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads)
+    r[i] = make_v<P>(r64[i].x, r64[i].y);
+}
+
+template <class P>
+int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st, bool fp32) {
+  using V = typename P::V;
+  const size_t nv = 2 * (size_t)c.nb;
+  const int gs = spmv_grid(c), gv = vec_grid(c);
+  double* pA = c.partials.get();
+  double* pB = pA + 2 * kMaxGrid;
+  double* pC = pA + 3 * kMaxGrid;
+  Mat<P> A;
+  V *r, *p, *q, *z;
+  DBuf<double> z64, ax64;
+  if (fp32) {
+    c.r32.alloc(nv);
+    c.p32.alloc(nv);
+    c.Ap32.alloc(nv);
+    c.x32.alloc(nv);
+    ensure_vals32(c);
+    r = (V*)c.r32.get();
+    p = (V*)c.p32.get();
+    q = (V*)c.Ap32.get();
+    z = (V*)c.x32.get();
+  } else {
+    z64.alloc(nv);
+    ax64.alloc(nv);
+    r = (V*)c.r.get();
+    p = (V*)c.p.get();
+    q = (V*)c.Ap.get();
+    z = (V*)z64.get();
+  }
+  {
+    void* vals = fp32 ? (void*)c.sell_val32.get() : (void*)c.sell_val.get();
+    A = Mat<P>{c.sell_off.get(), c.sell_col.get(), (const typename P::S*)vals, c.sell_row0.get(),
+               c.sell_nrow.get(), c.nslices, c.nb};
+  }
+  const Mat<F32> G{c.sell_off.get(), c.sell_col.get(), c.mpnn.gsell.get(), c.sell_row0.get(),
+                   c.sell_nrow.get(), c.nslices, c.nb};
+  // scratch of a refinement: r64 = b - A x and A x (FP64), apart from r, p, q, z
+  double* r64 = fp32 ? c.r.get() : c.work.get();
+  double* Ax = fp32 ? c.work.get() : ax64.get();
+  auto precond = [&](int cur, int test) {
+    hipLaunchKernelGGL(k_fcg_precond<P>, dim3(gs), dim3(kThreads), 0, c.stream, G, (const V*)r,
+                       (const V*)q, z, (const double*)pB, gv, pC, cur, c.scal.get(), test);
+  };
+  auto direction = [&](int restart) {
+    hipLaunchKernelGGL(k_fcg_direction<P>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                       (const V*)z, p, (const double*)pA, gs, (const double*)pC, gs,
+                       (const PcgScalars*)c.scal.get(), restart);
+  };
+  auto iteration = [&](int cur) {
+    hipLaunchKernelGGL(k_fcg_spmv<P>, dim3(gs), dim3(kThreads), 0, c.stream, A, (const V*)p,
+                       (const V*)r, q, pA, (const PcgScalars*)c.scal.get());
+    hipLaunchKernelGGL(k_fcg_update<P>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                       (double2*)c.x.get(), r, (const V*)p, (const V*)q, (const double*)pA, gs,
+                       pB, c.scal.get());
+    precond(cur, 1);
+    direction(0);
+  };
+  // FP64: refresh only where the plain recurrence would stop; FP32: frequent refreshes that
+  // keep the search direction (fewer iterations than restarts)
+  const double delta = o.inner_rtol > 0 ? o.inner_rtol : (fp32 ? 1e-1 : 0.0);
+  const int max_outer = o.max_outer > 0 ? o.max_outer : 200;
+  PcgScalars init{};
+  init.rtol = o.rtol;
+  init.max_iter = o.max_iter;
+  init.refine_mode = 1;
+  init.delta = delta;
+  init.rel_ref = 1.0;
+  init.best_rel[0] = init.best_rel[1] = 1.0;
+  init.stall_window = std::max(1000, o.max_iter / 20);
+  HIP_CHECK(hipMemcpyAsync(c.scal.get(), &init, sizeof init, hipMemcpyHostToDevice, c.stream));
+  hipEvent_t ev0, ev1;
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  HIP_CHECK(hipEventRecord(ev0, c.stream));
+  // x = 0, r = b, q = 0; bnorm; z = M r; p = z
+  hipLaunchKernelGGL(k_fcg_init<P>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                     (const double2*)c.b.get(), (double2*)c.x.get(), r, q, partB(c, 1));
+  hipLaunchKernelGGL(k_cg_init_finish, dim3(1), dim3(kThreads), 0, c.stream,
+                     (const double*)partB(c, 1), gv, c.scal.get(), (double*)nullptr);
+  precond(0, 0);
+  direction(1);
+  HIP_CHECK(hipGetLastError());
+  PcgScalars s = read_scalars(c);
+  const double bnorm = s.bnorm;
+  const bool zero_rhs = bnorm == 0.0;
+  int outer = 0, last_status = kRunning, iters = 0;
+  double true_rel = zero_rhs ? 0.0 : 1.0;
+  bool converged = zero_rhs, fallback = false;
+  if (!zero_rhs && o.max_iter > 0) {
+    const int chunk = c.n > 200000 ? 16 : 64;  // even
+    Chunk gr(c, chunk, iteration);
+    double prev_true = 1.0;
+    int stagnant = 0;
+    for (;;) {
+      run_until_stopped(c, gr, chunk, o.max_iter);
+      s = read_scalars(c);
+      last_status = s.status;
+      const int k = s.k_in[0] > s.k_in[1] ? s.k_in[0] : s.k_in[1];
+      iters = k;
+      if (s.status != kRefine) break;  // breakdown, stall or max_iter
+      // r64 = b - A x in FP64 (sum into sc->rr); the working residual follows
+      spmv_fp64(c, c.x.get(), Ax);
+      hipLaunchKernelGGL(k_residual, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                         (const double2*)c.b.get(), (const double2*)Ax, (double2*)r64, partX(c));
+      hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(kThreads), 0, c.stream, partX(c), gv,
+                         &c.scal.get()->rr);
+      hipLaunchKernelGGL(k_fcg_set_r<P>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                         (const double2*)r64, r);
+      HIP_CHECK(hipGetLastError());
+      const PcgScalars s2 = read_scalars(c);
+      true_rel = std::sqrt(s2.rr) / bnorm;
+      ++outer;
+      if (true_rel < o.rtol) {
+        converged = true;
+        break;
+      }
+      stagnant = true_rel > 0.5 * prev_true ? stagnant + 1 : 0;
+      prev_true = true_rel;
+      if (outer >= max_outer || k >= o.max_iter || stagnant >= 4) break;
+      PcgScalars upd = s2;
+      upd.status = kRunning;
+      upd.rel_ref = true_rel;
+      upd.best_rel[0] = upd.best_rel[1] = true_rel;  // progress is measured from the refresh
+      upd.best_k[0] = upd.best_k[1] = k;
+      HIP_CHECK(hipMemcpyAsync(c.scal.get(), &upd, sizeof upd, hipMemcpyHostToDevice, c.stream));
+      precond((k - 1) & 1, 0);  // z = M r with the refreshed residual, z'q against the last q
+      direction(0);             // p = z + beta p: the direction is kept
+      if (k & 1) iteration(1);  // the graph starts at parity 0
+      HIP_CHECK(hipGetLastError());
+      last_status = kRunning;
+    }
+  }
+  int fb_iters = 0, fb_outer = 0;
+  if (!converged) {
+    // breakdown or stall of the learned preconditioner: Jacobi-CG (FP32 + FP64 refinement)
+    // from the current x; the FP64 true residual decides
+    fallback = true;
+    cstfem_solver_options fo = o;
+    fo.max_iter = std::max(1, o.max_iter - iters);
+    fo.inner_rtol = 0.0;
+    fo.refresh = 0;
+    cstfem_solver_stats fs{};
+    solve_refined(c, fo, fs, true, true);
+    fb_iters = fs.iterations;
+    fb_outer = fs.outer_iterations;
+    converged = fs.converged != 0;
+    true_rel = fs.true_rel_residual;
+    last_status = converged ? kConverged : kMaxIter;
+  }
+  s = read_scalars(c);
+  HIP_CHECK(hipEventRecord(ev1, c.stream));
+  HIP_CHECK(hipEventSynchronize(ev1));
+  st.iterations = zero_rhs ? 0 : iters + fb_iters;
+  st.outer_iterations = outer + fb_outer;
+  st.converged = converged ? 1 : 0;
+  st.breakdown = last_status == kBreakdown ? 1 : 0;
+  st.fallback = fallback ? 1 : 0;
+  st.zero_rhs = zero_rhs ? 1 : 0;
+  st.rel_residual = zero_rhs ? 0.0 : s.rel;
+  st.true_rel_residual = true_rel;
+  st.spmv_calls = zero_rhs ? 0 : (long long)st.iterations + st.outer_iterations;
+  st.total = elapsed(ev0, ev1);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  return converged ? CSTFEM_OK : CSTFEM_NOT_CONVERGED;
+}
+
+}  // namespace
+
+int solve_fcg_learned(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
+                      bool fp32) {
+  if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: solve() called before setup()");
+  if (!c.mpnn.built)
+    throw Error(CSTFEM_ERR_ORDER, "fcg: build the learned preconditioner first (cstfem_mpnn_build)");
+  return fp32 ? fcg_run<F32>(c, o, st, true) : fcg_run<F64>(c, o, st, false);
 }
 
 // Average duration (seconds) of `reps` back-to-back launches of one hot kernel on the

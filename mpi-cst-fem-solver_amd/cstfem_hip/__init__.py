@@ -25,6 +25,7 @@ ERR_ARGUMENT, ERR_DEVICE, ERR_BAD_DIAGONAL, ERR_MISSING_ENTRY = -1, -2, -3, -4
 ERR_OUT_OF_RANGE, ERR_ALLOC, ERR_TRACTION_EDGE, ERR_CONFLICT, ERR_ORDER = -5, -6, -7, -8, -9
 PLANE_STRESS, PLANE_STRAIN = 0, 1
 METHOD_CG_FP64, METHOD_IR_FP32, METHOD_FCG_LEARNED, METHOD_CG_FP64_REFINED = 0, 1, 2, 3
+METHOD_FCG_LEARNED_FP32 = 4
 
 
 class CstfemError(RuntimeError):
@@ -87,6 +88,7 @@ _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
 _I64 = ctypes.POINTER(ctypes.c_int64)
+_F = ctypes.POINTER(ctypes.c_float)
 
 SIGNATURES = {
     "cstfem_last_error": (ctypes.c_char_p, []),
@@ -121,6 +123,16 @@ SIGNATURES = {
                                           ctypes.POINTER(BatchResult), _D, _D]),
     "cstfem_batch_autotune": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), ctypes.c_char_p,
                                              ctypes.POINTER(BatchConfig), _D]),
+    "cstfem_mpnn_num_params": (ctypes.c_int, []),
+    "cstfem_mpnn_init": (ctypes.c_int, [_P, ctypes.c_int]),
+    "cstfem_mpnn_set_params": (ctypes.c_int, [_P, _F, ctypes.c_int]),
+    "cstfem_mpnn_get_params": (ctypes.c_int, [_P, _F, ctypes.c_int]),
+    "cstfem_mpnn_build": (ctypes.c_int, [_P]),
+    "cstfem_mpnn_harvest": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I]),
+    "cstfem_mpnn_set_pairs": (ctypes.c_int, [_P, ctypes.c_int, _F, _F]),
+    "cstfem_mpnn_loss_grad": (ctypes.c_int, [_P, _D]),
+    "cstfem_mpnn_train_step": (ctypes.c_int, [_P, ctypes.c_float, _D]),
+    "cstfem_mpnn_download": (ctypes.c_int, [_P, ctypes.c_int, _F, ctypes.c_int64]),
 }
 
 
@@ -330,6 +342,54 @@ class Device:
         _check(lib().cstfem_profile_kernel(self._h, int(kind), int(reps), int(nbytes),
                                            ctypes.byref(t)))
         return t.value
+
+    # ---- learned preconditioner
+    def mpnn_num_params(self):
+        return int(lib().cstfem_mpnn_num_params())
+
+    def mpnn_init(self, seed=0):
+        _check(lib().cstfem_mpnn_init(self._h, int(seed)))
+
+    def mpnn_set_params(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        _check(lib().cstfem_mpnn_set_params(self._h, _ptr(t, ctypes.c_float), int(t.size)))
+
+    def mpnn_get_params(self):
+        t = np.empty(self.mpnn_num_params(), dtype=np.float32)
+        _check(lib().cstfem_mpnn_get_params(self._h, _ptr(t, ctypes.c_float), int(t.size)))
+        return t
+
+    def mpnn_build(self):
+        _check(lib().cstfem_mpnn_build(self._h))
+
+    def mpnn_harvest(self, k_first=1, k_last=50, stride=1):
+        got = ctypes.c_int()
+        _check(lib().cstfem_mpnn_harvest(self._h, int(k_first), int(k_last), int(stride),
+                                         ctypes.byref(got)))
+        return got.value
+
+    def mpnn_set_pairs(self, R, E):
+        R = None if R is None else np.ascontiguousarray(R, dtype=np.float32).reshape(-1, self.n)
+        E = None if E is None else np.ascontiguousarray(E, dtype=np.float32).reshape(-1, self.n)
+        K = (R if R is not None else E).shape[0]
+        _check(lib().cstfem_mpnn_set_pairs(self._h, int(K), _ptr(R, ctypes.c_float),
+                                           _ptr(E, ctypes.c_float)))
+
+    def mpnn_loss_grad(self):
+        loss = ctypes.c_double()
+        _check(lib().cstfem_mpnn_loss_grad(self._h, ctypes.byref(loss)))
+        return loss.value
+
+    def mpnn_train_step(self, lr=1e-3):
+        loss = ctypes.c_double()
+        _check(lib().cstfem_mpnn_train_step(self._h, float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def mpnn_download(self, what, count):
+        out = np.empty(int(count), dtype=np.float32)
+        _check(lib().cstfem_mpnn_download(self._h, int(what), _ptr(out, ctypes.c_float),
+                                          int(count)))
+        return out
 
     def spmv(self, x):
         x = _f64(x)
