@@ -8,10 +8,17 @@ pattern + assembly + loads + Dirichlet elimination + Jacobi setup + solve to a t
 relative residual below 1e-10, all on the device; the mesh and material are resident in
 HBM before the timed region starts.
 
-Prints one JSON line.  The roofline object covers the dominant kernel (the fused SpMV +
-p'Ap of every CG iteration), timed with HIP events on the library stream; cpu_baseline is
-the oracle (the CPU restatement of the reference's element loop and PCG, bitwise equal to
-the reference on uniform-E inputs) timed on a bounded sample on one host core.
+Methods: fp64 (the reference's FP64 Jacobi-CG recurrence with FP64 true-residual
+replacement), ir32 (FP32 Jacobi-CG under FP64 refinement), learned (flexible CG with the
+learned preconditioner, FP64 matrix and vectors) and learned32 (the same in FP32 with FP64
+refreshes).  The learned methods train the network once, before the timed steps, on a small
+plate of the same family (its time is reported as train_s); every timed step rebuilds M for
+the freshly assembled matrix (reported as precond_s, part of the step).
+
+Prints one JSON line.  The roofline object covers the dominant kernel of the best method
+(the fused FP64 SpMV of every iteration), timed with HIP events on the library stream;
+cpu_baseline is the oracle (the CPU restatement of the reference's element loop and PCG,
+bitwise equal to the reference on uniform-E inputs) timed on a bounded sample on one core.
 """
 import argparse
 import json
@@ -86,9 +93,12 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the hot kernels (for hardware-counter profiles)")
     ap.add_argument("--reps", type=int, default=10)
-    ap.add_argument("--methods", default="fp64,ir32",
-                    help="comma list of fp64 (FP64 Jacobi-CG, FP64 true-residual refinement) "
-                         "and ir32 (FP32 Jacobi-CG under FP64 refinement)")
+    ap.add_argument("--methods", default="fp64,learned,learned32",
+                    help="comma list of fp64 (FP64 Jacobi-CG, FP64 true-residual refinement), "
+                         "ir32 (FP32 Jacobi-CG under FP64 refinement), learned (FP64 flexible "
+                         "CG with the learned preconditioner), learned32 (its FP32 variant)")
+    ap.add_argument("--train-n", type=int, default=64, help="training plate nodes per side")
+    ap.add_argument("--train-steps", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole plate sits in one MI355X's HBM and this "
@@ -122,14 +132,35 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
-    methods = {"fp64": cstfem_hip.METHOD_CG_FP64_REFINED, "ir32": cstfem_hip.METHOD_IR_FP32}
+    methods = {"fp64": cstfem_hip.METHOD_CG_FP64_REFINED, "ir32": cstfem_hip.METHOD_IR_FP32,
+               "learned": cstfem_hip.METHOD_FCG_LEARNED,
+               "learned32": cstfem_hip.METHOD_FCG_LEARNED_FP32}
     chosen = [m for m in args.methods.split(",") if m]
+    theta, train_info = None, None
+    if any(m.startswith("learned") for m in chosen):
+        from cstfem_hip import learned
+        t0 = time.perf_counter()
+        with cstfem_hip.Device(0) as tdev:
+            theta, losses = learned.train(tdev, learned.training_plate(args.train_n),
+                                          steps=args.train_steps, log=log)
+        train_info = dict(train_s=time.perf_counter() - t0, plate_nodes=args.train_n ** 2,
+                          steps=args.train_steps, loss_first=losses[0], loss_last=losses[-1])
+        log(f"trained the preconditioner in {train_info['train_s']:.2f} s "
+            f"(loss {losses[0]:.4f} -> {losses[-1]:.4f})")
 
     def step(method):
         times = pipeline.PhaseTimes()
         pipeline.assemble_and_constrain(dev, prob, times)
-        _, st = pipeline.solve(dev, rtol=1e-10, max_iter=args.max_iter, method=methods[method],
-                               times=times, download=False)
+        t0 = time.perf_counter()
+        dev.setup()
+        if method.startswith("learned"):
+            dev.mpnn_set_params(theta)
+            dev.mpnn_build()
+        t1 = time.perf_counter()
+        _, st = dev.solve(rtol=1e-10, max_iter=args.max_iter, method=methods[method],
+                          download=False)
+        times.solve += time.perf_counter() - t0
+        st.precond_s = t1 - t0
         return st, times
 
     results = {}
@@ -152,15 +183,18 @@ def main():
         results[meth] = dict(s=sum(step_times) / len(step_times), iterations=st.iterations,
                              refinements=st.outer_iterations, verified=bool(verified),
                              true_rel_residual=st.true_rel_residual,
+                             fallback=bool(st.fallback),
                              s_per_iteration=times.solve / max(1, st.iterations),
                              phases_s={"pattern": times.pattern, "assembly": times.assembly,
-                                       "bcs": times.bcs, "solve": times.solve})
+                                       "bcs": times.bcs, "solve": times.solve,
+                                       "precond": st.precond_s})
     ok = {k: v for k, v in results.items() if v["verified"]} or results
     best = min(ok, key=lambda k: ok[k]["s"])
     per_step = results[best]["s"]
     fp64 = results.get("fp64")
     st_it = results[best]["iterations"]
-    times = pipeline.PhaseTimes(**results[best]["phases_s"])
+    times = pipeline.PhaseTimes(**{k: v for k, v in results[best]["phases_s"].items()
+                                   if k != "precond"})
 
     # dominant kernel: fused SpMV + p'Ap, average launch duration with HIP events
     n, nnz = dev.n, dev.nnz
@@ -170,6 +204,15 @@ def main():
     copy_bytes = 1 << 30
     t_copy = dev.profile_kernel(4, reps=20, nbytes=copy_bytes)
     t_spmv32 = dev.profile_kernel(5, reps=50)
+    fcg_kernels = None
+    if theta is not None:
+        dev.mpnn_set_params(theta)
+        dev.mpnn_build()
+        fcg_kernels = {name_k: dev.profile_kernel(kind, reps=50) for kind, name_k in
+                       ((6, "k_fcg_spmv<F64>"), (7, "k_fcg_precond<F64>"),
+                        (8, "k_fcg_update<F64>"), (9, "k_fcg_direction<F64>"))}
+        log("flexible CG kernels (us): " + ", ".join(f"{k} {v * 1e6:.1f}"
+                                                     for k, v in fcg_kernels.items()))
     b_spmv = spmv_bytes(n, nnz)
     achieved = b_spmv / t_spmv / 1e9
     copy_gbs = 2 * copy_bytes / t_copy / 1e9
@@ -195,7 +238,9 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if best == "ir32" else "f64",
+        "dtype": "f32+f64" if best in ("ir32", "learned32") else "f64",
+        "train": train_info,
+        "fcg_kernel_s": fcg_kernels,
         "data": "synthetic",
         "config": {
             "workload": (f"{prob.name}: {m.num_dofs} DOFs, {m.num_elements} CST elements, "

@@ -222,7 +222,8 @@ int cstfem_mpnn_download(cstfem_ctx* ctx, int what, float* out, int64_t count);
 /* Average seconds of `reps` back-to-back launches of one hot kernel on the library's
  * stream, timed with HIP events; solver state is untouched.  kind: 0 fused SpMV + p'Ap,
  * 1 plain SpMV, 2 CG update (axpys + dots), 3 CG direction, 4 copy of `bytes` bytes,
- * (achievable HBM bandwidth), 5 FP32 SpMV + p'Ap. */
+ * (achievable HBM bandwidth), 5 FP32 SpMV + p'Ap; flexible CG in FP64: 6 SpMV + p'q, p'r,
+ * 7 preconditioner z = M r + z'q (after cstfem_mpnn_build), 8 update, 9 direction. */
 int cstfem_profile_kernel(cstfem_ctx* ctx, int kind, int reps, int64_t bytes,
                           double* avg_seconds);
 

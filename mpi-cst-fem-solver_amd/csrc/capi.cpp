@@ -805,7 +805,7 @@ int cstfem_mpnn_download(cstfem_ctx* ctx, int what, float* out, int64_t count) {
 int cstfem_profile_kernel(cstfem_ctx* ctx, int kind, int reps, int64_t bytes, double* avg_seconds) {
   return guarded([&] {
     need(ctx && avg_seconds, "cstfem_profile_kernel: NULL argument");
-    need(kind >= 0 && kind <= 5, "cstfem_profile_kernel: kind must be 0..5");
+    need(kind >= 0 && kind <= 9, "cstfem_profile_kernel: kind must be 0..9");
     need(reps > 0, "cstfem_profile_kernel: reps must be positive");
     need(kind != 4 || bytes >= 32, "cstfem_profile_kernel: copy needs bytes >= 32");
     use_device(&ctx->c);

@@ -1323,7 +1323,9 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
         converged = true;
         break;
       }
-      stagnant = true_rel > 0.5 * prev_true ? stagnant + 1 : 0;
+      // near rtol the FP64 residual itself is at its rounding floor: refreshes there are
+      // not stagnation (max_outer bounds them)
+      stagnant = (true_rel > 0.5 * prev_true && true_rel > 4.0 * o.rtol) ? stagnant + 1 : 0;
       prev_true = true_rel;
       if (outer >= max_outer || k >= o.max_iter || stagnant >= 4) break;
       PcgScalars upd = s2;
@@ -1387,10 +1389,14 @@ int solve_fcg_learned(Context& c, const cstfem_solver_options& o, cstfem_solver_
 // Average duration (seconds) of `reps` back-to-back launches of one hot kernel on the
 // library stream, between two HIP events.  kind: 0 SpMV + p'Ap partials (k_cg_spmv, FP64),
 // 1 plain FP64 SpMV, 2 update (k_cg_update), 3 direction (k_cg_direction), 4 copy of
-// `bytes` bytes (achievable HBM bandwidth), 5 FP32 SpMV + partials.  Solver state is not
-// modified (private scalars and vectors).
+// `bytes` bytes (achievable HBM bandwidth), 5 FP32 SpMV + partials; flexible CG (FP64):
+// 6 SpMV + p'q, p'r (k_fcg_spmv), 7 z = M r + z'q (k_fcg_precond, needs a built M),
+// 8 update (k_fcg_update), 9 direction (k_fcg_direction).  Solver state is not modified
+// (private scalars and vectors).
 double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
   if (kind != 4 && !c.have_pattern) throw Error(CSTFEM_ERR_ORDER, "profile: no matrix on the device");
+  if (kind == 7 && !c.mpnn.built)
+    throw Error(CSTFEM_ERR_ORDER, "profile: build the learned preconditioner first");
   if (kind != 4) ensure_vectors(c);
   DBuf<PcgScalars> sc;
   sc.alloc(1);
@@ -1462,6 +1468,32 @@ double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
                            (const PcgScalars*)sc.get());
         break;
       }
+      case 6:
+        hipLaunchKernelGGL(k_fcg_spmv<F64>, dim3(gs), dim3(kThreads), 0, c.stream, mat64(c),
+                           (const double2*)b.get(), (const double2*)w.get(), (double2*)a.get(),
+                           part.get(), (const PcgScalars*)sc.get());
+        break;
+      case 7: {
+        const Mat<F32> G{c.sell_off.get(), c.sell_col.get(), c.mpnn.gsell.get(),
+                         c.sell_row0.get(), c.sell_nrow.get(), c.nslices, c.nb};
+        hipLaunchKernelGGL(k_fcg_precond<F64>, dim3(gs), dim3(kThreads), 0, c.stream, G,
+                           (const double2*)b.get(), (const double2*)w.get(), (double2*)a.get(),
+                           (const double*)part.get(), gv, part.get() + 3 * kMaxGrid, 0, sc.get(),
+                           0);
+        break;
+      }
+      case 8:
+        hipLaunchKernelGGL(k_fcg_update<F64>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                           (double2*)a.get(), (double2*)w.get(), (const double2*)b.get(),
+                           (const double2*)b.get(), (const double*)part.get(), gs,
+                           part.get() + 3 * kMaxGrid, sc.get());
+        break;
+      case 9:
+        hipLaunchKernelGGL(k_fcg_direction<F64>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                           (const double2*)b.get(), (double2*)a.get(), (const double*)part.get(),
+                           gs, (const double*)(part.get() + 3 * kMaxGrid), gs,
+                           (const PcgScalars*)sc.get(), 0);
+        break;
       default: {
         const int64_t n2 = bytes / 16;
         hipLaunchKernelGGL(k_copy, dim3(kMaxGrid), dim3(kThreads), 0, c.stream, n2,

@@ -136,3 +136,15 @@ def test_features_scale_invariant(oracle):
     g2 = mo.Graph(np.asarray(g.A.indptr), g.A.indices, g.A.data * 37.0, g.tri, g.nb,
                   g.E_elem * 37.0, g.fixed)
     np.testing.assert_allclose(g2.features(), f1, rtol=1e-12, atol=1e-12)
+
+
+def test_package_layout_matches_oracle():
+    """The package's parameter layout (cstfem_hip.learned) is the oracle's."""
+    from cstfem_hip import learned
+    for name in ("H", "F", "LIVE", "ROUNDS", "DEC", "W0", "B0", "WL", "BL", "WD", "BD", "WO",
+                 "BO", "NPARAMS"):
+        assert getattr(learned, name) == getattr(mo, name), name
+    th = learned.polynomial_start(mo.init_params(0), beta=0.375)
+    assert list(th[mo.BD:mo.WO]) == [1.0, 0.0, 1.0, -0.375]
+    assert list(th[mo.BO:]) == [0.0, 0.0, 0.0, -0.375]
+    assert not th[mo.WD:mo.BD].any() and not th[mo.WO:mo.BO].any()
