@@ -218,6 +218,28 @@ int cstfem_mpnn_train_step(cstfem_ctx* ctx, float lr, double* loss);
  * [a, b, d, c] per stored block (nblk x 4), 3 gradient (num_params).  count must match. */
 int cstfem_mpnn_download(cstfem_ctx* ctx, int what, float* out, int64_t count);
 
+/* ---- multilevel preconditioner (no reference counterpart) -------------------------------- */
+/* Aggregation hierarchy for CSTFEM_METHOD_FCG_LEARNED: greedy aggregation of the node graph,
+ * unsmoothed prolongators from the rigid-body modes (3 coarse DOFs per aggregate), Galerkin
+ * coarse matrices on the device, a dense coarsest solve, and a K-cycle (two flexible CG steps
+ * per coarse level).  fine_smoother: 0 the learned M (cstfem_mpnn_build first), 1 Chebyshev
+ * (degree 2) on D^-1 K.  Levels are added until a level has <= coarse_max DOFs or max_levels
+ * is reached (the coarsest must have <= 4096 DOFs).  Call after cstfem_setup; a new setup
+ * drops the hierarchy.  While a hierarchy exists, the FP64 flexible CG uses it. */
+int cstfem_mg_setup(cstfem_ctx* ctx, int max_levels, int coarse_max, int fine_smoother);
+int cstfem_mg_levels(cstfem_ctx* ctx, int* levels);
+/* DOFs, aggregates (0 on the coarsest), stored entries and the upper end of the smoothing
+ * interval (lambda_max estimate of D^-1 K; 0 where no Chebyshev smoother runs) of one level. */
+int cstfem_mg_level_info(cstfem_ctx* ctx, int level, int* n, int* nagg, int64_t* nnz,
+                         double* lam);
+/* Aggregate of every node of `level` (count = nodes of the level). */
+int cstfem_mg_download_aggregates(cstfem_ctx* ctx, int level, int* agg, int count);
+/* Coarse matrix of a level >= 1 as scalar CSR: row_ptr[n+1], cols/vals[nnz]. */
+int cstfem_mg_download_matrix(cstfem_ctx* ctx, int level, int* row_ptr, int* cols,
+                              double* vals);
+/* z = B r: one application of the preconditioner (host vectors of n doubles). */
+int cstfem_mg_apply(cstfem_ctx* ctx, const double* r, double* z, int n);
+
 /* ---- measurement (no reference counterpart) ------------------------------------------ */
 /* Average seconds of `reps` back-to-back launches of one hot kernel on the library's
  * stream, timed with HIP events; solver state is untouched.  kind: 0 fused SpMV + p'Ap,

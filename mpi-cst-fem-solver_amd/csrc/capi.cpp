@@ -802,6 +802,68 @@ int cstfem_mpnn_download(cstfem_ctx* ctx, int what, float* out, int64_t count) {
   });
 }
 
+int cstfem_mg_setup(cstfem_ctx* ctx, int max_levels, int coarse_max, int fine_smoother) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_mg_setup: NULL context");
+    use_device(&ctx->c);
+    mg_setup(ctx->c, max_levels, coarse_max, fine_smoother);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mg_levels(cstfem_ctx* ctx, int* levels) {
+  return guarded([&] {
+    need(ctx && levels, "cstfem_mg_levels: NULL argument");
+    *levels = mg_levels(ctx->c);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mg_level_info(cstfem_ctx* ctx, int level, int* n, int* nagg, int64_t* nnz,
+                         double* lam) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_mg_level_info: NULL context");
+    const int64_t z = mg_level_info(ctx->c, level, n, nagg, lam);
+    need(z >= 0, "cstfem_mg_level_info: no level " + std::to_string(level));
+    if (nnz) *nnz = z;
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mg_download_aggregates(cstfem_ctx* ctx, int level, int* agg, int count) {
+  return guarded([&] {
+    need(ctx && agg, "cstfem_mg_download_aggregates: NULL argument");
+    mg_download_agg(ctx->c, level, agg, count);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mg_download_matrix(cstfem_ctx* ctx, int level, int* row_ptr, int* cols, double* vals) {
+  return guarded([&] {
+    need(ctx && row_ptr && cols && vals, "cstfem_mg_download_matrix: NULL argument");
+    use_device(&ctx->c);
+    mg_download_matrix(ctx->c, level, row_ptr, cols, vals);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mg_apply(cstfem_ctx* ctx, const double* r, double* z, int n) {
+  return guarded([&] {
+    need(ctx && r && z, "cstfem_mg_apply: NULL argument");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(n == c.n, "cstfem_mg_apply: size mismatch");
+    DBuf<double> dr, dz;
+    dr.alloc(n);
+    dz.alloc(n);
+    HIP_CHECK(hipMemcpyAsync(dr.get(), r, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    mg_enqueue_apply(c, dr.get(), dz.get(), nullptr);
+    HIP_CHECK(hipMemcpyAsync(z, dz.get(), sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    return CSTFEM_OK;
+  });
+}
+
 int cstfem_profile_kernel(cstfem_ctx* ctx, int kind, int reps, int64_t bytes, double* avg_seconds) {
   return guarded([&] {
     need(ctx && avg_seconds, "cstfem_profile_kernel: NULL argument");

@@ -122,6 +122,48 @@ struct Mpnn {
   DBuf<float> R, E;
 };
 
+// One level of the aggregation multilevel preconditioner (mg.hip).  Level 0 is the context's
+// matrix (2x2-block SELL); levels >= 1 hold scalar SELL-64 matrices with 3 DOFs per node
+// (the rigid-body modes of one aggregate of the finer level).
+struct MgLevel {
+  int n = 0;      // DOFs
+  int nodes = 0;  // nodes = n / bs
+  int bs = 2;
+  // block pattern over the nodes (host; ascending columns), device copy for levels >= 1
+  std::vector<int> bptr, bcol;
+  DBuf<int> d_bptr, d_bcol;
+  // scalar SELL-64 matrix (levels >= 1): rows contiguous, slice s = rows 64s..64s+63
+  int nslices = 0;
+  std::vector<int64_t> h_off;
+  DBuf<int64_t> off;
+  DBuf<int> col;
+  DBuf<double> val;
+  DBuf<double> dinv;
+  double lam = 0.0;  // estimate of lambda_max(D^-1 A)
+  // restriction / prolongation to level + 1 (absent on the coarsest level)
+  int nagg = 0;
+  std::vector<int> h_agg;
+  DBuf<int> agg, mptr, mem;  // aggregate of every node; member nodes of every aggregate
+  DBuf<double> P;            // n x 3: rows of the tentative prolongator
+  // work vectors
+  DBuf<double> rhs, x, x2, d, res, kx, kres, kp, kq, kz;
+  DBuf<double> part;  // reduction partials of the K-cycle steps
+};
+
+struct Multigrid {
+  bool ready = false;
+  int fine_smoother = 0;  // 0: learned M, 1: Chebyshev on D^-1 A
+  std::vector<MgLevel*> lv;
+  int nc = 0;             // DOFs of the coarsest level (dense inverse)
+  DBuf<double> cinv;
+  ~Multigrid() { clear(); }
+  void clear() {
+    for (MgLevel* l : lv) delete l;
+    lv.clear();
+    ready = false;
+  }
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -183,6 +225,7 @@ struct Context {
   // constrained DOFs (union over apply_dirichlet calls since the last assembly)
   DBuf<unsigned char> fixed;
   Mpnn mpnn;
+  Multigrid mg;
 
   // reduction scratch
   DBuf<double> partials;   // 4 * max_blocks
@@ -220,6 +263,15 @@ int solve_fcg_learned(Context& c, const cstfem_solver_options& o, cstfem_solver_
                       bool fp32);
 void ensure_vectors(Context& c);
 void ensure_vals32(Context& c);
+// multilevel preconditioner (mg.hip)
+void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother);
+// z = B r on the library stream (level-0 vectors of 2*nb doubles); kernels return at once
+// when sc is given and its status is no longer running
+void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* sc);
+int64_t mg_level_info(Context& c, int level, int* n, int* nagg, double* lam);
+int mg_levels(Context& c);
+void mg_download_agg(Context& c, int level, int* out, int count);
+void mg_download_matrix(Context& c, int level, int* row_ptr, int* cols, double* vals);
 int mpnn_num_params();
 void mpnn_init(Context& c, int seed);
 void mpnn_set_params(Context& c, const float* theta, int n);

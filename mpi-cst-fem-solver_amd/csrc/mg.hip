@@ -1,0 +1,1132 @@
+// mg.hip -- aggregation multilevel preconditioner for the flexible CG.
+//
+// Hierarchy (set up once per matrix):
+//   aggregation   greedy over the node graph of each level, in node order: a node whose
+//                 neighbours are all free becomes a root and takes its whole neighbourhood;
+//                 leftovers join the aggregate of their first aggregated neighbour (column
+//                 order); nodes with none start their own.
+//   prolongator   unsmoothed: on each aggregate, the near-null space B restricted to its DOFs
+//                 is orthonormalised by modified Gram-Schmidt, P = Q (3 columns per
+//                 aggregate), and the coarse near-null space is R.  Level 0's B holds the three
+//                 global rigid-body modes (ux, uy, rotation about the centre of the mesh's
+//                 bounding box) with the rows of constrained DOFs zeroed; a column that vanishes (rank < 3) is dropped
+//                 as a dead coarse DOF (zero row and column of the coarse matrix).
+//   Galerkin      A_{l+1} = P^T A_l P on the device: one thread per coarse 3x3 block sums the
+//                 contributions of its aggregate's fine blocks in a fixed order (no atomics,
+//                 reproducible to the bit).
+//   coarsest      dense inverse (Cholesky on the host) applied as a matrix-vector product.
+// Cycle applied to r (all kernels on the library stream, no host synchronisation, so the
+// flexible CG captures it in its iteration graph):
+//   level 0:   z = S(r) [learned M, or 2 Chebyshev steps on D^-1 A]; z += P0 K_1(P0^T (r - A z));
+//              z = z + S(r - A z)
+//   level l:   2 Chebyshev steps, coarse correction by the K-cycle of level l+1 (two flexible
+//              CG steps preconditioned by the cycle of level l+1) or the dense solve, 2 steps.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "internal.hpp"
+
+namespace cstfem_hip {
+
+namespace {
+
+constexpr int kT = 256;
+constexpr int kW = kT / 64;
+constexpr int kPart = 256;  // partial slots per reduction on a coarse level
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+__device__ inline bool stopped(const PcgScalars* sc) { return sc && sc->status != kRunning; }
+
+__device__ inline double block_sum(double v, double* lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < kW; ++i) s += lds[i];
+  return s;
+}
+
+__device__ inline double sum_parts(const double* __restrict__ p, int n, double* lds) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += kT) s += p[i];
+  return block_sum(s, lds);
+}
+
+// ---- level 0 (2x2-block SELL, rows through row0/nrow) ----------------------------------
+
+struct Mat0 {
+  const int64_t* off;
+  const int* col;
+  const double* val;
+  const int* row0;
+  const int* nrow;
+  int nslices;
+};
+
+struct MatM {  // learned M: FP32 values in the same layout
+  const int64_t* off;
+  const int* col;
+  const float* val;
+  const int* row0;
+  const int* nrow;
+  int nslices;
+};
+
+__device__ inline void row0_spmv(const Mat0& A, const double2* __restrict__ x, int s, int lane,
+                                 double& y0, double& y1) {
+  const int64_t o = A.off[s];
+  const int w = (int)((A.off[s + 1] - o) >> 6);
+  const int* c = A.col + o + lane;
+  const v2d* v = reinterpret_cast<const v2d*>(A.val) + 2 * (o + lane);
+  double a0 = 0.0, a1 = 0.0;
+  for (int k = 0; k < w; ++k) {
+    const int m = c[(size_t)k * 64];
+    const v2d va = __builtin_nontemporal_load(v + (size_t)k * 128);
+    const v2d vb = __builtin_nontemporal_load(v + (size_t)k * 128 + 1);
+    const double2 xm = x[m];
+    a0 = a0 + va.x * xm.x;
+    a0 = a0 + va.y * xm.y;
+    a1 = a1 + vb.x * xm.x;
+    a1 = a1 + vb.y * xm.y;
+  }
+  y0 = a0;
+  y1 = a1;
+}
+
+__device__ inline void rowM_spmv(const MatM& A, const double2* __restrict__ x, int s, int lane,
+                                 double& y0, double& y1) {
+  const int64_t o = A.off[s];
+  const int w = (int)((A.off[s + 1] - o) >> 6);
+  const int* c = A.col + o + lane;
+  const float4* v = reinterpret_cast<const float4*>(A.val) + (o + lane);
+  double a0 = 0.0, a1 = 0.0;
+  for (int k = 0; k < w; ++k) {
+    const int m = c[(size_t)k * 64];
+    const float4 vv = v[(size_t)k * 64];
+    const double2 xm = x[m];
+    a0 = a0 + (double)vv.x * xm.x;
+    a0 = a0 + (double)vv.y * xm.y;
+    a1 = a1 + (double)vv.z * xm.x;
+    a1 = a1 + (double)vv.w * xm.y;
+  }
+  y0 = a0;
+  y1 = a1;
+}
+
+// res = r - A x
+__global__ __launch_bounds__(kT) void k0_resid(Mat0 A, const double2* __restrict__ x,
+                                               const double2* __restrict__ r,
+                                               double2* __restrict__ res,
+                                               const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  const int lane = threadIdx.x & 63;
+  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < A.nslices; s += gridDim.x * kW) {
+    double y0, y1;
+    row0_spmv(A, x, s, lane, y0, y1);
+    if (lane < A.nrow[s]) {
+      const int i = A.row0[s] + lane;
+      const double2 ri = r[i];
+      res[i] = make_double2(ri.x - y0, ri.y - y1);
+    }
+  }
+}
+
+// out = base + M in   (base == nullptr: out = M in)
+__global__ __launch_bounds__(kT) void k0_msmooth(MatM M, const double2* __restrict__ in,
+                                                 const double2* base, double2* out,
+                                                 const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  const int lane = threadIdx.x & 63;
+  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < M.nslices; s += gridDim.x * kW) {
+    double y0, y1;
+    rowM_spmv(M, in, s, lane, y0, y1);
+    if (lane < M.nrow[s]) {
+      const int i = M.row0[s] + lane;
+      double2 o = make_double2(y0, y1);
+      if (base) {
+        const double2 b = base[i];
+        o.x = b.x + o.x;
+        o.y = b.y + o.y;
+      }
+      out[i] = o;
+    }
+  }
+}
+
+// one Chebyshev step on D^-1 A: res = r - A xin (xin == nullptr: 0), d = cd d + cr D^-1 res,
+// xout = xin + d
+__global__ __launch_bounds__(kT) void k0_cheb(Mat0 A, const double2* __restrict__ xin,
+                                              double2* __restrict__ xout, double2* __restrict__ d,
+                                              const double2* __restrict__ r,
+                                              const double2* __restrict__ dinv, double cd,
+                                              double cr, const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  const int lane = threadIdx.x & 63;
+  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < A.nslices; s += gridDim.x * kW) {
+    double y0 = 0.0, y1 = 0.0;
+    if (xin) row0_spmv(A, xin, s, lane, y0, y1);
+    if (lane < A.nrow[s]) {
+      const int i = A.row0[s] + lane;
+      const double2 ri = r[i], di = dinv[i];
+      double2 dd = cd != 0.0 ? d[i] : make_double2(0.0, 0.0);
+      dd.x = cd * dd.x + cr * (di.x * (ri.x - y0));
+      dd.y = cd * dd.y + cr * (di.y * (ri.y - y1));
+      d[i] = dd;
+      const double2 xo = xin ? xin[i] : make_double2(0.0, 0.0);
+      xout[i] = make_double2(xo.x + dd.x, xo.y + dd.y);
+    }
+  }
+}
+
+// ---- levels >= 1: scalar SELL-64 ---------------------------------------------------------
+
+struct MatS {
+  const int64_t* off;
+  const int* col;
+  const double* val;
+  int n;
+};
+
+__device__ inline double rowS(const MatS& A, const double* __restrict__ x, int i) {
+  const int s = i >> 6, lane = i & 63;
+  const int64_t o = A.off[s];
+  const int w = (int)((A.off[s + 1] - o) >> 6);
+  double a = 0.0;
+  for (int k = 0; k < w; ++k) {
+    const int64_t q = o + (int64_t)k * 64 + lane;
+    a = a + A.val[q] * x[A.col[q]];
+  }
+  return a;
+}
+
+__global__ __launch_bounds__(kT) void ks_cheb(MatS A, const double* __restrict__ xin,
+                                              double* __restrict__ xout, double* __restrict__ d,
+                                              const double* __restrict__ r,
+                                              const double* __restrict__ dinv, double cd, double cr,
+                                              const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  const int i = blockIdx.x * kT + threadIdx.x;
+  if (i >= A.n) return;
+  const double y = xin ? rowS(A, xin, i) : 0.0;
+  const double dd = (cd != 0.0 ? cd * d[i] : 0.0) + cr * (dinv[i] * (r[i] - y));
+  d[i] = dd;
+  xout[i] = (xin ? xin[i] : 0.0) + dd;
+}
+
+__global__ __launch_bounds__(kT) void ks_resid(MatS A, const double* __restrict__ x,
+                                               const double* __restrict__ r,
+                                               double* __restrict__ res,
+                                               const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  const int i = blockIdx.x * kT + threadIdx.x;
+  if (i >= A.n) return;
+  res[i] = r[i] - rowS(A, x, i);
+}
+
+// K-cycle step: beta = first ? 0 : -z'q_old / p'q_old; q = A z + beta q; p = z + beta p;
+// partials p'q -> part[b], p'res -> part[kPart + b]
+__global__ __launch_bounds__(kT) void ks_spmv_dir(MatS A, const double* __restrict__ z,
+                                                  double* __restrict__ p, double* __restrict__ q,
+                                                  const double* __restrict__ res,
+                                                  double* __restrict__ part,
+                                                  const double* __restrict__ part_zq,
+                                                  const double* __restrict__ part_pq, int np,
+                                                  int first, const PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kW];
+  if (stopped(sc)) return;
+  double beta = 0.0;
+  if (!first) {
+    const double zq = sum_parts(part_zq, np, lds);
+    const double pq = sum_parts(part_pq, np, lds);
+    beta = pq > 0.0 ? -zq / pq : 0.0;
+  }
+  double a = 0.0, b = 0.0;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < A.n; i += gridDim.x * kT) {
+    const double y = rowS(A, z, i);
+    const double qi = first ? y : y + beta * q[i];
+    const double pi = first ? z[i] : z[i] + beta * p[i];
+    q[i] = qi;
+    p[i] = pi;
+    a += pi * qi;
+    b += pi * res[i];
+  }
+  const double sa = block_sum(a, lds);
+  const double sb = block_sum(b, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = sa;
+    part[kPart + blockIdx.x] = sb;
+  }
+}
+
+// alpha = p'res / p'q (0 unless p'q > 0); x += alpha p; res -= alpha q (if upd_res)
+__global__ __launch_bounds__(kT) void ks_update(int n, double* __restrict__ x,
+                                                double* __restrict__ res,
+                                                const double* __restrict__ p,
+                                                const double* __restrict__ q,
+                                                const double* __restrict__ part, int np,
+                                                int first, int upd_res,
+                                                const PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kW];
+  if (stopped(sc)) return;
+  const double pq = sum_parts(part, np, lds);
+  const double pr = sum_parts(part + kPart, np, lds);
+  const double alpha = pq > 0.0 ? pr / pq : 0.0;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
+    x[i] = (first ? 0.0 : x[i]) + alpha * p[i];
+    if (upd_res) res[i] = res[i] - alpha * q[i];
+  }
+}
+
+__global__ __launch_bounds__(kT) void ks_dot(int n, const double* __restrict__ a,
+                                             const double* __restrict__ b,
+                                             double* __restrict__ part,
+                                             const PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kW];
+  if (stopped(sc)) return;
+  double v = 0.0;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) v += a[i] * b[i];
+  const double s = block_sum(v, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kT) void ks_copy(int n, const double* __restrict__ a,
+                                              double* __restrict__ b,
+                                              const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) b[i] = a[i];
+}
+
+// rc[3a + c] = sum over member nodes i of a, k < bs:  P[(bs i + k) * 3 + c] * res[bs i + k]
+__global__ __launch_bounds__(kT) void k_restrict(int nagg, int bs, const int* __restrict__ mptr,
+                                                 const int* __restrict__ mem,
+                                                 const double* __restrict__ P,
+                                                 const double* __restrict__ res,
+                                                 double* __restrict__ rc,
+                                                 const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  const int t = blockIdx.x * kT + threadIdx.x;
+  if (t >= 3 * nagg) return;
+  const int a = t / 3, c = t % 3;
+  double s = 0.0;
+  for (int m = mptr[a]; m < mptr[a + 1]; ++m) {
+    const int i = mem[m];
+    for (int k = 0; k < bs; ++k) {
+      const int f = bs * i + k;
+      s = s + P[(size_t)f * 3 + c] * res[f];
+    }
+  }
+  rc[t] = s;
+}
+
+// x[f] += sum_c P[f * 3 + c] * ec[3 agg(node of f) + c]
+__global__ __launch_bounds__(kT) void k_prolong(int n, int bs, const int* __restrict__ agg,
+                                                const double* __restrict__ P,
+                                                const double* __restrict__ ec,
+                                                double* __restrict__ x,
+                                                const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  const int f = blockIdx.x * kT + threadIdx.x;
+  if (f >= n) return;
+  const int a = agg[f / bs];
+  const double* p = P + (size_t)f * 3;
+  const double* e = ec + 3 * (size_t)a;
+  x[f] = x[f] + (p[0] * e[0] + p[1] * e[1] + p[2] * e[2]);
+}
+
+// x = Cinv r (dense, row-major n x n)
+__global__ __launch_bounds__(kT) void k_dense(int n, const double* __restrict__ Cinv,
+                                              const double* __restrict__ r, double* __restrict__ x,
+                                              const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  const int i = blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const double* row = Cinv + (size_t)i * n;
+  double s = 0.0;
+  for (int j = 0; j < n; ++j) s = s + row[j] * r[j];
+  x[i] = s;
+}
+
+// ---- setup kernels -------------------------------------------------------------------
+
+// Galerkin product from level 0 (2x2 blocks): thread t = coarse block (crow[t], ccol[t]) at
+// position t - cptr[crow[t]] of its row; sums P_i^T A_q P_j over the aggregate's member nodes i
+// (member order) and their blocks q = (i, j) (column order) with agg(j) = ccol[t].
+__global__ void k_rap0(int ncblk, const int* __restrict__ crow, const int* __restrict__ cptr,
+                       const int* __restrict__ ccol, const int* __restrict__ mptr,
+                       const int* __restrict__ mem, const int* __restrict__ agg,
+                       const int* __restrict__ brow_ptr, const int* __restrict__ bcol,
+                       const int64_t* __restrict__ spos, const double* __restrict__ fval,
+                       const double* __restrict__ P, const int64_t* __restrict__ coff,
+                       double* __restrict__ cval) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncblk) return;
+  const int a = crow[t], b = ccol[t], pos = t - cptr[a];
+  double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int m = mptr[a]; m < mptr[a + 1]; ++m) {
+    const int i = mem[m];
+    const double* Pi = P + (size_t)(2 * i) * 3;
+    for (int q = brow_ptr[i]; q < brow_ptr[i + 1]; ++q) {
+      const int j = bcol[q];
+      if (agg[j] != b) continue;
+      const double* Aq = fval + 4 * spos[q];
+      const double* Pj = P + (size_t)(2 * j) * 3;
+      // T = A_q P_j (2x3), acc += P_i^T T
+      double T[2][3];
+      for (int k = 0; k < 2; ++k)
+        for (int c = 0; c < 3; ++c) T[k][c] = Aq[2 * k] * Pj[c] + Aq[2 * k + 1] * Pj[3 + c];
+      for (int c = 0; c < 3; ++c)
+        for (int cc = 0; cc < 3; ++cc) acc[c][cc] += Pi[c] * T[0][cc] + Pi[3 + c] * T[1][cc];
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    const int row = 3 * a + c, s = row >> 6, lane = row & 63;
+    for (int cc = 0; cc < 3; ++cc) cval[coff[s] + (int64_t)(3 * pos + cc) * 64 + lane] = acc[c][cc];
+  }
+}
+
+// Galerkin product from a level >= 1 (3x3 blocks in scalar SELL; block q of fine row i sits at
+// position q - fptr[i] of that row)
+__global__ void k_rapS(int ncblk, const int* __restrict__ crow, const int* __restrict__ cptr,
+                       const int* __restrict__ ccol, const int* __restrict__ mptr,
+                       const int* __restrict__ mem, const int* __restrict__ agg,
+                       const int* __restrict__ fptr, const int* __restrict__ fcol,
+                       const int64_t* __restrict__ foff, const double* __restrict__ fval,
+                       const double* __restrict__ P, const int64_t* __restrict__ coff,
+                       double* __restrict__ cval) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncblk) return;
+  const int a = crow[t], b = ccol[t], pos = t - cptr[a];
+  double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int m = mptr[a]; m < mptr[a + 1]; ++m) {
+    const int i = mem[m];
+    const double* Pi = P + (size_t)(3 * i) * 3;
+    for (int q = fptr[i]; q < fptr[i + 1]; ++q) {
+      const int j = fcol[q];
+      if (agg[j] != b) continue;
+      const int fp = q - fptr[i];
+      double Aq[3][3];
+      for (int k = 0; k < 3; ++k) {
+        const int row = 3 * i + k, s = row >> 6, lane = row & 63;
+        for (int kk = 0; kk < 3; ++kk) Aq[k][kk] = fval[foff[s] + (int64_t)(3 * fp + kk) * 64 + lane];
+      }
+      const double* Pj = P + (size_t)(3 * j) * 3;
+      double T[3][3];
+      for (int k = 0; k < 3; ++k)
+        for (int c = 0; c < 3; ++c)
+          T[k][c] = Aq[k][0] * Pj[c] + Aq[k][1] * Pj[3 + c] + Aq[k][2] * Pj[6 + c];
+      for (int c = 0; c < 3; ++c)
+        for (int cc = 0; cc < 3; ++cc)
+          acc[c][cc] += Pi[c] * T[0][cc] + Pi[3 + c] * T[1][cc] + Pi[6 + c] * T[2][cc];
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    const int row = 3 * a + c, s = row >> 6, lane = row & 63;
+    for (int cc = 0; cc < 3; ++cc) cval[coff[s] + (int64_t)(3 * pos + cc) * 64 + lane] = acc[c][cc];
+  }
+}
+
+// diagonal of a scalar SELL matrix (padding entries carry the row's own column and zero)
+__global__ void ks_dinv(MatS A, double* __restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  const int s = i >> 6, lane = i & 63;
+  const int64_t o = A.off[s];
+  const int w = (int)((A.off[s + 1] - o) >> 6);
+  double d = 0.0;
+  for (int k = 0; k < w; ++k) {
+    const int64_t q = o + (int64_t)k * 64 + lane;
+    if (A.col[q] == i) d += A.val[q];
+  }
+  dinv[i] = d > 0.0 ? 1.0 / d : 0.0;
+}
+
+// y = D^-1 A x, partial sums of y'y
+__global__ __launch_bounds__(kT) void ks_power(MatS A, const double* __restrict__ dinv,
+                                               const double* __restrict__ x,
+                                               double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double lds[kW];
+  const int i = blockIdx.x * kT + threadIdx.x;
+  double v = 0.0;
+  if (i < A.n) {
+    v = dinv[i] * rowS(A, x, i);
+    y[i] = v;
+  }
+  const double s = block_sum(v * v, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kT) void k0_power(Mat0 A, const double2* __restrict__ dinv,
+                                               const double2* __restrict__ x,
+                                               double2* __restrict__ y, double* __restrict__ part) {
+  __shared__ double lds[kW];
+  const int lane = threadIdx.x & 63;
+  double acc = 0.0;
+  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < A.nslices; s += gridDim.x * kW) {
+    double y0, y1;
+    row0_spmv(A, x, s, lane, y0, y1);
+    if (lane < A.nrow[s]) {
+      const int i = A.row0[s] + lane;
+      const double2 di = dinv[i];
+      const double2 v = make_double2(di.x * y0, di.y * y1);
+      y[i] = v;
+      acc += v.x * v.x + v.y * v.y;
+    }
+  }
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Gershgorin bound of lambda_max(D^-1 A): per-workgroup maxima of dinv_i sum_j |a_ij|
+__global__ __launch_bounds__(kT) void ks_gersh(MatS A, const double* __restrict__ dinv,
+                                               double* __restrict__ part) {
+  __shared__ double lds[kW];
+  double m = 0.0;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < A.n; i += gridDim.x * kT) {
+    const int s = i >> 6, lane = i & 63;
+    const int64_t o = A.off[s];
+    const int w = (int)((A.off[s + 1] - o) >> 6);
+    double a = 0.0;
+    for (int k = 0; k < w; ++k) a += fabs(A.val[o + (int64_t)k * 64 + lane]);
+    m = fmax(m, dinv[i] * a);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kW; ++i) t = fmax(t, lds[i]);
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k0_gersh(Mat0 A, const double2* __restrict__ dinv,
+                                               double* __restrict__ part) {
+  __shared__ double lds[kW];
+  const int lane = threadIdx.x & 63;
+  double m = 0.0;
+  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < A.nslices; s += gridDim.x * kW) {
+    const int64_t o = A.off[s];
+    const int w = (int)((A.off[s + 1] - o) >> 6);
+    const v2d* v = reinterpret_cast<const v2d*>(A.val) + 2 * (o + lane);
+    double a0 = 0.0, a1 = 0.0;
+    for (int k = 0; k < w; ++k) {
+      const v2d va = v[(size_t)k * 128], vb = v[(size_t)k * 128 + 1];
+      a0 += fabs(va.x) + fabs(va.y);
+      a1 += fabs(vb.x) + fabs(vb.y);
+    }
+    if (lane < A.nrow[s]) {
+      const double2 di = dinv[A.row0[s] + lane];
+      m = fmax(m, fmax(di.x * a0, di.y * a1));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kW; ++i) t = fmax(t, lds[i]);
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_scale(int64_t n, double* __restrict__ x, double f) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= f;
+}
+
+// ---- host helpers ------------------------------------------------------------------------
+
+int grid1(int64_t n) { return (int)std::max<int64_t>(1, (n + kT - 1) / kT); }
+
+Mat0 mat0(Context& c) {
+  return Mat0{c.sell_off.get(), c.sell_col.get(), c.sell_val.get(), c.sell_row0.get(),
+              c.sell_nrow.get(), c.nslices};
+}
+
+MatS mats(MgLevel& L) { return MatS{L.off.get(), L.col.get(), L.val.get(), L.n}; }
+
+int slice_grid(int nslices) { return std::max(1, std::min(2048, (nslices + kW - 1) / kW)); }
+
+// Greedy aggregation of a node graph (see the file header).
+int aggregate(const std::vector<int>& ptr, const std::vector<int>& col, std::vector<int>& agg) {
+  const int n = (int)ptr.size() - 1;
+  agg.assign(n, -1);
+  int na = 0;
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    bool free_nb = true;
+    for (int q = ptr[i]; q < ptr[i + 1] && free_nb; ++q) free_nb = agg[col[q]] < 0;
+    if (!free_nb) continue;
+    for (int q = ptr[i]; q < ptr[i + 1]; ++q) agg[col[q]] = na;
+    agg[i] = na;
+    ++na;
+  }
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    for (int q = ptr[i]; q < ptr[i + 1]; ++q)
+      if (agg[col[q]] >= 0) {
+        agg[i] = agg[col[q]];
+        break;
+      }
+  }
+  for (int i = 0; i < n; ++i)
+    if (agg[i] < 0) agg[i] = na++;
+  return na;
+}
+
+// Modified Gram-Schmidt on the rows `rows` (count m) of B (3 columns): Q into P (same rows),
+// R (3x3, row-major) into Rout.  A column whose remaining norm is <= 1e-10 of its original
+// norm (or zero) is dead: zero in Q and a zero row in R.
+void mgs(const std::vector<double>& B, const int* rows, int m, std::vector<double>& P,
+         double* Rout) {
+  double q[3][64 * 3];  // scratch per column (m <= 192 rows)
+  double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  bool alive[3] = {false, false, false};
+  for (int c = 0; c < 3; ++c) {
+    double n0 = 0.0;
+    for (int k = 0; k < m; ++k) {
+      q[c][k] = B[(size_t)rows[k] * 3 + c];
+      n0 += q[c][k] * q[c][k];
+    }
+    n0 = std::sqrt(n0);
+    for (int cc = 0; cc < c; ++cc) {
+      if (!alive[cc]) continue;
+      double dot = 0.0;
+      for (int k = 0; k < m; ++k) dot += q[cc][k] * q[c][k];
+      R[cc][c] = dot;
+      for (int k = 0; k < m; ++k) q[c][k] -= dot * q[cc][k];
+    }
+    double nr = 0.0;
+    for (int k = 0; k < m; ++k) nr += q[c][k] * q[c][k];
+    nr = std::sqrt(nr);
+    if (n0 > 0.0 && nr > 1e-10 * n0) {
+      alive[c] = true;
+      R[c][c] = nr;
+      for (int k = 0; k < m; ++k) q[c][k] /= nr;
+    } else {
+      for (int cc = 0; cc < 3; ++cc) R[c][cc] = 0.0;
+      for (int k = 0; k < m; ++k) q[c][k] = 0.0;
+    }
+  }
+  for (int c = 0; c < 3; ++c)
+    if (!alive[c])
+      for (int cc = 0; cc < 3; ++cc) R[c][cc] = 0.0;
+  for (int k = 0; k < m; ++k)
+    for (int c = 0; c < 3; ++c) P[(size_t)rows[k] * 3 + c] = q[c][k];
+  for (int c = 0; c < 3; ++c)
+    for (int cc = 0; cc < 3; ++cc) Rout[3 * c + cc] = R[c][cc];
+}
+
+// scalar SELL-64 layout of a 3x3-block pattern (cptr, ccol over na block rows)
+void sell_layout(MgLevel& L, const std::vector<int>& cptr, const std::vector<int>& ccol) {
+  const int n = L.n;
+  L.nslices = (n + 63) / 64;
+  L.h_off.assign(L.nslices + 1, 0);
+  for (int s = 0; s < L.nslices; ++s) {
+    int w = 0;
+    for (int r = 64 * s; r < std::min(n, 64 * s + 64); ++r) w = std::max(w, 3 * (cptr[r / 3 + 1] - cptr[r / 3]));
+    L.h_off[s + 1] = L.h_off[s] + 64 * (int64_t)w;
+  }
+  std::vector<int> col(L.h_off[L.nslices]);
+  for (int s = 0; s < L.nslices; ++s) {
+    const int w = (int)((L.h_off[s + 1] - L.h_off[s]) / 64);
+    for (int lane = 0; lane < 64; ++lane) {
+      const int r = 64 * s + lane;
+      for (int k = 0; k < w; ++k) {
+        int cidx = r < n ? r : n - 1;  // padding: the row's own column (value 0)
+        if (r < n) {
+          const int a = r / 3, nbk = cptr[a + 1] - cptr[a];
+          if (k < 3 * nbk) cidx = 3 * ccol[cptr[a] + k / 3] + k % 3;
+        }
+        col[L.h_off[s] + (int64_t)k * 64 + lane] = cidx;
+      }
+    }
+  }
+  L.off.alloc(L.h_off.size());
+  L.col.alloc(col.size());
+  L.val.alloc(col.size());
+  HIP_CHECK(hipMemcpy(L.off.get(), L.h_off.data(), sizeof(int64_t) * L.h_off.size(), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(L.col.get(), col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemset(L.val.get(), 0, sizeof(double) * col.size()));
+}
+
+template <class T>
+void upload(DBuf<T>& d, const std::vector<T>& h) {
+  d.alloc(h.size());
+  if (!h.empty()) HIP_CHECK(hipMemcpy(d.get(), h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+}
+
+// largest eigenvalue of D^-1 A by 20 power steps from a fixed vector
+double power_max(Context& c, int level) {
+  MgLevel& L = *c.mg.lv[level];
+  const int n = L.n;
+  DBuf<double> x, y, part;
+  x.alloc(n);
+  y.alloc(n);
+  std::vector<double> h(n);
+  for (int i = 0; i < n; ++i) h[i] = 1.0 + 0.5 * std::sin(0.7 * i + 0.3);
+  HIP_CHECK(hipMemcpy(x.get(), h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  const int g = level == 0 ? slice_grid(c.nslices) : grid1(n);
+  part.alloc(g);
+  std::vector<double> hp(g);
+  double lam = 0.0, xn = 0.0;
+  for (int i = 0; i < n; ++i) xn += h[i] * h[i];
+  xn = std::sqrt(xn);
+  for (int it = 0; it < 20; ++it) {
+    if (level == 0)
+      hipLaunchKernelGGL(k0_power, dim3(g), dim3(kT), 0, c.stream, mat0(c),
+                         (const double2*)c.inv_diag.get(), (const double2*)x.get(),
+                         (double2*)y.get(), part.get());
+    else
+      hipLaunchKernelGGL(ks_power, dim3(g), dim3(kT), 0, c.stream, mats(L),
+                         (const double*)L.dinv.get(), (const double*)x.get(), y.get(), part.get());
+    HIP_CHECK(hipMemcpyAsync(hp.data(), part.get(), sizeof(double) * g, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    double yn = 0.0;
+    for (double v : hp) yn += v;
+    yn = std::sqrt(yn);
+    lam = yn / xn;
+    if (!(yn > 0.0)) break;
+    hipLaunchKernelGGL(k_scale, dim3(grid1(n)), dim3(kT), 0, c.stream, (int64_t)n, y.get(), 1.0 / yn);
+    std::swap(x.p, y.p);
+    xn = 1.0;
+  }
+  HIP_CHECK(hipGetLastError());
+  return lam;
+}
+
+// Gershgorin upper bound of lambda_max(D^-1 A) (max over rows of dinv_i sum_j |a_ij|)
+double gersh_max(Context& c, int level) {
+  MgLevel& L = *c.mg.lv[level];
+  const int g = level == 0 ? slice_grid(c.nslices) : std::min(1024, grid1(L.n));
+  DBuf<double> part;
+  part.alloc(g);
+  if (level == 0)
+    hipLaunchKernelGGL(k0_gersh, dim3(g), dim3(kT), 0, c.stream, mat0(c),
+                       (const double2*)c.inv_diag.get(), part.get());
+  else
+    hipLaunchKernelGGL(ks_gersh, dim3(g), dim3(kT), 0, c.stream, mats(L), (const double*)L.dinv.get(),
+                       part.get());
+  std::vector<double> h(g);
+  HIP_CHECK(hipMemcpyAsync(h.data(), part.get(), sizeof(double) * g, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  double m = 0.0;
+  for (double v : h) m = std::max(m, v);
+  return m;
+}
+
+// Smoothing interval upper end: the power estimate with a margin, never above the Gershgorin
+// bound (which holds for every eigenvalue)
+double smoothing_max(Context& c, int level) {
+  const double gb = gersh_max(c, level);
+  const double pw = power_max(c, level);
+  return std::min(gb, 1.3 * pw);
+}
+
+// Chebyshev coefficients of the two smoothing steps on [lam / 30, lam]
+struct Cheb {
+  double c1r, c2d, c2r;
+};
+Cheb cheb(double lam) {
+  const double hi = lam, lo = hi / 30.0;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+  const double sigma = theta / delta, rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
+  return Cheb{1.0 / theta, rho1 * rho0, 2.0 * rho1 / delta};
+}
+
+// dense inverse of the coarsest matrix (dead DOFs: identity rows, zero solution)
+void coarsest_inverse(Context& c, MgLevel& L) {
+  const int n = L.n;
+  std::vector<double> val(L.val.n);
+  std::vector<int> col(L.val.n);
+  HIP_CHECK(hipMemcpy(val.data(), L.val.get(), sizeof(double) * val.size(), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(col.data(), L.col.get(), sizeof(int) * col.size(), hipMemcpyDeviceToHost));
+  std::vector<double> A((size_t)n * n, 0.0);
+  for (int s = 0; s < L.nslices; ++s) {
+    const int w = (int)((L.h_off[s + 1] - L.h_off[s]) / 64);
+    for (int lane = 0; lane < 64; ++lane) {
+      const int r = 64 * s + lane;
+      if (r >= n) continue;
+      for (int k = 0; k < w; ++k) {
+        const int64_t q = L.h_off[s] + (int64_t)k * 64 + lane;
+        A[(size_t)r * n + col[q]] += val[q];
+      }
+    }
+  }
+  std::vector<char> dead(n, 0);
+  for (int i = 0; i < n; ++i)
+    if (!(A[(size_t)i * n + i] > 0.0)) {
+      dead[i] = 1;
+      for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = A[(size_t)j * n + i] = 0.0;
+      A[(size_t)i * n + i] = 1.0;
+    }
+  // Cholesky A = L L^T (lower, in place), then the inverse column by column
+  for (int j = 0; j < n; ++j) {
+    double d = A[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(d > 0.0)) throw Error(CSTFEM_ERR_BAD_DIAGONAL, "multilevel: coarsest matrix is not positive definite");
+    const double ljj = std::sqrt(d);
+    A[(size_t)j * n + j] = ljj;
+    for (int i = j + 1; i < n; ++i) {
+      double s = A[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = s / ljj;
+    }
+  }
+  std::vector<double> inv((size_t)n * n, 0.0), e(n);
+  for (int col_i = 0; col_i < n; ++col_i) {
+    std::fill(e.begin(), e.end(), 0.0);
+    if (!dead[col_i]) e[col_i] = 1.0;
+    for (int i = 0; i < n; ++i) {  // L y = e
+      double s = e[i];
+      for (int k = 0; k < i; ++k) s -= A[(size_t)i * n + k] * e[k];
+      e[i] = s / A[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {  // L^T x = y
+      double s = e[i];
+      for (int k = i + 1; k < n; ++k) s -= A[(size_t)k * n + i] * e[k];
+      e[i] = s / A[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + col_i] = dead[i] ? 0.0 : e[i];
+  }
+  c.mg.nc = n;
+  upload(c.mg.cinv, inv);
+}
+
+void alloc_work(MgLevel& L) {
+  for (DBuf<double>* v : {&L.rhs, &L.x, &L.x2, &L.d, &L.res, &L.kx, &L.kres, &L.kp, &L.kq, &L.kz}) {
+    v->alloc(std::max(1, L.n));
+    HIP_CHECK(hipMemset(v->get(), 0, sizeof(double) * std::max(1, L.n)));
+  }
+  L.part.alloc(6 * kPart);
+}
+
+// ---- the cycle -----------------------------------------------------------------------------
+
+void kcycle(Context& c, int l, const PcgScalars* sc);
+
+// coarse correction of level l: lv[l+1].kx from lv[l+1].rhs
+void coarse_correction(Context& c, int l, const PcgScalars* sc) {
+  Multigrid& mg = c.mg;
+  MgLevel& C = *mg.lv[l + 1];
+  if (l + 1 == (int)mg.lv.size() - 1)
+    hipLaunchKernelGGL(k_dense, dim3(grid1(C.n)), dim3(kT), 0, c.stream, C.n,
+                       (const double*)mg.cinv.get(), (const double*)C.rhs.get(), C.kx.get(), sc);
+  else
+    kcycle(c, l + 1, sc);
+}
+
+// cycle on level l >= 1: out = B_l(rhs)
+void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars* sc) {
+  MgLevel& L = *c.mg.lv[l];
+  MgLevel& C = *c.mg.lv[l + 1];
+  const Cheb ch = cheb(L.lam);
+  const MatS A = mats(L);
+  const int g = grid1(L.n);
+  hipLaunchKernelGGL(ks_cheb, dim3(g), dim3(kT), 0, c.stream, A, (const double*)nullptr, L.x.get(),
+                     L.d.get(), rhs, (const double*)L.dinv.get(), 0.0, ch.c1r, sc);
+  hipLaunchKernelGGL(ks_cheb, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.x.get(), L.x2.get(),
+                     L.d.get(), rhs, (const double*)L.dinv.get(), ch.c2d, ch.c2r, sc);
+  hipLaunchKernelGGL(ks_resid, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.x2.get(), rhs,
+                     L.res.get(), sc);
+  hipLaunchKernelGGL(k_restrict, dim3(grid1(3 * L.nagg)), dim3(kT), 0, c.stream, L.nagg, L.bs,
+                     (const int*)L.mptr.get(), (const int*)L.mem.get(), (const double*)L.P.get(),
+                     (const double*)L.res.get(), C.rhs.get(), sc);
+  coarse_correction(c, l, sc);
+  hipLaunchKernelGGL(k_prolong, dim3(g), dim3(kT), 0, c.stream, L.n, L.bs, (const int*)L.agg.get(),
+                     (const double*)L.P.get(), (const double*)C.kx.get(), L.x2.get(), sc);
+  hipLaunchKernelGGL(ks_cheb, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.x2.get(), L.x.get(),
+                     L.d.get(), rhs, (const double*)L.dinv.get(), 0.0, ch.c1r, sc);
+  hipLaunchKernelGGL(ks_cheb, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.x.get(), out,
+                     L.d.get(), rhs, (const double*)L.dinv.get(), ch.c2d, ch.c2r, sc);
+}
+
+// two flexible CG steps on level l for rhs = lv[l].rhs, result in lv[l].kx
+void kcycle(Context& c, int l, const PcgScalars* sc) {
+  MgLevel& L = *c.mg.lv[l];
+  const MatS A = mats(L);
+  const int g = std::min(kPart, grid1(L.n));  // grid-stride kernels with <= kPart partials
+  double* s0 = L.part.get();             // p'q, p'r of step 1
+  double* s1 = L.part.get() + 2 * kPart;  // z'q_old
+  double* s2 = L.part.get() + 4 * kPart;  // p'q, p'r of step 2
+  hipLaunchKernelGGL(ks_copy, dim3(g), dim3(kT), 0, c.stream, L.n, (const double*)L.rhs.get(),
+                     L.kres.get(), sc);
+  cycle_s(c, l, L.kres.get(), L.kz.get(), sc);
+  hipLaunchKernelGGL(ks_spmv_dir, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.kz.get(),
+                     L.kp.get(), L.kq.get(), (const double*)L.kres.get(), s0, (const double*)nullptr,
+                     (const double*)nullptr, g, 1, sc);
+  hipLaunchKernelGGL(ks_update, dim3(g), dim3(kT), 0, c.stream, L.n, L.kx.get(), L.kres.get(),
+                     (const double*)L.kp.get(), (const double*)L.kq.get(), (const double*)s0, g, 1, 1,
+                     sc);
+  cycle_s(c, l, L.kres.get(), L.kz.get(), sc);
+  hipLaunchKernelGGL(ks_dot, dim3(g), dim3(kT), 0, c.stream, L.n, (const double*)L.kz.get(),
+                     (const double*)L.kq.get(), s1, sc);
+  hipLaunchKernelGGL(ks_spmv_dir, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.kz.get(),
+                     L.kp.get(), L.kq.get(), (const double*)L.kres.get(), s2, (const double*)s1,
+                     (const double*)s0, g, 0, sc);
+  hipLaunchKernelGGL(ks_update, dim3(g), dim3(kT), 0, c.stream, L.n, L.kx.get(), L.kres.get(),
+                     (const double*)L.kp.get(), (const double*)L.kq.get(), (const double*)s2, g, 0, 0,
+                     sc);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+
+void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* sc) {
+  Multigrid& mg = c.mg;
+  if (!mg.ready) throw Error(CSTFEM_ERR_ORDER, "multilevel: not set up");
+  MgLevel& L = *mg.lv[0];
+  MgLevel& C = *mg.lv[1];
+  const Mat0 A = mat0(c);
+  const int gs = slice_grid(c.nslices);
+  const double2* r2 = (const double2*)r;
+  double2* z2 = (double2*)z;
+  const bool learned = mg.fine_smoother == 0;
+  if (learned && !c.mpnn.built) throw Error(CSTFEM_ERR_ORDER, "multilevel: the learned smoother is not built");
+  const MatM M{c.sell_off.get(), c.sell_col.get(), c.mpnn.gsell.get(), c.sell_row0.get(),
+               c.sell_nrow.get(), c.nslices};
+  const Cheb ch = cheb(L.lam);
+  // pre-smoothing
+  if (learned) {
+    hipLaunchKernelGGL(k0_msmooth, dim3(gs), dim3(kT), 0, c.stream, M, r2, (const double2*)nullptr,
+                       z2, sc);
+  } else {
+    hipLaunchKernelGGL(k0_cheb, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)nullptr,
+                       (double2*)L.x.get(), (double2*)L.d.get(), r2, (const double2*)c.inv_diag.get(),
+                       0.0, ch.c1r, sc);
+    hipLaunchKernelGGL(k0_cheb, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)L.x.get(), z2,
+                       (double2*)L.d.get(), r2, (const double2*)c.inv_diag.get(), ch.c2d, ch.c2r, sc);
+  }
+  hipLaunchKernelGGL(k0_resid, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)z2, r2,
+                     (double2*)L.res.get(), sc);
+  hipLaunchKernelGGL(k_restrict, dim3(grid1(3 * L.nagg)), dim3(kT), 0, c.stream, L.nagg, 2,
+                     (const int*)L.mptr.get(), (const int*)L.mem.get(), (const double*)L.P.get(),
+                     (const double*)L.res.get(), C.rhs.get(), sc);
+  coarse_correction(c, 0, sc);
+  hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n, 2,
+                     (const int*)L.agg.get(), (const double*)L.P.get(), (const double*)C.kx.get(), z,
+                     sc);
+  // post-smoothing
+  if (learned) {
+    hipLaunchKernelGGL(k0_resid, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)z2, r2,
+                       (double2*)L.res.get(), sc);
+    hipLaunchKernelGGL(k0_msmooth, dim3(gs), dim3(kT), 0, c.stream, M, (const double2*)L.res.get(),
+                       (const double2*)z2, z2, sc);
+  } else {
+    hipLaunchKernelGGL(k0_cheb, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)z2,
+                       (double2*)L.x.get(), (double2*)L.d.get(), r2, (const double2*)c.inv_diag.get(),
+                       0.0, ch.c1r, sc);
+    hipLaunchKernelGGL(k0_cheb, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)L.x.get(), z2,
+                       (double2*)L.d.get(), r2, (const double2*)c.inv_diag.get(), ch.c2d, ch.c2r, sc);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother) {
+  if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "multilevel: call cstfem_setup first");
+  if (!c.mesh_pattern || c.n != 2 * c.nb)
+    throw Error(CSTFEM_ERR_ORDER, "multilevel: needs an assembled mesh system");
+  if (max_levels < 2 || max_levels > 16) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: max_levels must be 2..16");
+  if (coarse_max < 3 || coarse_max > 4096) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: coarse_max must be 3..4096");
+  if (fine_smoother != 0 && fine_smoother != 1) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: fine_smoother must be 0 or 1");
+  Multigrid& mg = c.mg;
+  mg.clear();
+  mg.fine_smoother = fine_smoother;
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  // level 0: the context's matrix and block pattern
+  MgLevel* L0 = new MgLevel;
+  mg.lv.push_back(L0);
+  L0->n = c.n;
+  L0->nodes = c.nb;
+  L0->bs = 2;
+  L0->bptr.resize(c.nb + 1);
+  L0->bcol.resize(c.nblk);
+  HIP_CHECK(hipMemcpy(L0->bptr.data(), c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(L0->bcol.data(), c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost));
+  std::vector<unsigned char> fixed(c.n, 0);
+  if (c.fixed.n == (size_t)c.n)
+    HIP_CHECK(hipMemcpy(fixed.data(), c.fixed.get(), c.n, hipMemcpyDeviceToHost));
+  alloc_work(*L0);
+  if (fine_smoother == 1) L0->lam = smoothing_max(c, 0);
+  // near-null space of level 0 is built per aggregate (centroid-relative rotation)
+  std::vector<double> B;
+  for (int l = 0;; ++l) {
+    MgLevel& L = *mg.lv[l];
+    const bool last = l + 1 >= max_levels - 1;
+    // aggregation and members (stable counting sort: ascending node order per aggregate)
+    L.nagg = aggregate(L.bptr, L.bcol, L.h_agg);
+    const int na = L.nagg;
+    std::vector<int> mptr(na + 1, 0), mem(L.nodes);
+    for (int i = 0; i < L.nodes; ++i) ++mptr[L.h_agg[i] + 1];
+    for (int a = 0; a < na; ++a) mptr[a + 1] += mptr[a];
+    {
+      std::vector<int> fill(mptr.begin(), mptr.end() - 1);
+      for (int i = 0; i < L.nodes; ++i) mem[fill[L.h_agg[i]]++] = i;
+    }
+    // near-null space rows of this level: the global rigid-body modes (rotation about the
+    // centre of the mesh's bounding box), so that every level's R carries the same modes
+    if (l == 0) {
+      B.assign((size_t)L.n * 3, 0.0);
+      double xlo = c.h_xy[0], xhi = c.h_xy[0], ylo = c.h_xy[1], yhi = c.h_xy[1];
+      for (int i = 0; i < L.nodes; ++i) {
+        xlo = std::min(xlo, c.h_xy[2 * (size_t)i]);
+        xhi = std::max(xhi, c.h_xy[2 * (size_t)i]);
+        ylo = std::min(ylo, c.h_xy[2 * (size_t)i + 1]);
+        yhi = std::max(yhi, c.h_xy[2 * (size_t)i + 1]);
+      }
+      const double cx = 0.5 * (xlo + xhi), cy = 0.5 * (ylo + yhi);
+      for (int i = 0; i < L.nodes; ++i) {
+        const double dx = c.h_xy[2 * (size_t)i] - cx, dy = c.h_xy[2 * (size_t)i + 1] - cy;
+        if (!fixed[2 * i]) {
+          B[(size_t)(2 * i) * 3 + 0] = 1.0;
+          B[(size_t)(2 * i) * 3 + 2] = -dy;
+        }
+        if (!fixed[2 * i + 1]) {
+          B[(size_t)(2 * i + 1) * 3 + 1] = 1.0;
+          B[(size_t)(2 * i + 1) * 3 + 2] = dx;
+        }
+      }
+    }
+    // tentative prolongator and the next near-null space
+    std::vector<double> P((size_t)L.n * 3, 0.0), Bn((size_t)na * 9, 0.0);
+    {
+      std::vector<int> rows;
+      for (int a = 0; a < na; ++a) {
+        rows.clear();
+        for (int m = mptr[a]; m < mptr[a + 1]; ++m)
+          for (int k = 0; k < L.bs; ++k) rows.push_back(L.bs * mem[m] + k);
+        if ((int)rows.size() > 192) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: aggregate too large");
+        mgs(B, rows.data(), (int)rows.size(), P, &Bn[(size_t)a * 9]);
+      }
+    }
+    // coarse block pattern
+    MgLevel* C = new MgLevel;
+    mg.lv.push_back(C);
+    C->nodes = na;
+    C->n = 3 * na;
+    C->bs = 3;
+    C->bptr.assign(na + 1, 0);
+    {
+      std::vector<int> mark(na, -1), row;
+      for (int a = 0; a < na; ++a) {
+        row.clear();
+        for (int m = mptr[a]; m < mptr[a + 1]; ++m) {
+          const int i = mem[m];
+          for (int q = L.bptr[i]; q < L.bptr[i + 1]; ++q) {
+            const int b = L.h_agg[L.bcol[q]];
+            if (mark[b] != a) {
+              mark[b] = a;
+              row.push_back(b);
+            }
+          }
+        }
+        std::sort(row.begin(), row.end());
+        C->bcol.insert(C->bcol.end(), row.begin(), row.end());
+        C->bptr[a + 1] = (int)C->bcol.size();
+      }
+    }
+    sell_layout(*C, C->bptr, C->bcol);
+    // device copies of the transfer and the Galerkin product
+    upload(L.agg, L.h_agg);
+    upload(L.mptr, mptr);
+    upload(L.mem, mem);
+    upload(L.P, P);
+    upload(C->d_bptr, C->bptr);
+    upload(C->d_bcol, C->bcol);
+    std::vector<int> crow(C->bcol.size());
+    for (int a = 0; a < na; ++a)
+      for (int t = C->bptr[a]; t < C->bptr[a + 1]; ++t) crow[t] = a;
+    DBuf<int> d_crow;
+    upload(d_crow, crow);
+    const int ncblk = (int)crow.size();
+    if (l == 0)
+      hipLaunchKernelGGL(k_rap0, dim3(grid1(ncblk)), dim3(kT), 0, c.stream, ncblk,
+                         (const int*)d_crow.get(), (const int*)C->d_bptr.get(),
+                         (const int*)C->d_bcol.get(), (const int*)L.mptr.get(),
+                         (const int*)L.mem.get(), (const int*)L.agg.get(), (const int*)c.brow_ptr.get(),
+                         (const int*)c.bcol.get(), (const int64_t*)c.sell_pos.get(),
+                         (const double*)c.sell_val.get(), (const double*)L.P.get(),
+                         (const int64_t*)C->off.get(), C->val.get());
+    else
+      hipLaunchKernelGGL(k_rapS, dim3(grid1(ncblk)), dim3(kT), 0, c.stream, ncblk,
+                         (const int*)d_crow.get(), (const int*)C->d_bptr.get(),
+                         (const int*)C->d_bcol.get(), (const int*)L.mptr.get(),
+                         (const int*)L.mem.get(), (const int*)L.agg.get(), (const int*)L.d_bptr.get(),
+                         (const int*)L.d_bcol.get(), (const int64_t*)L.off.get(),
+                         (const double*)L.val.get(), (const double*)L.P.get(),
+                         (const int64_t*)C->off.get(), C->val.get());
+    HIP_CHECK(hipGetLastError());
+    C->dinv.alloc(C->n);
+    hipLaunchKernelGGL(ks_dinv, dim3(grid1(C->n)), dim3(kT), 0, c.stream, mats(*C), C->dinv.get());
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    alloc_work(*C);
+    B.swap(Bn);
+    if (last || C->n <= coarse_max || na == L.nodes) {
+      if (C->n > 4096)
+        throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: coarsest level has " + std::to_string(C->n) +
+                                             " DOFs (> 4096): allow more levels");
+      coarsest_inverse(c, *C);
+      break;
+    }
+    C->lam = smoothing_max(c, l + 1);
+  }
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  mg.ready = true;
+}
+
+// aggregate of every node of `level` (host copy)
+void mg_download_agg(Context& c, int level, int* out, int count) {
+  if (!c.mg.ready) throw Error(CSTFEM_ERR_ORDER, "multilevel: not set up");
+  if (level < 0 || level >= (int)c.mg.lv.size() - 1)
+    throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: no aggregation on level " + std::to_string(level));
+  MgLevel& L = *c.mg.lv[level];
+  if (count != L.nodes) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: size mismatch");
+  std::copy(L.h_agg.begin(), L.h_agg.end(), out);
+}
+
+// matrix of a level >= 1 as scalar CSR (rows 3a + c: columns 3b + c' for the blocks b of a)
+void mg_download_matrix(Context& c, int level, int* row_ptr, int* cols, double* vals) {
+  if (!c.mg.ready) throw Error(CSTFEM_ERR_ORDER, "multilevel: not set up");
+  if (level < 1 || level >= (int)c.mg.lv.size())
+    throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: level " + std::to_string(level) + " has no coarse matrix");
+  MgLevel& L = *c.mg.lv[level];
+  std::vector<double> val(L.val.n);
+  HIP_CHECK(hipMemcpy(val.data(), L.val.get(), sizeof(double) * val.size(), hipMemcpyDeviceToHost));
+  int64_t k = 0;
+  row_ptr[0] = 0;
+  for (int r = 0; r < L.n; ++r) {
+    const int a = r / 3, s = r >> 6, lane = r & 63;
+    for (int t = L.bptr[a]; t < L.bptr[a + 1]; ++t)
+      for (int cc = 0; cc < 3; ++cc) {
+        const int e = 3 * (t - L.bptr[a]) + cc;
+        cols[k] = 3 * L.bcol[t] + cc;
+        vals[k] = val[L.h_off[s] + (int64_t)e * 64 + lane];
+        ++k;
+      }
+    row_ptr[r + 1] = (int)k;
+  }
+}
+
+int64_t mg_level_info(Context& c, int level, int* n, int* nagg, double* lam) {
+  if (!c.mg.ready) throw Error(CSTFEM_ERR_ORDER, "multilevel: not set up");
+  if (level < 0 || level >= (int)c.mg.lv.size()) return -1;
+  MgLevel& L = *c.mg.lv[level];
+  if (n) *n = L.n;
+  if (nagg) *nagg = L.nagg;
+  if (lam) *lam = L.lam;
+  return level == 0 ? c.nblk * 4 : (int64_t)L.bcol.size() * 9;
+}
+
+int mg_levels(Context& c) { return c.mg.ready ? (int)c.mg.lv.size() : 0; }
+
+}  // namespace cstfem_hip

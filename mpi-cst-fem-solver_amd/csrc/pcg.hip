@@ -587,6 +587,7 @@ void setup_solver(Context& c) {
   c.setup_done = true;
   c.vals32_valid = false;
   c.mpnn.built = false;  // M depends on the values and the Jacobi scaling
+  c.mg.clear();          // so does the multilevel hierarchy
 }
 
 // FP32 copy of the matrix values for the mixed-precision paths (rounded once per matrix)
@@ -1153,6 +1154,51 @@ __global__ __launch_bounds__(kThreads) void k_fcg_precond(
   if (threadIdx.x == 0) partC[blockIdx.x] = t;
 }
 
+// the convergence / refinement / stall test of k_fcg_precond alone (one workgroup), for
+// preconditioners applied by a kernel sequence
+__global__ __launch_bounds__(kThreads) void k_fcg_test(const double* __restrict__ partB, int nB,
+                                                       int cur, PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kWaves];
+  if (sc->status != kRunning) return;
+  const double rr = reduce_partials(partB, nB, 1, lds);
+  const int k = sc->k_in[cur ^ 1] + 1;
+  const double rel = sqrt(rr) / sc->bnorm;
+  const bool better = rel < sc->best_rel[cur ^ 1];
+  const double best = better ? rel : sc->best_rel[cur ^ 1];
+  const int best_k = better ? k : sc->best_k[cur ^ 1];
+  int status = kRunning;
+  if (rel < sc->rtol || rel < sc->delta * sc->rel_ref)
+    status = kRefine;
+  else if (k >= sc->max_iter)
+    status = kMaxIter;
+  else if (!(rel == rel) || k - best_k > sc->stall_window)
+    status = kStalled;
+  if (threadIdx.x == 0) {
+    sc->k_in[cur] = k;
+    sc->rel = rel;
+    sc->best_rel[cur] = best;
+    sc->best_k[cur] = best_k;
+    if (status != kRunning) sc->status = status;
+  }
+}
+
+// partials of z'q
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_fcg_zq(int nvec2, const typename P::V* __restrict__ z,
+                                                     const typename P::V* __restrict__ q,
+                                                     double* __restrict__ partC,
+                                                     const PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kWaves];
+  if (sc->status != kRunning) return;
+  double s = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const typename P::V zi = z[i], qi = q[i];
+    s += (double)zi.x * (double)qi.x + (double)zi.y * (double)qi.y;
+  }
+  const double t = block_sum_all(s, lds);
+  if (threadIdx.x == 0) partC[blockIdx.x] = t;
+}
+
 template <class P>
 __global__ __launch_bounds__(kThreads) void k_fcg_direction(
     int nvec2, const typename P::V* __restrict__ z, typename P::V* __restrict__ p,
@@ -1246,13 +1292,25 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
   // scratch of a refinement: r64 = b - A x and A x (FP64), apart from r, p, q, z
   double* r64 = fp32 ? c.r.get() : c.work.get();
   double* Ax = fp32 ? c.work.get() : ax64.get();
+  // the multilevel preconditioner (FP64 flexible CG only) or the learned M alone
+  const bool multilevel = !fp32 && c.mg.ready;
+  const int nC = multilevel ? gv : gs;  // partials of z'q
   auto precond = [&](int cur, int test) {
-    hipLaunchKernelGGL(k_fcg_precond<P>, dim3(gs), dim3(kThreads), 0, c.stream, G, (const V*)r,
-                       (const V*)q, z, (const double*)pB, gv, pC, cur, c.scal.get(), test);
+    if (multilevel) {
+      if (test)
+        hipLaunchKernelGGL(k_fcg_test, dim3(1), dim3(kThreads), 0, c.stream, (const double*)pB, gv,
+                           cur, c.scal.get());
+      mg_enqueue_apply(c, (const double*)r, (double*)z, c.scal.get());
+      hipLaunchKernelGGL(k_fcg_zq<P>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb, (const V*)z,
+                         (const V*)q, pC, (const PcgScalars*)c.scal.get());
+    } else {
+      hipLaunchKernelGGL(k_fcg_precond<P>, dim3(gs), dim3(kThreads), 0, c.stream, G, (const V*)r,
+                         (const V*)q, z, (const double*)pB, gv, pC, cur, c.scal.get(), test);
+    }
   };
   auto direction = [&](int restart) {
     hipLaunchKernelGGL(k_fcg_direction<P>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
-                       (const V*)z, p, (const double*)pA, gs, (const double*)pC, gs,
+                       (const V*)z, p, (const double*)pA, gs, (const double*)pC, nC,
                        (const PcgScalars*)c.scal.get(), restart);
   };
   auto iteration = [&](int cur) {
@@ -1381,7 +1439,9 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
 int solve_fcg_learned(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
                       bool fp32) {
   if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: solve() called before setup()");
-  if (!c.mpnn.built)
+  const bool multilevel = !fp32 && c.mg.ready;
+  const bool need_m = !multilevel || c.mg.fine_smoother == 0;
+  if (need_m && !c.mpnn.built)
     throw Error(CSTFEM_ERR_ORDER, "fcg: build the learned preconditioner first (cstfem_mpnn_build)");
   return fp32 ? fcg_run<F32>(c, o, st, true) : fcg_run<F64>(c, o, st, false);
 }

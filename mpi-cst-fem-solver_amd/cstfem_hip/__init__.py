@@ -133,6 +133,12 @@ SIGNATURES = {
     "cstfem_mpnn_loss_grad": (ctypes.c_int, [_P, _D]),
     "cstfem_mpnn_train_step": (ctypes.c_int, [_P, ctypes.c_float, _D]),
     "cstfem_mpnn_download": (ctypes.c_int, [_P, ctypes.c_int, _F, ctypes.c_int64]),
+    "cstfem_mg_setup": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "cstfem_mg_levels": (ctypes.c_int, [_P, _I]),
+    "cstfem_mg_level_info": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _I64, _D]),
+    "cstfem_mg_download_aggregates": (ctypes.c_int, [_P, ctypes.c_int, _I, ctypes.c_int]),
+    "cstfem_mg_download_matrix": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _D]),
+    "cstfem_mg_apply": (ctypes.c_int, [_P, _D, _D, ctypes.c_int]),
 }
 
 
@@ -390,6 +396,49 @@ class Device:
         _check(lib().cstfem_mpnn_download(self._h, int(what), _ptr(out, ctypes.c_float),
                                           int(count)))
         return out
+
+    # ---- multilevel preconditioner
+    def mg_setup(self, max_levels=10, coarse_max=500, fine_smoother=0):
+        """fine_smoother: 0 the learned M (mpnn_build first), 1 Chebyshev on D^-1 K."""
+        _check(lib().cstfem_mg_setup(self._h, int(max_levels), int(coarse_max),
+                                     int(fine_smoother)))
+
+    def mg_levels(self):
+        n = ctypes.c_int()
+        _check(lib().cstfem_mg_levels(self._h, ctypes.byref(n)))
+        return n.value
+
+    def mg_level_info(self, level):
+        """(DOFs, aggregates, stored entries, smoothing lambda_max) of one level."""
+        n, na, nnz, lam = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64(), ctypes.c_double()
+        _check(lib().cstfem_mg_level_info(self._h, int(level), ctypes.byref(n), ctypes.byref(na),
+                                          ctypes.byref(nnz), ctypes.byref(lam)))
+        return n.value, na.value, nnz.value, lam.value
+
+    def mg_download_aggregates(self, level):
+        n = self.mg_level_info(level)[0]
+        nodes = n // (2 if level == 0 else 3)
+        agg = np.empty(nodes, dtype=np.int32)
+        _check(lib().cstfem_mg_download_aggregates(self._h, int(level), _ptr(agg, ctypes.c_int),
+                                                   nodes))
+        return agg
+
+    def mg_download_matrix(self, level):
+        n, _, nnz, _ = self.mg_level_info(level)
+        rp = np.empty(n + 1, dtype=np.int32)
+        cols = np.empty(nnz, dtype=np.int32)
+        vals = np.empty(nnz, dtype=np.float64)
+        _check(lib().cstfem_mg_download_matrix(self._h, int(level), _ptr(rp, ctypes.c_int),
+                                               _ptr(cols, ctypes.c_int),
+                                               _ptr(vals, ctypes.c_double)))
+        return rp, cols, vals
+
+    def mg_apply(self, r):
+        r = _f64(r)
+        z = np.empty(self.n, dtype=np.float64)
+        _check(lib().cstfem_mg_apply(self._h, _ptr(r, ctypes.c_double), _ptr(z, ctypes.c_double),
+                                     self.n))
+        return z
 
     def spmv(self, x):
         x = _f64(x)
