@@ -9,11 +9,14 @@ relative residual below 1e-10, all on the device; the mesh and material are resi
 HBM before the timed region starts.
 
 Methods: fp64 (the reference's FP64 Jacobi-CG recurrence with FP64 true-residual
-replacement), ir32 (FP32 Jacobi-CG under FP64 refinement), learned (flexible CG with the
-learned preconditioner, FP64 matrix and vectors) and learned32 (the same in FP32 with FP64
-refreshes).  The learned methods train the network once, before the timed steps, on a small
-plate of the same family (its time is reported as train_s); every timed step rebuilds M for
-the freshly assembled matrix (reported as precond_s, part of the step).
+replacement), ir32 (FP32 Jacobi-CG under FP64 refinement), learned1 (flexible CG with the
+learned single-level preconditioner, FP64 matrix and vectors), learned32 (the same in FP32 with
+FP64 refreshes), learned_mg (flexible CG with the aggregation multilevel preconditioner whose
+fine-level smoother is the learned M) and mg_cheb (the same hierarchy with a Chebyshev fine
+smoother instead of the network).  The learned methods train the network once, before the
+timed steps, on a small plate of the same family (its time is reported as train_s); every
+timed step rebuilds M and the hierarchy for the freshly assembled matrix (reported as
+precond_s, part of the step).
 
 Prints one JSON line.  The roofline object covers the dominant kernel of the best method
 (the fused FP64 SpMV of every iteration), timed with HIP events on the library stream;
@@ -93,10 +96,15 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the hot kernels (for hardware-counter profiles)")
     ap.add_argument("--reps", type=int, default=10)
-    ap.add_argument("--methods", default="fp64,learned,learned32",
+    ap.add_argument("--methods", default="fp64,learned_mg,mg_cheb,learned1",
                     help="comma list of fp64 (FP64 Jacobi-CG, FP64 true-residual refinement), "
-                         "ir32 (FP32 Jacobi-CG under FP64 refinement), learned (FP64 flexible "
-                         "CG with the learned preconditioner), learned32 (its FP32 variant)")
+                         "ir32 (FP32 Jacobi-CG under FP64 refinement), learned1 (FP64 flexible "
+                         "CG with the learned preconditioner), learned32 (its FP32 variant), "
+                         "learned_mg (flexible CG, multilevel with the learned smoother), "
+                         "mg_cheb (multilevel with a Chebyshev smoother)")
+    ap.add_argument("--mg-levels", type=int, default=12)
+    ap.add_argument("--mg-coarse", type=int, default=500)
+    ap.add_argument("--mg-kcycle", type=int, default=16, help="coarse levels on the K-cycle")
     ap.add_argument("--train-n", type=int, default=64, help="training plate nodes per side")
     ap.add_argument("--train-steps", type=int, default=300)
     args = ap.parse_args()
@@ -133,11 +141,14 @@ def main():
         return
 
     methods = {"fp64": cstfem_hip.METHOD_CG_FP64_REFINED, "ir32": cstfem_hip.METHOD_IR_FP32,
-               "learned": cstfem_hip.METHOD_FCG_LEARNED,
-               "learned32": cstfem_hip.METHOD_FCG_LEARNED_FP32}
+               "learned1": cstfem_hip.METHOD_FCG_LEARNED,
+               "learned32": cstfem_hip.METHOD_FCG_LEARNED_FP32,
+               "learned_mg": cstfem_hip.METHOD_FCG_LEARNED,
+               "mg_cheb": cstfem_hip.METHOD_FCG_LEARNED}
+    uses_net = ("learned1", "learned32", "learned_mg")
     chosen = [m for m in args.methods.split(",") if m]
     theta, train_info = None, None
-    if any(m.startswith("learned") for m in chosen):
+    if any(m in uses_net for m in chosen):
         from cstfem_hip import learned
         t0 = time.perf_counter()
         with cstfem_hip.Device(0) as tdev:
@@ -153,9 +164,12 @@ def main():
         pipeline.assemble_and_constrain(dev, prob, times)
         t0 = time.perf_counter()
         dev.setup()
-        if method.startswith("learned"):
+        if method in uses_net:
             dev.mpnn_set_params(theta)
             dev.mpnn_build()
+        if method in ("learned_mg", "mg_cheb"):
+            dev.mg_setup(args.mg_levels, args.mg_coarse, 0 if method == "learned_mg" else 1,
+                         args.mg_kcycle)
         t1 = time.perf_counter()
         _, st = dev.solve(rtol=1e-10, max_iter=args.max_iter, method=methods[method],
                           download=False)
@@ -204,8 +218,19 @@ def main():
     copy_bytes = 1 << 30
     t_copy = dev.profile_kernel(4, reps=20, nbytes=copy_bytes)
     t_spmv32 = dev.profile_kernel(5, reps=50)
+    mg_levels = None
+    if "learned_mg" in chosen or "mg_cheb" in chosen:
+        dev.setup()
+        if theta is not None:
+            dev.mpnn_set_params(theta)
+            dev.mpnn_build()
+        dev.mg_setup(args.mg_levels, args.mg_coarse, 0 if theta is not None else 1,
+                     args.mg_kcycle)
+        mg_levels = [dict(zip(("dofs", "aggregates", "entries", "lam"), dev.mg_level_info(l)))
+                     for l in range(dev.mg_levels())]
     fcg_kernels = None
     if theta is not None:
+        dev.setup()
         dev.mpnn_set_params(theta)
         dev.mpnn_build()
         fcg_kernels = {name_k: dev.profile_kernel(kind, reps=50) for kind, name_k in
@@ -239,6 +264,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if best in ("ir32", "learned32") else "f64",
+        "mg_levels": mg_levels,
         "train": train_info,
         "fcg_kernel_s": fcg_kernels,
         "data": "synthetic",

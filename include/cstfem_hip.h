@@ -221,12 +221,14 @@ int cstfem_mpnn_download(cstfem_ctx* ctx, int what, float* out, int64_t count);
 /* ---- multilevel preconditioner (no reference counterpart) -------------------------------- */
 /* Aggregation hierarchy for CSTFEM_METHOD_FCG_LEARNED: greedy aggregation of the node graph,
  * unsmoothed prolongators from the rigid-body modes (3 coarse DOFs per aggregate), Galerkin
- * coarse matrices on the device, a dense coarsest solve, and a K-cycle (two flexible CG steps
- * per coarse level).  fine_smoother: 0 the learned M (cstfem_mpnn_build first), 1 Chebyshev
- * (degree 2) on D^-1 K.  Levels are added until a level has <= coarse_max DOFs or max_levels
- * is reached (the coarsest must have <= 4096 DOFs).  Call after cstfem_setup; a new setup
- * drops the hierarchy.  While a hierarchy exists, the FP64 flexible CG uses it. */
-int cstfem_mg_setup(cstfem_ctx* ctx, int max_levels, int coarse_max, int fine_smoother);
+ * coarse matrices on the device, a dense coarsest solve; coarse levels 1..kcycle_levels are
+ * solved by a K-cycle (two flexible CG steps each), deeper ones by one V-cycle.
+ * fine_smoother: 0 the learned M (cstfem_mpnn_build first), 1 Chebyshev (degree 2) on
+ * D^-1 K.  Levels are added until a level has <= coarse_max DOFs or max_levels is reached
+ * (the coarsest must have <= 4096 DOFs).  Call after cstfem_setup; a new setup drops the
+ * hierarchy.  While a hierarchy exists, the FP64 flexible CG uses it. */
+int cstfem_mg_setup(cstfem_ctx* ctx, int max_levels, int coarse_max, int fine_smoother,
+                    int kcycle_levels);
 int cstfem_mg_levels(cstfem_ctx* ctx, int* levels);
 /* DOFs, aggregates (0 on the coarsest), stored entries and the upper end of the smoothing
  * interval (lambda_max estimate of D^-1 K; 0 where no Chebyshev smoother runs) of one level. */

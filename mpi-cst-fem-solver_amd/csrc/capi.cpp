@@ -802,11 +802,12 @@ int cstfem_mpnn_download(cstfem_ctx* ctx, int what, float* out, int64_t count) {
   });
 }
 
-int cstfem_mg_setup(cstfem_ctx* ctx, int max_levels, int coarse_max, int fine_smoother) {
+int cstfem_mg_setup(cstfem_ctx* ctx, int max_levels, int coarse_max, int fine_smoother,
+                    int kcycle_levels) {
   return guarded([&] {
     need(ctx != nullptr, "cstfem_mg_setup: NULL context");
     use_device(&ctx->c);
-    mg_setup(ctx->c, max_levels, coarse_max, fine_smoother);
+    mg_setup(ctx->c, max_levels, coarse_max, fine_smoother, kcycle_levels);
     return CSTFEM_OK;
   });
 }

@@ -153,6 +153,7 @@ struct MgLevel {
 struct Multigrid {
   bool ready = false;
   int fine_smoother = 0;  // 0: learned M, 1: Chebyshev on D^-1 A
+  int kdepth = 16;        // levels 1..kdepth use the K-cycle, deeper ones the V-cycle
   std::vector<MgLevel*> lv;
   int nc = 0;             // DOFs of the coarsest level (dense inverse)
   DBuf<double> cinv;
@@ -264,7 +265,7 @@ int solve_fcg_learned(Context& c, const cstfem_solver_options& o, cstfem_solver_
 void ensure_vectors(Context& c);
 void ensure_vals32(Context& c);
 // multilevel preconditioner (mg.hip)
-void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother);
+void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int kdepth);
 // z = B r on the library stream (level-0 vectors of 2*nb doubles); kernels return at once
 // when sc is given and its status is no longer running
 void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* sc);

@@ -19,8 +19,10 @@
 // flexible CG captures it in its iteration graph):
 //   level 0:   z = S(r) [learned M, or 2 Chebyshev steps on D^-1 A]; z += P0 K_1(P0^T (r - A z));
 //              z = z + S(r - A z)
-//   level l:   2 Chebyshev steps, coarse correction by the K-cycle of level l+1 (two flexible
-//              CG steps preconditioned by the cycle of level l+1) or the dense solve, 2 steps.
+//   level l:   2 Chebyshev steps, coarse correction, 2 steps.  The coarse correction of level
+//              l is the dense solve on the coarsest level, the K-cycle of level l+1 (two
+//              flexible CG steps preconditioned by the cycle of level l+1) while l+1 <= kdepth,
+//              and one cycle of level l+1 (V-cycle) below that.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -814,15 +816,20 @@ void alloc_work(MgLevel& L) {
 
 void kcycle(Context& c, int l, const PcgScalars* sc);
 
-// coarse correction of level l: lv[l+1].kx from lv[l+1].rhs
+void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars* sc);
+
+// coarse correction of level l: lv[l+1].kx from lv[l+1].rhs -- the dense solve on the
+// coarsest level, the K-cycle on levels <= kdepth, one cycle (V) below
 void coarse_correction(Context& c, int l, const PcgScalars* sc) {
   Multigrid& mg = c.mg;
   MgLevel& C = *mg.lv[l + 1];
   if (l + 1 == (int)mg.lv.size() - 1)
     hipLaunchKernelGGL(k_dense, dim3(grid1(C.n)), dim3(kT), 0, c.stream, C.n,
                        (const double*)mg.cinv.get(), (const double*)C.rhs.get(), C.kx.get(), sc);
-  else
+  else if (l + 1 <= mg.kdepth)
     kcycle(c, l + 1, sc);
+  else
+    cycle_s(c, l + 1, C.rhs.get(), C.kx.get(), sc);
 }
 
 // cycle on level l >= 1: out = B_l(rhs)
@@ -932,16 +939,18 @@ void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* 
   HIP_CHECK(hipGetLastError());
 }
 
-void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother) {
+void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int kdepth) {
   if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "multilevel: call cstfem_setup first");
   if (!c.mesh_pattern || c.n != 2 * c.nb)
     throw Error(CSTFEM_ERR_ORDER, "multilevel: needs an assembled mesh system");
   if (max_levels < 2 || max_levels > 16) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: max_levels must be 2..16");
   if (coarse_max < 3 || coarse_max > 4096) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: coarse_max must be 3..4096");
   if (fine_smoother != 0 && fine_smoother != 1) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: fine_smoother must be 0 or 1");
+  if (kdepth < 0) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: kcycle_levels must be >= 0");
   Multigrid& mg = c.mg;
   mg.clear();
   mg.fine_smoother = fine_smoother;
+  mg.kdepth = kdepth;
   HIP_CHECK(hipStreamSynchronize(c.stream));
   // level 0: the context's matrix and block pattern
   MgLevel* L0 = new MgLevel;

@@ -199,13 +199,16 @@ __global__ void k_sum_chunks(int chunks, int len, const float* __restrict__ P,
   out[i] += s;
 }
 
-// column sums of G[N x M] (bias gradients), fixed order: one thread per column
-__global__ void k_colsum(int N, int M, const uint16_t* __restrict__ Gd, float* __restrict__ out) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+// column sums of G[N x M] (bias gradients): workgroup b sums rows [b*rows, (b+1)*rows) of
+// column threadIdx.x into P[b][M]; k_sum_chunks then adds the chunks in order
+__global__ void k_colsum(int N, int M, int rows, const uint16_t* __restrict__ Gd,
+                         float* __restrict__ P) {
+  const int m = threadIdx.x;
   if (m >= M) return;
+  const int n0 = blockIdx.x * rows, n1 = min(N, n0 + rows);
   float s = 0.f;
-  for (int n = 0; n < N; ++n) s += bf2f(Gd[(size_t)n * M + m]);
-  out[m] += s;
+  for (int n = n0; n < n1; ++n) s += bf2f(Gd[(size_t)n * M + m]);
+  P[(size_t)blockIdx.x * M + m] = s;
 }
 
 // X[i] = [H[i], mean_{j ~ i, j != i} H[j]]  (BF16 in, FP32 mean, BF16 out)
@@ -739,6 +742,14 @@ double mpnn_loss_grad(Context& c) {
                      c.bcol.get(), (const uint16_t*)m.H[kRounds].get(), (const float*)m.dghat.get(),
                      m.grad.get());
   const int chunks = std::min((N + 63) / 64, 256);  // workgroups of the dW reductions
+  const int crows = (N + chunks - 1) / chunks;
+  DBuf<float> CP;
+  CP.alloc((size_t)chunks * kH);
+  auto colsum = [&](const uint16_t* G, float* out) {
+    hipLaunchKernelGGL(k_colsum, dim3(chunks), dim3(kH), 0, c.stream, N, kH, crows, G, CP.get());
+    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(kH), 0, c.stream, chunks, kH,
+                       (const float*)CP.get(), out);
+  };
   DBuf<float> P;
   P.alloc((size_t)chunks * 2 * kH * kH);
   for (int l = kRounds; l >= 1; --l) {
@@ -750,8 +761,7 @@ double mpnn_loss_grad(Context& c) {
                        (const uint16_t*)m.X[l].get(), (const uint16_t*)dZ.get(), P.get());
     hipLaunchKernelGGL(k_sum_chunks, dim3((2 * kH * kH + 255) / 256), dim3(256), 0, c.stream, chunks,
                        2 * kH * kH, (const float*)P.get(), m.grad.get() + kWl + (l - 1) * 2 * kH * kH);
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(kH), 0, c.stream, N, kH, (const uint16_t*)dZ.get(),
-                       m.grad.get() + kBl + (l - 1) * kH);
+    colsum(dZ.get(), m.grad.get() + kBl + (l - 1) * kH);
     // dX_l = dZ W_l^T  ([N x kH] x [kH x 2kH]): B = W_l as [in x out] = [2kH rows x kH]
     hipLaunchKernelGGL((k_dense<kH, 2 * kH, false, 1>), dim3(rows_grid(N)), dim3(kT), 0, c.stream, N,
                        (const uint16_t*)dZ.get(), (const uint16_t*)m.wn[l].get(), (const float*)nullptr,
@@ -768,8 +778,7 @@ double mpnn_loss_grad(Context& c) {
                      (const uint16_t*)m.feat.get(), (const uint16_t*)dZ.get(), P.get());
   hipLaunchKernelGGL(k_sum_chunks, dim3((kF * kH + 255) / 256), dim3(256), 0, c.stream, chunks, kF * kH,
                      (const float*)P.get(), m.grad.get() + kW0);
-  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(kH), 0, c.stream, N, kH, (const uint16_t*)dZ.get(),
-                     m.grad.get() + kB0);
+  colsum(dZ.get(), m.grad.get() + kB0);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c.stream));
   return loss;

@@ -133,7 +133,8 @@ SIGNATURES = {
     "cstfem_mpnn_loss_grad": (ctypes.c_int, [_P, _D]),
     "cstfem_mpnn_train_step": (ctypes.c_int, [_P, ctypes.c_float, _D]),
     "cstfem_mpnn_download": (ctypes.c_int, [_P, ctypes.c_int, _F, ctypes.c_int64]),
-    "cstfem_mg_setup": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "cstfem_mg_setup": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int]),
     "cstfem_mg_levels": (ctypes.c_int, [_P, _I]),
     "cstfem_mg_level_info": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _I64, _D]),
     "cstfem_mg_download_aggregates": (ctypes.c_int, [_P, ctypes.c_int, _I, ctypes.c_int]),
@@ -398,10 +399,11 @@ class Device:
         return out
 
     # ---- multilevel preconditioner
-    def mg_setup(self, max_levels=10, coarse_max=500, fine_smoother=0):
-        """fine_smoother: 0 the learned M (mpnn_build first), 1 Chebyshev on D^-1 K."""
+    def mg_setup(self, max_levels=10, coarse_max=500, fine_smoother=0, kcycle_levels=16):
+        """fine_smoother: 0 the learned M (mpnn_build first), 1 Chebyshev on D^-1 K;
+        coarse levels 1..kcycle_levels use the K-cycle, deeper ones the V-cycle."""
         _check(lib().cstfem_mg_setup(self._h, int(max_levels), int(coarse_max),
-                                     int(fine_smoother)))
+                                     int(fine_smoother), int(kcycle_levels)))
 
     def mg_levels(self):
         n = ctypes.c_int()

@@ -18,9 +18,10 @@ preconditioner; this module restates the device algorithm step by step:
   cycle         level 0: z = S(r); z += P K_1(P^T (r - A z)); z += S(r - A z), with S = the
                 learned M (given as a matrix) or two Chebyshev steps on D^-1 A over [lam/30, lam];
                 levels >= 1: two Chebyshev steps, coarse correction, two steps; the coarse
-                correction of level l is the dense solve on the coarsest level, else the
-                K-cycle of level l+1: two flexible CG steps (x = 0, alpha = p'r/p'q with 0 when
-                p'q <= 0, beta = -z'q/p'q) preconditioned by the cycle of level l+1.
+                correction of level l is the dense solve on the coarsest level, else, for
+                levels l+1 <= kdepth, the K-cycle of level l+1: two flexible CG steps (x = 0,
+                alpha = p'r/p'q with 0 when p'q <= 0, beta = -z'q/p'q) preconditioned by the
+                cycle of level l+1, and below that one cycle of level l+1 (V-cycle).
 
 Pinning (tests/test_oracle_mg.py): known answers for the aggregation and the Gram-Schmidt
 step; the prolongator is orthonormal per aggregate and reproduces the near-null space
@@ -191,36 +192,38 @@ def cheb2(L, x, r, lam):
     return x + d
 
 
-def coarse_correction(levels, l, rc, lams):
+def coarse_correction(levels, l, rc, lams, kdepth=99):
     if l + 1 == len(levels) - 1:
         return levels[-1].inv @ rc
-    return kcycle(levels, l + 1, rc, lams)
+    if l + 1 <= kdepth:
+        return kcycle(levels, l + 1, rc, lams, kdepth)
+    return cycle(levels, l + 1, rc, lams, kdepth=kdepth)
 
 
-def cycle(levels, l, r, lams, M=None):
+def cycle(levels, l, r, lams, M=None, kdepth=99):
     L = levels[l]
     if l == 0 and M is not None:
         x = M @ r
     else:
         x = cheb2(L, None, r, lams[l])
     rc = L.P.T @ (r - L.A @ x)
-    x = x + L.P @ coarse_correction(levels, l, rc, lams)
+    x = x + L.P @ coarse_correction(levels, l, rc, lams, kdepth)
     if l == 0 and M is not None:
         return x + M @ (r - L.A @ x)
     return cheb2(L, x, r, lams[l])
 
 
-def kcycle(levels, l, rhs, lams):
+def kcycle(levels, l, rhs, lams, kdepth=99):
     A = levels[l].A
     res = rhs.copy()
-    z = cycle(levels, l, res, lams)
+    z = cycle(levels, l, res, lams, kdepth=kdepth)
     p = z.copy()
     q = A @ p
     pq, pr = p @ q, p @ res
     alpha = pr / pq if pq > 0 else 0.0
     x = alpha * p
     res = res - alpha * q
-    z = cycle(levels, l, res, lams)
+    z = cycle(levels, l, res, lams, kdepth=kdepth)
     zq = z @ q
     beta = -zq / pq if pq > 0 else 0.0
     q = A @ z + beta * q

@@ -3,7 +3,7 @@
 - aggregates of every level equal the oracle's (integers, exact);
 - coarse matrices (Galerkin products on the device) equal the oracle's to 1e-12 relative;
 - one application of the cycle equals the oracle's cycle (FP64; reductions in a different
-  order) to 1e-9 relative, for hierarchies of 2 to 5 levels;
+  order) to 1e-9 relative, for hierarchies of 2 to 5 levels and K-cycle depths 1 to all;
 - FCG with the multilevel preconditioner reaches the FP64 true residual in a small fraction
   of Jacobi-CG's iterations, with the Chebyshev and with the learned fine smoother.
 No reference counterpart exists for this component.
@@ -46,11 +46,11 @@ def system(dev, prob):
     return A, prob.mesh.xy, fixed, bptr, bcol
 
 
-@pytest.mark.parametrize("n,max_levels,coarse_max", [(16, 2, 4096), (24, 3, 40),
-                                                     (40, 5, 30)])
-def test_hierarchy_and_cycle_match_oracle(dev, n, max_levels, coarse_max):
+@pytest.mark.parametrize("n,max_levels,coarse_max,kdepth", [(16, 2, 4096, 2), (24, 3, 40, 2),
+                                                            (40, 5, 30, 1), (40, 5, 30, 9)])
+def test_hierarchy_and_cycle_match_oracle(dev, n, max_levels, coarse_max, kdepth):
     A, xy, fixed, bptr, bcol = system(dev, hetero(n))
-    dev.mg_setup(max_levels, coarse_max, 1)
+    dev.mg_setup(max_levels, coarse_max, 1, kdepth)
     levels = mgo.setup(A, xy, fixed, max_levels=max_levels, coarse_max=coarse_max,
                        ptr0=bptr, col0=bcol)
     nlev = dev.mg_levels()
@@ -72,7 +72,7 @@ def test_hierarchy_and_cycle_match_oracle(dev, n, max_levels, coarse_max):
     for _ in range(2):
         r = rng.standard_normal(dev.n)
         z = dev.mg_apply(r)
-        zo = mgo.cycle(levels, 0, r, lams)
+        zo = mgo.cycle(levels, 0, r, lams, kdepth=kdepth)
         assert np.linalg.norm(z - zo) <= 1e-9 * np.linalg.norm(zo)
     # the cycle is deterministic
     np.testing.assert_array_equal(dev.mg_apply(r), z)
