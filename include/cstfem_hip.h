@@ -151,6 +151,26 @@ int cstfem_download_solution(cstfem_ctx* ctx, double* x, int n);
 /* y = A x with the device matrix (FP64, per-row summation in column order). */
 int cstfem_spmv(cstfem_ctx* ctx, const double* x, double* y, int n);
 
+/* ---- stresses and Kirsch metrics -------------------------------------------------------- */
+/* Replaces compute_element_stresses (postprocess.cpp:35-60) for the last solution on the
+ * device: sig[3*ne] = (sxx, syy, txy) per element (element.cpp:57-69, the reference's
+ * operation order), von_mises[ne] and szz[ne] (element.cpp:72-81), cxy[2*ne] centroids and
+ * area[ne]; every output but sig may be NULL.  A per-element modulus is honoured. */
+int cstfem_element_stresses(cstfem_ctx* ctx, double* sig, double* von_mises, double* szz,
+                            double* cxy, double* area);
+/* Mirrors KirschMetrics (postprocess.hpp) */
+typedef struct {
+  double l2_rel_stress_error;       /* centroid stresses vs the analytic field, area-weighted */
+  double l2_rel_displacement_error; /* degree-2 rule */
+  double energy_rel_error;          /* complementary energy norm, degree-2 rule */
+  int scf_element;                  /* element whose centroid is nearest to (0, a) */
+  int pad_;
+  double scf_x, scf_y, scf_numerical, scf_analytical_centroid, scf_theoretical;
+} cstfem_kirsch_result;
+/* Replaces kirsch_metrics (postprocess.cpp:81-150) for the last solution: uniform modulus
+ * (cstfem_set_material without E_elem), sigma_inf and hole radius a. */
+int cstfem_kirsch_metrics(cstfem_ctx* ctx, double sigma_inf, double a, cstfem_kirsch_result* out);
+
 /* ---- batches of independent plates (no reference counterpart) ------------------------- */
 /* Per-element modulus and Poisson ratio (plates of a batch differ in both); D_e is
  * constitutive_matrix(E_e, nu_e, plane) (element.cpp:7-15). */

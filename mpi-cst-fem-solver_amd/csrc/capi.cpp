@@ -715,6 +715,25 @@ int cstfem_batch_autotune(cstfem_ctx* ctx, const cstfem_solver_options* opt,
   });
 }
 
+int cstfem_element_stresses(cstfem_ctx* ctx, double* sig, double* von_mises, double* szz,
+                            double* cxy, double* area) {
+  return guarded([&] {
+    need(ctx && sig, "cstfem_element_stresses: NULL argument");
+    use_device(&ctx->c);
+    element_stresses(ctx->c, sig, von_mises, szz, cxy, area);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_kirsch_metrics(cstfem_ctx* ctx, double sigma_inf, double a, cstfem_kirsch_result* out) {
+  return guarded([&] {
+    need(ctx && out, "cstfem_kirsch_metrics: NULL argument");
+    use_device(&ctx->c);
+    kirsch_metrics(ctx->c, sigma_inf, a, out);
+    return CSTFEM_OK;
+  });
+}
+
 int cstfem_mpnn_num_params(void) { return mpnn_num_params(); }
 
 int cstfem_mpnn_init(cstfem_ctx* ctx, int seed) {

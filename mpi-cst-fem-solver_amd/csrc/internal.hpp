@@ -284,6 +284,8 @@ double mpnn_loss_grad(Context& c);
 double mpnn_train_step(Context& c, float lr);
 void mpnn_download(Context& c, int what, float* out, int64_t count);
 double profile_kernel(Context& c, int kind, int reps, int64_t bytes);
+void element_stresses(Context& c, double* sig, double* vm, double* szz, double* cxy, double* area);
+void kirsch_metrics(Context& c, double sigma_inf, double a, cstfem_kirsch_result* out);
 void batch_define(Context& c, int count, const int* node_off);
 double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batch_config& cfg,
                    cstfem_batch_result* results);

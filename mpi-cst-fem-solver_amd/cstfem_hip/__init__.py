@@ -61,6 +61,15 @@ class BatchConfig(ctypes.Structure):
     _fields_ = [("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int)]
 
 
+class KirschMetricsC(ctypes.Structure):
+    _fields_ = [("l2_rel_stress_error", ctypes.c_double),
+                ("l2_rel_displacement_error", ctypes.c_double),
+                ("energy_rel_error", ctypes.c_double), ("scf_element", ctypes.c_int),
+                ("pad_", ctypes.c_int), ("scf_x", ctypes.c_double), ("scf_y", ctypes.c_double),
+                ("scf_numerical", ctypes.c_double), ("scf_analytical_centroid", ctypes.c_double),
+                ("scf_theoretical", ctypes.c_double)]
+
+
 class BatchResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
                 ("refinements", ctypes.c_int), ("status", ctypes.c_int),
@@ -133,6 +142,9 @@ SIGNATURES = {
     "cstfem_mpnn_loss_grad": (ctypes.c_int, [_P, _D]),
     "cstfem_mpnn_train_step": (ctypes.c_int, [_P, ctypes.c_float, _D]),
     "cstfem_mpnn_download": (ctypes.c_int, [_P, ctypes.c_int, _F, ctypes.c_int64]),
+    "cstfem_element_stresses": (ctypes.c_int, [_P, _D, _D, _D, _D, _D]),
+    "cstfem_kirsch_metrics": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double,
+                                             ctypes.POINTER(KirschMetricsC)]),
     "cstfem_mg_setup": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int]),
     "cstfem_mg_levels": (ctypes.c_int, [_P, _I]),
@@ -349,6 +361,23 @@ class Device:
         _check(lib().cstfem_profile_kernel(self._h, int(kind), int(reps), int(nbytes),
                                            ctypes.byref(t)))
         return t.value
+
+    # ---- stresses and Kirsch metrics (postprocess.cpp)
+    def element_stresses(self, ne):
+        """(sig [ne x 3], von_mises, szz, centroids [ne x 2], area) of the last solution."""
+        sig = np.empty(3 * ne)
+        vm, szz, area = np.empty(ne), np.empty(ne), np.empty(ne)
+        cxy = np.empty(2 * ne)
+        _check(lib().cstfem_element_stresses(self._h, _ptr(sig, ctypes.c_double),
+                                             _ptr(vm, ctypes.c_double), _ptr(szz, ctypes.c_double),
+                                             _ptr(cxy, ctypes.c_double),
+                                             _ptr(area, ctypes.c_double)))
+        return sig.reshape(ne, 3), vm, szz, cxy.reshape(ne, 2), area
+
+    def kirsch_metrics(self, sigma_inf, a):
+        m = KirschMetricsC()
+        _check(lib().cstfem_kirsch_metrics(self._h, float(sigma_inf), float(a), ctypes.byref(m)))
+        return {k: getattr(m, k) for k, _ in KirschMetricsC._fields_ if k != "pad_"}
 
     # ---- learned preconditioner
     def mpnn_num_params(self):
