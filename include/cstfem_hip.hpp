@@ -47,13 +47,27 @@ inline Mat3 constitutive_matrix(double E, double nu, PlaneCondition plane) {
           c * (1.0 - 2.0 * nu) / 2.0};
 }
 
-// The reference's Mesh (mesh.hpp:20-31) reduced to what the path reads.
+struct PhysicalGroup {  // mesh.hpp:13-17
+  int dim = 0;
+  int tag = 0;
+  std::string name;
+};
+
+// The reference's Mesh (mesh.hpp:20-31).  Node and element indices are 0-based.
 struct Mesh {
   std::vector<double> xy;                                          // 2 per node
+  std::vector<std::size_t> node_tags;                              // Gmsh tag per node (read)
   std::vector<int> triangles;                                      // 3 per element, CCW
   std::map<std::string, std::vector<std::array<int, 2>>> boundary_edges;
+  std::vector<PhysicalGroup> physical_groups;                      // from $PhysicalNames
+  int flipped_triangles = 0;  // triangles reoriented to counter-clockwise on read
   std::size_t num_nodes() const { return xy.size() / 2; }
   std::size_t num_triangles() const { return triangles.size() / 3; }
+  bool has_physical_group(const std::string& name) const {
+    for (const auto& g : physical_groups)
+      if (g.name == name) return true;
+    return false;
+  }
 };
 
 // csr.hpp:14-32 (host copy; the device keeps its own)
@@ -230,6 +244,36 @@ inline void apply_dirichlet(Device& dev, const DirichletDofs& dofs, CsrMatrix& K
   check(cstfem_download_rhs(dev.get(), b.data(), static_cast<int>(b.size())));
 }
 
+// SolverStats with the reference's messages (solver.cpp:46-104, 123-129) from the C struct
+inline SolverStats make_stats(const cstfem_solver_stats& s, double rtol, int max_iter) {
+  SolverStats st;
+  st.iterations = s.iterations;
+  st.rel_residual = s.rel_residual;
+  st.true_rel_residual = s.true_rel_residual;
+  st.converged = s.converged != 0;
+  st.outer_iterations = s.outer_iterations;
+  std::ostringstream msg;
+  if (s.zero_rhs)
+    msg << "zero right-hand side, x = 0";
+  else if (s.breakdown)
+    msg << "CG breakdown at iteration " << s.iterations + 1 << ": p'Ap = " << s.breakdown_pap
+        << " (matrix not SPD?)";
+  else if (s.converged)
+    msg << "converged in " << s.iterations << " iterations, relative residual " << s.rel_residual;
+  else
+    msg << "NOT converged: reached max_iter = " << max_iter << ", relative residual "
+        << s.rel_residual << " (rtol " << rtol << ")";
+  st.message = msg.str();
+  if (s.warning) {
+    std::ostringstream w;
+    w << "WARNING: CG recursive residual converged (" << s.rel_residual
+      << ") but the true residual ||b - Ax|| / ||b|| = " << s.true_rel_residual
+      << " exceeds 10 * rtol = " << 10.0 * rtol;
+    st.warning = w.str();
+  }
+  return st;
+}
+
 // solver.hpp:120-128
 class LinearSolver {
  public:
@@ -271,32 +315,7 @@ class HipCgJacobiSolver : public LinearSolver {
     x.assign(b.size(), 0.0);
     std::vector<double> hist(options_.record_history ? options_.max_iter + 1 : 0);
     check(cstfem_solve(dev_->get(), &o, &s, x.data(), hist.empty() ? nullptr : hist.data()));
-    stats_ = SolverStats{};
-    stats_.iterations = s.iterations;
-    stats_.rel_residual = s.rel_residual;
-    stats_.true_rel_residual = s.true_rel_residual;
-    stats_.converged = s.converged != 0;
-    stats_.outer_iterations = s.outer_iterations;
-    std::ostringstream msg;  // solver.cpp:46-104
-    if (s.zero_rhs)
-      msg << "zero right-hand side, x = 0";
-    else if (s.breakdown)
-      msg << "CG breakdown at iteration " << s.iterations + 1 << ": p'Ap = " << s.breakdown_pap
-          << " (matrix not SPD?)";
-    else if (s.converged)
-      msg << "converged in " << s.iterations << " iterations, relative residual "
-          << s.rel_residual;
-    else
-      msg << "NOT converged: reached max_iter = " << options_.max_iter << ", relative residual "
-          << s.rel_residual << " (rtol " << options_.rtol << ")";
-    stats_.message = msg.str();
-    if (s.warning) {  // solver.cpp:123-129
-      std::ostringstream w;
-      w << "WARNING: CG recursive residual converged (" << s.rel_residual
-        << ") but the true residual ||b - Ax|| / ||b|| = " << s.true_rel_residual
-        << " exceeds 10 * rtol = " << 10.0 * options_.rtol;
-      stats_.warning = w.str();
-    }
+    stats_ = make_stats(s, options_.rtol, options_.max_iter);
     if (options_.record_history)
       stats_.residual_history.assign(hist.begin(),
                                      hist.begin() + (s.zero_rhs ? 1 : s.iterations + 1));
