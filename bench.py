@@ -135,6 +135,14 @@ def main():
         for kind, name_k in ((0, "k_cg_spmv<F64>"), (5, "k_cg_spmv<F32>"), (2, "k_cg_update"),
                              (3, "k_cg_direction<F64>")):
             out[name_k] = dev.profile_kernel(kind, reps=args.reps)
+        # flexible-CG kernels, with M at the polynomial starting point of the network
+        from cstfem_hip import learned
+        dev.mpnn_init(0)
+        dev.mpnn_set_params(learned.polynomial_start(dev.mpnn_get_params()))
+        dev.mpnn_build()
+        for kind, name_k in ((6, "k_fcg_spmv<F64>"), (7, "k_fcg_precond<F64>"),
+                             (8, "k_fcg_update<F64>"), (9, "k_fcg_direction<F64>")):
+            out[name_k] = dev.profile_kernel(kind, reps=args.reps)
         out["k_copy(256MiB)"] = dev.profile_kernel(4, reps=args.reps, nbytes=256 << 20)
         out["spmv_bytes"] = spmv_bytes(n, nnz)
         print(json.dumps(out), flush=True)

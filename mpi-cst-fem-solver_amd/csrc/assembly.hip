@@ -452,23 +452,29 @@ __device__ inline void constitutive(double E, double nu, int strain, double D[9]
 
 // Rows 2a..2a+1, columns 2b..2b+1 of cst_stiffness (element.cpp:21-55), computed as the
 // reference computes the upper triangle (entry (i,j) with i > j is the mirrored (j,i)).
+__device__ inline double pick3(int i, double v0, double v1, double v2) {
+  return i == 0 ? v0 : (i == 1 ? v1 : v2);
+}
+
+// Column q (0..5) of the 3x6 strain-displacement matrix B of element.cpp:21-35, built with
+// selects rather than an indexed local array, so that nothing is placed in scratch memory.
+__device__ inline void b_column(int q, double bi, double ci, double inv2A, double col[3]) {
+  const double vb = bi * inv2A, vc = ci * inv2A;
+  const bool ux = (q & 1) == 0;
+  col[0] = ux ? vb : 0.0;
+  col[1] = ux ? 0.0 : vc;
+  col[2] = ux ? vc : vb;
+}
+
+// Rows 2a..2a+1, columns 2b..2b+1 of cst_stiffness (element.cpp:21-55), computed as the
+// reference computes the upper triangle (entry (i,j) with i > j is the mirrored (j,i)).
 __device__ inline void ke_block(const double xy[6], const double D[9], double t, int a, int b,
                                 double out[4]) {
   const double x1 = xy[0], y1 = xy[1], x2 = xy[2], y2 = xy[3], x3 = xy[4], y3 = xy[5];
   const double area = 0.5 * ((x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1));
   const double inv2A = 1.0 / (2.0 * area);
-  const double bb[3] = {y2 - y3, y3 - y1, y1 - y2};
-  const double cc[3] = {x3 - x2, x1 - x3, x2 - x1};
-  double B[18];
-#pragma unroll
-  for (int i = 0; i < 18; ++i) B[i] = 0.0;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    B[0 * 6 + 2 * i] = bb[i] * inv2A;
-    B[1 * 6 + 2 * i + 1] = cc[i] * inv2A;
-    B[2 * 6 + 2 * i] = cc[i] * inv2A;
-    B[2 * 6 + 2 * i + 1] = bb[i] * inv2A;
-  }
+  const double b0 = y2 - y3, b1 = y3 - y1, b2 = y1 - y2;
+  const double c0 = x3 - x2, c1 = x1 - x3, c2 = x2 - x1;
   const double scale = t * fabs(area);
 #pragma unroll
   for (int ci = 0; ci < 2; ++ci)
@@ -476,19 +482,24 @@ __device__ inline void ke_block(const double xy[6], const double D[9], double t,
     for (int cj = 0; cj < 2; ++cj) {
       const int i0 = 2 * a + ci, j0 = 2 * b + cj;
       const int p = i0 < j0 ? i0 : j0, q = i0 < j0 ? j0 : i0;
+      double Bp[3], Bq[3];
+      b_column(p, pick3(p >> 1, b0, b1, b2), pick3(p >> 1, c0, c1, c2), inv2A, Bp);
+      b_column(q, pick3(q >> 1, b0, b1, b2), pick3(q >> 1, c0, c1, c2), inv2A, Bq);
       double s = 0.0;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         double db = 0.0;  // DB[k][q] = sum_m D[k][m] B[m][q]
 #pragma unroll
-        for (int m = 0; m < 3; ++m) db += D[k * 3 + m] * B[m * 6 + q];
-        s += B[k * 6 + p] * db;
+        for (int m = 0; m < 3; ++m) db += D[k * 3 + m] * Bq[m];
+        s += Bp[k] * db;
       }
       out[ci * 2 + cj] = scale * s;
     }
 }
 
-// one wavefront per SELL slice, lane = block row; writes are coalesced 32-byte blocks
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+// one wavefront per SELL slice, lane = block row; writes are coalesced
 __global__ void k_assemble(const int* __restrict__ row0, const int* __restrict__ nrow,
                            int nslices, const int64_t* __restrict__ off,
                            const int* __restrict__ brow_ptr, const int* __restrict__ slot_ptr,
@@ -525,11 +536,11 @@ __global__ void k_assemble(const int* __restrict__ row0, const int* __restrict__
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[i] += kb[i];
     }
-    double* v = val + 4 * (o + (int64_t)k * kSlice + lane);
-    v[0] = acc[0];
-    v[1] = acc[1];
-    v[2] = acc[2];
-    v[3] = acc[3];
+    // planar block (sell_vidx): the wavefront writes the 1 KB of row-0 pairs, then the
+    // 1 KB of row-1 pairs, each as one coalesced 16-byte-per-lane store
+    double* vk = val + 4 * (o + (int64_t)k * kSlice) + 2 * lane;
+    *reinterpret_cast<v2d*>(vk) = v2d{acc[0], acc[1]};
+    *reinterpret_cast<v2d*>(vk + 2 * kSlice) = v2d{acc[2], acc[3]};
   }
 }
 
@@ -542,11 +553,11 @@ __global__ void k_export_csr(int nb, const int* __restrict__ brow_ptr,
   const int b0 = brow_ptr[n], d = brow_ptr[n + 1] - b0;
   const int64_t r0 = 4 * (int64_t)b0, r1 = r0 + 2 * d;
   for (int k = 0; k < d; ++k) {
-    const double* v = val + 4 * spos[b0 + k];
-    out[r0 + 2 * k] = v[0];
-    out[r0 + 2 * k + 1] = v[1];
-    out[r1 + 2 * k] = v[2];
-    out[r1 + 2 * k + 1] = v[3];
+    const int64_t q = spos[b0 + k];
+    out[r0 + 2 * k] = val[sell_vidx(q, 0)];
+    out[r0 + 2 * k + 1] = val[sell_vidx(q, 1)];
+    out[r1 + 2 * k] = val[sell_vidx(q, 2)];
+    out[r1 + 2 * k + 1] = val[sell_vidx(q, 3)];
   }
 }
 

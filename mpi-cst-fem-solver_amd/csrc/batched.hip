@@ -86,12 +86,12 @@ __device__ inline void row_spmv(const BatchArgs& A, const double2* __restrict__ 
   const int64_t o = A.off[s];
   const int w = (int)((A.off[s + 1] - o) >> 6);
   const int* c = A.col + o + lane;
-  const v2d* v = reinterpret_cast<const v2d*>(A.val64) + 2 * (o + lane);
+  const v2d* v = reinterpret_cast<const v2d*>(A.val64 + 4 * o) + lane;
   double a0 = 0.0, a1 = 0.0;
   for (int k = 0; k < w; ++k) {
     const int m = c[(size_t)k * 64];
     const v2d va = v[(size_t)k * 128];
-    const v2d vb = v[(size_t)k * 128 + 1];
+    const v2d vb = v[(size_t)k * 128 + 64];
     const double2 xm = x[m];
     a0 = a0 + va.x * xm.x;
     a0 = a0 + va.y * xm.y;
@@ -339,12 +339,6 @@ __global__ void k_reset_queue(int* q) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
 }
 
-__global__ void k_round(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = (float)a[i];
-}
-
 }  // namespace
 
 void batch_define(Context& c, int count, const int* node_off) {
@@ -444,13 +438,7 @@ double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batc
     c.r32.alloc(nv);
     c.p32.alloc(nv);
     c.Ap32.alloc(nv);
-    if (!c.vals32_valid) {
-      c.sell_val32.alloc(4 * c.sell_blocks);
-      hipLaunchKernelGGL(k_round, dim3(2048), dim3(256), 0, c.stream, (int64_t)(4 * c.sell_blocks),
-                         (const double*)c.sell_val.get(), c.sell_val32.get());
-      HIP_CHECK(hipGetLastError());
-      c.vals32_valid = true;
-    }
+    ensure_vals32(c);
   }
   const BatchArgs args = batch_args(c, o);
   int grid = cfg.groups_per_cu * (c.cu_count > 0 ? c.cu_count : 256);

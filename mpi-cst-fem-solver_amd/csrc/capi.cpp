@@ -364,7 +364,7 @@ int cstfem_upload_csr(cstfem_ctx* ctx, int n, const int* row_ptr, const int* col
       for (int k = row_ptr[rr]; k < row_ptr[rr + 1]; ++k) {
         const int J = cols[k] / 2, cj = cols[k] % 2;
         while (bc[q] != J) ++q;
-        const int64_t vi = 4 * spos[q] + 2 * ci + cj;
+        const int64_t vi = sell_vidx(spos[q], 2 * ci + cj);
         sval[vi] = vals[k];
         c.h_generic_map[k] = vi;
       }

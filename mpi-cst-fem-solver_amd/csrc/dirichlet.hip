@@ -37,13 +37,13 @@ __global__ void k_dirichlet(int nb, int n, const int* __restrict__ brow_ptr,
     if (i >= n) break;
     if (fixed[i]) {
       const int qd = bdiag[nd];
-      const double diag = qd >= 0 ? val[4 * spos[qd] + 3 * c] : 0.0;
+      const double diag = qd >= 0 ? val[sell_vidx(spos[qd], 3 * c)] : 0.0;
       if (!(diag > 0.0)) {
         atomicMin(err, i);
         continue;
       }
       for (int q = q0; q < q1; ++q) {
-        double* v = val + 4 * spos[q] + 2 * c;
+        double* v = val + sell_vidx(spos[q], 2 * c);  // row c: v[0], v[1]
         const int m = bcol[q];
         if (2 * m != i) v[0] = 0.0;
         if (2 * m + 1 != i) v[1] = 0.0;
@@ -53,7 +53,7 @@ __global__ void k_dirichlet(int nb, int n, const int* __restrict__ brow_ptr,
       double bi = b[i];
       for (int q = q0; q < q1; ++q) {
         const int m = bcol[q];
-        double* v = val + 4 * spos[q] + 2 * c;
+        double* v = val + sell_vidx(spos[q], 2 * c);  // row c: v[0], v[1]
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
           const int j = 2 * m + d;

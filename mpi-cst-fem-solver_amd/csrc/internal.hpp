@@ -10,9 +10,13 @@
 //              element, encoded e*9 + a*3 + b (a, b = local node of the row / column).
 //   matrix:    SELL-64 of 2x2 blocks: slice s holds block rows 64s..64s+63; its w_s block
 //              columns are stored column-major with stride 64, so lane l of a wavefront
-//              reads block k of row 64s+l at  sell_off[s] + 64k + l  (32 contiguous bytes
-//              of values, 4 bytes of column index).  Padding blocks are zero with the row's
-//              own column.  sell_pos[q] maps CSR block q to its SELL slot.
+//              reads block k of row 64s+l at SELL position  sell_off[s] + 64k + l  (4 bytes
+//              of column index).  FP64 values are planar per (slice, k): the 64 row-0 pairs
+//              (a00 a01, 1 KB) then the 64 row-1 pairs (a10 a11, 1 KB), so that each half
+//              is one fully coalesced 16-byte-per-lane load (sell_vidx).  FP32 copies keep
+//              the 4 values of a block together (16 bytes per lane, already coalesced).
+//              Padding blocks are zero with the row's own column.  sell_pos[q] maps CSR
+//              block q to its SELL position.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,6 +31,12 @@
 namespace cstfem_hip {
 
 constexpr int kSlice = 64;  // block rows per SELL slice = one wavefront
+
+// Index in the FP64 SELL value array of entry j (0 a00, 1 a01, 2 a10, 3 a11) of the 2x2 block
+// at SELL position pos (slice offsets are multiples of 64, so pos & 63 is the lane).
+__host__ __device__ inline int64_t sell_vidx(int64_t pos, int j) {
+  return 4 * (pos & ~int64_t(63)) + ((j >> 1) << 7) + 2 * (pos & 63) + (j & 1);
+}
 
 // 3x3 row-major constitutive matrix, passed to kernels by value
 struct Mat3 {
@@ -204,9 +214,9 @@ struct Context {
   std::vector<int64_t> h_sell_off;
   DBuf<int64_t> sell_off;
   DBuf<int> sell_col;
-  DBuf<double> sell_val;   // 4 per block, row-major 2x2 (v00 v01 v10 v11)
+  DBuf<double> sell_val;   // 4 per block, planar per (slice, k): see sell_vidx
   DBuf<int64_t> sell_pos;  // CSR block -> SELL slot
-  DBuf<float> sell_val32;  // FP32 copy for the mixed-precision path
+  DBuf<float> sell_val32;  // FP32 copy (v00 v01 v10 v11 at 4*pos) for the mixed-precision path
   bool vals32_valid = false;
 
   // generic CSR upload: scalar entry -> SELL value index (host side, for download)

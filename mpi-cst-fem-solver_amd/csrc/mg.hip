@@ -86,12 +86,12 @@ __device__ inline void row0_spmv(const Mat0& A, const double2* __restrict__ x, i
   const int64_t o = A.off[s];
   const int w = (int)((A.off[s + 1] - o) >> 6);
   const int* c = A.col + o + lane;
-  const v2d* v = reinterpret_cast<const v2d*>(A.val) + 2 * (o + lane);
+  const v2d* v = reinterpret_cast<const v2d*>(A.val + 4 * o) + lane;
   double a0 = 0.0, a1 = 0.0;
   for (int k = 0; k < w; ++k) {
     const int m = c[(size_t)k * 64];
     const v2d va = __builtin_nontemporal_load(v + (size_t)k * 128);
-    const v2d vb = __builtin_nontemporal_load(v + (size_t)k * 128 + 1);
+    const v2d vb = __builtin_nontemporal_load(v + (size_t)k * 128 + 64);
     const double2 xm = x[m];
     a0 = a0 + va.x * xm.x;
     a0 = a0 + va.y * xm.y;
@@ -377,7 +377,9 @@ __global__ void k_rap0(int ncblk, const int* __restrict__ crow, const int* __res
     for (int q = brow_ptr[i]; q < brow_ptr[i + 1]; ++q) {
       const int j = bcol[q];
       if (agg[j] != b) continue;
-      const double* Aq = fval + 4 * spos[q];
+      const int64_t sq = spos[q];
+      const double Aq[4] = {fval[sell_vidx(sq, 0)], fval[sell_vidx(sq, 1)], fval[sell_vidx(sq, 2)],
+                            fval[sell_vidx(sq, 3)]};
       const double* Pj = P + (size_t)(2 * j) * 3;
       // T = A_q P_j (2x3), acc += P_i^T T
       double T[2][3];
@@ -517,10 +519,10 @@ __global__ __launch_bounds__(kT) void k0_gersh(Mat0 A, const double2* __restrict
   for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < A.nslices; s += gridDim.x * kW) {
     const int64_t o = A.off[s];
     const int w = (int)((A.off[s + 1] - o) >> 6);
-    const v2d* v = reinterpret_cast<const v2d*>(A.val) + 2 * (o + lane);
+    const v2d* v = reinterpret_cast<const v2d*>(A.val + 4 * o) + lane;
     double a0 = 0.0, a1 = 0.0;
     for (int k = 0; k < w; ++k) {
-      const v2d va = v[(size_t)k * 128], vb = v[(size_t)k * 128 + 1];
+      const v2d va = v[(size_t)k * 128], vb = v[(size_t)k * 128 + 64];
       a0 += fabs(va.x) + fabs(va.y);
       a1 += fabs(vb.x) + fabs(vb.y);
     }

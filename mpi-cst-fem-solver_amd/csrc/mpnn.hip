@@ -90,8 +90,8 @@ __global__ void k_features(int nb, const int* __restrict__ brow_ptr, const int* 
   }
   f[3] = fixed ? (float)fixed[2 * i] : 0.f;
   f[4] = fixed ? (float)fixed[2 * i + 1] : 0.f;
-  const double* d = val + 4 * spos[bdiag[i]];
-  const double a00 = d[0], a01 = d[1], a11 = d[3];
+  const int64_t sd = spos[bdiag[i]];
+  const double a00 = val[sell_vidx(sd, 0)], a01 = val[sell_vidx(sd, 1)], a11 = val[sell_vidx(sd, 3)];
   f[5] = (float)(a01 / sqrt(a00 * a11));
   f[6] = (float)log(a00 / a11);
   f[7] = (float)(brow_ptr[i + 1] - brow_ptr[i] - 1) / 6.f;
@@ -101,8 +101,8 @@ __global__ void k_features(int nb, const int* __restrict__ brow_ptr, const int* 
   for (int q = brow_ptr[i]; q < brow_ptr[i + 1]; ++q) {
     const int j = bcol[q];
     if (j == i) continue;
-    const double* dj = val + 4 * spos[bdiag[j]];
-    ln_nb += log(dj[0] + dj[3]);
+    const int64_t sj = spos[bdiag[j]];
+    ln_nb += log(val[sell_vidx(sj, 0)] + val[sell_vidx(sj, 3)]);
     ++cnt;
   }
   f[8] = cnt ? (float)(log(a00 + a11) - ln_nb / cnt) : 0.f;
@@ -283,8 +283,10 @@ __global__ void k_decode(int nb, const int* __restrict__ brow_ptr, const int* __
     for (int u = 0; u < kDec; ++u) ghat[kDec * (size_t)q + u] = o[u];
     const double dj0 = invd[2 * j], dj1 = invd[2 * j + 1];
     const float sj0 = (float)sqrt(dj0), sj1 = (float)sqrt(dj1);
-    const double* a = aval + 4 * spos[q];
-    float* g = gsell + 4 * spos[q];
+    const int64_t sq = spos[q];
+    const double a[4] = {aval[sell_vidx(sq, 0)], aval[sell_vidx(sq, 1)], aval[sell_vidx(sq, 2)],
+                         aval[sell_vidx(sq, 3)]};
+    float* g = gsell + 4 * sq;
     g[0] = si0 * o[0] * sj0 + o[3] * (float)(di0 * a[0] * dj0);
     g[1] = si0 * o[1] * sj1 + o[3] * (float)(di0 * a[1] * dj1);
     g[2] = si1 * o[1] * sj0 + o[3] * (float)(di1 * a[2] * dj0);
@@ -366,7 +368,9 @@ __global__ void k_dghat(int nb, int K, int n2, const int* __restrict__ brow_ptr,
       g10 += w * v.y * r.x;
       g11 += w * v.y * r.y;
     }
-    const double* a = aval + 4 * spos[q];
+    const int64_t sq = spos[q];
+    const double a[4] = {aval[sell_vidx(sq, 0)], aval[sell_vidx(sq, 1)], aval[sell_vidx(sq, 2)],
+                         aval[sell_vidx(sq, 3)]};
     float* d = dghat + kDec * (size_t)q;
     d[0] = (float)(g00 * si0 * sj0);
     d[1] = (float)(g01 * si0 * sj1 + g10 * si1 * sj0);

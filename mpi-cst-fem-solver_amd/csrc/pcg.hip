@@ -100,13 +100,13 @@ __device__ inline void sell_row(const int64_t* __restrict__ off, const int* __re
   const int64_t o = off[s];
   const int w = (int)((off[s + 1] - o) >> 6);
   const int* c = col + o + lane;
-  const v2d* v = reinterpret_cast<const v2d*>(val) + 2 * (o + lane);
+  const v2d* v = reinterpret_cast<const v2d*>(val + 4 * o) + lane;
   double a0 = 0.0, a1 = 0.0;
 #pragma unroll 4
   for (int k = 0; k < w; ++k) {
     const int m = c[(size_t)k * 64];
     const v2d va = __builtin_nontemporal_load(v + (size_t)k * 128);
-    const v2d vb = __builtin_nontemporal_load(v + (size_t)k * 128 + 1);
+    const v2d vb = __builtin_nontemporal_load(v + (size_t)k * 128 + 64);
     const double2 xm = x[m];
     a0 = a0 + va.x * xm.x;
     a0 = a0 + va.y * xm.y;
@@ -490,7 +490,7 @@ __global__ void k_inv_diag(int nb, int n, const int* __restrict__ bdiag,
     const int i = 2 * nd + c;
     double v = 0.0;
     if (i < n) {
-      const double d = q >= 0 ? val[4 * spos[q] + 3 * c] : 0.0;
+      const double d = q >= 0 ? val[sell_vidx(spos[q], 3 * c)] : 0.0;
       if (!(d > 0.0))
         atomicMin(err, i);
       else
@@ -501,10 +501,14 @@ __global__ void k_inv_diag(int nb, int n, const int* __restrict__ bdiag,
   }
 }
 
-__global__ void k_round_values(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = (float)a[i];
+// FP32 copy of the FP64 values: planar FP64 (sell_vidx) -> 4 consecutive floats per block
+__global__ void k_round_values(int64_t nblocks, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nblocks;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 r0 = *reinterpret_cast<const double2*>(a + sell_vidx(i, 0));
+    const double2 r1 = *reinterpret_cast<const double2*>(a + sell_vidx(i, 2));
+    reinterpret_cast<float4*>(b)[i] = make_float4((float)r0.x, (float)r0.y, (float)r1.x, (float)r1.y);
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void k_copy(int64_t n2, const double2* __restrict__ a,
@@ -595,7 +599,7 @@ void ensure_vals32(Context& c) {
   if (c.vals32_valid) return;
   c.sell_val32.alloc(4 * c.sell_blocks);
   hipLaunchKernelGGL(k_round_values, dim3(2048), dim3(256), 0, c.stream,
-                     (int64_t)(4 * c.sell_blocks), (const double*)c.sell_val.get(),
+                     (int64_t)c.sell_blocks, (const double*)c.sell_val.get(),
                      c.sell_val32.get());
   HIP_CHECK(hipGetLastError());
   c.vals32_valid = true;
@@ -1490,7 +1494,7 @@ double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
       a32.alloc(nv);
       v32.alloc(4 * c.sell_blocks);
       hipLaunchKernelGGL(k_round_values, dim3(2048), dim3(256), 0, c.stream,
-                         (int64_t)(4 * c.sell_blocks), (const double*)c.sell_val.get(), v32.get());
+                         (int64_t)c.sell_blocks, (const double*)c.sell_val.get(), v32.get());
       HIP_CHECK(hipMemsetAsync(a32.get(), 0, sizeof(float) * nv, c.stream));
     }
   }
