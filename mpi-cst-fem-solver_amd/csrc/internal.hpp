@@ -155,6 +155,7 @@ struct MgLevel {
   std::vector<int> h_agg;
   DBuf<int> agg, mptr, mem;  // aggregate of every node; member nodes of every aggregate
   DBuf<double> P;            // n x 3: rows of the tentative prolongator
+  DBuf<double> nns;          // n x 3 near-null space of levels >= 1 (R factors of the level above)
   // work vectors
   DBuf<double> rhs, x, x2, d, res, kx, kres, kp, kq, kz;
   DBuf<double> part;  // reduction partials of the K-cycle steps
@@ -167,7 +168,12 @@ struct Multigrid {
   std::vector<MgLevel*> lv;
   int nc = 0;             // DOFs of the coarsest level (dense inverse)
   DBuf<double> cinv;
-  ~Multigrid() { clear(); }
+  int* pinned = nullptr;  // page-locked staging of the fine node graph, kept across set-ups
+  size_t pinned_n = 0;
+  ~Multigrid() {
+    clear();
+    if (pinned) (void)hipHostFree(pinned);
+  }
   void clear() {
     for (MgLevel* l : lv) delete l;
     lv.clear();

@@ -1,6 +1,9 @@
 // mg.hip -- aggregation multilevel preconditioner for the flexible CG.
 //
 // Hierarchy (set up once per matrix):
+// Set-up: the aggregation runs on the host (it is sequential by definition) over a
+// page-locked copy of the node graph; everything else (Gram-Schmidt, coarse pattern, SELL
+// layout, Galerkin product, spectral bounds) runs on the device.
 //   aggregation   greedy over the node graph of each level, in node order: a node whose
 //                 neighbours are all free becomes a root and takes its whole neighbourhood;
 //                 leftovers join the aggregate of their first aggregated neighbour (column
@@ -24,8 +27,11 @@
 //              flexible CG steps preconditioned by the cycle of level l+1) while l+1 <= kdepth,
 //              and one cycle of level l+1 (V-cycle) below that.
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -451,6 +457,151 @@ __global__ void ks_dinv(MatS A, double* __restrict__ dinv) {
   dinv[i] = d > 0.0 ? 1.0 / d : 0.0;
 }
 
+// ---- hierarchy set-up on the device ------------------------------------------------------
+
+// Entry (row, c) of a level's near-null space: level 0 (B == nullptr) evaluates the global
+// rigid-body modes from the node coordinates (x translation, y translation, rotation about
+// (cx, cy)), zero on constrained DOFs; coarser levels read the R factors of the level above.
+__device__ inline double nns_entry(int row, int c, const double* __restrict__ B,
+                                   const double* __restrict__ xy,
+                                   const unsigned char* __restrict__ fixed, double cx, double cy) {
+  if (B) return B[(size_t)row * 3 + c];
+  if (fixed && fixed[row]) return 0.0;
+  const int i = row >> 1;
+  if ((row & 1) == 0) return c == 0 ? 1.0 : (c == 1 ? 0.0 : -(xy[2 * (size_t)i + 1] - cy));
+  return c == 0 ? 0.0 : (c == 1 ? 1.0 : xy[2 * (size_t)i] - cx);
+}
+
+// Modified Gram-Schmidt of the near-null space on each aggregate (one thread per aggregate,
+// rows in member order, then DOF order): Q into P (n x 3), R (3x3 row-major) into Rn[9a..].
+// A column whose remaining norm is <= 1e-10 of its original norm (or zero) is dead: zero in Q
+// and a zero row in R.
+__global__ void k_mgs(int na, int bs, const int* __restrict__ mptr, const int* __restrict__ mem,
+                      const double* __restrict__ B, const double* __restrict__ xy,
+                      const unsigned char* __restrict__ fixed, double cx, double cy,
+                      double* __restrict__ P, double* __restrict__ Rn) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= na) return;
+  const int m0 = mptr[a], m = bs * (mptr[a + 1] - m0);
+  double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  bool alive[3] = {false, false, false};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double n0 = 0.0;
+    for (int k = 0; k < m; ++k) {
+      const int row = bs * mem[m0 + k / bs] + k % bs;
+      const double v = nns_entry(row, c, B, xy, fixed, cx, cy);
+      P[(size_t)row * 3 + c] = v;
+      n0 += v * v;
+    }
+    n0 = sqrt(n0);
+#pragma unroll
+    for (int cc = 0; cc < c; ++cc) {
+      if (!alive[cc]) continue;
+      double dot = 0.0;
+      for (int k = 0; k < m; ++k) {
+        const int row = bs * mem[m0 + k / bs] + k % bs;
+        dot += P[(size_t)row * 3 + cc] * P[(size_t)row * 3 + c];
+      }
+      R[cc][c] = dot;
+      for (int k = 0; k < m; ++k) {
+        const int row = bs * mem[m0 + k / bs] + k % bs;
+        P[(size_t)row * 3 + c] -= dot * P[(size_t)row * 3 + cc];
+      }
+    }
+    double nr = 0.0;
+    for (int k = 0; k < m; ++k) {
+      const int row = bs * mem[m0 + k / bs] + k % bs;
+      nr += P[(size_t)row * 3 + c] * P[(size_t)row * 3 + c];
+    }
+    nr = sqrt(nr);
+    const bool ok = n0 > 0.0 && nr > 1e-10 * n0;
+    alive[c] = ok;
+    if (ok) R[c][c] = nr;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+      if (!ok) R[c][cc] = 0.0;
+    for (int k = 0; k < m; ++k) {
+      const int row = bs * mem[m0 + k / bs] + k % bs;
+      P[(size_t)row * 3 + c] = ok ? P[(size_t)row * 3 + c] / nr : 0.0;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) Rn[9 * (size_t)a + 3 * c + cc] = alive[c] ? R[c][cc] : 0.0;
+}
+
+constexpr int kMaxCoarseDeg = 128;  // block columns of one coarse block row
+
+// Sorted distinct aggregates adjacent to aggregate a (over its members' block rows); -1 if
+// there are more than kMaxCoarseDeg.
+__device__ inline int coarse_row(int a, const int* __restrict__ mptr, const int* __restrict__ mem,
+                                 const int* __restrict__ bptr, const int* __restrict__ bcol,
+                                 const int* __restrict__ agg, int* out) {
+  int len = 0;
+  for (int m = mptr[a]; m < mptr[a + 1]; ++m) {
+    const int i = mem[m];
+    for (int q = bptr[i]; q < bptr[i + 1]; ++q) {
+      const int b = agg[bcol[q]];
+      int j = len - 1;
+      while (j >= 0 && out[j] > b) --j;
+      if (j >= 0 && out[j] == b) continue;
+      if (len == kMaxCoarseDeg) return -1;
+      for (int t = len; t > j + 1; --t) out[t] = out[t - 1];
+      out[j + 1] = b;
+      ++len;
+    }
+  }
+  return len;
+}
+
+__global__ void k_cpat_count(int na, const int* __restrict__ mptr, const int* __restrict__ mem,
+                             const int* __restrict__ bptr, const int* __restrict__ bcol,
+                             const int* __restrict__ agg, int* __restrict__ deg,
+                             int* __restrict__ err) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= na) return;
+  int buf[kMaxCoarseDeg];
+  const int len = coarse_row(a, mptr, mem, bptr, bcol, agg, buf);
+  if (len < 0) atomicMax(err, 1);
+  deg[a] = len < 0 ? 0 : len;
+}
+
+__global__ void k_cpat_fill(int na, const int* __restrict__ mptr, const int* __restrict__ mem,
+                            const int* __restrict__ bptr, const int* __restrict__ bcol,
+                            const int* __restrict__ agg, const int* __restrict__ cptr,
+                            int* __restrict__ ccol, int* __restrict__ crow) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= na) return;
+  int buf[kMaxCoarseDeg];
+  const int len = coarse_row(a, mptr, mem, bptr, bcol, agg, buf);
+  const int o = cptr[a];
+  for (int t = 0; t < len; ++t) {
+    ccol[o + t] = buf[t];
+    crow[o + t] = a;
+  }
+}
+
+// column indices of the scalar SELL-64 layout of a 3x3-block pattern: lane = row, block
+// column t of block row a gives the columns 3 ccol[t] + c'; padding repeats the row's own
+// column (value 0)
+__global__ void k_sell_cols(int n, int nslices, const int64_t* __restrict__ off,
+                            const int* __restrict__ cptr, const int* __restrict__ ccol,
+                            int* __restrict__ col) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= 64 * nslices) return;
+  const int s = r >> 6, lane = r & 63;
+  const int w = (int)((off[s + 1] - off[s]) >> 6);
+  const int a = r / 3;
+  const int nbk = r < n ? cptr[a + 1] - cptr[a] : 0;
+  for (int k = 0; k < w; ++k) {
+    int cidx = r < n ? r : n - 1;
+    if (k < 3 * nbk) cidx = 3 * ccol[cptr[a] + k / 3] + k % 3;
+    col[off[s] + (int64_t)k * 64 + lane] = cidx;
+  }
+}
+
 // y = D^-1 A x, partial sums of y'y
 __global__ __launch_bounds__(kT) void ks_power(MatS A, const double* __restrict__ dinv,
                                                const double* __restrict__ x,
@@ -542,9 +693,37 @@ __global__ __launch_bounds__(kT) void k0_gersh(Mat0 A, const double2* __restrict
   }
 }
 
-__global__ void k_scale(int64_t n, double* __restrict__ x, double f) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) x[i] *= f;
+// start vector of the power iteration: 1 + 0.5 sin(0.7 i + 0.3)
+__global__ void k_power_init(int n, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 1.0 + 0.5 * sin(0.7 * (double)i + 0.3);
+}
+
+__global__ __launch_bounds__(kT) void k_sumsq(int n, const double* __restrict__ x,
+                                              double* __restrict__ part) {
+  __shared__ double lds[kW];
+  double v = 0.0;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) v += x[i] * x[i];
+  const double s = block_sum(v, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// y /= ||y|| from the partials of y'y (summed in the same order by every workgroup).
+// st[0] = ||y|| when `record`; st[1] != 0 ends the iteration (set once ||y|| is not positive,
+// by workgroup 0, while every workgroup of that launch already skips the scaling itself).
+__global__ __launch_bounds__(kT) void k_power_scale(int n, double* __restrict__ y,
+                                                    const double* __restrict__ part, int nparts,
+                                                    double* __restrict__ st, int record) {
+  __shared__ double lds[kW];
+  if (st[1] != 0.0) return;
+  const double yn = sqrt(sum_parts(part, nparts, lds));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (record) st[0] = yn;
+    if (!(yn > 0.0)) st[1] = 1.0;
+  }
+  if (!(yn > 0.0)) return;
+  const double f = 1.0 / yn;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) y[i] *= f;
 }
 
 // ---- host helpers ------------------------------------------------------------------------
@@ -561,8 +740,7 @@ MatS mats(MgLevel& L) { return MatS{L.off.get(), L.col.get(), L.val.get(), L.n};
 int slice_grid(int nslices) { return std::max(1, std::min(2048, (nslices + kW - 1) / kW)); }
 
 // Greedy aggregation of a node graph (see the file header).
-int aggregate(const std::vector<int>& ptr, const std::vector<int>& col, std::vector<int>& agg) {
-  const int n = (int)ptr.size() - 1;
+int aggregate(const int* ptr, const int* col, int n, std::vector<int>& agg) {
   agg.assign(n, -1);
   int na = 0;
   for (int i = 0; i < n; ++i) {
@@ -587,80 +765,30 @@ int aggregate(const std::vector<int>& ptr, const std::vector<int>& col, std::vec
   return na;
 }
 
-// Modified Gram-Schmidt on the rows `rows` (count m) of B (3 columns): Q into P (same rows),
-// R (3x3, row-major) into Rout.  A column whose remaining norm is <= 1e-10 of its original
-// norm (or zero) is dead: zero in Q and a zero row in R.
-void mgs(const std::vector<double>& B, const int* rows, int m, std::vector<double>& P,
-         double* Rout) {
-  double q[3][64 * 3];  // scratch per column (m <= 192 rows)
-  double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  bool alive[3] = {false, false, false};
-  for (int c = 0; c < 3; ++c) {
-    double n0 = 0.0;
-    for (int k = 0; k < m; ++k) {
-      q[c][k] = B[(size_t)rows[k] * 3 + c];
-      n0 += q[c][k] * q[c][k];
-    }
-    n0 = std::sqrt(n0);
-    for (int cc = 0; cc < c; ++cc) {
-      if (!alive[cc]) continue;
-      double dot = 0.0;
-      for (int k = 0; k < m; ++k) dot += q[cc][k] * q[c][k];
-      R[cc][c] = dot;
-      for (int k = 0; k < m; ++k) q[c][k] -= dot * q[cc][k];
-    }
-    double nr = 0.0;
-    for (int k = 0; k < m; ++k) nr += q[c][k] * q[c][k];
-    nr = std::sqrt(nr);
-    if (n0 > 0.0 && nr > 1e-10 * n0) {
-      alive[c] = true;
-      R[c][c] = nr;
-      for (int k = 0; k < m; ++k) q[c][k] /= nr;
-    } else {
-      for (int cc = 0; cc < 3; ++cc) R[c][cc] = 0.0;
-      for (int k = 0; k < m; ++k) q[c][k] = 0.0;
-    }
-  }
-  for (int c = 0; c < 3; ++c)
-    if (!alive[c])
-      for (int cc = 0; cc < 3; ++cc) R[c][cc] = 0.0;
-  for (int k = 0; k < m; ++k)
-    for (int c = 0; c < 3; ++c) P[(size_t)rows[k] * 3 + c] = q[c][k];
-  for (int c = 0; c < 3; ++c)
-    for (int cc = 0; cc < 3; ++cc) Rout[3 * c + cc] = R[c][cc];
-}
-
-// scalar SELL-64 layout of a 3x3-block pattern (cptr, ccol over na block rows)
-void sell_layout(MgLevel& L, const std::vector<int>& cptr, const std::vector<int>& ccol) {
+// slice offsets of the scalar SELL-64 layout of a 3x3-block pattern (cptr over n / 3 block
+// rows): the width of a slice is that of its longest row
+void sell_offsets(MgLevel& L, const std::vector<int>& cptr) {
   const int n = L.n;
   L.nslices = (n + 63) / 64;
   L.h_off.assign(L.nslices + 1, 0);
   for (int s = 0; s < L.nslices; ++s) {
     int w = 0;
-    for (int r = 64 * s; r < std::min(n, 64 * s + 64); ++r) w = std::max(w, 3 * (cptr[r / 3 + 1] - cptr[r / 3]));
+    for (int a = (64 * s) / 3; a <= (std::min(n, 64 * s + 64) - 1) / 3; ++a)
+      w = std::max(w, 3 * (cptr[a + 1] - cptr[a]));
     L.h_off[s + 1] = L.h_off[s] + 64 * (int64_t)w;
   }
-  std::vector<int> col(L.h_off[L.nslices]);
-  for (int s = 0; s < L.nslices; ++s) {
-    const int w = (int)((L.h_off[s + 1] - L.h_off[s]) / 64);
-    for (int lane = 0; lane < 64; ++lane) {
-      const int r = 64 * s + lane;
-      for (int k = 0; k < w; ++k) {
-        int cidx = r < n ? r : n - 1;  // padding: the row's own column (value 0)
-        if (r < n) {
-          const int a = r / 3, nbk = cptr[a + 1] - cptr[a];
-          if (k < 3 * nbk) cidx = 3 * ccol[cptr[a] + k / 3] + k % 3;
-        }
-        col[L.h_off[s] + (int64_t)k * 64 + lane] = cidx;
-      }
-    }
+}
+
+// page-locked staging for pattern downloads, kept across set-ups
+int* pinned_ints(Multigrid& mg, size_t count) {
+  if (count > mg.pinned_n) {
+    if (mg.pinned) HIP_CHECK(hipHostFree(mg.pinned));
+    mg.pinned = nullptr;
+    mg.pinned_n = 0;
+    HIP_CHECK(hipHostMalloc((void**)&mg.pinned, count * sizeof(int), hipHostMallocDefault));
+    mg.pinned_n = count;
   }
-  L.off.alloc(L.h_off.size());
-  L.col.alloc(col.size());
-  L.val.alloc(col.size());
-  HIP_CHECK(hipMemcpy(L.off.get(), L.h_off.data(), sizeof(int64_t) * L.h_off.size(), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(L.col.get(), col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemset(L.val.get(), 0, sizeof(double) * col.size()));
+  return mg.pinned;
 }
 
 template <class T>
@@ -669,22 +797,23 @@ void upload(DBuf<T>& d, const std::vector<T>& h) {
   if (!h.empty()) HIP_CHECK(hipMemcpy(d.get(), h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
 }
 
-// largest eigenvalue of D^-1 A by 20 power steps from a fixed vector
+// largest eigenvalue of D^-1 A by 20 power steps from a fixed vector, all on the device
+// (one host synchronisation at the end)
 double power_max(Context& c, int level) {
   MgLevel& L = *c.mg.lv[level];
   const int n = L.n;
-  DBuf<double> x, y, part;
+  DBuf<double> x, y, part, st;
   x.alloc(n);
   y.alloc(n);
-  std::vector<double> h(n);
-  for (int i = 0; i < n; ++i) h[i] = 1.0 + 0.5 * std::sin(0.7 * i + 0.3);
-  HIP_CHECK(hipMemcpy(x.get(), h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  st.alloc(2);
   const int g = level == 0 ? slice_grid(c.nslices) : grid1(n);
-  part.alloc(g);
-  std::vector<double> hp(g);
-  double lam = 0.0, xn = 0.0;
-  for (int i = 0; i < n; ++i) xn += h[i] * h[i];
-  xn = std::sqrt(xn);
+  const int gn = std::min(kPart, grid1(n));
+  part.alloc(std::max(g, gn));
+  HIP_CHECK(hipMemsetAsync(st.get(), 0, 2 * sizeof(double), c.stream));
+  hipLaunchKernelGGL(k_power_init, dim3(grid1(n)), dim3(kT), 0, c.stream, n, x.get());
+  hipLaunchKernelGGL(k_sumsq, dim3(gn), dim3(kT), 0, c.stream, n, (const double*)x.get(), part.get());
+  hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x.get(),
+                     (const double*)part.get(), gn, st.get(), 0);
   for (int it = 0; it < 20; ++it) {
     if (level == 0)
       hipLaunchKernelGGL(k0_power, dim3(g), dim3(kT), 0, c.stream, mat0(c),
@@ -693,18 +822,14 @@ double power_max(Context& c, int level) {
     else
       hipLaunchKernelGGL(ks_power, dim3(g), dim3(kT), 0, c.stream, mats(L),
                          (const double*)L.dinv.get(), (const double*)x.get(), y.get(), part.get());
-    HIP_CHECK(hipMemcpyAsync(hp.data(), part.get(), sizeof(double) * g, hipMemcpyDeviceToHost, c.stream));
-    HIP_CHECK(hipStreamSynchronize(c.stream));
-    double yn = 0.0;
-    for (double v : hp) yn += v;
-    yn = std::sqrt(yn);
-    lam = yn / xn;
-    if (!(yn > 0.0)) break;
-    hipLaunchKernelGGL(k_scale, dim3(grid1(n)), dim3(kT), 0, c.stream, (int64_t)n, y.get(), 1.0 / yn);
+    hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, y.get(),
+                       (const double*)part.get(), g, st.get(), 1);
     std::swap(x.p, y.p);
-    xn = 1.0;
   }
   HIP_CHECK(hipGetLastError());
+  double lam = 0.0;
+  HIP_CHECK(hipMemcpyAsync(&lam, st.get(), sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
   return lam;
 }
 
@@ -805,6 +930,21 @@ void coarsest_inverse(Context& c, MgLevel& L) {
   c.mg.nc = n;
   upload(c.mg.cinv, inv);
 }
+
+// Phase times of the setup on stderr when CSTFEM_MG_TRACE is set (each mark synchronises the
+// stream, so the trace itself slows the setup down a little).
+struct SetupTrace {
+  bool on = std::getenv("CSTFEM_MG_TRACE") != nullptr;
+  std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+  void mark(Context& c, int level, const char* what) {
+    if (!on) return;
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[mg setup] level %d %-20s %8.3f ms\n", level, what,
+                 std::chrono::duration<double, std::milli>(now - last).count());
+    last = now;
+  }
+};
 
 void alloc_work(MgLevel& L) {
   for (DBuf<double>* v : {&L.rhs, &L.x, &L.x2, &L.d, &L.res, &L.kx, &L.kres, &L.kp, &L.kq, &L.kz}) {
@@ -953,143 +1093,151 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
   mg.clear();
   mg.fine_smoother = fine_smoother;
   mg.kdepth = kdepth;
+  SetupTrace tr;
   HIP_CHECK(hipStreamSynchronize(c.stream));
-  // level 0: the context's matrix and block pattern
+  // level 0: the context's matrix; its node graph is read on the host (aggregation) from a
+  // page-locked copy
   MgLevel* L0 = new MgLevel;
   mg.lv.push_back(L0);
   L0->n = c.n;
   L0->nodes = c.nb;
   L0->bs = 2;
-  L0->bptr.resize(c.nb + 1);
-  L0->bcol.resize(c.nblk);
-  HIP_CHECK(hipMemcpy(L0->bptr.data(), c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost));
-  HIP_CHECK(hipMemcpy(L0->bcol.data(), c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost));
-  std::vector<unsigned char> fixed(c.n, 0);
-  if (c.fixed.n == (size_t)c.n)
-    HIP_CHECK(hipMemcpy(fixed.data(), c.fixed.get(), c.n, hipMemcpyDeviceToHost));
+  int* pin = pinned_ints(mg, (size_t)c.nb + 1 + (size_t)c.nblk);
+  HIP_CHECK(hipMemcpyAsync(pin, c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1, c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  tr.mark(c, 0, "download pattern");
   alloc_work(*L0);
   if (fine_smoother == 1) L0->lam = smoothing_max(c, 0);
-  // near-null space of level 0 is built per aggregate (centroid-relative rotation)
-  std::vector<double> B;
+  tr.mark(c, 0, "work, smoothing");
+  // rigid-body rotation about the centre of the mesh's bounding box
+  double xlo = c.h_xy[0], xhi = c.h_xy[0], ylo = c.h_xy[1], yhi = c.h_xy[1];
+  for (int i = 0; i < c.nb; ++i) {
+    xlo = std::min(xlo, c.h_xy[2 * (size_t)i]);
+    xhi = std::max(xhi, c.h_xy[2 * (size_t)i]);
+    ylo = std::min(ylo, c.h_xy[2 * (size_t)i + 1]);
+    yhi = std::max(yhi, c.h_xy[2 * (size_t)i + 1]);
+  }
+  const double cx = 0.5 * (xlo + xhi), cy = 0.5 * (ylo + yhi);
+  const unsigned char* fixed = c.fixed.n == (size_t)c.n ? c.fixed.get() : nullptr;
+  DBuf<int> flag;
+  flag.alloc(1);
   for (int l = 0;; ++l) {
     MgLevel& L = *mg.lv[l];
     const bool last = l + 1 >= max_levels - 1;
-    // aggregation and members (stable counting sort: ascending node order per aggregate)
-    L.nagg = aggregate(L.bptr, L.bcol, L.h_agg);
+    // aggregation (host, greedy in node order) and members (ascending node order)
+    const int* hp = l == 0 ? pin : L.bptr.data();
+    const int* hc = l == 0 ? pin + c.nb + 1 : L.bcol.data();
+    L.nagg = aggregate(hp, hc, L.nodes, L.h_agg);
     const int na = L.nagg;
     std::vector<int> mptr(na + 1, 0), mem(L.nodes);
     for (int i = 0; i < L.nodes; ++i) ++mptr[L.h_agg[i] + 1];
     for (int a = 0; a < na; ++a) mptr[a + 1] += mptr[a];
+    for (int a = 0; a < na; ++a)
+      if (L.bs * (mptr[a + 1] - mptr[a]) > 192) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: aggregate too large");
     {
       std::vector<int> fill(mptr.begin(), mptr.end() - 1);
       for (int i = 0; i < L.nodes; ++i) mem[fill[L.h_agg[i]]++] = i;
     }
-    // near-null space rows of this level: the global rigid-body modes (rotation about the
-    // centre of the mesh's bounding box), so that every level's R carries the same modes
-    if (l == 0) {
-      B.assign((size_t)L.n * 3, 0.0);
-      double xlo = c.h_xy[0], xhi = c.h_xy[0], ylo = c.h_xy[1], yhi = c.h_xy[1];
-      for (int i = 0; i < L.nodes; ++i) {
-        xlo = std::min(xlo, c.h_xy[2 * (size_t)i]);
-        xhi = std::max(xhi, c.h_xy[2 * (size_t)i]);
-        ylo = std::min(ylo, c.h_xy[2 * (size_t)i + 1]);
-        yhi = std::max(yhi, c.h_xy[2 * (size_t)i + 1]);
-      }
-      const double cx = 0.5 * (xlo + xhi), cy = 0.5 * (ylo + yhi);
-      for (int i = 0; i < L.nodes; ++i) {
-        const double dx = c.h_xy[2 * (size_t)i] - cx, dy = c.h_xy[2 * (size_t)i + 1] - cy;
-        if (!fixed[2 * i]) {
-          B[(size_t)(2 * i) * 3 + 0] = 1.0;
-          B[(size_t)(2 * i) * 3 + 2] = -dy;
-        }
-        if (!fixed[2 * i + 1]) {
-          B[(size_t)(2 * i + 1) * 3 + 1] = 1.0;
-          B[(size_t)(2 * i + 1) * 3 + 2] = dx;
-        }
-      }
-    }
-    // tentative prolongator and the next near-null space
-    std::vector<double> P((size_t)L.n * 3, 0.0), Bn((size_t)na * 9, 0.0);
-    {
-      std::vector<int> rows;
-      for (int a = 0; a < na; ++a) {
-        rows.clear();
-        for (int m = mptr[a]; m < mptr[a + 1]; ++m)
-          for (int k = 0; k < L.bs; ++k) rows.push_back(L.bs * mem[m] + k);
-        if ((int)rows.size() > 192) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: aggregate too large");
-        mgs(B, rows.data(), (int)rows.size(), P, &Bn[(size_t)a * 9]);
-      }
-    }
-    // coarse block pattern
+    upload(L.agg, L.h_agg);
+    upload(L.mptr, mptr);
+    upload(L.mem, mem);
+    tr.mark(c, l, "aggregate, members");
     MgLevel* C = new MgLevel;
     mg.lv.push_back(C);
     C->nodes = na;
     C->n = 3 * na;
     C->bs = 3;
-    C->bptr.assign(na + 1, 0);
-    {
-      std::vector<int> mark(na, -1), row;
-      for (int a = 0; a < na; ++a) {
-        row.clear();
-        for (int m = mptr[a]; m < mptr[a + 1]; ++m) {
-          const int i = mem[m];
-          for (int q = L.bptr[i]; q < L.bptr[i + 1]; ++q) {
-            const int b = L.h_agg[L.bcol[q]];
-            if (mark[b] != a) {
-              mark[b] = a;
-              row.push_back(b);
-            }
-          }
-        }
-        std::sort(row.begin(), row.end());
-        C->bcol.insert(C->bcol.end(), row.begin(), row.end());
-        C->bptr[a + 1] = (int)C->bcol.size();
-      }
-    }
-    sell_layout(*C, C->bptr, C->bcol);
-    // device copies of the transfer and the Galerkin product
-    upload(L.agg, L.h_agg);
-    upload(L.mptr, mptr);
-    upload(L.mem, mem);
-    upload(L.P, P);
-    upload(C->d_bptr, C->bptr);
-    upload(C->d_bcol, C->bcol);
-    std::vector<int> crow(C->bcol.size());
-    for (int a = 0; a < na; ++a)
-      for (int t = C->bptr[a]; t < C->bptr[a + 1]; ++t) crow[t] = a;
-    DBuf<int> d_crow;
-    upload(d_crow, crow);
-    const int ncblk = (int)crow.size();
-    if (l == 0)
-      hipLaunchKernelGGL(k_rap0, dim3(grid1(ncblk)), dim3(kT), 0, c.stream, ncblk,
-                         (const int*)d_crow.get(), (const int*)C->d_bptr.get(),
-                         (const int*)C->d_bcol.get(), (const int*)L.mptr.get(),
-                         (const int*)L.mem.get(), (const int*)L.agg.get(), (const int*)c.brow_ptr.get(),
-                         (const int*)c.bcol.get(), (const int64_t*)c.sell_pos.get(),
-                         (const double*)c.sell_val.get(), (const double*)L.P.get(),
-                         (const int64_t*)C->off.get(), C->val.get());
-    else
-      hipLaunchKernelGGL(k_rapS, dim3(grid1(ncblk)), dim3(kT), 0, c.stream, ncblk,
-                         (const int*)d_crow.get(), (const int*)C->d_bptr.get(),
-                         (const int*)C->d_bcol.get(), (const int*)L.mptr.get(),
-                         (const int*)L.mem.get(), (const int*)L.agg.get(), (const int*)L.d_bptr.get(),
-                         (const int*)L.d_bcol.get(), (const int64_t*)L.off.get(),
-                         (const double*)L.val.get(), (const double*)L.P.get(),
-                         (const int64_t*)C->off.get(), C->val.get());
+    // tentative prolongator and the next near-null space (device)
+    L.P.alloc((size_t)L.n * 3);
+    C->nns.alloc((size_t)na * 9);
+    hipLaunchKernelGGL(k_mgs, dim3(grid1(na)), dim3(kT), 0, c.stream, na, L.bs,
+                       (const int*)L.mptr.get(), (const int*)L.mem.get(),
+                       (const double*)(l == 0 ? nullptr : L.nns.get()), (const double*)c.xy.get(),
+                       fixed, cx, cy, L.P.get(), C->nns.get());
     HIP_CHECK(hipGetLastError());
+    tr.mark(c, l, "gram-schmidt");
+    // coarse block pattern (device): sorted distinct neighbour aggregates of each aggregate
+    const int* dbp = l == 0 ? c.brow_ptr.get() : L.d_bptr.get();
+    const int* dbc = l == 0 ? c.bcol.get() : L.d_bcol.get();
+    DBuf<int> deg;
+    deg.alloc(na);
+    HIP_CHECK(hipMemsetAsync(flag.get(), 0, sizeof(int), c.stream));
+    hipLaunchKernelGGL(k_cpat_count, dim3(grid1(na)), dim3(kT), 0, c.stream, na,
+                       (const int*)L.mptr.get(), (const int*)L.mem.get(), dbp, dbc,
+                       (const int*)L.agg.get(), deg.get(), flag.get());
+    HIP_CHECK(hipGetLastError());
+    std::vector<int> hdeg(na);
+    int hflag = 0;
+    HIP_CHECK(hipMemcpyAsync(hdeg.data(), deg.get(), sizeof(int) * na, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(&hflag, flag.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    if (hflag)
+      throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: an aggregate touches more than " +
+                                           std::to_string(kMaxCoarseDeg) + " aggregates");
+    C->bptr.assign(na + 1, 0);
+    for (int a = 0; a < na; ++a) C->bptr[a + 1] = C->bptr[a] + hdeg[a];
+    const int ncblk = C->bptr[na];
+    upload(C->d_bptr, C->bptr);
+    C->d_bcol.alloc(std::max(ncblk, 1));
+    DBuf<int> d_crow;
+    d_crow.alloc(std::max(ncblk, 1));
+    hipLaunchKernelGGL(k_cpat_fill, dim3(grid1(na)), dim3(kT), 0, c.stream, na,
+                       (const int*)L.mptr.get(), (const int*)L.mem.get(), dbp, dbc,
+                       (const int*)L.agg.get(), (const int*)C->d_bptr.get(), C->d_bcol.get(),
+                       d_crow.get());
+    HIP_CHECK(hipGetLastError());
+    C->bcol.resize(ncblk);
+    HIP_CHECK(hipMemcpyAsync(C->bcol.data(), C->d_bcol.get(), sizeof(int) * ncblk, hipMemcpyDeviceToHost, c.stream));
+    tr.mark(c, l, "coarse pattern");
+    // scalar SELL-64 layout of the coarse matrix
+    sell_offsets(*C, C->bptr);
+    upload(C->off, C->h_off);
+    const int64_t nent = C->h_off[C->nslices];
+    C->col.alloc(std::max<int64_t>(nent, 1));
+    C->val.alloc(std::max<int64_t>(nent, 1));
+    HIP_CHECK(hipMemsetAsync(C->val.get(), 0, sizeof(double) * std::max<int64_t>(nent, 1), c.stream));
+    hipLaunchKernelGGL(k_sell_cols, dim3(grid1(64 * (int64_t)C->nslices)), dim3(kT), 0, c.stream, C->n,
+                       C->nslices, (const int64_t*)C->off.get(), (const int*)C->d_bptr.get(),
+                       (const int*)C->d_bcol.get(), C->col.get());
+    HIP_CHECK(hipGetLastError());
+    tr.mark(c, l, "sell layout");
+    // Galerkin product
+    if (ncblk > 0) {
+      if (l == 0)
+        hipLaunchKernelGGL(k_rap0, dim3(grid1(ncblk)), dim3(kT), 0, c.stream, ncblk,
+                           (const int*)d_crow.get(), (const int*)C->d_bptr.get(),
+                           (const int*)C->d_bcol.get(), (const int*)L.mptr.get(),
+                           (const int*)L.mem.get(), (const int*)L.agg.get(), (const int*)c.brow_ptr.get(),
+                           (const int*)c.bcol.get(), (const int64_t*)c.sell_pos.get(),
+                           (const double*)c.sell_val.get(), (const double*)L.P.get(),
+                           (const int64_t*)C->off.get(), C->val.get());
+      else
+        hipLaunchKernelGGL(k_rapS, dim3(grid1(ncblk)), dim3(kT), 0, c.stream, ncblk,
+                           (const int*)d_crow.get(), (const int*)C->d_bptr.get(),
+                           (const int*)C->d_bcol.get(), (const int*)L.mptr.get(),
+                           (const int*)L.mem.get(), (const int*)L.agg.get(), (const int*)L.d_bptr.get(),
+                           (const int*)L.d_bcol.get(), (const int64_t*)L.off.get(),
+                           (const double*)L.val.get(), (const double*)L.P.get(),
+                           (const int64_t*)C->off.get(), C->val.get());
+      HIP_CHECK(hipGetLastError());
+    }
     C->dinv.alloc(C->n);
     hipLaunchKernelGGL(ks_dinv, dim3(grid1(C->n)), dim3(kT), 0, c.stream, mats(*C), C->dinv.get());
-    HIP_CHECK(hipStreamSynchronize(c.stream));
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(c.stream));  // C->bcol on the host, C->val for the inverse
     alloc_work(*C);
-    B.swap(Bn);
+    tr.mark(c, l, "galerkin, dinv, work");
     if (last || C->n <= coarse_max || na == L.nodes) {
       if (C->n > 4096)
         throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: coarsest level has " + std::to_string(C->n) +
                                              " DOFs (> 4096): allow more levels");
       coarsest_inverse(c, *C);
+      tr.mark(c, l + 1, "coarsest inverse");
       break;
     }
     C->lam = smoothing_max(c, l + 1);
+    tr.mark(c, l + 1, "smoothing bound");
   }
   HIP_CHECK(hipStreamSynchronize(c.stream));
   mg.ready = true;
