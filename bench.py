@@ -104,7 +104,7 @@ def main():
                          "mg_cheb (multilevel with a Chebyshev smoother)")
     ap.add_argument("--mg-levels", type=int, default=12)
     ap.add_argument("--mg-coarse", type=int, default=500)
-    ap.add_argument("--mg-kcycle", type=int, default=16, help="coarse levels on the K-cycle")
+    ap.add_argument("--mg-kcycle", type=int, default=3, help="coarse levels on the K-cycle")
     ap.add_argument("--train-n", type=int, default=64, help="training plate nodes per side")
     ap.add_argument("--train-steps", type=int, default=300)
     args = ap.parse_args()

@@ -164,9 +164,10 @@ struct MgLevel {
 struct Multigrid {
   bool ready = false;
   int fine_smoother = 0;  // 0: learned M, 1: Chebyshev on D^-1 A
-  int kdepth = 16;        // levels 1..kdepth use the K-cycle, deeper ones the V-cycle
+  int kdepth = 3;         // levels 1..kdepth use the K-cycle, deeper ones the V-cycle
   std::vector<MgLevel*> lv;
   int nc = 0;             // DOFs of the coarsest level (dense inverse)
+  int tail_from = -1;     // first level of the fused single-workgroup tail cycle (-1: none)
   DBuf<double> cinv;
   int* pinned = nullptr;  // page-locked staging of the fine node graph, kept across set-ups
   size_t pinned_n = 0;
@@ -177,6 +178,7 @@ struct Multigrid {
   void clear() {
     for (MgLevel* l : lv) delete l;
     lv.clear();
+    tail_from = -1;
     ready = false;
   }
 };

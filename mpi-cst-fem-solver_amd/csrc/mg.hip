@@ -43,7 +43,7 @@ namespace {
 
 constexpr int kT = 256;
 constexpr int kW = kT / 64;
-constexpr int kPart = 256;  // partial slots per reduction on a coarse level
+constexpr int kPart = 2048;  // partial slots per reduction on a coarse level
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
@@ -214,32 +214,63 @@ __device__ inline double rowS(const MatS& A, const double* __restrict__ x, int i
   return a;
 }
 
+// Row i of A x computed by LPR consecutive lanes (lane j takes the entries k = j, j + LPR, ...;
+// the partial sums are combined by a butterfly, so every lane of the group returns the sum).
+// The LPR lanes of a group are always active together.
+template <int LPR>
+__device__ inline double rowS_lanes(const MatS& A, const double* __restrict__ x, int i, int j) {
+  if (LPR == 1) return rowS(A, x, i);
+  const int s = i >> 6, lane = i & 63;
+  const int64_t o = A.off[s];
+  const int w = (int)((A.off[s + 1] - o) >> 6);
+  double a = 0.0;
+  for (int k = j; k < w; k += LPR) {
+    const int64_t q = o + (int64_t)k * 64 + lane;
+    a = a + A.val[q] * x[A.col[q]];
+  }
+#pragma unroll
+  for (int m = 1; m < LPR; m <<= 1) a += __shfl_xor(a, m, 64);
+  return a;
+}
+
+// Lanes per row on a scalar level of n rows (about 21 entries per row): one lane per row while
+// there are enough rows to fill the chip, more lanes on small levels, where one lane's chain
+// of loads would set the kernel's duration.
+inline int lanes_per_row(int n) { return n >= 200000 ? 1 : n >= 50000 ? 2 : n >= 10000 ? 4 : 8; }
+
+template <int LPR>
 __global__ __launch_bounds__(kT) void ks_cheb(MatS A, const double* __restrict__ xin,
                                               double* __restrict__ xout, double* __restrict__ d,
                                               const double* __restrict__ r,
                                               const double* __restrict__ dinv, double cd, double cr,
                                               const PcgScalars* __restrict__ sc) {
   if (stopped(sc)) return;
-  const int i = blockIdx.x * kT + threadIdx.x;
+  const int t = blockIdx.x * kT + threadIdx.x;
+  const int i = t / LPR, j = t % LPR;
   if (i >= A.n) return;
-  const double y = xin ? rowS(A, xin, i) : 0.0;
+  const double y = xin ? rowS_lanes<LPR>(A, xin, i, j) : 0.0;
+  if (j != 0) return;
   const double dd = (cd != 0.0 ? cd * d[i] : 0.0) + cr * (dinv[i] * (r[i] - y));
   d[i] = dd;
   xout[i] = (xin ? xin[i] : 0.0) + dd;
 }
 
+template <int LPR>
 __global__ __launch_bounds__(kT) void ks_resid(MatS A, const double* __restrict__ x,
                                                const double* __restrict__ r,
                                                double* __restrict__ res,
                                                const PcgScalars* __restrict__ sc) {
   if (stopped(sc)) return;
-  const int i = blockIdx.x * kT + threadIdx.x;
+  const int t = blockIdx.x * kT + threadIdx.x;
+  const int i = t / LPR, j = t % LPR;
   if (i >= A.n) return;
-  res[i] = r[i] - rowS(A, x, i);
+  const double y = rowS_lanes<LPR>(A, x, i, j);
+  if (j == 0) res[i] = r[i] - y;
 }
 
 // K-cycle step: beta = first ? 0 : -z'q_old / p'q_old; q = A z + beta q; p = z + beta p;
 // partials p'q -> part[b], p'res -> part[kPart + b]
+template <int LPR>
 __global__ __launch_bounds__(kT) void ks_spmv_dir(MatS A, const double* __restrict__ z,
                                                   double* __restrict__ p, double* __restrict__ q,
                                                   const double* __restrict__ res,
@@ -256,14 +287,18 @@ __global__ __launch_bounds__(kT) void ks_spmv_dir(MatS A, const double* __restri
     beta = pq > 0.0 ? -zq / pq : 0.0;
   }
   double a = 0.0, b = 0.0;
-  for (int i = blockIdx.x * kT + threadIdx.x; i < A.n; i += gridDim.x * kT) {
-    const double y = rowS(A, z, i);
-    const double qi = first ? y : y + beta * q[i];
-    const double pi = first ? z[i] : z[i] + beta * p[i];
-    q[i] = qi;
-    p[i] = pi;
-    a += pi * qi;
-    b += pi * res[i];
+  const int j = threadIdx.x % LPR;
+  const int stride = gridDim.x * (kT / LPR);
+  for (int i = (blockIdx.x * kT + threadIdx.x) / LPR; i < A.n; i += stride) {
+    const double y = rowS_lanes<LPR>(A, z, i, j);
+    if (j == 0) {
+      const double qi = first ? y : y + beta * q[i];
+      const double pi = first ? z[i] : z[i] + beta * p[i];
+      q[i] = qi;
+      p[i] = pi;
+      a += pi * qi;
+      b += pi * res[i];
+    }
   }
   const double sa = block_sum(a, lds);
   const double sb = block_sum(b, lds);
@@ -311,7 +346,9 @@ __global__ __launch_bounds__(kT) void ks_copy(int n, const double* __restrict__ 
   for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) b[i] = a[i];
 }
 
-// rc[3a + c] = sum over member nodes i of a, k < bs:  P[(bs i + k) * 3 + c] * res[bs i + k]
+// rc[3a + c] = sum over member nodes i of a, k < bs:  P[(bs i + k) * 3 + c] * res[bs i + k];
+// LPR lanes per coarse DOF split the member rows (butterfly sum, as in rowS_lanes)
+template <int LPR>
 __global__ __launch_bounds__(kT) void k_restrict(int nagg, int bs, const int* __restrict__ mptr,
                                                  const int* __restrict__ mem,
                                                  const double* __restrict__ P,
@@ -319,18 +356,19 @@ __global__ __launch_bounds__(kT) void k_restrict(int nagg, int bs, const int* __
                                                  double* __restrict__ rc,
                                                  const PcgScalars* __restrict__ sc) {
   if (stopped(sc)) return;
-  const int t = blockIdx.x * kT + threadIdx.x;
+  const int tt = blockIdx.x * kT + threadIdx.x;
+  const int t = tt / LPR, j = tt % LPR;
   if (t >= 3 * nagg) return;
   const int a = t / 3, c = t % 3;
+  const int m0 = mptr[a], rows = bs * (mptr[a + 1] - m0);
   double s = 0.0;
-  for (int m = mptr[a]; m < mptr[a + 1]; ++m) {
-    const int i = mem[m];
-    for (int k = 0; k < bs; ++k) {
-      const int f = bs * i + k;
-      s = s + P[(size_t)f * 3 + c] * res[f];
-    }
+  for (int k = j; k < rows; k += LPR) {
+    const int f = bs * mem[m0 + k / bs] + k % bs;
+    s = s + P[(size_t)f * 3 + c] * res[f];
   }
-  rc[t] = s;
+#pragma unroll
+  for (int m = 1; m < LPR; m <<= 1) s += __shfl_xor(s, m, 64);
+  if (j == 0) rc[t] = s;
 }
 
 // x[f] += sum_c P[f * 3 + c] * ec[3 agg(node of f) + c]
@@ -359,6 +397,108 @@ __global__ __launch_bounds__(kT) void k_dense(int n, const double* __restrict__ 
   double s = 0.0;
   for (int j = 0; j < n; ++j) s = s + row[j] * r[j];
   x[i] = s;
+}
+
+// ---- fused tail: the V-cycle of the smallest levels in one workgroup ------------------------
+
+constexpr int kTailLevels = 4;      // cycle levels in the fused tail (plus the dense coarsest)
+constexpr int kTailThreads = 1024;
+constexpr int kTailMaxN = 3072;     // largest level that runs inside the tail
+
+struct TailLevel {
+  MatS A;
+  const double* dinv;
+  const int* mptr;
+  const int* mem;
+  const int* agg;
+  const double* P;
+  int nagg, bs;
+  double* rhs;  // right-hand side (filled by the restriction of the level above, except the top)
+  double* out;  // result of this level's cycle
+  double *x, *x2, *d, *res;
+  double c1r, c2d, c2r;
+};
+
+struct Tail {
+  int nlev;
+  TailLevel L[kTailLevels];
+  const double* cinv;
+  int nc;
+  double* crhs;
+  double* ckx;
+};
+
+// The same arithmetic as cycle_s (two Chebyshev steps, residual, restriction, coarse
+// correction, prolongation, two steps) on each tail level in turn, with the dense solve at the
+// bottom; one workgroup, phases separated by barriers (every vector and matrix of the tail is
+// small enough to stay in the L2 cache).
+__global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  const int tid = threadIdx.x;
+  for (int l = 0; l < T.nlev; ++l) {
+    const TailLevel& L = T.L[l];
+    const int n = L.A.n;
+    for (int i = tid; i < n; i += kTailThreads) {
+      const double dd = 0.0 + L.c1r * (L.dinv[i] * (L.rhs[i] - 0.0));
+      L.d[i] = dd;
+      L.x[i] = 0.0 + dd;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += kTailThreads) {
+      const double y = rowS(L.A, L.x, i);
+      const double dd = (L.c2d != 0.0 ? L.c2d * L.d[i] : 0.0) + L.c2r * (L.dinv[i] * (L.rhs[i] - y));
+      L.d[i] = dd;
+      L.x2[i] = L.x[i] + dd;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += kTailThreads) L.res[i] = L.rhs[i] - rowS(L.A, L.x2, i);
+    __syncthreads();
+    double* rc = l + 1 < T.nlev ? T.L[l + 1].rhs : T.crhs;
+    for (int t = tid; t < 3 * L.nagg; t += kTailThreads) {
+      const int a = t / 3, c = t % 3;
+      const int m0 = L.mptr[a], rows = L.bs * (L.mptr[a + 1] - m0);
+      double s = 0.0;
+      for (int k = 0; k < rows; ++k) {
+        const int f = L.bs * L.mem[m0 + k / L.bs] + k % L.bs;
+        s = s + L.P[(size_t)f * 3 + c] * L.res[f];
+      }
+      rc[t] = s;
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < T.nc; i += kTailThreads) {
+    const double* row = T.cinv + (size_t)i * T.nc;
+    double s = 0.0;
+    for (int j = 0; j < T.nc; ++j) s = s + row[j] * T.crhs[j];
+    T.ckx[i] = s;
+  }
+  __syncthreads();
+  for (int l = T.nlev - 1; l >= 0; --l) {
+    const TailLevel& L = T.L[l];
+    const int n = L.A.n;
+    const double* e = l + 1 < T.nlev ? T.L[l + 1].out : T.ckx;
+    for (int f = tid; f < n; f += kTailThreads) {
+      const int a = L.agg[f / L.bs];
+      const double* p = L.P + (size_t)f * 3;
+      const double* ea = e + 3 * (size_t)a;
+      L.x2[f] = L.x2[f] + (p[0] * ea[0] + p[1] * ea[1] + p[2] * ea[2]);
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += kTailThreads) {
+      const double y = rowS(L.A, L.x2, i);
+      const double dd = 0.0 + L.c1r * (L.dinv[i] * (L.rhs[i] - y));
+      L.d[i] = dd;
+      L.x[i] = L.x2[i] + dd;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += kTailThreads) {
+      const double y = rowS(L.A, L.x, i);
+      const double dd = (L.c2d != 0.0 ? L.c2d * L.d[i] : 0.0) + L.c2r * (L.dinv[i] * (L.rhs[i] - y));
+      L.d[i] = dd;
+      L.out[i] = L.x[i] + dd;
+    }
+    __syncthreads();
+  }
 }
 
 // ---- setup kernels -------------------------------------------------------------------
@@ -960,6 +1100,90 @@ void kcycle(Context& c, int l, const PcgScalars* sc);
 
 void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars* sc);
 
+// ---- launches with the lane count chosen per level ------------------------------------------
+
+#define CSTFEM_LPR_SWITCH(lpr, KERNEL, grid, ...)                                               \
+  switch (lpr) {                                                                              \
+    case 1: hipLaunchKernelGGL(KERNEL<1>, dim3(grid), dim3(kT), 0, c.stream, __VA_ARGS__); break; \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, dim3(grid), dim3(kT), 0, c.stream, __VA_ARGS__); break; \
+    case 4: hipLaunchKernelGGL(KERNEL<4>, dim3(grid), dim3(kT), 0, c.stream, __VA_ARGS__); break; \
+    default: hipLaunchKernelGGL(KERNEL<8>, dim3(grid), dim3(kT), 0, c.stream, __VA_ARGS__); break; \
+  }
+
+void launch_cheb(Context& c, MgLevel& L, const double* xin, double* xout, const double* r,
+                 double cd, double cr, const PcgScalars* sc) {
+  const int lpr = lanes_per_row(L.n);
+  CSTFEM_LPR_SWITCH(lpr, ks_cheb, grid1((int64_t)L.n * lpr), mats(L), xin, xout, L.d.get(), r,
+                    (const double*)L.dinv.get(), cd, cr, sc);
+}
+
+void launch_resid(Context& c, MgLevel& L, const double* x, const double* r, double* res,
+                  const PcgScalars* sc) {
+  const int lpr = lanes_per_row(L.n);
+  CSTFEM_LPR_SWITCH(lpr, ks_resid, grid1((int64_t)L.n * lpr), mats(L), x, r, res, sc);
+}
+
+// restriction of level L's residual res into rc (3 per aggregate)
+void launch_restrict(Context& c, MgLevel& L, const double* res, double* rc, const PcgScalars* sc) {
+  const int nc = 3 * L.nagg;
+  const int lpr = nc >= 100000 ? 1 : nc >= 20000 ? 2 : 8;
+  CSTFEM_LPR_SWITCH(lpr, k_restrict, grid1((int64_t)nc * lpr), L.nagg, L.bs,
+                    (const int*)L.mptr.get(), (const int*)L.mem.get(), (const double*)L.P.get(),
+                    res, rc, sc);
+}
+
+// grid of the K-cycle kernels of a level (all of them share it: their partials are summed
+// by count)
+int kcycle_grid(const MgLevel& L) {
+  return std::min<int64_t>(kPart, grid1((int64_t)L.n * lanes_per_row(L.n)));
+}
+
+// the fused tail of the hierarchy (levels tail_from .. coarsest), or -1 when it does not apply
+int tail_start(const Multigrid& mg) {
+  if (std::getenv("CSTFEM_MG_TAIL") && std::getenv("CSTFEM_MG_TAIL")[0] == '0') return -1;
+  const int last = (int)mg.lv.size() - 1;
+  int t = last;
+  while (t - 1 >= 1 && t - 1 > mg.kdepth && last - (t - 1) <= kTailLevels &&
+         mg.lv[t - 1]->n <= kTailMaxN)
+    --t;
+  return t < last ? t : -1;
+}
+
+void launch_tail(Context& c, int t, const double* rhs, double* out, const PcgScalars* sc) {
+  Multigrid& mg = c.mg;
+  const int last = (int)mg.lv.size() - 1;
+  Tail T{};
+  T.nlev = last - t;
+  for (int k = 0; k < T.nlev; ++k) {
+    MgLevel& L = *mg.lv[t + k];
+    const Cheb ch = cheb(L.lam);
+    TailLevel& q = T.L[k];
+    q.A = mats(L);
+    q.dinv = L.dinv.get();
+    q.mptr = L.mptr.get();
+    q.mem = L.mem.get();
+    q.agg = L.agg.get();
+    q.P = L.P.get();
+    q.nagg = L.nagg;
+    q.bs = L.bs;
+    q.rhs = k == 0 ? const_cast<double*>(rhs) : L.rhs.get();
+    q.out = k == 0 ? out : L.kx.get();
+    q.x = L.x.get();
+    q.x2 = L.x2.get();
+    q.d = L.d.get();
+    q.res = L.res.get();
+    q.c1r = ch.c1r;
+    q.c2d = ch.c2d;
+    q.c2r = ch.c2r;
+  }
+  MgLevel& Cst = *mg.lv[last];
+  T.cinv = mg.cinv.get();
+  T.nc = Cst.n;
+  T.crhs = Cst.rhs.get();
+  T.ckx = Cst.kx.get();
+  hipLaunchKernelGGL(k_tail, dim3(1), dim3(kTailThreads), 0, c.stream, T, sc);
+}
+
 // coarse correction of level l: lv[l+1].kx from lv[l+1].rhs -- the dense solve on the
 // coarsest level, the K-cycle on levels <= kdepth, one cycle (V) below
 void coarse_correction(Context& c, int l, const PcgScalars* sc) {
@@ -976,52 +1200,48 @@ void coarse_correction(Context& c, int l, const PcgScalars* sc) {
 
 // cycle on level l >= 1: out = B_l(rhs)
 void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars* sc) {
+  if (l == c.mg.tail_from) {
+    launch_tail(c, l, rhs, out, sc);
+    return;
+  }
   MgLevel& L = *c.mg.lv[l];
   MgLevel& C = *c.mg.lv[l + 1];
   const Cheb ch = cheb(L.lam);
-  const MatS A = mats(L);
-  const int g = grid1(L.n);
-  hipLaunchKernelGGL(ks_cheb, dim3(g), dim3(kT), 0, c.stream, A, (const double*)nullptr, L.x.get(),
-                     L.d.get(), rhs, (const double*)L.dinv.get(), 0.0, ch.c1r, sc);
-  hipLaunchKernelGGL(ks_cheb, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.x.get(), L.x2.get(),
-                     L.d.get(), rhs, (const double*)L.dinv.get(), ch.c2d, ch.c2r, sc);
-  hipLaunchKernelGGL(ks_resid, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.x2.get(), rhs,
-                     L.res.get(), sc);
-  hipLaunchKernelGGL(k_restrict, dim3(grid1(3 * L.nagg)), dim3(kT), 0, c.stream, L.nagg, L.bs,
-                     (const int*)L.mptr.get(), (const int*)L.mem.get(), (const double*)L.P.get(),
-                     (const double*)L.res.get(), C.rhs.get(), sc);
+  launch_cheb(c, L, nullptr, L.x.get(), rhs, 0.0, ch.c1r, sc);
+  launch_cheb(c, L, L.x.get(), L.x2.get(), rhs, ch.c2d, ch.c2r, sc);
+  launch_resid(c, L, L.x2.get(), rhs, L.res.get(), sc);
+  launch_restrict(c, L, L.res.get(), C.rhs.get(), sc);
   coarse_correction(c, l, sc);
-  hipLaunchKernelGGL(k_prolong, dim3(g), dim3(kT), 0, c.stream, L.n, L.bs, (const int*)L.agg.get(),
-                     (const double*)L.P.get(), (const double*)C.kx.get(), L.x2.get(), sc);
-  hipLaunchKernelGGL(ks_cheb, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.x2.get(), L.x.get(),
-                     L.d.get(), rhs, (const double*)L.dinv.get(), 0.0, ch.c1r, sc);
-  hipLaunchKernelGGL(ks_cheb, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.x.get(), out,
-                     L.d.get(), rhs, (const double*)L.dinv.get(), ch.c2d, ch.c2r, sc);
+  hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n, L.bs,
+                     (const int*)L.agg.get(), (const double*)L.P.get(), (const double*)C.kx.get(),
+                     L.x2.get(), sc);
+  launch_cheb(c, L, L.x2.get(), L.x.get(), rhs, 0.0, ch.c1r, sc);
+  launch_cheb(c, L, L.x.get(), out, rhs, ch.c2d, ch.c2r, sc);
 }
 
 // two flexible CG steps on level l for rhs = lv[l].rhs, result in lv[l].kx
 void kcycle(Context& c, int l, const PcgScalars* sc) {
   MgLevel& L = *c.mg.lv[l];
   const MatS A = mats(L);
-  const int g = std::min(kPart, grid1(L.n));  // grid-stride kernels with <= kPart partials
+  const int lpr = lanes_per_row(L.n);
+  const int g = kcycle_grid(L);           // grid-stride kernels with <= kPart partials
   double* s0 = L.part.get();             // p'q, p'r of step 1
   double* s1 = L.part.get() + 2 * kPart;  // z'q_old
   double* s2 = L.part.get() + 4 * kPart;  // p'q, p'r of step 2
   hipLaunchKernelGGL(ks_copy, dim3(g), dim3(kT), 0, c.stream, L.n, (const double*)L.rhs.get(),
                      L.kres.get(), sc);
   cycle_s(c, l, L.kres.get(), L.kz.get(), sc);
-  hipLaunchKernelGGL(ks_spmv_dir, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.kz.get(),
-                     L.kp.get(), L.kq.get(), (const double*)L.kres.get(), s0, (const double*)nullptr,
-                     (const double*)nullptr, g, 1, sc);
+  CSTFEM_LPR_SWITCH(lpr, ks_spmv_dir, g, A, (const double*)L.kz.get(), L.kp.get(), L.kq.get(),
+                    (const double*)L.kres.get(), s0, (const double*)nullptr, (const double*)nullptr,
+                    g, 1, sc);
   hipLaunchKernelGGL(ks_update, dim3(g), dim3(kT), 0, c.stream, L.n, L.kx.get(), L.kres.get(),
                      (const double*)L.kp.get(), (const double*)L.kq.get(), (const double*)s0, g, 1, 1,
                      sc);
   cycle_s(c, l, L.kres.get(), L.kz.get(), sc);
   hipLaunchKernelGGL(ks_dot, dim3(g), dim3(kT), 0, c.stream, L.n, (const double*)L.kz.get(),
                      (const double*)L.kq.get(), s1, sc);
-  hipLaunchKernelGGL(ks_spmv_dir, dim3(g), dim3(kT), 0, c.stream, A, (const double*)L.kz.get(),
-                     L.kp.get(), L.kq.get(), (const double*)L.kres.get(), s2, (const double*)s1,
-                     (const double*)s0, g, 0, sc);
+  CSTFEM_LPR_SWITCH(lpr, ks_spmv_dir, g, A, (const double*)L.kz.get(), L.kp.get(), L.kq.get(),
+                    (const double*)L.kres.get(), s2, (const double*)s1, (const double*)s0, g, 0, sc);
   hipLaunchKernelGGL(ks_update, dim3(g), dim3(kT), 0, c.stream, L.n, L.kx.get(), L.kres.get(),
                      (const double*)L.kp.get(), (const double*)L.kq.get(), (const double*)s2, g, 0, 0,
                      sc);
@@ -1058,9 +1278,7 @@ void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* 
   }
   hipLaunchKernelGGL(k0_resid, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)z2, r2,
                      (double2*)L.res.get(), sc);
-  hipLaunchKernelGGL(k_restrict, dim3(grid1(3 * L.nagg)), dim3(kT), 0, c.stream, L.nagg, 2,
-                     (const int*)L.mptr.get(), (const int*)L.mem.get(), (const double*)L.P.get(),
-                     (const double*)L.res.get(), C.rhs.get(), sc);
+  launch_restrict(c, L, L.res.get(), C.rhs.get(), sc);
   coarse_correction(c, 0, sc);
   hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n, 2,
                      (const int*)L.agg.get(), (const double*)L.P.get(), (const double*)C.kx.get(), z,
@@ -1240,6 +1458,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     tr.mark(c, l + 1, "smoothing bound");
   }
   HIP_CHECK(hipStreamSynchronize(c.stream));
+  mg.tail_from = tail_start(mg);
   mg.ready = true;
 }
 

@@ -428,7 +428,7 @@ class Device:
         return out
 
     # ---- multilevel preconditioner
-    def mg_setup(self, max_levels=10, coarse_max=500, fine_smoother=0, kcycle_levels=16):
+    def mg_setup(self, max_levels=10, coarse_max=500, fine_smoother=0, kcycle_levels=3):
         """fine_smoother: 0 the learned M (mpnn_build first), 1 Chebyshev on D^-1 K;
         coarse levels 1..kcycle_levels use the K-cycle, deeper ones the V-cycle."""
         _check(lib().cstfem_mg_setup(self._h, int(max_levels), int(coarse_max),

@@ -247,9 +247,9 @@ int run_case(const std::string& case_path, std::ostream& out, std::ostream& err,
     if (gmethod == "fcg_learned_multilevel") {
       check_rc(cstfem_mpnn_set_params(ctx, theta.data(), (int)theta.size()));
       check_rc(cstfem_mpnn_build(ctx));
-      check_rc(cstfem_mg_setup(ctx, 12, 500, 0, 16));
+      check_rc(cstfem_mg_setup(ctx, 12, 500, 0, 3));
     } else if (gmethod == "fcg_multilevel") {
-      check_rc(cstfem_mg_setup(ctx, 12, 500, 1, 16));
+      check_rc(cstfem_mg_setup(ctx, 12, 500, 1, 3));
     }
     cstfem_solver_options o{};
     o.rtol = rtol;

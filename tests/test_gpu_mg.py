@@ -109,11 +109,28 @@ def test_setup_order_and_arguments(dev):
 
 
 def test_coarse_level_above_one_k_cycle_grid(dev):
-    """A first coarse level larger than the K-cycle's reduction grid covers (256 workgroups x
-    256 threads = 65536 DOFs): every K-cycle kernel must stride over the whole level."""
-    system(dev, hetero(460, e_seed=7, seed=9))
+    """A first coarse level larger than the K-cycle's reduction grid covers (2048 workgroups x
+    256 threads = 524288 DOFs): every K-cycle kernel must stride over the whole level."""
+    system(dev, hetero(1300, e_seed=7, seed=9))
     dev.mg_setup(10, 300, 1)
-    assert dev.mg_level_info(1)[0] > 65536
+    assert dev.mg_level_info(1)[0] > 524288
     x, s = dev.solve(rtol=1e-10, max_iter=2000, method=FCG)
     assert s.converged and not s.fallback and s.true_rel_residual < 1e-10
     assert s.iterations < 150, s.iterations
+
+
+@pytest.mark.parametrize("kdepth", [1, 2])
+def test_fused_tail_matches_separate_kernels(dev, monkeypatch, kdepth):
+    """The smallest V-cycle levels run in one single-workgroup kernel; switching that off
+    (CSTFEM_MG_TAIL=0, read at set-up) must give the same cycle up to summation order."""
+    system(dev, hetero(60))
+    dev.mg_setup(8, 20, 1, kdepth)
+    sizes = [dev.mg_level_info(l)[0] for l in range(dev.mg_levels())]
+    assert len(sizes) >= kdepth + 3 and sizes[kdepth + 1] <= 3072, sizes
+    r = np.random.default_rng(3).standard_normal(dev.n)
+    z_tail = dev.mg_apply(r)
+    monkeypatch.setenv("CSTFEM_MG_TAIL", "0")
+    dev.mg_setup(8, 20, 1, kdepth)
+    z_kern = dev.mg_apply(r)
+    assert np.linalg.norm(z_tail - z_kern) <= 1e-12 * np.linalg.norm(z_kern)
+    assert np.linalg.norm(z_kern) > 0
