@@ -83,6 +83,21 @@ def cpu_baseline(prob, iterations, sample_iters):
                 assembly_s=t1 - t0, s_per_iteration=per_it)
 
 
+def measured_traffic(dofs):
+    """HBM bytes per launch of k_cg_spmv<F64> from the committed rocprofv3 --pmc measurement
+    (FETCH_SIZE and WRITE_SIZE runs, scaled by the copy-kernel calibration) when it was taken
+    on this workload; (None, None) otherwise."""
+    path = os.path.join(ROOT, "profiles", "k_cg_spmv_hetero4M_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    if rec.get("dofs") != dofs:
+        return None, None
+    return rec["traffic_bytes"], "profiles/k_cg_spmv_hetero4M_traffic.json (rocprofv3 --pmc)"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,7 +119,10 @@ def main():
                          "mg_cheb (multilevel with a Chebyshev smoother)")
     ap.add_argument("--mg-levels", type=int, default=12)
     ap.add_argument("--mg-coarse", type=int, default=500)
-    ap.add_argument("--mg-kcycle", type=int, default=3, help="coarse levels on the K-cycle")
+    ap.add_argument("--mg-kcycle", type=int, default=3,
+                    help="coarse levels on the K-cycle (learned fine smoother)")
+    ap.add_argument("--mg-kcycle-cheb", type=int, default=16,
+                    help="coarse levels on the K-cycle (Chebyshev fine smoother)")
     ap.add_argument("--train-n", type=int, default=64, help="training plate nodes per side")
     ap.add_argument("--train-steps", type=int, default=300)
     args = ap.parse_args()
@@ -176,8 +194,9 @@ def main():
             dev.mpnn_set_params(theta)
             dev.mpnn_build()
         if method in ("learned_mg", "mg_cheb"):
-            dev.mg_setup(args.mg_levels, args.mg_coarse, 0 if method == "learned_mg" else 1,
-                         args.mg_kcycle)
+            learned_fine = method == "learned_mg"
+            dev.mg_setup(args.mg_levels, args.mg_coarse, 0 if learned_fine else 1,
+                         args.mg_kcycle if learned_fine else args.mg_kcycle_cheb)
         t1 = time.perf_counter()
         _, st = dev.solve(rtol=1e-10, max_iter=args.max_iter, method=methods[method],
                           download=False)
@@ -260,6 +279,7 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(prob, (fp64 or results[best])["iterations"], args.cpu_iters)
 
+    traffic, traffic_src = measured_traffic(m.num_dofs)
     out = {
         "metric": METRIC,
         "value": per_step,
@@ -292,7 +312,8 @@ def main():
         "spmv32_gbs": (4 * nnz + nnz + 4 * (n // 2 + 1) + 8 * n) / t_spmv32 / 1e9,
         "fp64_iteration_gbs": (iter_bytes / fp64["s_per_iteration"] / 1e9) if fp64 else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k_cg_spmv", "bytes_per_launch": b_spmv,
                      "avg_launch_s": t_spmv},
         "cpu_baseline": cpu,

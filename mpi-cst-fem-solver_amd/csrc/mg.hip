@@ -202,32 +202,70 @@ struct MatS {
   int n;
 };
 
-__device__ inline double rowS(const MatS& A, const double* __restrict__ x, int i) {
+// Entries k = k0, k0 + step, ... (k < w) of one SELL row in ascending order, accumulated
+// into a in that order.  Loads are issued in groups of 8 (all column indices and values, then
+// all x entries) so that a row costs two memory round trips per group instead of two per entry.
+// x entries read straight from a vector
+struct XVec {
+  const double* __restrict__ x;
+  __device__ double operator()(int j) const { return x[j]; }
+};
+
+// x = the first Chebyshev step from zero, c1r D^-1 rhs, evaluated where it is needed (the
+// same expression as the separate step, so the same bits)
+struct XCheb1 {
+  const double* __restrict__ dinv;
+  const double* __restrict__ rhs;
+  double c1r;
+  __device__ double operator()(int j) const { return 0.0 + c1r * (dinv[j] * (rhs[j] - 0.0)); }
+};
+
+template <int G, class XF>
+__device__ inline double row_entries(const MatS& A, const XF& xf, int64_t base, int w, int k0,
+                                     int step, double a) {
+  for (int k = k0; k < w; k += G * step) {
+    int cidx[G];
+    double v[G], xv[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const int kk = k + u * step;
+      const int64_t q = base + (int64_t)(kk < w ? kk : k0) * 64;  // in-row index for the tail
+      cidx[u] = A.col[q];
+      v[u] = A.val[q];
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) xv[u] = xf(cidx[u]);
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+      if (k + u * step < w) a = a + v[u] * xv[u];
+  }
+  return a;
+}
+
+template <class XF>
+__device__ inline double rowS_f(const MatS& A, const XF& xf, int i) {
   const int s = i >> 6, lane = i & 63;
   const int64_t o = A.off[s];
   const int w = (int)((A.off[s + 1] - o) >> 6);
-  double a = 0.0;
-  for (int k = 0; k < w; ++k) {
-    const int64_t q = o + (int64_t)k * 64 + lane;
-    a = a + A.val[q] * x[A.col[q]];
-  }
-  return a;
+  return row_entries<8>(A, xf, o + lane, w, 0, 1, 0.0);
+}
+
+__device__ inline double rowS(const MatS& A, const double* __restrict__ x, int i) {
+  return rowS_f(A, XVec{x}, i);
 }
 
 // Row i of A x computed by LPR consecutive lanes (lane j takes the entries k = j, j + LPR, ...;
 // the partial sums are combined by a butterfly, so every lane of the group returns the sum).
 // The LPR lanes of a group are always active together.
-template <int LPR>
-__device__ inline double rowS_lanes(const MatS& A, const double* __restrict__ x, int i, int j) {
-  if (LPR == 1) return rowS(A, x, i);
+template <int LPR, class XF>
+__device__ inline double rowS_lanes(const MatS& A, const XF& xf, int i, int j) {
+  if (LPR == 1) return rowS_f(A, xf, i);
   const int s = i >> 6, lane = i & 63;
   const int64_t o = A.off[s];
   const int w = (int)((A.off[s + 1] - o) >> 6);
-  double a = 0.0;
-  for (int k = j; k < w; k += LPR) {
-    const int64_t q = o + (int64_t)k * 64 + lane;
-    a = a + A.val[q] * x[A.col[q]];
-  }
+  // rows hold about 21 entries: with 4 or more lanes per row, groups of 4 per lane cover a row
+  // in one or two rounds
+  double a = j < w ? row_entries<(LPR >= 4 ? 4 : 8)>(A, xf, o + lane, w, j, LPR, 0.0) : 0.0;
 #pragma unroll
   for (int m = 1; m < LPR; m <<= 1) a += __shfl_xor(a, m, 64);
   return a;
@@ -248,11 +286,34 @@ __global__ __launch_bounds__(kT) void ks_cheb(MatS A, const double* __restrict__
   const int t = blockIdx.x * kT + threadIdx.x;
   const int i = t / LPR, j = t % LPR;
   if (i >= A.n) return;
-  const double y = xin ? rowS_lanes<LPR>(A, xin, i, j) : 0.0;
+  const double y = xin ? rowS_lanes<LPR>(A, XVec{xin}, i, j) : 0.0;
   if (j != 0) return;
   const double dd = (cd != 0.0 ? cd * d[i] : 0.0) + cr * (dinv[i] * (r[i] - y));
   d[i] = dd;
   xout[i] = (xin ? xin[i] : 0.0) + dd;
+}
+
+// Both pre-smoothing steps from x = 0 in one pass: the first step's x (= c1r D^-1 rhs) is
+// evaluated at every column the second step's row sum needs.  Writes d and x2 of the second
+// step; identical in value to the two separate steps.
+template <int LPR>
+__global__ __launch_bounds__(kT) void ks_cheb_pre(MatS A, double* __restrict__ x2,
+                                                  double* __restrict__ d,
+                                                  const double* __restrict__ r,
+                                                  const double* __restrict__ dinv, double c1r,
+                                                  double c2d, double c2r,
+                                                  const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  const int t = blockIdx.x * kT + threadIdx.x;
+  const int i = t / LPR, j = t % LPR;
+  if (i >= A.n) return;
+  const XCheb1 x1{dinv, r, c1r};
+  const double y = rowS_lanes<LPR>(A, x1, i, j);
+  if (j != 0) return;
+  const double d1 = 0.0 + c1r * (dinv[i] * (r[i] - 0.0));
+  const double dd = (c2d != 0.0 ? c2d * d1 : 0.0) + c2r * (dinv[i] * (r[i] - y));
+  d[i] = dd;
+  x2[i] = (0.0 + d1) + dd;
 }
 
 template <int LPR>
@@ -264,7 +325,7 @@ __global__ __launch_bounds__(kT) void ks_resid(MatS A, const double* __restrict_
   const int t = blockIdx.x * kT + threadIdx.x;
   const int i = t / LPR, j = t % LPR;
   if (i >= A.n) return;
-  const double y = rowS_lanes<LPR>(A, x, i, j);
+  const double y = rowS_lanes<LPR>(A, XVec{x}, i, j);
   if (j == 0) res[i] = r[i] - y;
 }
 
@@ -290,7 +351,7 @@ __global__ __launch_bounds__(kT) void ks_spmv_dir(MatS A, const double* __restri
   const int j = threadIdx.x % LPR;
   const int stride = gridDim.x * (kT / LPR);
   for (int i = (blockIdx.x * kT + threadIdx.x) / LPR; i < A.n; i += stride) {
-    const double y = rowS_lanes<LPR>(A, z, i, j);
+    const double y = rowS_lanes<LPR>(A, XVec{z}, i, j);
     if (j == 0) {
       const double qi = first ? y : y + beta * q[i];
       const double pi = first ? z[i] : z[i] + beta * p[i];
@@ -346,6 +407,33 @@ __global__ __launch_bounds__(kT) void ks_copy(int n, const double* __restrict__ 
   for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) b[i] = a[i];
 }
 
+// sum over the member rows k = k0, k0 + step, ... of aggregate a (rows in member order, then
+// DOF order) of P[f * 3 + c] res[f], in that order; loads in groups of 8
+__device__ inline double restrict_sum(int a, int c, int bs, const int* __restrict__ mptr,
+                                      const int* __restrict__ mem, const double* __restrict__ P,
+                                      const double* __restrict__ res, int k0, int step) {
+  const int m0 = mptr[a], rows = bs * (mptr[a + 1] - m0);
+  double s = 0.0;
+  for (int k = k0; k < rows; k += 8 * step) {
+    int f[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int kk = k + u * step < rows ? k + u * step : k;
+      f[u] = bs * mem[m0 + kk / bs] + kk % bs;
+    }
+    double p[8], r[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      p[u] = P[(size_t)f[u] * 3 + c];
+      r[u] = res[f[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k + u * step < rows) s = s + p[u] * r[u];
+  }
+  return s;
+}
+
 // rc[3a + c] = sum over member nodes i of a, k < bs:  P[(bs i + k) * 3 + c] * res[bs i + k];
 // LPR lanes per coarse DOF split the member rows (butterfly sum, as in rowS_lanes)
 template <int LPR>
@@ -360,12 +448,7 @@ __global__ __launch_bounds__(kT) void k_restrict(int nagg, int bs, const int* __
   const int t = tt / LPR, j = tt % LPR;
   if (t >= 3 * nagg) return;
   const int a = t / 3, c = t % 3;
-  const int m0 = mptr[a], rows = bs * (mptr[a + 1] - m0);
-  double s = 0.0;
-  for (int k = j; k < rows; k += LPR) {
-    const int f = bs * mem[m0 + k / bs] + k % bs;
-    s = s + P[(size_t)f * 3 + c] * res[f];
-  }
+  double s = restrict_sum(a, c, bs, mptr, mem, P, res, j, LPR);
 #pragma unroll
   for (int m = 1; m < LPR; m <<= 1) s += __shfl_xor(s, m, 64);
   if (j == 0) rc[t] = s;
@@ -386,6 +469,25 @@ __global__ __launch_bounds__(kT) void k_prolong(int n, int bs, const int* __rest
   x[f] = x[f] + (p[0] * e[0] + p[1] * e[1] + p[2] * e[2]);
 }
 
+// sum_j row[j] r[j] in ascending j, loads issued in groups of 8
+__device__ inline double dense_row(const double* __restrict__ row, const double* __restrict__ r,
+                                   int n) {
+  double s = 0.0;
+  for (int j = 0; j < n; j += 8) {
+    double a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int jj = j + u < n ? j + u : j;
+      a[u] = row[jj];
+      b[u] = r[jj];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (j + u < n) s = s + a[u] * b[u];
+  }
+  return s;
+}
+
 // x = Cinv r (dense, row-major n x n)
 __global__ __launch_bounds__(kT) void k_dense(int n, const double* __restrict__ Cinv,
                                               const double* __restrict__ r, double* __restrict__ x,
@@ -393,17 +495,15 @@ __global__ __launch_bounds__(kT) void k_dense(int n, const double* __restrict__ 
   if (stopped(sc)) return;
   const int i = blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
-  const double* row = Cinv + (size_t)i * n;
-  double s = 0.0;
-  for (int j = 0; j < n; ++j) s = s + row[j] * r[j];
-  x[i] = s;
+  x[i] = dense_row(Cinv + (size_t)i * n, r, n);
 }
 
 // ---- fused tail: the V-cycle of the smallest levels in one workgroup ------------------------
 
-constexpr int kTailLevels = 4;      // cycle levels in the fused tail (plus the dense coarsest)
+constexpr int kTailLevels = 4;       // cycle levels in the fused tail (plus the dense coarsest)
 constexpr int kTailThreads = 1024;
-constexpr int kTailMaxN = 3072;     // largest level that runs inside the tail
+constexpr int kTailLds = 64 * 1024;  // bytes of LDS for the tail's vectors
+constexpr int kTailLanes = 8;        // lanes per coarse DOF (restriction) and per dense row
 
 struct TailLevel {
   MatS A;
@@ -413,9 +513,7 @@ struct TailLevel {
   const int* agg;
   const double* P;
   int nagg, bs;
-  double* rhs;  // right-hand side (filled by the restriction of the level above, except the top)
-  double* out;  // result of this level's cycle
-  double *x, *x2, *d, *res;
+  int lds;  // offset (doubles) of this level's vectors rhs, x, x2, d, res in LDS
   double c1r, c2d, c2r;
 };
 
@@ -424,78 +522,96 @@ struct Tail {
   TailLevel L[kTailLevels];
   const double* cinv;
   int nc;
-  double* crhs;
-  double* ckx;
+  int lds_c;          // offset of the coarsest rhs and solution in LDS
+  const double* rhs;  // right-hand side of the top tail level (global)
+  double* out;        // its result (global)
 };
 
-// The same arithmetic as cycle_s (two Chebyshev steps, residual, restriction, coarse
-// correction, prolongation, two steps) on each tail level in turn, with the dense solve at the
-// bottom; one workgroup, phases separated by barriers (every vector and matrix of the tail is
-// small enough to stay in the L2 cache).
+// The same steps as cycle_s (two Chebyshev steps, residual, restriction, coarse correction,
+// prolongation, two steps) on each tail level in turn, with the dense solve at the bottom, in
+// one workgroup: the vectors live in LDS, the matrices are read from L2, and barriers separate
+// the phases.  The restriction and the dense rows use kTailLanes lanes each (butterfly sums).
 __global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars* __restrict__ sc) {
+  extern __shared__ double lds[];
   if (stopped(sc)) return;
   const int tid = threadIdx.x;
+  {
+    double* rhs0 = lds + T.L[0].lds;
+    for (int i = tid; i < T.L[0].A.n; i += kTailThreads) rhs0[i] = T.rhs[i];
+  }
+  __syncthreads();
   for (int l = 0; l < T.nlev; ++l) {
     const TailLevel& L = T.L[l];
     const int n = L.A.n;
+    double* rhs = lds + L.lds;
+    double *x = rhs + n, *x2 = x + n, *d = x2 + n, *res = d + n;
     for (int i = tid; i < n; i += kTailThreads) {
-      const double dd = 0.0 + L.c1r * (L.dinv[i] * (L.rhs[i] - 0.0));
-      L.d[i] = dd;
-      L.x[i] = 0.0 + dd;
+      const double dd = 0.0 + L.c1r * (L.dinv[i] * (rhs[i] - 0.0));
+      d[i] = dd;
+      x[i] = 0.0 + dd;
     }
     __syncthreads();
     for (int i = tid; i < n; i += kTailThreads) {
-      const double y = rowS(L.A, L.x, i);
-      const double dd = (L.c2d != 0.0 ? L.c2d * L.d[i] : 0.0) + L.c2r * (L.dinv[i] * (L.rhs[i] - y));
-      L.d[i] = dd;
-      L.x2[i] = L.x[i] + dd;
+      const double y = rowS_f(L.A, XVec{x}, i);
+      const double dd = (L.c2d != 0.0 ? L.c2d * d[i] : 0.0) + L.c2r * (L.dinv[i] * (rhs[i] - y));
+      d[i] = dd;
+      x2[i] = x[i] + dd;
     }
     __syncthreads();
-    for (int i = tid; i < n; i += kTailThreads) L.res[i] = L.rhs[i] - rowS(L.A, L.x2, i);
+    for (int i = tid; i < n; i += kTailThreads) res[i] = rhs[i] - rowS_f(L.A, XVec{x2}, i);
     __syncthreads();
-    double* rc = l + 1 < T.nlev ? T.L[l + 1].rhs : T.crhs;
-    for (int t = tid; t < 3 * L.nagg; t += kTailThreads) {
-      const int a = t / 3, c = t % 3;
-      const int m0 = L.mptr[a], rows = L.bs * (L.mptr[a + 1] - m0);
-      double s = 0.0;
-      for (int k = 0; k < rows; ++k) {
-        const int f = L.bs * L.mem[m0 + k / L.bs] + k % L.bs;
-        s = s + L.P[(size_t)f * 3 + c] * L.res[f];
-      }
-      rc[t] = s;
+    double* rc = lds + (l + 1 < T.nlev ? T.L[l + 1].lds : T.lds_c);
+    for (int tt = tid; tt < 3 * L.nagg * kTailLanes; tt += kTailThreads) {
+      const int t = tt / kTailLanes, j = tt % kTailLanes;
+      double sum = restrict_sum(t / 3, t % 3, L.bs, L.mptr, L.mem, L.P, res, j, kTailLanes);
+#pragma unroll
+      for (int m = 1; m < kTailLanes; m <<= 1) sum += __shfl_xor(sum, m, 64);
+      if (j == 0) rc[t] = sum;
     }
     __syncthreads();
   }
-  for (int i = tid; i < T.nc; i += kTailThreads) {
-    const double* row = T.cinv + (size_t)i * T.nc;
-    double s = 0.0;
-    for (int j = 0; j < T.nc; ++j) s = s + row[j] * T.crhs[j];
-    T.ckx[i] = s;
+  {
+    const double* crhs = lds + T.lds_c;
+    double* ckx = lds + T.lds_c + T.nc;
+    for (int tt = tid; tt < T.nc * kTailLanes; tt += kTailThreads) {
+      const int i = tt / kTailLanes, j = tt % kTailLanes;
+      const double* row = T.cinv + (size_t)i * T.nc;
+      double sum = 0.0;
+      for (int k = j; k < T.nc; k += kTailLanes) sum = sum + row[k] * crhs[k];
+#pragma unroll
+      for (int m = 1; m < kTailLanes; m <<= 1) sum += __shfl_xor(sum, m, 64);
+      if (j == 0) ckx[i] = sum;
+    }
   }
   __syncthreads();
   for (int l = T.nlev - 1; l >= 0; --l) {
     const TailLevel& L = T.L[l];
     const int n = L.A.n;
-    const double* e = l + 1 < T.nlev ? T.L[l + 1].out : T.ckx;
+    double* rhs = lds + L.lds;
+    double *x = rhs + n, *x2 = x + n, *d = x2 + n, *res = d + n;
+    // the level below left its result in its res vector (the coarsest: after its rhs)
+    const double* e = l + 1 < T.nlev ? lds + T.L[l + 1].lds + 4 * T.L[l + 1].A.n
+                                     : lds + T.lds_c + T.nc;
     for (int f = tid; f < n; f += kTailThreads) {
       const int a = L.agg[f / L.bs];
       const double* p = L.P + (size_t)f * 3;
-      const double* ea = e + 3 * (size_t)a;
-      L.x2[f] = L.x2[f] + (p[0] * ea[0] + p[1] * ea[1] + p[2] * ea[2]);
+      const double* ea = e + 3 * a;
+      x2[f] = x2[f] + (p[0] * ea[0] + p[1] * ea[1] + p[2] * ea[2]);
     }
     __syncthreads();
     for (int i = tid; i < n; i += kTailThreads) {
-      const double y = rowS(L.A, L.x2, i);
-      const double dd = 0.0 + L.c1r * (L.dinv[i] * (L.rhs[i] - y));
-      L.d[i] = dd;
-      L.x[i] = L.x2[i] + dd;
+      const double y = rowS_f(L.A, XVec{x2}, i);
+      const double dd = 0.0 + L.c1r * (L.dinv[i] * (rhs[i] - y));
+      d[i] = dd;
+      x[i] = x2[i] + dd;
     }
     __syncthreads();
+    double* out = l == 0 ? T.out : res;
     for (int i = tid; i < n; i += kTailThreads) {
-      const double y = rowS(L.A, L.x, i);
-      const double dd = (L.c2d != 0.0 ? L.c2d * L.d[i] : 0.0) + L.c2r * (L.dinv[i] * (L.rhs[i] - y));
-      L.d[i] = dd;
-      L.out[i] = L.x[i] + dd;
+      const double y = rowS_f(L.A, XVec{x}, i);
+      const double dd = (L.c2d != 0.0 ? L.c2d * d[i] : 0.0) + L.c2r * (L.dinv[i] * (rhs[i] - y));
+      d[i] = dd;
+      out[i] = x[i] + dd;
     }
     __syncthreads();
   }
@@ -1117,6 +1233,13 @@ void launch_cheb(Context& c, MgLevel& L, const double* xin, double* xout, const 
                     (const double*)L.dinv.get(), cd, cr, sc);
 }
 
+void launch_cheb_pre(Context& c, MgLevel& L, double* x2, const double* r, const Cheb& ch,
+                     const PcgScalars* sc) {
+  const int lpr = lanes_per_row(L.n);
+  CSTFEM_LPR_SWITCH(lpr, ks_cheb_pre, grid1((int64_t)L.n * lpr), mats(L), x2, L.d.get(), r,
+                    (const double*)L.dinv.get(), ch.c1r, ch.c2d, ch.c2r, sc);
+}
+
 void launch_resid(Context& c, MgLevel& L, const double* x, const double* r, double* res,
                   const PcgScalars* sc) {
   const int lpr = lanes_per_row(L.n);
@@ -1138,13 +1261,23 @@ int kcycle_grid(const MgLevel& L) {
   return std::min<int64_t>(kPart, grid1((int64_t)L.n * lanes_per_row(L.n)));
 }
 
+// LDS doubles the fused tail needs from level t down (5 vectors per cycle level, rhs and
+// solution of the coarsest)
+int64_t tail_lds_doubles(const Multigrid& mg, int t) {
+  const int last = (int)mg.lv.size() - 1;
+  int64_t s = 2 * (int64_t)mg.lv[last]->n;
+  for (int l = t; l < last; ++l) s += 5 * (int64_t)mg.lv[l]->n;
+  return s;
+}
+
 // the fused tail of the hierarchy (levels tail_from .. coarsest), or -1 when it does not apply
 int tail_start(const Multigrid& mg) {
-  if (std::getenv("CSTFEM_MG_TAIL") && std::getenv("CSTFEM_MG_TAIL")[0] == '0') return -1;
+  const char* env = std::getenv("CSTFEM_MG_TAIL");
+  if (env && env[0] == '0') return -1;
   const int last = (int)mg.lv.size() - 1;
   int t = last;
   while (t - 1 >= 1 && t - 1 > mg.kdepth && last - (t - 1) <= kTailLevels &&
-         mg.lv[t - 1]->n <= kTailMaxN)
+         tail_lds_doubles(mg, t - 1) * 8 <= kTailLds)
     --t;
   return t < last ? t : -1;
 }
@@ -1154,6 +1287,7 @@ void launch_tail(Context& c, int t, const double* rhs, double* out, const PcgSca
   const int last = (int)mg.lv.size() - 1;
   Tail T{};
   T.nlev = last - t;
+  int off = 0;
   for (int k = 0; k < T.nlev; ++k) {
     MgLevel& L = *mg.lv[t + k];
     const Cheb ch = cheb(L.lam);
@@ -1166,22 +1300,27 @@ void launch_tail(Context& c, int t, const double* rhs, double* out, const PcgSca
     q.P = L.P.get();
     q.nagg = L.nagg;
     q.bs = L.bs;
-    q.rhs = k == 0 ? const_cast<double*>(rhs) : L.rhs.get();
-    q.out = k == 0 ? out : L.kx.get();
-    q.x = L.x.get();
-    q.x2 = L.x2.get();
-    q.d = L.d.get();
-    q.res = L.res.get();
+    q.lds = off;
+    off += 5 * L.n;
     q.c1r = ch.c1r;
     q.c2d = ch.c2d;
     q.c2r = ch.c2r;
   }
-  MgLevel& Cst = *mg.lv[last];
   T.cinv = mg.cinv.get();
-  T.nc = Cst.n;
-  T.crhs = Cst.rhs.get();
-  T.ckx = Cst.kx.get();
-  hipLaunchKernelGGL(k_tail, dim3(1), dim3(kTailThreads), 0, c.stream, T, sc);
+  T.nc = mg.lv[last]->n;
+  T.lds_c = off;
+  T.rhs = rhs;
+  T.out = out;
+  const size_t bytes = sizeof(double) * (size_t)tail_lds_doubles(mg, t);
+  hipLaunchKernelGGL(k_tail, dim3(1), dim3(kTailThreads), bytes, c.stream, T, sc);
+}
+
+// where the restriction of level l's residual goes: straight into the K-cycle's residual when
+// level l+1 runs the K-cycle (its right-hand side is not needed otherwise), else its rhs
+double* coarse_rhs(Multigrid& mg, int l) {
+  MgLevel& C = *mg.lv[l + 1];
+  const bool kc = l + 1 < (int)mg.lv.size() - 1 && l + 1 <= mg.kdepth;
+  return kc ? C.kres.get() : C.rhs.get();
 }
 
 // coarse correction of level l: lv[l+1].kx from lv[l+1].rhs -- the dense solve on the
@@ -1207,10 +1346,9 @@ void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars
   MgLevel& L = *c.mg.lv[l];
   MgLevel& C = *c.mg.lv[l + 1];
   const Cheb ch = cheb(L.lam);
-  launch_cheb(c, L, nullptr, L.x.get(), rhs, 0.0, ch.c1r, sc);
-  launch_cheb(c, L, L.x.get(), L.x2.get(), rhs, ch.c2d, ch.c2r, sc);
+  launch_cheb_pre(c, L, L.x2.get(), rhs, ch, sc);
   launch_resid(c, L, L.x2.get(), rhs, L.res.get(), sc);
-  launch_restrict(c, L, L.res.get(), C.rhs.get(), sc);
+  launch_restrict(c, L, L.res.get(), coarse_rhs(c.mg, l), sc);
   coarse_correction(c, l, sc);
   hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n, L.bs,
                      (const int*)L.agg.get(), (const double*)L.P.get(), (const double*)C.kx.get(),
@@ -1219,7 +1357,8 @@ void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars
   launch_cheb(c, L, L.x.get(), out, rhs, ch.c2d, ch.c2r, sc);
 }
 
-// two flexible CG steps on level l for rhs = lv[l].rhs, result in lv[l].kx
+// two flexible CG steps on level l for the right-hand side in lv[l].kres (overwritten by the
+// residual), result in lv[l].kx
 void kcycle(Context& c, int l, const PcgScalars* sc) {
   MgLevel& L = *c.mg.lv[l];
   const MatS A = mats(L);
@@ -1228,9 +1367,7 @@ void kcycle(Context& c, int l, const PcgScalars* sc) {
   double* s0 = L.part.get();             // p'q, p'r of step 1
   double* s1 = L.part.get() + 2 * kPart;  // z'q_old
   double* s2 = L.part.get() + 4 * kPart;  // p'q, p'r of step 2
-  hipLaunchKernelGGL(ks_copy, dim3(g), dim3(kT), 0, c.stream, L.n, (const double*)L.rhs.get(),
-                     L.kres.get(), sc);
-  cycle_s(c, l, L.kres.get(), L.kz.get(), sc);
+  cycle_s(c, l, L.kres.get(), L.kz.get(), sc);  // the restriction filled kres
   CSTFEM_LPR_SWITCH(lpr, ks_spmv_dir, g, A, (const double*)L.kz.get(), L.kp.get(), L.kq.get(),
                     (const double*)L.kres.get(), s0, (const double*)nullptr, (const double*)nullptr,
                     g, 1, sc);
@@ -1278,7 +1415,7 @@ void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* 
   }
   hipLaunchKernelGGL(k0_resid, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)z2, r2,
                      (double2*)L.res.get(), sc);
-  launch_restrict(c, L, L.res.get(), C.rhs.get(), sc);
+  launch_restrict(c, L, L.res.get(), coarse_rhs(mg, 0), sc);
   coarse_correction(c, 0, sc);
   hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n, 2,
                      (const int*)L.agg.get(), (const double*)L.P.get(), (const double*)C.kx.get(), z,
