@@ -178,7 +178,8 @@ def main():
         from cstfem_hip import learned
         t0 = time.perf_counter()
         with cstfem_hip.Device(0) as tdev:
-            theta, losses = learned.train(tdev, learned.training_plate(args.train_n),
+            theta, losses = learned.train(tdev, learned.training_problem(args.config,
+                                                                         args.train_n),
                                           steps=args.train_steps, log=log)
         train_info = dict(train_s=time.perf_counter() - t0, plate_nodes=args.train_n ** 2,
                           steps=args.train_steps, loss_first=losses[0], loss_last=losses[-1])
@@ -224,7 +225,7 @@ def main():
         results[meth] = dict(s=sum(step_times) / len(step_times), iterations=st.iterations,
                              refinements=st.outer_iterations, verified=bool(verified),
                              true_rel_residual=st.true_rel_residual,
-                             fallback=bool(st.fallback),
+                             fallback=int(st.fallback),
                              s_per_iteration=times.solve / max(1, st.iterations),
                              phases_s={"pattern": times.pattern, "assembly": times.assembly,
                                        "bcs": times.bcs, "solve": times.solve,

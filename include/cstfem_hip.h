@@ -81,7 +81,9 @@ typedef struct {
   int breakdown;           /* 1 if p'Ap <= 0 stopped CG */
   int zero_rhs;
   int outer_iterations;    /* IR / FCG outer steps; 0 for plain CG */
-  int fallback;            /* FCG: 1 if it fell back to Jacobi-CG on stagnation */
+  int fallback;            /* FCG: 0 none; on stagnation or breakdown 1 = fell back to
+                              Jacobi-CG, 2 = to the multilevel cycle with the Chebyshev fine
+                              smoother (learned multilevel only) */
   int warning;             /* 1 if converged but true residual > 10*rtol */
   int pad_;
   double rel_residual;     /* recursive ||r||/||b|| at exit */

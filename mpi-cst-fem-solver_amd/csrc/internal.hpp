@@ -289,6 +289,9 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
 void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* sc);
 int64_t mg_level_info(Context& c, int level, int* n, int* nagg, double* lam);
 int mg_levels(Context& c);
+// switches the level-0 smoother (0 learned M, 1 Chebyshev; its spectral bound is computed on
+// first use)
+void mg_set_fine_smoother(Context& c, int smoother);
 void mg_download_agg(Context& c, int level, int* out, int count);
 void mg_download_matrix(Context& c, int level, int* row_ptr, int* cols, double* vals);
 int mpnn_num_params();

@@ -1644,4 +1644,11 @@ int64_t mg_level_info(Context& c, int level, int* n, int* nagg, double* lam) {
 
 int mg_levels(Context& c) { return c.mg.ready ? (int)c.mg.lv.size() : 0; }
 
+void mg_set_fine_smoother(Context& c, int smoother) {
+  Multigrid& mg = c.mg;
+  if (!mg.ready) throw Error(CSTFEM_ERR_ORDER, "multilevel: not set up");
+  if (smoother == 1 && !(mg.lv[0]->lam > 0.0)) mg.lv[0]->lam = smoothing_max(c, 0);
+  mg.fine_smoother = smoother;
+}
+
 }  // namespace cstfem_hip

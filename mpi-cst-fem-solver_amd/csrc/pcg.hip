@@ -1219,6 +1219,10 @@ __global__ __launch_bounds__(kThreads) void k_fcg_direction(
   const typename P::S b = (typename P::S)beta;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
     const typename P::V zi = z[i];
+    if (restart) {  // p = z: the old direction is not read (it may hold a failed solve's NaN)
+      p[i] = zi;
+      continue;
+    }
     typename P::V pi = p[i];
     pi.x = zi.x + b * pi.x;
     pi.y = zi.y + b * pi.y;
@@ -1249,6 +1253,29 @@ __global__ __launch_bounds__(kThreads) void k_fcg_init(int nvec2, const double2*
   }
 }
 
+// warm start from the current x: r = b - A x (Ax in FP64), q = 0; partials (b'b, 0) as
+// k_fcg_init
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_fcg_init_warm(int nvec2, const double2* __restrict__ b,
+                                                            const double2* __restrict__ Ax,
+                                                            typename P::V* __restrict__ r,
+                                                            typename P::V* __restrict__ q,
+                                                            double* __restrict__ part) {
+  __shared__ double lds[kWaves];
+  double bb = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nvec2; i += gridDim.x * kThreads) {
+    const double2 bi = b[i], ai = Ax[i];
+    r[i] = make_v<P>(bi.x - ai.x, bi.y - ai.y);
+    q[i] = make_v<P>(0.0, 0.0);
+    bb += bi.x * bi.x + bi.y * bi.y;
+  }
+  const double s0 = block_sum_all(bb, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s0;
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
 // after a refinement: the working residual from r64 = b - A x (FP64)
 template <class P>
 __global__ __launch_bounds__(kThreads) void k_fcg_set_r(int nvec2, const double2* __restrict__ r64,
@@ -1257,8 +1284,10 @@ __global__ __launch_bounds__(kThreads) void k_fcg_set_r(int nvec2, const double2
     r[i] = make_v<P>(r64[i].x, r64[i].y);
 }
 
+// warm: start from the current x (r = b - A x) instead of x = 0
 template <class P>
-int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st, bool fp32) {
+int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st, bool fp32,
+            bool warm = false) {
   using V = typename P::V;
   const size_t nv = 2 * (size_t)c.nb;
   const int gs = spmv_grid(c), gv = vec_grid(c);
@@ -1337,15 +1366,23 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
   init.delta = delta;
   init.rel_ref = 1.0;
   init.best_rel[0] = init.best_rel[1] = 1.0;
-  init.stall_window = std::max(1000, o.max_iter / 20);
+  // the multilevel preconditioner gains an order of magnitude every 10-20 iterations: 200
+  // iterations without a new best residual are a stall
+  init.stall_window = multilevel ? 200 : std::max(1000, o.max_iter / 20);
   HIP_CHECK(hipMemcpyAsync(c.scal.get(), &init, sizeof init, hipMemcpyHostToDevice, c.stream));
   hipEvent_t ev0, ev1;
   HIP_CHECK(hipEventCreate(&ev0));
   HIP_CHECK(hipEventCreate(&ev1));
   HIP_CHECK(hipEventRecord(ev0, c.stream));
-  // x = 0, r = b, q = 0; bnorm; z = M r; p = z
-  hipLaunchKernelGGL(k_fcg_init<P>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
-                     (const double2*)c.b.get(), (double2*)c.x.get(), r, q, partB(c, 1));
+  // x = 0 (or the current x), r = b - A x, q = 0; bnorm; z = M r; p = z
+  if (warm) {
+    spmv_fp64(c, c.x.get(), Ax);
+    hipLaunchKernelGGL(k_fcg_init_warm<P>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                       (const double2*)c.b.get(), (const double2*)Ax, r, q, partB(c, 1));
+  } else {
+    hipLaunchKernelGGL(k_fcg_init<P>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                       (const double2*)c.b.get(), (double2*)c.x.get(), r, q, partB(c, 1));
+  }
   hipLaunchKernelGGL(k_cg_init_finish, dim3(1), dim3(kThreads), 0, c.stream,
                      (const double*)partB(c, 1), gv, c.scal.get(), (double*)nullptr);
   precond(0, 0);
@@ -1403,11 +1440,33 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
       last_status = kRunning;
     }
   }
-  int fb_iters = 0, fb_outer = 0;
-  if (!converged) {
+  int fb_iters = 0, fb_outer = 0, fb_kind = 0;
+  if (!converged && !zero_rhs && multilevel && c.mg.fine_smoother == 0) {
+    // the learned smoother failed on this matrix: the same hierarchy with the Chebyshev
+    // smoother, from the current x (its own stall falls back to Jacobi-CG)
+    cstfem_solver_options fo = o;
+    fo.max_iter = std::max(1, o.max_iter - iters);
+    cstfem_solver_stats fs{};
+    mg_set_fine_smoother(c, 1);
+    try {
+      fcg_run<P>(c, fo, fs, false, true);
+    } catch (...) {
+      mg_set_fine_smoother(c, 0);
+      throw;
+    }
+    mg_set_fine_smoother(c, 0);
+    fb_iters = fs.iterations;
+    fb_outer = fs.outer_iterations;
+    converged = fs.converged != 0;
+    true_rel = fs.true_rel_residual;
+    last_status = converged ? kConverged : kMaxIter;
+    fallback = true;
+    fb_kind = fs.fallback ? fs.fallback : 2;
+  } else if (!converged) {
     // breakdown or stall of the learned preconditioner: Jacobi-CG (FP32 + FP64 refinement)
     // from the current x; the FP64 true residual decides
     fallback = true;
+    fb_kind = 1;
     cstfem_solver_options fo = o;
     fo.max_iter = std::max(1, o.max_iter - iters);
     fo.inner_rtol = 0.0;
@@ -1427,7 +1486,7 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
   st.outer_iterations = outer + fb_outer;
   st.converged = converged ? 1 : 0;
   st.breakdown = last_status == kBreakdown ? 1 : 0;
-  st.fallback = fallback ? 1 : 0;
+  st.fallback = fallback ? fb_kind : 0;
   st.zero_rhs = zero_rhs ? 1 : 0;
   st.rel_residual = zero_rhs ? 0.0 : s.rel;
   st.true_rel_residual = true_rel;

@@ -87,7 +87,7 @@ class SolverStats:
     warning: str = ""
     residual_history: list = field(default_factory=list)
     outer_iterations: int = 0
-    fallback: bool = False
+    fallback: int = 0  # 1 fell back to Jacobi-CG, 2 to the Chebyshev-smoothed multilevel cycle
     breakdown: dict = field(default_factory=dict)
 
 
@@ -483,7 +483,7 @@ def _stats_from(s: SolverStatsC, o: SolverOptions, rc: int, hist) -> SolverStats
     """Builds the reference's SolverStats, messages included (solver.cpp:46-130)."""
     st = SolverStats(iterations=s.iterations, rel_residual=s.rel_residual,
                      true_rel_residual=s.true_rel_residual, converged=bool(s.converged),
-                     outer_iterations=s.outer_iterations, fallback=bool(s.fallback))
+                     outer_iterations=s.outer_iterations, fallback=int(s.fallback))
     if s.zero_rhs:
         st.message = "zero right-hand side, x = 0"
     elif s.breakdown:

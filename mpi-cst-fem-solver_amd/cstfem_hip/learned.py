@@ -78,3 +78,16 @@ def training_plate(n: int = 64, seed: int = 5, e_seed: int = 101):
     return pipeline.plate_problem(n, n, Lx=2.0, Ly=1.0, jitter=0.2, seed=seed, E=1.0, nu=0.3,
                                   plane=PLANE_STRAIN, tx=1.0, E_sigma=1.0, E_seed=e_seed,
                                   body=(0.0, -1.0), name=f"train{n}")
+
+
+def training_problem(config: int, n: int = 64):
+    """A small plate of the same family as benchmark config `config` (not the benchmark
+    matrix itself) to train on: the network's features are local, so it carries over to the
+    large plate of the family, but not across families (plane stress against plane strain,
+    graded against uniform meshes)."""
+    from . import pipeline
+    if config == 1:
+        return pipeline.config_problem(1, n / 700.0)  # Kirsch quarter plate, n x n nodes
+    if config == 0:
+        return pipeline.plate_problem(n, n, seed=11, name=f"train{n}")
+    return training_plate(n)

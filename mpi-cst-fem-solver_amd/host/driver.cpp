@@ -361,7 +361,7 @@ int run_case(const std::string& case_path, std::ostream& out, std::ostream& err,
        << "    \"compute_units\": " << cus << ",\n"
        << "    \"method\": " << str(gmethod) << ",\n"
        << "    \"outer_iterations\": " << s.outer_iterations << ",\n"
-       << "    \"fallback\": " << (s.fallback ? "true" : "false") << ",\n"
+       << "    \"fallback\": " << (s.fallback == 0 ? "null" : s.fallback == 1 ? "\"jacobi_cg\"" : "\"multilevel_chebyshev\"") << ",\n"
        << "    \"solve_device_s\": " << num(s.total) << ",\n"
        << "    \"context_init_s\": " << num(t_gpu_init) << ",\n"
        << "    \"max_von_mises\": " << num(*std::max_element(vm.begin(), vm.end())) << "\n"

@@ -5,7 +5,11 @@
 - one application of the cycle equals the oracle's cycle (FP64; reductions in a different
   order) to 1e-9 relative, for hierarchies of 2 to 5 levels and K-cycle depths 1 to all;
 - FCG with the multilevel preconditioner reaches the FP64 true residual in a small fraction
-  of Jacobi-CG's iterations, with the Chebyshev and with the learned fine smoother.
+  of Jacobi-CG's iterations, with the Chebyshev and with the learned fine smoother;
+- the trained smoother needs fewer iterations than the untrained (polynomial) one and than
+  Chebyshev, on a plate it was not trained on;
+- a learned smoother that breaks down hands over to the same hierarchy with the Chebyshev
+  smoother, which still converges.
 No reference counterpart exists for this component.
 """
 import numpy as np
@@ -134,3 +138,40 @@ def test_fused_tail_matches_separate_kernels(dev, monkeypatch, kdepth):
     z_kern = dev.mg_apply(r)
     assert np.linalg.norm(z_tail - z_kern) <= 1e-12 * np.linalg.norm(z_kern)
     assert np.linalg.norm(z_kern) > 0
+
+
+def test_trained_smoother_beats_untrained_and_chebyshev(dev):
+    theta, losses = learned.train(dev, learned.training_plate(64), steps=300)
+    assert losses[-1] < 0.8 * losses[0]
+    system(dev, hetero(200, e_seed=7, seed=9))
+    its = {}
+    for name, th, smoother in (("trained", theta, 0), ("untrained", learned.polynomial_start(theta), 0),
+                               ("chebyshev", None, 1)):
+        dev.setup()
+        if th is not None:
+            dev.mpnn_set_params(th)
+            dev.mpnn_build()
+        dev.mg_setup(12, 500, smoother, 3)
+        _, s = dev.solve(rtol=1e-10, max_iter=2000, method=FCG)
+        assert s.converged and not s.fallback and s.true_rel_residual < 1e-10, name
+        its[name] = s.iterations
+    assert its["trained"] < its["untrained"] and its["trained"] < its["chebyshev"], its
+
+
+@pytest.mark.parametrize("bad", ["nan", "indefinite"])
+def test_bad_learned_smoother_falls_back_to_chebyshev_cycle(dev, bad):
+    A, xy, fixed, bptr, bcol = system(dev, hetero(120, e_seed=7, seed=9))
+    dev.mpnn_init(0)
+    th = dev.mpnn_get_params()
+    if bad == "nan":
+        th[learned.BD] = np.nan      # M = NaN: breakdown at the first iteration
+    else:
+        th[learned.BD + 2] = -1.0    # M = diag(+D^-1, -D^-1): indefinite
+    dev.mpnn_set_params(th)
+    dev.mpnn_build()
+    dev.mg_setup(10, 300, 0)
+    x, s = dev.solve(rtol=1e-10, max_iter=5000, method=FCG)
+    assert s.converged and s.fallback == 2 and s.true_rel_residual < 1e-10, (s.fallback, s.iterations)
+    assert s.iterations < 600, s.iterations
+    b = dev.download_rhs()
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10

@@ -156,7 +156,7 @@ def test_fallback_on_bad_network(dev, bad):
     dev.mpnn_build()
     for method in (FCG, FCG32):
         x, s = dev.solve(rtol=1e-10, max_iter=20000, method=method)
-        assert s.fallback and s.converged, (bad, method)
+        assert s.fallback == 1 and s.converged, (bad, method)  # Jacobi-CG
         assert s.true_rel_residual < 1e-10
         if bad == "nan":
             assert s.breakdown or s.iterations >= s_ref.iterations
