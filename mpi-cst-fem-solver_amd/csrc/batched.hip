@@ -8,9 +8,9 @@
 // decreasing estimated cost n^1.5) and runs that plate's whole CG with workgroup barriers
 // only, so no workgroup ever waits for another and every workgroup exits when the queue is
 // empty.  Per-iteration reductions are per-slice partial sums (a fixed 64-lane butterfly)
-// combined by a fixed pairwise tree over the slice index: the result does not depend on the
-// workgroup size or the number of resident workgroups, which is what lets the autotuner pick
-// those for speed alone.
+// combined by fixed butterflies over groups of 64 slices and over the group sums: the result
+// does not depend on the workgroup size or the number of resident workgroups, which is what
+// lets the autotuner pick those for speed alone.
 //
 // Every plate runs the reference's Jacobi-PCG recurrence (solver.cpp:35-105) and is verified
 // by an FP64 true residual b - A x; when the recursive residual reaches rtol (or drops by
@@ -67,20 +67,29 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Fixed pairwise tree over a[0..n): the same additions in the same order for any workgroup
-// size.  Every thread returns the total.
-__device__ inline double tree_sum(double* a, int n) {
-  __syncthreads();
-  for (int stride = 1; stride < n; stride <<= 1) {
-    for (int i = threadIdx.x * 2 * stride; i < n; i += blockDim.x * 2 * stride)
-      if (i + stride < n) a[i] += a[i + stride];
-    __syncthreads();
+// Sum of a[0..n) (n <= kMaxSlicesPerPlate): consecutive groups of 64 entries are summed by a
+// 64-lane butterfly (zeros beyond n), then the group sums by one more butterfly.  The
+// arithmetic is the same for any workgroup size, which wave sums which group does not
+// matter, and it costs two barriers.  Every thread returns the total.
+__device__ inline double fixed_sum(const double* a, int n) {
+  __shared__ double grp[kMaxSlicesPerPlate / 64];
+  __syncthreads();  // the partials are written
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const int G = (n + 63) >> 6;
+  for (int q = wave; q < G; q += W) {
+    const int i = 64 * q + lane;
+    const double v = wave_sum(i < n ? a[i] : 0.0);
+    if (lane == 0) grp[q] = v;
   }
-  const double t = a[0];
   __syncthreads();
-  return t;
+  return wave_sum(lane < G ? grp[lane] : 0.0);
 }
 
+constexpr int kGroup = 4;  // entries per group of loads (8 would spill registers here)
+
+// Rows of one slice times x, entries in column order; loads are issued in groups of kGroup
+// (all indices and values, then all x entries) so that a slice costs two memory round trips
+// per group rather than two per entry.
 __device__ inline void row_spmv(const BatchArgs& A, const double2* __restrict__ x, int s, int lane,
                                 double& y0, double& y1) {
   const int64_t o = A.off[s];
@@ -88,15 +97,28 @@ __device__ inline void row_spmv(const BatchArgs& A, const double2* __restrict__ 
   const int* c = A.col + o + lane;
   const v2d* v = reinterpret_cast<const v2d*>(A.val64 + 4 * o) + lane;
   double a0 = 0.0, a1 = 0.0;
-  for (int k = 0; k < w; ++k) {
-    const int m = c[(size_t)k * 64];
-    const v2d va = v[(size_t)k * 128];
-    const v2d vb = v[(size_t)k * 128 + 64];
-    const double2 xm = x[m];
-    a0 = a0 + va.x * xm.x;
-    a0 = a0 + va.y * xm.y;
-    a1 = a1 + vb.x * xm.x;
-    a1 = a1 + vb.y * xm.y;
+  for (int k = 0; k < w; k += kGroup) {
+    int m[kGroup];
+    v2d va[kGroup], vb[kGroup];
+    double2 xm[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const int kk = k + u < w ? k + u : k;
+      m[u] = c[(size_t)kk * 64];
+      va[u] = v[(size_t)kk * 128];
+      vb[u] = v[(size_t)kk * 128 + 64];
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) xm[u] = x[m[u]];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      if (k + u < w) {
+        a0 = a0 + va[u].x * xm[u].x;
+        a0 = a0 + va[u].y * xm[u].y;
+        a1 = a1 + vb[u].x * xm[u].x;
+        a1 = a1 + vb[u].y * xm[u].y;
+      }
+    }
   }
   y0 = a0;
   y1 = a1;
@@ -109,14 +131,27 @@ __device__ inline void row_spmv(const BatchArgs& A, const float2* __restrict__ x
   const int* c = A.col + o + lane;
   const v4f* v = reinterpret_cast<const v4f*>(A.val32) + (o + lane);
   float a0 = 0.f, a1 = 0.f;
-  for (int k = 0; k < w; ++k) {
-    const int m = c[(size_t)k * 64];
-    const v4f vv = v[(size_t)k * 64];
-    const float2 xm = x[m];
-    a0 = a0 + vv.x * xm.x;
-    a0 = a0 + vv.y * xm.y;
-    a1 = a1 + vv.z * xm.x;
-    a1 = a1 + vv.w * xm.y;
+  for (int k = 0; k < w; k += kGroup) {
+    int m[kGroup];
+    v4f vv[kGroup];
+    float2 xm[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const int kk = k + u < w ? k + u : k;
+      m[u] = c[(size_t)kk * 64];
+      vv[u] = v[(size_t)kk * 64];
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) xm[u] = x[m[u]];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      if (k + u < w) {
+        a0 = a0 + vv[u].x * xm[u].x;
+        a0 = a0 + vv[u].y * xm[u].y;
+        a1 = a1 + vv[u].z * xm[u].x;
+        a1 = a1 + vv[u].w * xm[u].y;
+      }
+    }
   }
   y0 = a0;
   y1 = a1;
@@ -177,8 +212,8 @@ __device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* pa
       part2[s - s0] = rz;
     }
   }
-  const double bnorm = sqrt(tree_sum(part1, ns));
-  double rz = tree_sum(part2, ns);
+  const double bnorm = sqrt(fixed_sum(part1, ns));
+  double rz = fixed_sum(part2, ns);
   int it = 0, refreshes = 0, status = kRunning;
   double rel = 1.0, true_rel = 0.0, rel_ref = 1.0, prev_true = 1.0;
   int stagnant = 0;
@@ -204,7 +239,7 @@ __device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* pa
       t = wave_sum(t);
       if (lane == 0) part0[s - s0] = t;
     }
-    const double pAp = tree_sum(part0, ns);
+    const double pAp = fixed_sum(part0, ns);
     if (!(pAp > 0.0)) {
       status = kBreakdown;
       break;
@@ -240,8 +275,8 @@ __device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* pa
         part2[s - s0] = rz_s;
       }
     }
-    const double rr = tree_sum(part1, ns);
-    double rz_new = tree_sum(part2, ns);
+    const double rr = fixed_sum(part1, ns);
+    double rz_new = fixed_sum(part2, ns);
     ++it;
     rel = sqrt(rr) / bnorm;
     bool restart = false;
@@ -272,8 +307,8 @@ __device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* pa
           part2[s - s0] = t_rz;
         }
       }
-      true_rel = sqrt(tree_sum(part1, ns)) / bnorm;
-      rz_new = tree_sum(part2, ns);
+      true_rel = sqrt(fixed_sum(part1, ns)) / bnorm;
+      rz_new = fixed_sum(part2, ns);
       ++refreshes;
       if (true_rel < A.rtol) {
         status = kConverged;
