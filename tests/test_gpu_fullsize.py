@@ -1,0 +1,93 @@
+"""The benchmark plates at their full sizes.
+
+- Config 2 (heterogeneous plate, 3,993,138 DOFs) and config 1 (Kirsch quarter plate,
+  980,000 DOFs): the pattern, assembled values, loads and the Dirichlet-eliminated K and b
+  are bit-identical to the oracle's (which builds them in about a second).
+- The solutions are checked through properties that do not need the oracle's CG (minutes at
+  these sizes):
+  - the FP64 true relative residual, recomputed on the host with scipy, is below 1e-10 for
+    FP64 Jacobi-CG and for the learned multilevel FCG;
+  - the Kirsch plate's stress concentration lies near the theoretical 3.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import cstfem_hip
+from cstfem_hip import learned, pipeline
+
+pytestmark = pytest.mark.gpu
+
+
+def device_arrays(dev, prob):
+    pipeline.load_problem(dev, prob)
+    dev.build_stiffness_pattern()
+    rp, c = dev.download_pattern()
+    dev.assemble_stiffness()
+    v = dev.download_values()
+    dev.set_rhs(np.zeros(dev.n))
+    for g, t in prob.tractions:
+        dev.add_traction(prob.mesh.groups[g], **t)
+    if prob.body is not None:
+        dev.add_body_force(*prob.body)
+    f = dev.download_rhs()
+    d, val = prob.dirichlet_dofs()
+    dev.apply_dirichlet(d, val)
+    K = dev.download_values()
+    b = dev.download_rhs()
+    return dict(rp=rp, c=c, v=v, f=f, K=K, b=b)
+
+
+def oracle_arrays(oracle, prob):
+    m = prob.mesh
+    strain = 1 if prob.plane == cstfem_hip.PLANE_STRAIN else 0
+    rp, c = oracle.build_pattern(m.num_nodes, m.tri)
+    v = oracle.assemble(m.xy, m.tri, rp, c, E=prob.E, nu=prob.nu, strain=strain,
+                        t=prob.thickness, E_elem=prob.E_elem)
+    f = np.zeros(m.num_dofs)
+    for g, t in prob.tractions:
+        oracle.add_traction(m.xy, m.tri, m.groups[g], f, t=prob.thickness, **t)
+    if prob.body is not None:
+        oracle.add_body_force(m.xy, m.tri, f, prob.thickness, *prob.body)
+    d, val = prob.dirichlet_dofs()
+    K, b = oracle.apply_dirichlet(rp, c, v, f, d, val)
+    return dict(rp=rp, c=c, v=v, f=f, K=K, b=b)
+
+
+def true_rel(r, x):
+    A = sp.csr_matrix((r["K"], r["c"], r["rp"]), shape=(len(r["b"]), len(r["b"])))
+    return np.linalg.norm(r["b"] - A @ x) / np.linalg.norm(r["b"])
+
+
+@pytest.mark.parametrize("config", [2, 1])
+def test_full_size_system_bitwise_and_solutions(dev, oracle, config):
+    prob = pipeline.config_problem(config)
+    r = device_arrays(dev, prob)
+    o = oracle_arrays(oracle, prob)
+    for key in ("rp", "c", "v", "f", "K", "b"):
+        assert np.array_equal(r[key], o[key]), key
+    del o
+    dev.setup()
+    x, s = dev.solve(rtol=1e-10, max_iter=200000, method=cstfem_hip.METHOD_CG_FP64_REFINED)
+    assert s.converged and s.true_rel_residual < 1e-10
+    assert true_rel(r, x) < 1e-10
+    print(f"config {config}: Jacobi-CG {s.iterations} iterations", flush=True)
+    if config == 1:
+        k = dev.kirsch_metrics(1.0, 1.0)
+        print("kirsch", k, flush=True)
+        assert 2.9 < k["scf_numerical"] < 3.2, k
+    # learned multilevel FCG, network trained on a small plate of the same family
+    tdev = cstfem_hip.Device(0)
+    try:
+        theta, _ = learned.train(tdev, learned.training_problem(config, 64), steps=300)
+    finally:
+        tdev.close()
+    dev.setup()
+    dev.mpnn_set_params(theta)
+    dev.mpnn_build()
+    dev.mg_setup(12, 500, 0, 3)
+    x2, s2 = dev.solve(rtol=1e-10, max_iter=2000, method=cstfem_hip.METHOD_FCG_LEARNED)
+    print(f"config {config}: learned multilevel FCG {s2.iterations} iterations", flush=True)
+    assert s2.converged and not s2.fallback and s2.true_rel_residual < 1e-10
+    assert s2.iterations * 100 < s.iterations
+    assert true_rel(r, x2) < 1e-10
