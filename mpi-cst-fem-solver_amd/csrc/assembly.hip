@@ -87,9 +87,10 @@ void exclusive_scan_int(Context& c, const int* in, int* out, int64_t count, int6
   }
   scan_rec(c, in, out, count);
   if (total) {
-    int last_in = 0, last_out = 0;
-    HIP_CHECK(hipMemcpy(&last_in, in + count - 1, sizeof(int), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(&last_out, out + count - 1, sizeof(int), hipMemcpyDeviceToHost));
+    int last_in = 0, last_out = 0;  // read in stream order after the scan kernels
+    HIP_CHECK(hipMemcpyAsync(&last_in, in + count - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(&last_out, out + count - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
     *total = (int64_t)last_in + last_out;
   }
 }

@@ -26,6 +26,7 @@ void hip_check(hipError_t e, const char* what, const char* file, int line) {
 
 Context::~Context() {
   if (h_scal) (void)hipHostFree(h_scal);
+  if (aux) (void)hipStreamDestroy(aux);
   if (stream) (void)hipStreamDestroy(stream);
 }
 

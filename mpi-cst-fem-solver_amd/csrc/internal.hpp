@@ -186,6 +186,7 @@ struct Multigrid {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;  // copies that may overlap the main stream (created on first use)
   int cu_count = 0;
 
   // mesh (host copies kept for boundary loads)

@@ -1449,18 +1449,20 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
   mg.fine_smoother = fine_smoother;
   mg.kdepth = kdepth;
   SetupTrace tr;
-  HIP_CHECK(hipStreamSynchronize(c.stream));
   // level 0: the context's matrix; its node graph is read on the host (aggregation) from a
-  // page-locked copy
+  // page-locked copy.  The pattern is complete since build_pattern returned, so the copy runs
+  // on the auxiliary stream and the host aggregation overlaps whatever the main stream is
+  // still doing (building M, typically); the device work below follows it in stream order.
   MgLevel* L0 = new MgLevel;
   mg.lv.push_back(L0);
   L0->n = c.n;
   L0->nodes = c.nb;
   L0->bs = 2;
+  if (!c.aux) HIP_CHECK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
   int* pin = pinned_ints(mg, (size_t)c.nb + 1 + (size_t)c.nblk);
-  HIP_CHECK(hipMemcpyAsync(pin, c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1, c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  HIP_CHECK(hipMemcpyAsync(pin, c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost, c.aux));
+  HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1, c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost, c.aux));
+  HIP_CHECK(hipStreamSynchronize(c.aux));
   tr.mark(c, 0, "download pattern");
   alloc_work(*L0);
   if (fine_smoother == 1) L0->lam = smoothing_max(c, 0);
@@ -1616,6 +1618,7 @@ void mg_download_matrix(Context& c, int level, int* row_ptr, int* cols, double* 
     throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: level " + std::to_string(level) + " has no coarse matrix");
   MgLevel& L = *c.mg.lv[level];
   std::vector<double> val(L.val.n);
+  HIP_CHECK(hipStreamSynchronize(c.stream));
   HIP_CHECK(hipMemcpy(val.data(), L.val.get(), sizeof(double) * val.size(), hipMemcpyDeviceToHost));
   int64_t k = 0;
   row_ptr[0] = 0;

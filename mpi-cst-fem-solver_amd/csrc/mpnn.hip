@@ -602,8 +602,7 @@ void mpnn_forward(Context& c) {
                      (const float*)m.theta.get(), (const double*)c.inv_diag.get(),
                      (const double*)c.sell_val.get(), m.ghat.get(), m.gsell.get());
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(c.stream));
-  m.built = true;
+  m.built = true;  // M is ready in stream order; no host synchronisation (set-up may overlap)
 }
 
 void mpnn_set_pairs(Context& c, int K, const float* R, const float* E) {
@@ -804,6 +803,7 @@ double mpnn_train_step(Context& c, float lr) {
 
 void mpnn_download(Context& c, int what, float* out, int64_t count) {
   Mpnn& m = c.mpnn;
+  HIP_CHECK(hipStreamSynchronize(c.stream));  // the arrays are written on the library stream
   const float* src = nullptr;
   int64_t n = 0;
   DBuf<float> tmp;
