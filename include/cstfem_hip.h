@@ -249,12 +249,15 @@ int cstfem_mpnn_download(cstfem_ctx* ctx, int what, float* out, int64_t count);
  * fine_smoother: 0 the learned M (cstfem_mpnn_build first), 1 Chebyshev (degree 2) on
  * D^-1 K.  Levels are added until a level has <= coarse_max DOFs or max_levels is reached
  * (the coarsest must have <= 4096 DOFs).  Call after cstfem_setup; a new setup drops the
- * hierarchy.  While a hierarchy exists, the FP64 flexible CG uses it. */
+ * hierarchy.  While a hierarchy exists, the FP64 flexible CG uses it.  With
+ * fine_smoother = 0 and M built, the set-up estimates lambda_max(M K) (12 power steps); above
+ * 1.5 (or not finite) the learned smoother is replaced by Chebyshev for this hierarchy. */
 int cstfem_mg_setup(cstfem_ctx* ctx, int max_levels, int coarse_max, int fine_smoother,
                     int kcycle_levels);
 int cstfem_mg_levels(cstfem_ctx* ctx, int* levels);
 /* DOFs, aggregates (0 on the coarsest), stored entries and the upper end of the smoothing
- * interval (lambda_max estimate of D^-1 K; 0 where no Chebyshev smoother runs) of one level. */
+ * interval (lambda_max estimate of D^-1 K; 0 where no Chebyshev smoother runs) of one level.
+ * On level 0 with a learned smoother it is the set-up's estimate of lambda_max(M K). */
 int cstfem_mg_level_info(cstfem_ctx* ctx, int level, int* n, int* nagg, int64_t* nnz,
                          double* lam);
 /* Aggregate of every node of `level` (count = nodes of the level). */

@@ -165,9 +165,11 @@ struct Multigrid {
   bool ready = false;
   int fine_smoother = 0;  // 0: learned M, 1: Chebyshev on D^-1 A
   int kdepth = 3;         // levels 1..kdepth use the K-cycle, deeper ones the V-cycle
+  int kdepth_set = 3;     // the set-up's value while a Chebyshev fallback widens kdepth
   std::vector<MgLevel*> lv;
   int nc = 0;             // DOFs of the coarsest level (dense inverse)
   int tail_from = -1;     // first level of the fused single-workgroup tail cycle (-1: none)
+  double m_bound = 0.0;   // lambda_max(M A) of the learned level-0 smoother, before scaling
   DBuf<double> cinv;
   int* pinned = nullptr;  // page-locked staging of the fine node graph, kept across set-ups
   size_t pinned_n = 0;
@@ -179,6 +181,7 @@ struct Multigrid {
     for (MgLevel* l : lv) delete l;
     lv.clear();
     tail_from = -1;
+    m_bound = 0.0;
     ready = false;
   }
 };

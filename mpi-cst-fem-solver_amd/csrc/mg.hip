@@ -500,6 +500,9 @@ __global__ __launch_bounds__(kT) void k_dense(int n, const double* __restrict__ 
 
 // ---- fused tail: the V-cycle of the smallest levels in one workgroup ------------------------
 
+constexpr int kSmootherSteps = 12;       // power steps for lambda_max(M A) of the learned smoother
+constexpr double kSmootherAccept = 1.5;  // largest lambda_max(M A) accepted for the smoother
+
 constexpr int kTailLevels = 4;       // cycle levels in the fused tail (plus the dense coarsest)
 constexpr int kTailThreads = 1024;
 constexpr int kTailLds = 64 * 1024;  // bytes of LDS for the tail's vectors
@@ -894,6 +897,36 @@ __global__ __launch_bounds__(kT) void k0_power(Mat0 A, const double2* __restrict
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// y = A x on level 0
+__global__ __launch_bounds__(kT) void k0_apply(Mat0 A, const double2* __restrict__ x,
+                                               double2* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < A.nslices; s += gridDim.x * kW) {
+    double y0, y1;
+    row0_spmv(A, x, s, lane, y0, y1);
+    if (lane < A.nrow[s]) y[A.row0[s] + lane] = make_double2(y0, y1);
+  }
+}
+
+// z = M y with partial sums of z'z
+__global__ __launch_bounds__(kT) void k0_mapply_sq(MatM M, const double2* __restrict__ y,
+                                                   double2* __restrict__ z,
+                                                   double* __restrict__ part) {
+  __shared__ double lds[kW];
+  const int lane = threadIdx.x & 63;
+  double acc = 0.0;
+  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < M.nslices; s += gridDim.x * kW) {
+    double z0, z1;
+    rowM_spmv(M, y, s, lane, z0, z1);
+    if (lane < M.nrow[s]) {
+      z[M.row0[s] + lane] = make_double2(z0, z1);
+      acc += z0 * z0 + z1 * z1;
+    }
+  }
+  const double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // Gershgorin bound of lambda_max(D^-1 A): per-workgroup maxima of dinv_i sum_j |a_ij|
 __global__ __launch_bounds__(kT) void ks_gersh(MatS A, const double* __restrict__ dinv,
                                                double* __restrict__ part) {
@@ -1115,6 +1148,37 @@ double smoothing_max(Context& c, int level) {
   const double gb = gersh_max(c, level);
   const double pw = power_max(c, level);
   return std::min(gb, 1.3 * pw);
+}
+
+// largest eigenvalue of M A (the learned smoother on level 0) by `iters` power steps
+double learned_smoother_bound(Context& c, int iters) {
+  const int n = c.n, gs = slice_grid(c.nslices);
+  const int gn = std::min(kPart, grid1(n));
+  const MatM M{c.sell_off.get(), c.sell_col.get(), c.mpnn.gsell.get(), c.sell_row0.get(),
+               c.sell_nrow.get(), c.nslices};
+  DBuf<double> x, y, part, st;
+  x.alloc(n);
+  y.alloc(n);
+  st.alloc(2);
+  part.alloc(std::max(gs, gn));
+  HIP_CHECK(hipMemsetAsync(st.get(), 0, 2 * sizeof(double), c.stream));
+  hipLaunchKernelGGL(k_power_init, dim3(grid1(n)), dim3(kT), 0, c.stream, n, x.get());
+  hipLaunchKernelGGL(k_sumsq, dim3(gn), dim3(kT), 0, c.stream, n, (const double*)x.get(), part.get());
+  hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x.get(),
+                     (const double*)part.get(), gn, st.get(), 0);
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k0_apply, dim3(gs), dim3(kT), 0, c.stream, mat0(c),
+                       (const double2*)x.get(), (double2*)y.get());
+    hipLaunchKernelGGL(k0_mapply_sq, dim3(gs), dim3(kT), 0, c.stream, M, (const double2*)y.get(),
+                       (double2*)x.get(), part.get());
+    hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x.get(),
+                       (const double*)part.get(), gs, st.get(), 1);
+  }
+  HIP_CHECK(hipGetLastError());
+  double lam = 0.0;
+  HIP_CHECK(hipMemcpyAsync(&lam, st.get(), sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  return lam;
 }
 
 // Chebyshev coefficients of the two smoothing steps on [lam / 30, lam]
@@ -1448,6 +1512,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
   mg.clear();
   mg.fine_smoother = fine_smoother;
   mg.kdepth = kdepth;
+  mg.kdepth_set = kdepth;
   SetupTrace tr;
   // level 0: the context's matrix; its node graph is read on the host (aggregation) from a
   // page-locked copy.  The pattern is complete since build_pattern returned, so the copy runs
@@ -1598,6 +1663,17 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
   }
   HIP_CHECK(hipStreamSynchronize(c.stream));
   mg.tail_from = tail_start(mg);
+  if (fine_smoother == 0 && c.mpnn.built) {
+    // A good learned smoother has lambda_max(M A) close to 1 (1.14-1.16 on the plates it was
+    // trained for); one far from its training data overshoots on some modes.  Such an M is
+    // replaced by the Chebyshev smoother before any iteration is spent on it.
+    mg.m_bound = learned_smoother_bound(c, kSmootherSteps);
+    if (!(mg.m_bound <= kSmootherAccept)) {
+      mg.ready = true;
+      mg_set_fine_smoother(c, 1);
+    }
+    tr.mark(c, 0, "smoother bound");
+  }
   mg.ready = true;
 }
 
@@ -1641,17 +1717,29 @@ int64_t mg_level_info(Context& c, int level, int* n, int* nagg, double* lam) {
   MgLevel& L = *c.mg.lv[level];
   if (n) *n = L.n;
   if (nagg) *nagg = L.nagg;
-  if (lam) *lam = L.lam;
+  // level 0: lambda_max(M A) when a learned smoother was checked, else that of D^-1 A
+  if (lam) *lam = (level == 0 && c.mg.m_bound != 0.0) ? c.mg.m_bound : L.lam;
   return level == 0 ? c.nblk * 4 : (int64_t)L.bcol.size() * 9;
 }
 
 int mg_levels(Context& c) { return c.mg.ready ? (int)c.mg.lv.size() : 0; }
 
+// Switches the level-0 smoother of a hierarchy set up for the learned one.  The Chebyshev
+// smoother takes the K-cycle on every level with it: with V-cycles below level 3 its
+// iteration count on the heterogeneous plate depends on rounding (78-189 against 72).
 void mg_set_fine_smoother(Context& c, int smoother) {
   Multigrid& mg = c.mg;
   if (!mg.ready) throw Error(CSTFEM_ERR_ORDER, "multilevel: not set up");
-  if (smoother == 1 && !(mg.lv[0]->lam > 0.0)) mg.lv[0]->lam = smoothing_max(c, 0);
+  if (smoother == mg.fine_smoother) return;
+  if (smoother == 1) {
+    if (!(mg.lv[0]->lam > 0.0)) mg.lv[0]->lam = smoothing_max(c, 0);
+    mg.kdepth_set = mg.kdepth;
+    mg.kdepth = std::max(mg.kdepth, (int)mg.lv.size());
+  } else {
+    mg.kdepth = mg.kdepth_set;
+  }
   mg.fine_smoother = smoother;
+  mg.tail_from = tail_start(mg);
 }
 
 }  // namespace cstfem_hip

@@ -1441,15 +1441,30 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
     }
   }
   int fb_iters = 0, fb_outer = 0, fb_kind = 0;
+  // a fallback continues from the current x only if its FP64 true residual is below that of
+  // x = 0 (a diverged iterate would leave the FP64 residual floor above rtol); else from 0
+  auto fallback_start = [&]() {
+    spmv_fp64(c, c.x.get(), Ax);
+    hipLaunchKernelGGL(k_residual, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                       (const double2*)c.b.get(), (const double2*)Ax, (double2*)r64, partX(c));
+    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(kThreads), 0, c.stream, partX(c), gv,
+                       &c.scal.get()->rr);
+    HIP_CHECK(hipGetLastError());
+    const double cur = std::sqrt(read_scalars(c).rr) / bnorm;
+    if (cur < 1.0) return true;
+    HIP_CHECK(hipMemsetAsync(c.x.get(), 0, sizeof(double) * nv, c.stream));
+    return false;
+  };
   if (!converged && !zero_rhs && multilevel && c.mg.fine_smoother == 0) {
     // the learned smoother failed on this matrix: the same hierarchy with the Chebyshev
     // smoother, from the current x (its own stall falls back to Jacobi-CG)
     cstfem_solver_options fo = o;
     fo.max_iter = std::max(1, o.max_iter - iters);
     cstfem_solver_stats fs{};
+    const bool warm_x = fallback_start();
     mg_set_fine_smoother(c, 1);
     try {
-      fcg_run<P>(c, fo, fs, false, true);
+      fcg_run<P>(c, fo, fs, false, warm_x);
     } catch (...) {
       mg_set_fine_smoother(c, 0);
       throw;
@@ -1472,7 +1487,8 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
     fo.inner_rtol = 0.0;
     fo.refresh = 0;
     cstfem_solver_stats fs{};
-    solve_refined(c, fo, fs, true, true);
+    const bool warm_x = fallback_start();
+    solve_refined(c, fo, fs, true, warm_x);
     fb_iters = fs.iterations;
     fb_outer = fs.outer_iterations;
     converged = fs.converged != 0;

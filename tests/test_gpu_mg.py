@@ -8,8 +8,10 @@
   of Jacobi-CG's iterations, with the Chebyshev and with the learned fine smoother;
 - the trained smoother needs fewer iterations than the untrained (polynomial) one and than
   Chebyshev, on a plate it was not trained on;
-- a learned smoother that breaks down hands over to the same hierarchy with the Chebyshev
-  smoother, which still converges.
+- a learned smoother that overshoots (lambda_max(M A) > 1.5) is replaced by Chebyshev at
+  set-up; one that breaks down during the solve hands over to the same hierarchy with the
+  Chebyshev smoother; either way the solve converges;
+- a good trained smoother passes the set-up check.
 No reference counterpart exists for this component.
 """
 import numpy as np
@@ -152,6 +154,8 @@ def test_trained_smoother_beats_untrained_and_chebyshev(dev):
             dev.mpnn_set_params(th)
             dev.mpnn_build()
         dev.mg_setup(12, 500, smoother, 3)
+        if smoother == 0:
+            assert 0.5 < dev.mg_level_info(0)[3] <= 1.5, name  # accepted by the set-up check
         _, s = dev.solve(rtol=1e-10, max_iter=2000, method=FCG)
         assert s.converged and not s.fallback and s.true_rel_residual < 1e-10, name
         its[name] = s.iterations
@@ -170,8 +174,12 @@ def test_bad_learned_smoother_falls_back_to_chebyshev_cycle(dev, bad):
     dev.mpnn_set_params(th)
     dev.mpnn_build()
     dev.mg_setup(10, 300, 0)
+    # the set-up rejects a smoother whose lambda_max(M A) is above 1.5 (or not finite) and
+    # uses Chebyshev from the start; one that passes and then fails hands over at run time
+    rejected = not (dev.mg_level_info(0)[3] <= 1.5)
     x, s = dev.solve(rtol=1e-10, max_iter=5000, method=FCG)
-    assert s.converged and s.fallback == 2 and s.true_rel_residual < 1e-10, (s.fallback, s.iterations)
+    assert s.converged and s.true_rel_residual < 1e-10, (s.fallback, s.iterations)
+    assert (rejected and s.fallback == 0) or s.fallback == 2, (rejected, s.fallback)
     assert s.iterations < 600, s.iterations
     b = dev.download_rhs()
     assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10
