@@ -1552,18 +1552,32 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     const int* hc = l == 0 ? pin + c.nb + 1 : L.bcol.data();
     L.nagg = aggregate(hp, hc, L.nodes, L.h_agg);
     const int na = L.nagg;
-    std::vector<int> mptr(na + 1, 0), mem(L.nodes);
+    // members and the aggregate map go up from page-locked memory; the staging area follows
+    // the node graph, which the host no longer reads once the aggregation is done
+    const size_t need = 2 * (size_t)L.nodes + na + 1;
+    // the uploads of the level above read the staging area: let them finish (on level 0 only
+    // the node-graph download used it, and that was waited for)
+    if (l > 0 || need > mg.pinned_n) HIP_CHECK(hipStreamSynchronize(c.stream));
+    int* stage = pinned_ints(mg, std::max(mg.pinned_n, need));
+    int* mptr = stage;
+    int* mem = stage + na + 1;
+    int* agg = mem + L.nodes;
+    std::fill(mptr, mptr + na + 1, 0);
     for (int i = 0; i < L.nodes; ++i) ++mptr[L.h_agg[i] + 1];
     for (int a = 0; a < na; ++a) mptr[a + 1] += mptr[a];
     for (int a = 0; a < na; ++a)
       if (L.bs * (mptr[a + 1] - mptr[a]) > 192) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: aggregate too large");
     {
-      std::vector<int> fill(mptr.begin(), mptr.end() - 1);
+      std::vector<int> fill(mptr, mptr + na);
       for (int i = 0; i < L.nodes; ++i) mem[fill[L.h_agg[i]]++] = i;
     }
-    upload(L.agg, L.h_agg);
-    upload(L.mptr, mptr);
-    upload(L.mem, mem);
+    std::copy(L.h_agg.begin(), L.h_agg.end(), agg);
+    L.agg.alloc(L.nodes);
+    L.mptr.alloc(na + 1);
+    L.mem.alloc(L.nodes);
+    HIP_CHECK(hipMemcpyAsync(L.agg.get(), agg, sizeof(int) * L.nodes, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(L.mptr.get(), mptr, sizeof(int) * (na + 1), hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(L.mem.get(), mem, sizeof(int) * L.nodes, hipMemcpyHostToDevice, c.stream));
     tr.mark(c, l, "aggregate, members");
     MgLevel* C = new MgLevel;
     mg.lv.push_back(C);
