@@ -400,13 +400,6 @@ __global__ __launch_bounds__(kT) void ks_dot(int n, const double* __restrict__ a
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kT) void ks_copy(int n, const double* __restrict__ a,
-                                              double* __restrict__ b,
-                                              const PcgScalars* __restrict__ sc) {
-  if (stopped(sc)) return;
-  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) b[i] = a[i];
-}
-
 // sum over the member rows k = k0, k0 + step, ... of aggregate a (rows in member order, then
 // DOF order) of P[f * 3 + c] res[f], in that order; loads in groups of 8
 __device__ inline double restrict_sum(int a, int c, int bs, const int* __restrict__ mptr,

@@ -833,12 +833,4 @@ void mpnn_download(Context& c, int what, float* out, int64_t count) {
   HIP_CHECK(hipMemcpy(out, src, sizeof(float) * n, hipMemcpyDeviceToHost));
 }
 
-// z = M r on FP32 vectors (used by the flexible CG)
-void mpnn_apply(Context& c, const float* r, float* z) {
-  const int gs = std::max(1, std::min(2048, (c.nslices + 3) / 4));
-  hipLaunchKernelGGL(k_gspmv, dim3(gs), dim3(kT), 0, c.stream, c.nslices, c.sell_row0.get(),
-                     c.sell_nrow.get(), c.sell_off.get(), c.sell_col.get(),
-                     (const float*)c.mpnn.gsell.get(), (const float2*)r, (float2*)z);
-}
-
 }  // namespace cstfem_hip
