@@ -148,6 +148,7 @@ struct MgLevel {
   DBuf<int64_t> off;
   DBuf<int> col;
   DBuf<double> val;
+  DBuf<float> val32;  // FP32 copy of val for the cycle's row sums (when enabled)
   DBuf<double> dinv;
   double lam = 0.0;  // estimate of lambda_max(D^-1 A)
   // restriction / prolongation to level + 1 (absent on the coarsest level)
@@ -169,7 +170,8 @@ struct Multigrid {
   std::vector<MgLevel*> lv;
   int nc = 0;             // DOFs of the coarsest level (dense inverse)
   int tail_from = -1;     // first level of the fused single-workgroup tail cycle (-1: none)
-  double m_bound = 0.0;   // lambda_max(M A) of the learned level-0 smoother, before scaling
+  double m_bound = 0.0;   // lambda_max(M A) of the learned level-0 smoother
+  bool coarse_fp32 = false;  // levels >= 1 hold FP32 copies of their values (learned smoother)
   DBuf<double> cinv;
   int* pinned = nullptr;  // page-locked staging of the fine node graph, kept across set-ups
   size_t pinned_n = 0;

@@ -195,16 +195,18 @@ __global__ __launch_bounds__(kT) void k0_cheb(Mat0 A, const double2* __restrict_
 
 // ---- levels >= 1: scalar SELL-64 ---------------------------------------------------------
 
-struct MatS {
+// Scalar SELL-64 matrix of a level >= 1 with values of type T (the FP64 values, or an FP32
+// copy for the cycle's row sums)
+template <class T>
+struct MatST {
   const int64_t* off;
   const int* col;
-  const double* val;
+  const T* val;
   int n;
 };
+using MatS = MatST<double>;
+using MatS32 = MatST<float>;
 
-// Entries k = k0, k0 + step, ... (k < w) of one SELL row in ascending order, accumulated
-// into a in that order.  Loads are issued in groups of 8 (all column indices and values, then
-// all x entries) so that a row costs two memory round trips per group instead of two per entry.
 // x entries read straight from a vector
 struct XVec {
   const double* __restrict__ x;
@@ -220,8 +222,12 @@ struct XCheb1 {
   __device__ double operator()(int j) const { return 0.0 + c1r * (dinv[j] * (rhs[j] - 0.0)); }
 };
 
-template <int G, class XF>
-__device__ inline double row_entries(const MatS& A, const XF& xf, int64_t base, int w, int k0,
+// Entries k = k0, k0 + step, ... (k < w) of one SELL row in ascending order, accumulated
+// into a in that order (FP64).  Loads are issued in groups of G (all column indices and
+// values, then all x entries) so that a row costs two memory round trips per group instead
+// of two per entry.
+template <int G, class XF, class T>
+__device__ inline double row_entries(const MatST<T>& A, const XF& xf, int64_t base, int w, int k0,
                                      int step, double a) {
   for (int k = k0; k < w; k += G * step) {
     int cidx[G];
@@ -231,7 +237,7 @@ __device__ inline double row_entries(const MatS& A, const XF& xf, int64_t base, 
       const int kk = k + u * step;
       const int64_t q = base + (int64_t)(kk < w ? kk : k0) * 64;  // in-row index for the tail
       cidx[u] = A.col[q];
-      v[u] = A.val[q];
+      v[u] = (double)A.val[q];
     }
 #pragma unroll
     for (int u = 0; u < G; ++u) xv[u] = xf(cidx[u]);
@@ -242,8 +248,8 @@ __device__ inline double row_entries(const MatS& A, const XF& xf, int64_t base, 
   return a;
 }
 
-template <class XF>
-__device__ inline double rowS_f(const MatS& A, const XF& xf, int i) {
+template <class XF, class T>
+__device__ inline double rowS_f(const MatST<T>& A, const XF& xf, int i) {
   const int s = i >> 6, lane = i & 63;
   const int64_t o = A.off[s];
   const int w = (int)((A.off[s + 1] - o) >> 6);
@@ -257,8 +263,8 @@ __device__ inline double rowS(const MatS& A, const double* __restrict__ x, int i
 // Row i of A x computed by LPR consecutive lanes (lane j takes the entries k = j, j + LPR, ...;
 // the partial sums are combined by a butterfly, so every lane of the group returns the sum).
 // The LPR lanes of a group are always active together.
-template <int LPR, class XF>
-__device__ inline double rowS_lanes(const MatS& A, const XF& xf, int i, int j) {
+template <int LPR, class XF, class T>
+__device__ inline double rowS_lanes(const MatST<T>& A, const XF& xf, int i, int j) {
   if (LPR == 1) return rowS_f(A, xf, i);
   const int s = i >> 6, lane = i & 63;
   const int64_t o = A.off[s];
@@ -276,8 +282,8 @@ __device__ inline double rowS_lanes(const MatS& A, const XF& xf, int i, int j) {
 // of loads would set the kernel's duration.
 inline int lanes_per_row(int n) { return n >= 200000 ? 1 : n >= 50000 ? 2 : n >= 10000 ? 4 : 8; }
 
-template <int LPR>
-__global__ __launch_bounds__(kT) void ks_cheb(MatS A, const double* __restrict__ xin,
+template <int LPR, class T>
+__global__ __launch_bounds__(kT) void ks_cheb(MatST<T> A, const double* __restrict__ xin,
                                               double* __restrict__ xout, double* __restrict__ d,
                                               const double* __restrict__ r,
                                               const double* __restrict__ dinv, double cd, double cr,
@@ -296,8 +302,8 @@ __global__ __launch_bounds__(kT) void ks_cheb(MatS A, const double* __restrict__
 // Both pre-smoothing steps from x = 0 in one pass: the first step's x (= c1r D^-1 rhs) is
 // evaluated at every column the second step's row sum needs.  Writes d and x2 of the second
 // step; identical in value to the two separate steps.
-template <int LPR>
-__global__ __launch_bounds__(kT) void ks_cheb_pre(MatS A, double* __restrict__ x2,
+template <int LPR, class T>
+__global__ __launch_bounds__(kT) void ks_cheb_pre(MatST<T> A, double* __restrict__ x2,
                                                   double* __restrict__ d,
                                                   const double* __restrict__ r,
                                                   const double* __restrict__ dinv, double c1r,
@@ -316,8 +322,8 @@ __global__ __launch_bounds__(kT) void ks_cheb_pre(MatS A, double* __restrict__ x
   x2[i] = (0.0 + d1) + dd;
 }
 
-template <int LPR>
-__global__ __launch_bounds__(kT) void ks_resid(MatS A, const double* __restrict__ x,
+template <int LPR, class T>
+__global__ __launch_bounds__(kT) void ks_resid(MatST<T> A, const double* __restrict__ x,
                                                const double* __restrict__ r,
                                                double* __restrict__ res,
                                                const PcgScalars* __restrict__ sc) {
@@ -331,8 +337,8 @@ __global__ __launch_bounds__(kT) void ks_resid(MatS A, const double* __restrict_
 
 // K-cycle step: beta = first ? 0 : -z'q_old / p'q_old; q = A z + beta q; p = z + beta p;
 // partials p'q -> part[b], p'res -> part[kPart + b]
-template <int LPR>
-__global__ __launch_bounds__(kT) void ks_spmv_dir(MatS A, const double* __restrict__ z,
+template <int LPR, class T>
+__global__ __launch_bounds__(kT) void ks_spmv_dir(MatST<T> A, const double* __restrict__ z,
                                                   double* __restrict__ p, double* __restrict__ q,
                                                   const double* __restrict__ res,
                                                   double* __restrict__ part,
@@ -975,6 +981,12 @@ __global__ __launch_bounds__(kT) void k0_gersh(Mat0 A, const double2* __restrict
   }
 }
 
+__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
 // start vector of the power iteration: 1 + 0.5 sin(0.7 i + 0.3)
 __global__ void k_power_init(int n, double* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1018,6 +1030,7 @@ Mat0 mat0(Context& c) {
 }
 
 MatS mats(MgLevel& L) { return MatS{L.off.get(), L.col.get(), L.val.get(), L.n}; }
+MatS32 mats32(MgLevel& L) { return MatS32{L.off.get(), L.col.get(), L.val32.get(), L.n}; }
 
 int slice_grid(int nslices) { return std::max(1, std::min(2048, (nslices + kW - 1) / kW)); }
 
@@ -1283,24 +1296,45 @@ void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars
     default: hipLaunchKernelGGL(KERNEL<8>, dim3(grid), dim3(kT), 0, c.stream, __VA_ARGS__); break; \
   }
 
+// the same with the matrix value type as the kernel's second template argument: FP32 when
+// the level holds an FP32 copy of its values and the learned smoother is in use
+#define CSTFEM_LPR_SWITCH_M(L, lpr, KERNEL, grid, ...)                                           \
+  if ((L).val32.n && c.mg.fine_smoother == 0) {                                               \
+    const MatS32 A_ = mats32(L);                                                              \
+    switch (lpr) {                                                                            \
+      case 1: hipLaunchKernelGGL((KERNEL<1, float>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
+      case 2: hipLaunchKernelGGL((KERNEL<2, float>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
+      case 4: hipLaunchKernelGGL((KERNEL<4, float>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL((KERNEL<8, float>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
+    }                                                                                         \
+  } else {                                                                                    \
+    const MatS A_ = mats(L);                                                                  \
+    switch (lpr) {                                                                            \
+      case 1: hipLaunchKernelGGL((KERNEL<1, double>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
+      case 2: hipLaunchKernelGGL((KERNEL<2, double>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
+      case 4: hipLaunchKernelGGL((KERNEL<4, double>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL((KERNEL<8, double>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
+    }                                                                                         \
+  }
+
 void launch_cheb(Context& c, MgLevel& L, const double* xin, double* xout, const double* r,
                  double cd, double cr, const PcgScalars* sc) {
   const int lpr = lanes_per_row(L.n);
-  CSTFEM_LPR_SWITCH(lpr, ks_cheb, grid1((int64_t)L.n * lpr), mats(L), xin, xout, L.d.get(), r,
-                    (const double*)L.dinv.get(), cd, cr, sc);
+  CSTFEM_LPR_SWITCH_M(L, lpr, ks_cheb, grid1((int64_t)L.n * lpr), xin, xout, L.d.get(), r,
+                      (const double*)L.dinv.get(), cd, cr, sc);
 }
 
 void launch_cheb_pre(Context& c, MgLevel& L, double* x2, const double* r, const Cheb& ch,
                      const PcgScalars* sc) {
   const int lpr = lanes_per_row(L.n);
-  CSTFEM_LPR_SWITCH(lpr, ks_cheb_pre, grid1((int64_t)L.n * lpr), mats(L), x2, L.d.get(), r,
+  CSTFEM_LPR_SWITCH_M(L, lpr, ks_cheb_pre, grid1((int64_t)L.n * lpr), x2, L.d.get(), r,
                     (const double*)L.dinv.get(), ch.c1r, ch.c2d, ch.c2r, sc);
 }
 
 void launch_resid(Context& c, MgLevel& L, const double* x, const double* r, double* res,
                   const PcgScalars* sc) {
   const int lpr = lanes_per_row(L.n);
-  CSTFEM_LPR_SWITCH(lpr, ks_resid, grid1((int64_t)L.n * lpr), mats(L), x, r, res, sc);
+  CSTFEM_LPR_SWITCH_M(L, lpr, ks_resid, grid1((int64_t)L.n * lpr), x, r, res, sc);
 }
 
 // restriction of level L's residual res into rc (3 per aggregate)
@@ -1418,24 +1452,24 @@ void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars
 // residual), result in lv[l].kx
 void kcycle(Context& c, int l, const PcgScalars* sc) {
   MgLevel& L = *c.mg.lv[l];
-  const MatS A = mats(L);
   const int lpr = lanes_per_row(L.n);
   const int g = kcycle_grid(L);           // grid-stride kernels with <= kPart partials
   double* s0 = L.part.get();             // p'q, p'r of step 1
   double* s1 = L.part.get() + 2 * kPart;  // z'q_old
   double* s2 = L.part.get() + 4 * kPart;  // p'q, p'r of step 2
   cycle_s(c, l, L.kres.get(), L.kz.get(), sc);  // the restriction filled kres
-  CSTFEM_LPR_SWITCH(lpr, ks_spmv_dir, g, A, (const double*)L.kz.get(), L.kp.get(), L.kq.get(),
-                    (const double*)L.kres.get(), s0, (const double*)nullptr, (const double*)nullptr,
-                    g, 1, sc);
+  CSTFEM_LPR_SWITCH_M(L, lpr, ks_spmv_dir, g, (const double*)L.kz.get(), L.kp.get(), L.kq.get(),
+                      (const double*)L.kres.get(), s0, (const double*)nullptr,
+                      (const double*)nullptr, g, 1, sc);
   hipLaunchKernelGGL(ks_update, dim3(g), dim3(kT), 0, c.stream, L.n, L.kx.get(), L.kres.get(),
                      (const double*)L.kp.get(), (const double*)L.kq.get(), (const double*)s0, g, 1, 1,
                      sc);
   cycle_s(c, l, L.kres.get(), L.kz.get(), sc);
   hipLaunchKernelGGL(ks_dot, dim3(g), dim3(kT), 0, c.stream, L.n, (const double*)L.kz.get(),
                      (const double*)L.kq.get(), s1, sc);
-  CSTFEM_LPR_SWITCH(lpr, ks_spmv_dir, g, A, (const double*)L.kz.get(), L.kp.get(), L.kq.get(),
-                    (const double*)L.kres.get(), s2, (const double*)s1, (const double*)s0, g, 0, sc);
+  CSTFEM_LPR_SWITCH_M(L, lpr, ks_spmv_dir, g, (const double*)L.kz.get(), L.kp.get(), L.kq.get(),
+                      (const double*)L.kres.get(), s2, (const double*)s1, (const double*)s0, g, 0,
+                      sc);
   hipLaunchKernelGGL(ks_update, dim3(g), dim3(kT), 0, c.stream, L.n, L.kx.get(), L.kres.get(),
                      (const double*)L.kp.get(), (const double*)L.kq.get(), (const double*)s2, g, 0, 0,
                      sc);
@@ -1506,6 +1540,13 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
   mg.fine_smoother = fine_smoother;
   mg.kdepth = kdepth;
   mg.kdepth_set = kdepth;
+  {
+    // FP32 copies of the coarse values for the learned smoother's cycle (CSTFEM_MG_COARSE32=0
+    // keeps FP64); the Chebyshev-smoothed cycle always reads FP64: its iteration count at the
+    // FP64 residual floor moves with such perturbations (72 -> 90 on the 4M plate)
+    const char* env = std::getenv("CSTFEM_MG_COARSE32");
+    mg.coarse_fp32 = fine_smoother == 0 && !(env && env[0] == '0');
+  }
   SetupTrace tr;
   // level 0: the context's matrix; its node graph is read on the host (aggregation) from a
   // page-locked copy.  The pattern is complete since build_pattern returned, so the copy runs
@@ -1654,6 +1695,12 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     C->dinv.alloc(C->n);
     hipLaunchKernelGGL(ks_dinv, dim3(grid1(C->n)), dim3(kT), 0, c.stream, mats(*C), C->dinv.get());
     HIP_CHECK(hipGetLastError());
+    if (mg.coarse_fp32) {  // FP32 copy of the values for the cycle's row sums
+      C->val32.alloc(C->val.n);
+      hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(kT), 0, c.stream, (int64_t)C->val.n,
+                         (const double*)C->val.get(), C->val32.get());
+      HIP_CHECK(hipGetLastError());
+    }
     HIP_CHECK(hipStreamSynchronize(c.stream));  // C->bcol on the host, C->val for the inverse
     alloc_work(*C);
     tr.mark(c, l, "galerkin, dinv, work");

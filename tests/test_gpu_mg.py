@@ -183,3 +183,21 @@ def test_bad_learned_smoother_falls_back_to_chebyshev_cycle(dev, bad):
     assert s.iterations < 600, s.iterations
     b = dev.download_rhs()
     assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10
+
+
+def test_fp32_coarse_values_match_fp64_cycle(dev, monkeypatch):
+    """The learned smoother's cycle reads FP32 copies of the coarse matrices by default;
+    CSTFEM_MG_COARSE32=0 (read at set-up) keeps FP64.  One application differs by the FP32
+    rounding of the coarse values only."""
+    system(dev, hetero(60))
+    dev.mpnn_init(0)
+    dev.mpnn_set_params(learned.polynomial_start(dev.mpnn_get_params()))
+    dev.mpnn_build()
+    r = np.random.default_rng(4).standard_normal(dev.n)
+    dev.mg_setup(8, 20, 0, 2)
+    z32 = dev.mg_apply(r)
+    monkeypatch.setenv("CSTFEM_MG_COARSE32", "0")
+    dev.mg_setup(8, 20, 0, 2)
+    z64 = dev.mg_apply(r)
+    err = np.linalg.norm(z32 - z64) / np.linalg.norm(z64)
+    assert 0 < err < 1e-5, err
