@@ -146,6 +146,25 @@ __global__ __launch_bounds__(kT) void k0_resid(Mat0 A, const double2* __restrict
   }
 }
 
+// res = r - A x with the FP32 copy of the matrix values (4 floats per block, the layout M
+// uses), FP64 sums
+__global__ __launch_bounds__(kT) void k0_resid32(MatM A, const double2* __restrict__ x,
+                                                 const double2* __restrict__ r,
+                                                 double2* __restrict__ res,
+                                                 const PcgScalars* __restrict__ sc) {
+  if (stopped(sc)) return;
+  const int lane = threadIdx.x & 63;
+  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < A.nslices; s += gridDim.x * kW) {
+    double y0, y1;
+    rowM_spmv(A, x, s, lane, y0, y1);
+    if (lane < A.nrow[s]) {
+      const int i = A.row0[s] + lane;
+      const double2 ri = r[i];
+      res[i] = make_double2(ri.x - y0, ri.y - y1);
+    }
+  }
+}
+
 // out = base + M in   (base == nullptr: out = M in)
 __global__ __launch_bounds__(kT) void k0_msmooth(MatM M, const double2* __restrict__ in,
                                                  const double2* base, double2* out,
@@ -1504,8 +1523,19 @@ void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* 
     hipLaunchKernelGGL(k0_cheb, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)L.x.get(), z2,
                        (double2*)L.d.get(), r2, (const double2*)c.inv_diag.get(), ch.c2d, ch.c2r, sc);
   }
-  hipLaunchKernelGGL(k0_resid, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)z2, r2,
-                     (double2*)L.res.get(), sc);
+  // the learned smoother's cycle computes its level-0 residuals with the FP32 values too
+  const bool v32 = learned && mg.coarse_fp32;
+  const MatM A32{c.sell_off.get(), c.sell_col.get(), c.sell_val32.get(), c.sell_row0.get(),
+                 c.sell_nrow.get(), c.nslices};
+  auto resid0 = [&]() {
+    if (v32)
+      hipLaunchKernelGGL(k0_resid32, dim3(gs), dim3(kT), 0, c.stream, A32, (const double2*)z2, r2,
+                         (double2*)L.res.get(), sc);
+    else
+      hipLaunchKernelGGL(k0_resid, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)z2, r2,
+                         (double2*)L.res.get(), sc);
+  };
+  resid0();
   launch_restrict(c, L, L.res.get(), coarse_rhs(mg, 0), sc);
   coarse_correction(c, 0, sc);
   hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n, 2,
@@ -1513,8 +1543,7 @@ void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* 
                      sc);
   // post-smoothing
   if (learned) {
-    hipLaunchKernelGGL(k0_resid, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)z2, r2,
-                       (double2*)L.res.get(), sc);
+    resid0();
     hipLaunchKernelGGL(k0_msmooth, dim3(gs), dim3(kT), 0, c.stream, M, (const double2*)L.res.get(),
                        (const double2*)z2, z2, sc);
   } else {
@@ -1717,6 +1746,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
   }
   HIP_CHECK(hipStreamSynchronize(c.stream));
   mg.tail_from = tail_start(mg);
+  if (mg.coarse_fp32) ensure_vals32(c);
   if (fine_smoother == 0 && c.mpnn.built) {
     // A good learned smoother has lambda_max(M A) close to 1 (1.14-1.16 on the plates it was
     // trained for); one far from its training data overshoots on some modes.  Such an M is
