@@ -27,8 +27,15 @@
  *                        L0/L1 iteration counts 148 / 295 and bitwise vs oracle/_ref
  *   oc_element_stresses  linear-field stress recovery (closed form)
  *   oc_kirsch_*          golden Kirsch metrics of docs/verification_results.json
+ *   oc_pcg_jacobi_mt     the same recurrence on several threads (OpenMP), for the timed CPU
+ *                        baseline only: its dot products sum per-thread partials in thread
+ *                        order, so its rounding differs from oc_pcg_jacobi; the tests check
+ *                        that it converges to the same solution (1e-8)
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -486,6 +493,129 @@ int oc_pcg_jacobi(int n, const int* row_ptr, const int* cols, const double* vals
   st->true_rel_residual = sqrt(dot(n, r, r)) / bnorm;
   free(inv_diag); free(r); free(z); free(p); free(Ap);
   return st->converged ? OC_OK : OC_NOT_CONVERGED;
+}
+
+/* Jacobi-PCG of solver.cpp:35-131 on `threads` OpenMP threads (timed baseline; see the
+ * header).  Rows are split statically; each dot product sums per-thread partials in thread
+ * order.  Returns as oc_pcg_jacobi (history is not recorded). */
+static double dot_mt(int n, const double* a, const double* b, double* part, int threads) {
+#pragma omp parallel num_threads(threads)
+  {
+    double s = 0.0;
+#pragma omp for schedule(static)
+    for (int i = 0; i < n; ++i) s += a[i] * b[i];
+#ifdef _OPENMP
+    part[omp_get_thread_num()] = s;
+#else
+    part[0] = s;
+#endif
+  }
+  double t = 0.0;
+  for (int k = 0; k < threads; ++k) t += part[k];
+  return t;
+}
+
+int oc_pcg_jacobi_mt(int n, const int* row_ptr, const int* cols, const double* vals,
+                     const double* b, double* x, double rtol, int max_iter, int threads,
+                     oc_stats* st) {
+  memset(st, 0, sizeof *st);
+  if (threads < 1) threads = 1;
+#ifndef _OPENMP
+  threads = 1;
+#endif
+  double* part = (double*)calloc((size_t)threads, sizeof(double));
+  double* inv_diag = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+  double* r = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+  double* p = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+  double* Ap = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+  if (!part || !inv_diag || !r || !p || !Ap) {
+    free(part); free(inv_diag); free(r); free(p); free(Ap);
+    return OC_ERR_ALLOC;
+  }
+  int bad = -1;
+  for (int i = 0; i < n; ++i) {
+    const int kd = csr_find(row_ptr, cols, i, i);
+    const double d = kd >= 0 ? vals[kd] : 0.0;
+    if (!(d > 0.0)) { bad = i; break; }
+    inv_diag[i] = 1.0 / d;
+  }
+  int rc = OC_OK;
+  if (bad >= 0) {
+    st->iterations = bad;
+    rc = OC_ERR_BAD_DIAGONAL;
+    goto done;
+  }
+#pragma omp parallel for num_threads(threads) schedule(static)
+  for (int i = 0; i < n; ++i) {
+    x[i] = 0.0;
+    r[i] = b[i];
+    p[i] = inv_diag[i] * b[i];
+  }
+  {
+    const double bnorm = sqrt(dot_mt(n, b, b, part, threads));
+    if (bnorm == 0.0) {
+      st->converged = 1;
+      st->zero_rhs = 1;
+      goto done;
+    }
+    double rz = dot_mt(n, r, p, part, threads);
+    st->rel_residual = 1.0;
+    for (int k = 1; k <= max_iter; ++k) {
+#pragma omp parallel for num_threads(threads) schedule(static)
+      for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) s += vals[q] * p[cols[q]];
+        Ap[i] = s;
+      }
+      const double pAp = dot_mt(n, p, Ap, part, threads);
+      if (!(pAp > 0.0)) {
+        st->breakdown = 1;
+        st->breakdown_pap = pAp;
+        break;
+      }
+      const double alpha = rz / pAp;
+#pragma omp parallel for num_threads(threads) schedule(static)
+      for (int i = 0; i < n; ++i) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * Ap[i];
+      }
+      st->iterations = k;
+      st->rel_residual = sqrt(dot_mt(n, r, r, part, threads)) / bnorm;
+      if (st->rel_residual < rtol) {
+        st->converged = 1;
+        break;
+      }
+      /* z = D^-1 r is folded into the r'z product and the direction update */
+      double rz_new = 0.0;
+#pragma omp parallel num_threads(threads)
+      {
+        double s = 0.0;
+#pragma omp for schedule(static)
+        for (int i = 0; i < n; ++i) s += r[i] * (inv_diag[i] * r[i]);
+#ifdef _OPENMP
+        part[omp_get_thread_num()] = s;
+#else
+        part[0] = s;
+#endif
+      }
+      for (int t = 0; t < threads; ++t) rz_new += part[t];
+      const double beta = rz_new / rz;
+      rz = rz_new;
+#pragma omp parallel for num_threads(threads) schedule(static)
+      for (int i = 0; i < n; ++i) p[i] = inv_diag[i] * r[i] + beta * p[i];
+    }
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) s += vals[q] * x[cols[q]];
+      r[i] = b[i] - s;
+    }
+    st->true_rel_residual = sqrt(dot_mt(n, r, r, part, threads)) / bnorm;
+    rc = st->converged ? OC_OK : OC_NOT_CONVERGED;
+  }
+done:
+  free(part); free(inv_diag); free(r); free(p); free(Ap);
+  return rc;
 }
 
 /* postprocess.cpp:35-60: constant stress of each element (sxx, syy, txy), centroid, area. */

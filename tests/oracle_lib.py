@@ -60,6 +60,10 @@ class Oracle:
         self._stresses.argtypes = [ctypes.c_int64, _I, _D, _D, _D, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_int, _D, _D, _D]
         if prefix == "oc":
+            self._pcg_mt = L.oc_pcg_jacobi_mt
+            self._pcg_mt.restype = ctypes.c_int
+            self._pcg_mt.argtypes = [ctypes.c_int, _I, _I, _D, _D, _D, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_int, ctypes.POINTER(Stats)]
             self._body = L.oc_add_body_force
             self._body.argtypes = [ctypes.c_int64, _I, _D, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, _D]
@@ -166,6 +170,19 @@ class Oracle:
         if history:
             out["history"] = h[: (1 if st.zero_rhs else st.iterations + 1)].copy()
         return x, out
+
+    def pcg_jacobi_mt(self, row_ptr, cols, vals, b, rtol=1e-10, max_iter=20000, threads=1):
+        """The same recurrence on `threads` OpenMP threads (timed CPU baseline)."""
+        x = np.zeros(b.size)
+        st = Stats()
+        rc = self._pcg_mt(b.size, _p(row_ptr, ctypes.c_int), _p(cols, ctypes.c_int), _p(vals),
+                          _p(np.ascontiguousarray(b, float)), _p(x), rtol, max_iter,
+                          int(threads), ctypes.byref(st))
+        if rc < 0:
+            raise RuntimeError(f"pcg_jacobi_mt failed ({rc})")
+        return x, dict(iterations=st.iterations, converged=bool(st.converged),
+                       breakdown=bool(st.breakdown), zero_rhs=bool(st.zero_rhs),
+                       rel_residual=st.rel_residual, true_rel_residual=st.true_rel_residual)
 
     # postprocess.cpp
     def element_stresses(self, xy, tri, u, E, nu, strain=0, E_elem=None):

@@ -281,6 +281,23 @@ def test_pcg_edge_cases(oracle):
     assert not st["converged"] and st["iterations"] == 2
 
 
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_pcg_baseline(oracle, threads):
+    """The threaded PCG (timed CPU baseline) runs the same recurrence; only its dot-product
+    summation order differs, so it converges to the same solution in about as many
+    iterations; with one thread it is the sequential oracle bit for bit."""
+    A = random_spd(150, np.random.default_rng(11))
+    b = np.random.default_rng(12).uniform(-1, 1, 150)
+    x1, s1 = oracle.pcg_jacobi(*dense_to_csr(A), b, 1e-12, 2000)
+    x2, s2 = oracle.pcg_jacobi_mt(*dense_to_csr(A), b, 1e-12, 2000, threads=threads)
+    assert s2["converged"] and abs(s2["iterations"] - s1["iterations"]) <= 2
+    assert np.max(np.abs(x2 - x1)) <= 1e-8 * np.max(np.abs(x1))
+    if threads == 1:
+        assert np.array_equal(x1, x2) and s1["iterations"] == s2["iterations"]
+    _, s0 = oracle.pcg_jacobi_mt(*dense_to_csr(A), np.zeros(150), threads=threads)
+    assert s0["converged"] and s0["zero_rhs"] and s0["iterations"] == 0
+
+
 def test_pcg_matches_reference_bitwise(oracle, ref):
     A = random_spd(60, np.random.default_rng(9))
     b = np.random.default_rng(10).uniform(-1, 1, 60)
