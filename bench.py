@@ -18,8 +18,10 @@ timed steps, on a small plate of the same family (its time is reported as train_
 timed step rebuilds M and the hierarchy for the freshly assembled matrix (reported as
 precond_s, part of the step).
 
-Prints one JSON line.  The roofline object covers the dominant kernel of the best method
-(the fused FP64 SpMV of every iteration), timed with HIP events on the library stream;
+Prints one JSON line.  The roofline object covers the FP64 Jacobi-CG's SpMV (k_cg_spmv, the
+dominant kernel of the baseline method, whose bandwidth the target is stated for), timed
+with HIP events on the library stream; roofline_fcg gives the same for the flexible CG's SpMV
+(q = A p plus p'r), which every iteration of the learned methods runs;
 cpu_baseline is the oracle (the CPU restatement of the reference's element loop and PCG,
 bitwise equal to the reference on uniform-E inputs) timed on a bounded sample, its PCG on
 all host threads (OMP_NUM_THREADS, else every core up to 16); the one-core figures are kept
@@ -337,6 +339,13 @@ def main():
                      "traffic_source": traffic_src,
                      "kernel": "k_cg_spmv", "bytes_per_launch": b_spmv,
                      "avg_launch_s": t_spmv},
+        "roofline_fcg": ({"kernel": "k_fcg_spmv<F64>", "bound": "hbm",
+                          "bytes_per_launch": b_spmv + 8 * n,
+                          "avg_launch_s": fcg_kernels["k_fcg_spmv<F64>"],
+                          "achieved": (b_spmv + 8 * n) / fcg_kernels["k_fcg_spmv<F64>"] / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": (b_spmv + 8 * n) / fcg_kernels["k_fcg_spmv<F64>"] / 1e9
+                          / HBM_PEAK_GBS} if fcg_kernels else None),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
