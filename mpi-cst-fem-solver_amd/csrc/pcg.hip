@@ -1395,7 +1395,9 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
   double true_rel = zero_rhs ? 0.0 : 1.0;
   bool converged = zero_rhs, fallback = false;
   if (!zero_rhs && o.max_iter > 0) {
-    const int chunk = c.n > 200000 ? 16 : 64;  // even
+    // a multilevel iteration is ~2 ms of ~160 launches: two of them hide the host's status
+    // read, and a short chunk keeps the graph small and the launches queued past a stop few
+    const int chunk = multilevel ? 2 : (c.n > 200000 ? 16 : 64);  // even
     Chunk gr(c, chunk, iteration);
     double prev_true = 1.0;
     int stagnant = 0;
