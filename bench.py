@@ -210,7 +210,8 @@ def main():
 
     def step(method):
         times = pipeline.PhaseTimes()
-        pipeline.assemble_and_constrain(dev, prob, times)
+        pipeline.assemble_and_constrain(dev, prob, times,
+                                        multilevel=method in ("learned_mg", "mg_cheb"))
         t0 = time.perf_counter()
         dev.setup()
         if method in uses_net:

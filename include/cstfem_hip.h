@@ -254,6 +254,13 @@ int cstfem_mpnn_download(cstfem_ctx* ctx, int what, float* out, int64_t count);
  * 1.5 (or not finite) the learned smoother is replaced by Chebyshev for this hierarchy. */
 int cstfem_mg_setup(cstfem_ctx* ctx, int max_levels, int coarse_max, int fine_smoother,
                     int kcycle_levels);
+/* Starts the set-up's first step ahead of time: the greedy aggregation of the fine node graph,
+ * which depends on the pattern only.  The graph is copied to the host on a second stream and
+ * aggregated on a worker thread, while the caller goes on with assembly, loads, Dirichlet,
+ * cstfem_setup and cstfem_mpnn_build; the next cstfem_mg_setup waits for it and uses it.
+ * Call after cstfem_build_stiffness_pattern; a new pattern (or mesh, or CSR upload) drops a
+ * started aggregation.  The result is the same as without it. */
+int cstfem_mg_prepare(cstfem_ctx* ctx);
 int cstfem_mg_levels(cstfem_ctx* ctx, int* levels);
 /* DOFs, aggregates (0 on the coarsest), stored entries and the upper end of the smoothing
  * interval (lambda_max estimate of D^-1 K; 0 where no Chebyshev smoother runs) of one level.

@@ -234,6 +234,7 @@ __global__ void k_adj_fill(int nn, const int* __restrict__ tri, const int* __res
 }  // namespace
 
 void build_pattern(Context& c) {
+  mg_drop_prepared(c);  // a level-0 aggregation under way reads the old pattern
   build_node_elements(c);
   const int nn = c.nn;
   DBuf<int> deg;

@@ -25,6 +25,7 @@ void hip_check(hipError_t e, const char* what, const char* file, int line) {
 }
 
 Context::~Context() {
+  mg.join_prepared();  // its download runs on the auxiliary stream
   if (h_scal) (void)hipHostFree(h_scal);
   if (aux) (void)hipStreamDestroy(aux);
   if (stream) (void)hipStreamDestroy(stream);
@@ -163,6 +164,7 @@ int cstfem_upload_mesh(cstfem_ctx* ctx, int nn, const double* xy, int64_t ne, co
                                            " references node " + std::to_string(tri[k]) + " out of range");
     Context& c = ctx->c;
     use_device(&c);
+    mg_drop_prepared(c);
     c.nn = nn;
     c.ne = ne;
     c.h_xy.assign(xy, xy + 2 * (size_t)nn);
@@ -376,6 +378,7 @@ int cstfem_upload_csr(cstfem_ctx* ctx, int n, const int* row_ptr, const int* col
     c.nslices = nslices;
     c.sell_blocks = sb;
     c.h_sell_off = off;
+    mg_drop_prepared(c);  // a level-0 aggregation under way reads the old pattern
     c.brow_ptr.alloc(nb + 1);
     c.bcol.alloc(std::max<int64_t>(nblk, 1));
     c.bdiag.alloc(nb);
@@ -828,6 +831,15 @@ int cstfem_mg_setup(cstfem_ctx* ctx, int max_levels, int coarse_max, int fine_sm
     need(ctx != nullptr, "cstfem_mg_setup: NULL context");
     use_device(&ctx->c);
     mg_setup(ctx->c, max_levels, coarse_max, fine_smoother, kcycle_levels);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mg_prepare(cstfem_ctx* ctx) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_mg_prepare: NULL context");
+    use_device(&ctx->c);
+    mg_prepare(ctx->c);
     return CSTFEM_OK;
   });
 }

@@ -22,8 +22,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "cstfem_hip.h"
@@ -160,6 +162,17 @@ struct MgLevel {
   // work vectors
   DBuf<double> rhs, x, x2, d, res, kx, kres, kp, kq, kz;
   DBuf<double> part;  // reduction partials of the K-cycle steps
+  // before reuse by a new hierarchy: sizes and bounds cleared, and the buffer that only some
+  // hierarchies fill (val32, read whenever present) released; the others are reallocated by
+  // the set-up, which keeps a buffer of the same size as it is
+  void reset() {
+    n = nodes = 0;
+    bs = 2;
+    nslices = 0;
+    lam = 0.0;
+    nagg = 0;
+    val32.release();
+  }
 };
 
 struct Multigrid {
@@ -168,6 +181,9 @@ struct Multigrid {
   int kdepth = 3;         // levels 1..kdepth use the K-cycle, deeper ones the V-cycle
   int kdepth_set = 3;     // the set-up's value while a Chebyshev fallback widens kdepth
   std::vector<MgLevel*> lv;
+  // level objects of the dropped hierarchy, in level order: the next set-up takes them back,
+  // so that a hierarchy of the same sizes (the same mesh with new values) allocates nothing
+  std::vector<MgLevel*> spare;
   int nc = 0;             // DOFs of the coarsest level (dense inverse)
   int tail_from = -1;     // first level of the fused single-workgroup tail cycle (-1: none)
   double m_bound = 0.0;   // lambda_max(M A) of the learned level-0 smoother
@@ -175,12 +191,46 @@ struct Multigrid {
   DBuf<double> cinv;
   int* pinned = nullptr;  // page-locked staging of the fine node graph, kept across set-ups
   size_t pinned_n = 0;
+  // level-0 aggregation started ahead of the set-up (mg_prepare): a worker thread waits for
+  // the node graph's download (auxiliary stream, event pre_done), runs the greedy pass into
+  // pre_agg and stages the member lists; the set-up joins it.  Uses `pinned` until joined.
+  std::thread pre_worker;
+  bool pre_started = false;
+  std::vector<int> pre_agg;
+  int pre_nagg = 0;
+  bool pre_staged = false;  // the worker also left the member lists in `pinned`
+  std::exception_ptr pre_error;
+  hipEvent_t pre_queued = nullptr, pre_done = nullptr;
+  void join_prepared() {
+    if (pre_worker.joinable()) pre_worker.join();
+    pre_started = false;
+  }
+  MgLevel* take_level() {
+    MgLevel* L;
+    if (!spare.empty()) {
+      L = spare.front();
+      spare.erase(spare.begin());
+    } else {
+      L = new MgLevel;
+    }
+    L->reset();
+    lv.push_back(L);
+    return L;
+  }
+  void drop_spare() {
+    for (MgLevel* l : spare) delete l;
+    spare.clear();
+  }
   ~Multigrid() {
+    join_prepared();
     clear();
+    drop_spare();
     if (pinned) (void)hipHostFree(pinned);
+    if (pre_queued) (void)hipEventDestroy(pre_queued);
+    if (pre_done) (void)hipEventDestroy(pre_done);
   }
   void clear() {
-    for (MgLevel* l : lv) delete l;
+    spare.insert(spare.begin(), lv.begin(), lv.end());
     lv.clear();
     tail_from = -1;
     m_bound = 0.0;
@@ -290,6 +340,11 @@ void ensure_vectors(Context& c);
 void ensure_vals32(Context& c);
 // multilevel preconditioner (mg.hip)
 void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int kdepth);
+// starts the level-0 aggregation of the current mesh pattern (node-graph download on the
+// auxiliary stream, greedy pass on a host thread); the next mg_setup takes its result
+void mg_prepare(Context& c);
+// waits for a started level-0 aggregation and drops it (before the pattern changes)
+void mg_drop_prepared(Context& c);
 // z = B r on the library stream (level-0 vectors of 2*nb doubles); kernels return at once
 // when sc is given and its status is no longer running
 void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* sc);

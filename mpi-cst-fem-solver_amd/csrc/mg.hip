@@ -1279,11 +1279,15 @@ void coarsest_inverse(Context& c, MgLevel& L) {
 // Phase times of the setup on stderr when CSTFEM_MG_TRACE is set (each mark synchronises the
 // stream, so the trace itself slows the setup down a little).
 struct SetupTrace {
-  bool on = std::getenv("CSTFEM_MG_TRACE") != nullptr;
+  // CSTFEM_MG_TRACE=1: each mark waits for the stream (device time per phase); =2: host
+  // timestamps only (where the host thread spends the set-up)
+  const char* env = std::getenv("CSTFEM_MG_TRACE");
+  bool on = env != nullptr;
+  bool sync = env != nullptr && env[0] != '2';
   std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
   void mark(Context& c, int level, const char* what) {
     if (!on) return;
-    HIP_CHECK(hipStreamSynchronize(c.stream));
+    if (sync) HIP_CHECK(hipStreamSynchronize(c.stream));
     const auto now = std::chrono::steady_clock::now();
     std::fprintf(stderr, "[mg setup] level %d %-20s %8.3f ms\n", level, what,
                  std::chrono::duration<double, std::milli>(now - last).count());
@@ -1291,10 +1295,10 @@ struct SetupTrace {
   }
 };
 
-void alloc_work(MgLevel& L) {
+void alloc_work(Context& c, MgLevel& L) {
   for (DBuf<double>* v : {&L.rhs, &L.x, &L.x2, &L.d, &L.res, &L.kx, &L.kres, &L.kp, &L.kq, &L.kz}) {
     v->alloc(std::max(1, L.n));
-    HIP_CHECK(hipMemset(v->get(), 0, sizeof(double) * std::max(1, L.n)));
+    HIP_CHECK(hipMemsetAsync(v->get(), 0, sizeof(double) * std::max(1, L.n), c.stream));
   }
   L.part.alloc(6 * kPart);
 }
@@ -1556,6 +1560,74 @@ void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* 
   HIP_CHECK(hipGetLastError());
 }
 
+namespace {
+
+// member lists of the aggregates into page-locked staging: mptr[na + 1] (aggregate offsets),
+// mem[nodes] (member nodes, ascending within an aggregate), agg[nodes] (a copy of the map)
+void stage_members(const std::vector<int>& h_agg, int nodes, int na, int bs, int* stage) {
+  int* mptr = stage;
+  int* mem = stage + na + 1;
+  int* agg = mem + nodes;
+  std::fill(mptr, mptr + na + 1, 0);
+  for (int i = 0; i < nodes; ++i) ++mptr[h_agg[i] + 1];
+  for (int a = 0; a < na; ++a) mptr[a + 1] += mptr[a];
+  for (int a = 0; a < na; ++a)
+    if (bs * (mptr[a + 1] - mptr[a]) > 192) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: aggregate too large");
+  {
+    std::vector<int> fill(mptr, mptr + na);
+    for (int i = 0; i < nodes; ++i) mem[fill[h_agg[i]]++] = i;
+  }
+  std::copy(h_agg.begin(), h_agg.end(), agg);
+}
+
+}  // namespace
+
+void mg_drop_prepared(Context& c) {
+  c.mg.join_prepared();
+  c.mg.pre_error = nullptr;
+  c.mg.pre_agg.clear();
+}
+
+void mg_prepare(Context& c) {
+  if (!c.have_pattern || !c.mesh_pattern || c.n != 2 * c.nb)
+    throw Error(CSTFEM_ERR_ORDER, "multilevel: prepare needs a mesh pattern (build_stiffness_pattern first)");
+  Multigrid& mg = c.mg;
+  mg_drop_prepared(c);
+  if (!c.aux) HIP_CHECK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
+  if (!mg.pre_queued) HIP_CHECK(hipEventCreateWithFlags(&mg.pre_queued, hipEventDisableTiming));
+  if (!mg.pre_done) HIP_CHECK(hipEventCreateWithFlags(&mg.pre_done, hipEventDisableTiming));
+  const size_t need = (size_t)c.nb + 1 + (size_t)c.nblk;
+  // growing the staging frees the old one, which uploads of an earlier set-up may still read
+  if (need > mg.pinned_n) HIP_CHECK(hipStreamSynchronize(c.stream));
+  int* pin = pinned_ints(mg, need);
+  // the download follows the work queued so far on the main stream (the pattern's kernels,
+  // the uploads of an earlier set-up from the same staging)
+  HIP_CHECK(hipEventRecord(mg.pre_queued, c.stream));
+  HIP_CHECK(hipStreamWaitEvent(c.aux, mg.pre_queued, 0));
+  HIP_CHECK(hipMemcpyAsync(pin, c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost, c.aux));
+  HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1, c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost, c.aux));
+  HIP_CHECK(hipEventRecord(mg.pre_done, c.aux));
+  const int nb = c.nb;
+  const hipEvent_t done = mg.pre_done;
+  mg.pre_error = nullptr;
+  mg.pre_staged = false;
+  const size_t room = mg.pinned_n;
+  mg.pre_worker = std::thread([&mg, pin, nb, done, room]() {
+    try {
+      HIP_CHECK(hipEventSynchronize(done));
+      mg.pre_nagg = aggregate(pin, pin + nb + 1, nb, mg.pre_agg);
+      // the member lists too, over the node graph (no longer read) when they fit there
+      if (2 * (size_t)nb + mg.pre_nagg + 1 <= room) {
+        stage_members(mg.pre_agg, nb, mg.pre_nagg, 2, pin);
+        mg.pre_staged = true;
+      }
+    } catch (...) {
+      mg.pre_error = std::current_exception();
+    }
+  });
+  mg.pre_started = true;
+}
+
 void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int kdepth) {
   if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "multilevel: call cstfem_setup first");
   if (!c.mesh_pattern || c.n != 2 * c.nb)
@@ -1565,7 +1637,9 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
   if (fine_smoother != 0 && fine_smoother != 1) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: fine_smoother must be 0 or 1");
   if (kdepth < 0) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: kcycle_levels must be >= 0");
   Multigrid& mg = c.mg;
+  SetupTrace tr;
   mg.clear();
+  tr.mark(c, 0, "clear");
   mg.fine_smoother = fine_smoother;
   mg.kdepth = kdepth;
   mg.kdepth_set = kdepth;
@@ -1576,23 +1650,37 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     const char* env = std::getenv("CSTFEM_MG_COARSE32");
     mg.coarse_fp32 = fine_smoother == 0 && !(env && env[0] == '0');
   }
-  SetupTrace tr;
   // level 0: the context's matrix; its node graph is read on the host (aggregation) from a
   // page-locked copy.  The pattern is complete since build_pattern returned, so the copy runs
   // on the auxiliary stream and the host aggregation overlaps whatever the main stream is
   // still doing (building M, typically); the device work below follows it in stream order.
-  MgLevel* L0 = new MgLevel;
-  mg.lv.push_back(L0);
+  MgLevel* L0 = mg.take_level();
   L0->n = c.n;
   L0->nodes = c.nb;
   L0->bs = 2;
-  if (!c.aux) HIP_CHECK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
-  int* pin = pinned_ints(mg, (size_t)c.nb + 1 + (size_t)c.nblk);
-  HIP_CHECK(hipMemcpyAsync(pin, c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost, c.aux));
-  HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1, c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost, c.aux));
-  HIP_CHECK(hipStreamSynchronize(c.aux));
-  tr.mark(c, 0, "download pattern");
-  alloc_work(*L0);
+  // a level-0 aggregation started by mg_prepare belongs to the current pattern (every
+  // pattern change drops it): take its result
+  const bool prepared = mg.pre_started;
+  int* pin = nullptr;
+  if (prepared) {
+    mg.join_prepared();
+    if (mg.pre_error) {
+      const std::exception_ptr e = mg.pre_error;
+      mg.pre_error = nullptr;
+      std::rethrow_exception(e);
+    }
+    L0->h_agg.swap(mg.pre_agg);
+    L0->nagg = mg.pre_nagg;
+    tr.mark(c, 0, "join prepared aggregation");
+  } else {
+    if (!c.aux) HIP_CHECK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
+    pin = pinned_ints(mg, (size_t)c.nb + 1 + (size_t)c.nblk);
+    HIP_CHECK(hipMemcpyAsync(pin, c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost, c.aux));
+    HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1, c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost, c.aux));
+    HIP_CHECK(hipStreamSynchronize(c.aux));
+    tr.mark(c, 0, "download pattern");
+  }
+  alloc_work(c, *L0);
   if (fine_smoother == 1) L0->lam = smoothing_max(c, 0);
   tr.mark(c, 0, "work, smoothing");
   // rigid-body rotation about the centre of the mesh's bounding box
@@ -1611,30 +1699,28 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     MgLevel& L = *mg.lv[l];
     const bool last = l + 1 >= max_levels - 1;
     // aggregation (host, greedy in node order) and members (ascending node order)
-    const int* hp = l == 0 ? pin : L.bptr.data();
-    const int* hc = l == 0 ? pin + c.nb + 1 : L.bcol.data();
-    L.nagg = aggregate(hp, hc, L.nodes, L.h_agg);
+    if (!(l == 0 && prepared)) {
+      const int* hp = l == 0 ? pin : L.bptr.data();
+      const int* hc = l == 0 ? pin + c.nb + 1 : L.bcol.data();
+      L.nagg = aggregate(hp, hc, L.nodes, L.h_agg);
+    }
     const int na = L.nagg;
     // members and the aggregate map go up from page-locked memory; the staging area follows
     // the node graph, which the host no longer reads once the aggregation is done
     const size_t need = 2 * (size_t)L.nodes + na + 1;
-    // the uploads of the level above read the staging area: let them finish (on level 0 only
-    // the node-graph download used it, and that was waited for)
-    if (l > 0 || need > mg.pinned_n) HIP_CHECK(hipStreamSynchronize(c.stream));
-    int* stage = pinned_ints(mg, std::max(mg.pinned_n, need));
+    int* stage;
+    if (l == 0 && prepared && mg.pre_staged) {
+      stage = mg.pinned;  // the worker staged them
+    } else {
+      // the uploads of the level above read the staging area: let them finish (on level 0
+      // only the node-graph download used it, and that was waited for)
+      if (l > 0 || need > mg.pinned_n) HIP_CHECK(hipStreamSynchronize(c.stream));
+      stage = pinned_ints(mg, std::max(mg.pinned_n, need));
+      stage_members(L.h_agg, L.nodes, na, L.bs, stage);
+    }
     int* mptr = stage;
     int* mem = stage + na + 1;
     int* agg = mem + L.nodes;
-    std::fill(mptr, mptr + na + 1, 0);
-    for (int i = 0; i < L.nodes; ++i) ++mptr[L.h_agg[i] + 1];
-    for (int a = 0; a < na; ++a) mptr[a + 1] += mptr[a];
-    for (int a = 0; a < na; ++a)
-      if (L.bs * (mptr[a + 1] - mptr[a]) > 192) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: aggregate too large");
-    {
-      std::vector<int> fill(mptr, mptr + na);
-      for (int i = 0; i < L.nodes; ++i) mem[fill[L.h_agg[i]]++] = i;
-    }
-    std::copy(L.h_agg.begin(), L.h_agg.end(), agg);
     L.agg.alloc(L.nodes);
     L.mptr.alloc(na + 1);
     L.mem.alloc(L.nodes);
@@ -1642,8 +1728,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     HIP_CHECK(hipMemcpyAsync(L.mptr.get(), mptr, sizeof(int) * (na + 1), hipMemcpyHostToDevice, c.stream));
     HIP_CHECK(hipMemcpyAsync(L.mem.get(), mem, sizeof(int) * L.nodes, hipMemcpyHostToDevice, c.stream));
     tr.mark(c, l, "aggregate, members");
-    MgLevel* C = new MgLevel;
-    mg.lv.push_back(C);
+    MgLevel* C = mg.take_level();
     C->nodes = na;
     C->n = 3 * na;
     C->bs = 3;
@@ -1731,7 +1816,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
       HIP_CHECK(hipGetLastError());
     }
     HIP_CHECK(hipStreamSynchronize(c.stream));  // C->bcol on the host, C->val for the inverse
-    alloc_work(*C);
+    alloc_work(c, *C);
     tr.mark(c, l, "galerkin, dinv, work");
     if (last || C->n <= coarse_max || na == L.nodes) {
       if (C->n > 4096)
@@ -1758,6 +1843,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     }
     tr.mark(c, 0, "smoother bound");
   }
+  mg.drop_spare();  // levels of a deeper earlier hierarchy
   mg.ready = true;
 }
 

@@ -118,40 +118,49 @@ __global__ __launch_bounds__(kT) void k_dense(int N, const uint16_t* __restrict_
                                               const uint16_t* __restrict__ B,
                                               const float* __restrict__ bias, void* __restrict__ Yv) {
   const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
-  const int row0 = wave * 16;
-  if (row0 >= N) return;
-  const int ar = row0 + (lane & 15);        // A fragment row
   const int kq = 4 * (lane >> 4);          // A/B fragment k offset within a 16-step
-  v4f acc[M / 16];
-#pragma unroll
-  for (int t = 0; t < M / 16; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    v4s a = {0, 0, 0, 0};
-    if (ar < N) a = *reinterpret_cast<const v4s*>(X + (size_t)ar * K + k0 + kq);
-#pragma unroll
-    for (int t = 0; t < M / 16; ++t) {
-      const int bn = t * 16 + (lane & 15);
-      const v4s b = *reinterpret_cast<const v4s*>(B + (size_t)bn * K + k0 + kq);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-    }
-  }
-  // lane holds D[m = 4*(lane/16) + j][n = lane % 16] of each 16x16 tile
+  // every B fragment of the layer stays in registers while the wavefront walks its row tiles
+  v4s bf[M / 16][K / 16];
+  float bb[M / 16];
 #pragma unroll
   for (int t = 0; t < M / 16; ++t) {
-    const int col = t * 16 + (lane & 15);
-    const float bb = bias ? bias[col] : 0.f;
+    const int bn = t * 16 + (lane & 15);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int row = row0 + 4 * (lane >> 4) + j;
-      if (row >= N) continue;
-      float v = acc[t][j] + bb;
-      if (kRelu) v = v > 0.f ? v : 0.f;
-      if (kOut == 0)
-        reinterpret_cast<uint16_t*>(Yv)[(size_t)row * M + col] = f2bf(v);
-      else
-        reinterpret_cast<float*>(Yv)[(size_t)row * M + col] = v;
+    for (int k = 0; k < K / 16; ++k)
+      bf[t][k] = *reinterpret_cast<const v4s*>(B + (size_t)bn * K + 16 * k + kq);
+    bb[t] = bias ? bias[bn] : 0.f;
+  }
+  const int tiles = (N + 15) / 16;
+  for (int tile = blockIdx.x * (kT / 64) + (threadIdx.x >> 6); tile < tiles;
+       tile += gridDim.x * (kT / 64)) {
+    const int row0 = tile * 16;
+    const int ar = row0 + (lane & 15);      // A fragment row
+    v4f acc[M / 16];
+#pragma unroll
+    for (int t = 0; t < M / 16; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < K / 16; ++k) {
+      v4s a = {0, 0, 0, 0};
+      if (ar < N) a = *reinterpret_cast<const v4s*>(X + (size_t)ar * K + 16 * k + kq);
+#pragma unroll
+      for (int t = 0; t < M / 16; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bf[t][k], acc[t], 0, 0, 0);
+    }
+    // lane holds D[m = 4*(lane/16) + j][n = lane % 16] of each 16x16 tile
+#pragma unroll
+    for (int t = 0; t < M / 16; ++t) {
+      const int col = t * 16 + (lane & 15);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = row0 + 4 * (lane >> 4) + j;
+        if (row >= N) continue;
+        float v = acc[t][j] + bb[t];
+        if (kRelu) v = v > 0.f ? v : 0.f;
+        if (kOut == 0)
+          reinterpret_cast<uint16_t*>(Yv)[(size_t)row * M + col] = f2bf(v);
+        else
+          reinterpret_cast<float*>(Yv)[(size_t)row * M + col] = v;
+      }
     }
   }
 }
@@ -212,21 +221,50 @@ __global__ void k_colsum(int N, int M, int rows, const uint16_t* __restrict__ Gd
 }
 
 // X[i] = [H[i], mean_{j ~ i, j != i} H[j]]  (BF16 in, FP32 mean, BF16 out)
-__global__ void k_aggregate(int nb, const int* __restrict__ brow_ptr, const int* __restrict__ bcol,
-                            const uint16_t* __restrict__ Hh, uint16_t* __restrict__ X) {
-  const int i = blockIdx.x * (blockDim.x / kH) + threadIdx.x / kH;
-  const int c = threadIdx.x % kH;
+// X[i] = [H[i], mean over neighbours j != i of H[j]] (rows of 2 kH BF16): 8 lanes per node,
+// 8 channels (one 16-byte load) per lane; the neighbour indices of a row are read 4 at a time
+// ahead of their loads, and every channel sums its neighbours in pattern order
+__global__ __launch_bounds__(kT) void k_aggregate(int nb, const int* __restrict__ brow_ptr,
+                                                  const int* __restrict__ bcol,
+                                                  const uint16_t* __restrict__ Hh,
+                                                  uint16_t* __restrict__ X) {
+  const int t = blockIdx.x * kT + threadIdx.x;
+  const int i = t >> 3, part = t & 7;
   if (i >= nb) return;
-  float s = 0.f;
+  const uint4* H4 = reinterpret_cast<const uint4*>(Hh);  // row j: H4[8 j + part]
+  float s[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s[k] = 0.f;
   int cnt = 0;
-  for (int q = brow_ptr[i]; q < brow_ptr[i + 1]; ++q) {
-    const int j = bcol[q];
-    if (j == i) continue;
-    s += bf2f(Hh[(size_t)j * kH + c]);
-    ++cnt;
+  const int q1 = brow_ptr[i + 1];
+  for (int q = brow_ptr[i]; q < q1; q += 4) {
+    int jj[4];
+    uint4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) jj[u] = q + u < q1 ? bcol[q + u] : i;  // i: not summed
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = H4[8 * (size_t)jj[u] + part];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (jj[u] == i) continue;
+      const uint32_t w[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s[2 * k] += __builtin_bit_cast(float, w[k] << 16);
+        s[2 * k + 1] += __builtin_bit_cast(float, w[k] & 0xffff0000u);
+      }
+      ++cnt;
+    }
   }
-  X[(size_t)i * 2 * kH + c] = Hh[(size_t)i * kH + c];
-  X[(size_t)i * 2 * kH + kH + c] = f2bf(cnt ? s / cnt : 0.f);
+  uint4* X4 = reinterpret_cast<uint4*>(X);  // row i: X4[16 i + 0..7] = H[i], [8..15] = mean
+  X4[16 * (size_t)i + part] = H4[8 * (size_t)i + part];
+  uint32_t m[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float lo = cnt ? s[2 * k] / cnt : 0.f, hi = cnt ? s[2 * k + 1] / cnt : 0.f;
+    m[k] = (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+  }
+  X4[16 * (size_t)i + 8 + part] = make_uint4(m[0], m[1], m[2], m[3]);
 }
 
 // backward of k_aggregate: dH[j] += dX[j][:kH] + sum_{i ~ j, i != j} dX[i][kH:] / deg_i
@@ -254,7 +292,31 @@ __global__ void k_relu_bwd(int64_t n, const float* __restrict__ dH, const uint16
   dZ[i] = bf2f(Hh[i]) > 0.f ? f2bf(dH[i]) : (uint16_t)0;
 }
 
-// decoder: per stored block q = (i, j) of row i: [a, b, d, c] and the SELL values of M
+// decoder output of one block: o = b + W^T (h_i * h_j), channels in ascending order; h_j comes
+// in 16-byte loads of 8 channels
+__device__ inline void decode_block(const float* __restrict__ hi, const uint4* __restrict__ hj,
+                                    const float* __restrict__ W, const float* __restrict__ bv,
+                                    float o[kDec]) {
+#pragma unroll
+  for (int u = 0; u < kDec; ++u) o[u] = bv[u];
+#pragma unroll
+  for (int c8 = 0; c8 < kH / 8; ++c8) {
+    const uint4 v = hj[c8];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int c = 8 * c8 + k;
+      const float h = __builtin_bit_cast(float, (k & 1) ? (w[k / 2] & 0xffff0000u) : (w[k / 2] << 16));
+      const float p = hi[c] * h;
+#pragma unroll
+      for (int u = 0; u < kDec; ++u) o[u] += W[c * kDec + u] * p;
+    }
+  }
+}
+
+// decoder: per stored block q = (i, j) of row i: [a, b, d, c] and the SELL values of M.
+// One thread per row; the off-diagonal blocks share the weights Wo (uniform loads), the
+// diagonal block uses Wd
 __global__ void k_decode(int nb, const int* __restrict__ brow_ptr, const int* __restrict__ bcol,
                          const int64_t* __restrict__ spos, const uint16_t* __restrict__ Hh,
                          const float* __restrict__ theta, const double* __restrict__ invd,
@@ -262,23 +324,21 @@ __global__ void k_decode(int nb, const int* __restrict__ brow_ptr, const int* __
                          float* __restrict__ gsell) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
+  const uint4* H4 = reinterpret_cast<const uint4*>(Hh);
   float hi[kH];
 #pragma unroll
-  for (int c = 0; c < kH; ++c) hi[c] = bf2f(Hh[(size_t)i * kH + c]);
+  for (int c8 = 0; c8 < kH / 8; ++c8) {
+    const uint4 v = H4[8 * (size_t)i + c8];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      hi[8 * c8 + 2 * k] = __builtin_bit_cast(float, w[k] << 16);
+      hi[8 * c8 + 2 * k + 1] = __builtin_bit_cast(float, w[k] & 0xffff0000u);
+    }
+  }
   const double di0 = invd[2 * i], di1 = invd[2 * i + 1];
   const float si0 = (float)sqrt(di0), si1 = (float)sqrt(di1);
-  for (int q = brow_ptr[i]; q < brow_ptr[i + 1]; ++q) {
-    const int j = bcol[q];
-    const float* W = theta + (j == i ? kWd : kWo);
-    const float* bv = theta + (j == i ? kBd : kBo);
-    float o[kDec];
-#pragma unroll
-    for (int u = 0; u < kDec; ++u) o[u] = bv[u];
-    for (int c = 0; c < kH; ++c) {
-      const float p = hi[c] * bf2f(Hh[(size_t)j * kH + c]);
-#pragma unroll
-      for (int u = 0; u < kDec; ++u) o[u] += W[c * kDec + u] * p;
-    }
+  auto store = [&](int q, int j, const float o[kDec]) {
 #pragma unroll
     for (int u = 0; u < kDec; ++u) ghat[kDec * (size_t)q + u] = o[u];
     const double dj0 = invd[2 * j], dj1 = invd[2 * j + 1];
@@ -291,6 +351,22 @@ __global__ void k_decode(int nb, const int* __restrict__ brow_ptr, const int* __
     g[1] = si0 * o[1] * sj1 + o[3] * (float)(di0 * a[1] * dj1);
     g[2] = si1 * o[1] * sj0 + o[3] * (float)(di1 * a[2] * dj0);
     g[3] = si1 * o[2] * sj1 + o[3] * (float)(di1 * a[3] * dj1);
+  };
+  int qd = -1;
+  for (int q = brow_ptr[i]; q < brow_ptr[i + 1]; ++q) {
+    const int j = bcol[q];
+    if (j == i) {
+      qd = q;
+      continue;
+    }
+    float o[kDec];
+    decode_block(hi, H4 + 8 * (size_t)j, theta + kWo, theta + kBo, o);
+    store(q, j, o);
+  }
+  if (qd >= 0) {
+    float o[kDec];
+    decode_block(hi, H4 + 8 * (size_t)i, theta + kWd, theta + kBd, o);
+    store(qd, i, o);
   }
 }
 
@@ -566,7 +642,9 @@ void weights_bf16(Context& c) {
   HIP_CHECK(hipGetLastError());
 }
 
-int rows_grid(int N) { return (N + 63) / 64; }  // 4 wavefronts x 16 rows per workgroup
+// workgroups of k_dense: 4 wavefronts each, every wavefront walks 16-row tiles; capped so that
+// a wavefront's registers of weights serve several tiles
+int rows_grid(int N) { return std::max(1, std::min((N + 63) / 64, 2048)); }
 
 }  // namespace
 
@@ -588,7 +666,7 @@ void mpnn_forward(Context& c) {
                      (const uint16_t*)m.feat.get(), (const uint16_t*)m.wt[0].get(),
                      (const float*)m.theta.get() + kB0, (void*)m.H[0].get());
   for (int l = 1; l <= kRounds; ++l) {
-    hipLaunchKernelGGL(k_aggregate, dim3((N + 3) / 4), dim3(4 * kH), 0, c.stream, N,
+    hipLaunchKernelGGL(k_aggregate, dim3((8 * (int64_t)N + kT - 1) / kT), dim3(kT), 0, c.stream, N,
                        c.brow_ptr.get(), c.bcol.get(), (const uint16_t*)m.H[l - 1].get(), m.X[l].get());
     hipLaunchKernelGGL((k_dense<2 * kH, kH, true, 0>), dim3(rows_grid(N)), dim3(kT), 0, c.stream, N,
                        (const uint16_t*)m.X[l].get(), (const uint16_t*)m.wt[l].get(),

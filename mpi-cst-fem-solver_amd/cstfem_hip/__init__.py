@@ -147,6 +147,7 @@ SIGNATURES = {
                                              ctypes.POINTER(KirschMetricsC)]),
     "cstfem_mg_setup": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int]),
+    "cstfem_mg_prepare": (ctypes.c_int, [_P]),
     "cstfem_mg_levels": (ctypes.c_int, [_P, _I]),
     "cstfem_mg_level_info": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _I64, _D]),
     "cstfem_mg_download_aggregates": (ctypes.c_int, [_P, ctypes.c_int, _I, ctypes.c_int]),
@@ -433,6 +434,11 @@ class Device:
         coarse levels 1..kcycle_levels use the K-cycle, deeper ones the V-cycle."""
         _check(lib().cstfem_mg_setup(self._h, int(max_levels), int(coarse_max),
                                      int(fine_smoother), int(kcycle_levels)))
+
+    def mg_prepare(self):
+        """Starts the fine level's aggregation (pattern only) on a host thread; the next
+        mg_setup uses it.  Call after build_stiffness_pattern."""
+        _check(lib().cstfem_mg_prepare(self._h))
 
     def mg_levels(self):
         n = ctypes.c_int()

@@ -163,10 +163,15 @@ def load_problem(dev: Device, prob: Problem):
     dev.set_material(prob.E, prob.nu, prob.plane, prob.thickness, prob.E_elem)
 
 
-def assemble_and_constrain(dev: Device, prob: Problem, times: PhaseTimes | None = None):
-    """Pattern, assembly, loads and Dirichlet elimination, all on the device."""
+def assemble_and_constrain(dev: Device, prob: Problem, times: PhaseTimes | None = None,
+                           multilevel: bool = False):
+    """Pattern, assembly, loads and Dirichlet elimination, all on the device.  With
+    multilevel, the fine level's aggregation starts right after the pattern (mg_prepare) and
+    runs on a host thread meanwhile."""
     t0 = time.perf_counter()
     dev.build_stiffness_pattern()
+    if multilevel:
+        dev.mg_prepare()
     t1 = time.perf_counter()
     dev.assemble_stiffness()
     t2 = time.perf_counter()

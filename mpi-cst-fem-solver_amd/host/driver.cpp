@@ -187,6 +187,8 @@ int run_case(const std::string& case_path, std::ostream& out, std::ostream& err,
     check_rc(cstfem_set_material(ctx, E, nu, plane, t, nullptr));
     int64_t nnz = 0;
     check_rc(cstfem_build_pattern(ctx, &nnz));
+    // the multilevel set-up's fine aggregation needs the pattern only: start it now
+    if (gmethod == "fcg_learned_multilevel" || gmethod == "fcg_multilevel") check_rc(cstfem_mg_prepare(ctx));
     check_rc(cstfem_assemble(ctx));
     const double t_assembly = since(t0);
 

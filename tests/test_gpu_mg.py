@@ -114,6 +114,79 @@ def test_setup_order_and_arguments(dev):
         dev.mg_apply(np.ones(dev.n))
 
 
+def hierarchy(dev):
+    """Every level's size, aggregates and coarse matrix, and one cycle applied to a fixed r."""
+    out = []
+    for l in range(dev.mg_levels()):
+        info = dev.mg_level_info(l)
+        agg = dev.mg_download_aggregates(l) if l < dev.mg_levels() - 1 else None
+        mat = dev.mg_download_matrix(l) if l >= 1 else None
+        out.append((info, agg, mat))
+    z = dev.mg_apply(np.random.default_rng(11).standard_normal(dev.n))
+    return out, z
+
+
+def assert_same_hierarchy(a, b):
+    (la, za), (lb, zb) = a, b
+    assert len(la) == len(lb)
+    for (ia, aa, ma), (ib, ab, mb) in zip(la, lb):
+        assert ia == ib
+        if aa is not None:
+            np.testing.assert_array_equal(aa, ab)
+        if ma is not None:
+            for x, y in zip(ma, mb):
+                np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(za, zb)
+
+
+def test_prepared_aggregation_is_the_inline_one(dev):
+    """mg_prepare starts the fine level's aggregation right after the pattern (node-graph
+    download on a second stream, greedy pass on a host thread) while assembly and the loads go
+    on.  The set-up that takes it must build the same hierarchy, bit for bit, as one that
+    aggregates inline.  A new pattern or mesh drops a started aggregation: the set-up after
+    one must not use the stale result."""
+    big, small = hetero(300, e_seed=4, seed=6), hetero(90, e_seed=5, seed=7)
+    ref = {}
+    for name, prob in (("big", big), ("small", small)):
+        pipeline.load_problem(dev, prob)
+        pipeline.assemble_and_constrain(dev, prob)
+        dev.setup()
+        dev.mg_setup(10, 200, 1)
+        ref[name] = hierarchy(dev)
+    # prepared: the same hierarchy
+    pipeline.load_problem(dev, big)
+    pipeline.assemble_and_constrain(dev, big, multilevel=True)
+    dev.setup()
+    dev.mg_setup(10, 200, 1)
+    assert_same_hierarchy(hierarchy(dev), ref["big"])
+    # prepared twice, then the pattern is built again (assemble_and_constrain does): dropped
+    dev.build_stiffness_pattern()
+    dev.mg_prepare()
+    dev.mg_prepare()
+    pipeline.assemble_and_constrain(dev, big)
+    dev.setup()
+    dev.mg_setup(10, 200, 1)
+    assert_same_hierarchy(hierarchy(dev), ref["big"])
+    # prepared on the big plate, then another mesh: the small plate's own hierarchy
+    dev.build_stiffness_pattern()
+    dev.mg_prepare()
+    pipeline.load_problem(dev, small)
+    pipeline.assemble_and_constrain(dev, small)
+    dev.setup()
+    dev.mg_setup(10, 200, 1)
+    assert_same_hierarchy(hierarchy(dev), ref["small"])
+    # a prepared aggregation that no set-up takes is joined when the context closes
+    d2 = cstfem_hip.Device(0)
+    try:
+        with pytest.raises(cstfem_hip.CstfemLogicError):
+            d2.mg_prepare()  # no pattern yet
+        pipeline.load_problem(d2, small)
+        d2.build_stiffness_pattern()
+        d2.mg_prepare()
+    finally:
+        d2.close()
+
+
 def test_coarse_level_above_one_k_cycle_grid(dev):
     """A first coarse level larger than the K-cycle's reduction grid covers (2048 workgroups x
     256 threads = 524288 DOFs): every K-cycle kernel must stride over the whole level."""
