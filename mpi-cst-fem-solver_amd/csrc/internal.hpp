@@ -189,16 +189,21 @@ struct Multigrid {
   double m_bound = 0.0;   // lambda_max(M A) of the learned level-0 smoother
   bool coarse_fp32 = false;  // levels >= 1 hold FP32 copies of their values (learned smoother)
   DBuf<double> cinv;
+  // device workspaces of the set-up, grown on demand and kept, so that repeated set-ups
+  // allocate nothing: member-list scratch, coarse degrees and error flags (iwork), the coarse
+  // row of every coarse block (crow), power-iteration vectors (dwork)
+  DBuf<int> iwork, crow;
+  DBuf<double> dwork;
   int* pinned = nullptr;  // page-locked staging of the fine node graph, kept across set-ups
   size_t pinned_n = 0;
   // level-0 aggregation started ahead of the set-up (mg_prepare): a worker thread waits for
   // the node graph's download (auxiliary stream, event pre_done), runs the greedy pass into
-  // pre_agg and stages the member lists; the set-up joins it.  Uses `pinned` until joined.
+  // pre_agg and stages the map for upload; the set-up joins it.  Uses `pinned` until joined.
   std::thread pre_worker;
   bool pre_started = false;
   std::vector<int> pre_agg;
   int pre_nagg = 0;
-  bool pre_staged = false;  // the worker also left the member lists in `pinned`
+  bool pre_staged = false;  // the worker also copied the aggregate map into `pinned`
   std::exception_ptr pre_error;
   hipEvent_t pre_queued = nullptr, pre_done = nullptr;
   void join_prepared() {

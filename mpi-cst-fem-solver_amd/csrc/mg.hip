@@ -809,6 +809,39 @@ __global__ void k_mgs(int na, int bs, const int* __restrict__ mptr, const int* _
     for (int cc = 0; cc < 3; ++cc) Rn[9 * (size_t)a + 3 * c + cc] = alive[c] ? R[c][cc] : 0.0;
 }
 
+// member lists of the aggregates: counts, offsets (a scan), members in arrival order, then
+// every list sorted ascending (the order of the Gram-Schmidt step, the restriction and the
+// Galerkin sums); err = 1 if an aggregate holds more than 192 DOFs
+__global__ void k_member_count(int n, const int* __restrict__ agg, int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[agg[i]], 1);
+}
+
+__global__ void k_member_fill(int n, const int* __restrict__ agg, const int* __restrict__ mptr,
+                              int* __restrict__ cursor, int* __restrict__ mem) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = agg[i];
+  mem[mptr[a] + atomicAdd(&cursor[a], 1)] = i;
+}
+
+__global__ void k_member_sort(int na, int bs, const int* __restrict__ mptr, int* __restrict__ mem,
+                              int* __restrict__ err) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= na) return;
+  const int m0 = mptr[a], m1 = mptr[a + 1];
+  if (bs * (m1 - m0) > 192) atomicMax(err, 1);
+  for (int k = m0 + 1; k < m1; ++k) {
+    const int v = mem[k];
+    int j = k - 1;
+    while (j >= m0 && mem[j] > v) {
+      mem[j + 1] = mem[j];
+      --j;
+    }
+    mem[j + 1] = v;
+  }
+}
+
 constexpr int kMaxCoarseDeg = 128;  // block columns of one coarse block row
 
 // Sorted distinct aggregates adjacent to aggregate a (over its members' block rows); -1 if
@@ -1105,6 +1138,13 @@ int* pinned_ints(Multigrid& mg, size_t count) {
   return mg.pinned;
 }
 
+// a workspace of at least `count` elements (reallocated only to grow)
+template <class T>
+T* grow(DBuf<T>& b, size_t count) {
+  if (!b.get() || b.n < count) b.alloc(std::max<size_t>(count, 1));
+  return b.get();
+}
+
 template <class T>
 void upload(DBuf<T>& d, const std::vector<T>& h) {
   d.alloc(h.size());
@@ -1116,33 +1156,33 @@ void upload(DBuf<T>& d, const std::vector<T>& h) {
 double power_max(Context& c, int level) {
   MgLevel& L = *c.mg.lv[level];
   const int n = L.n;
-  DBuf<double> x, y, part, st;
-  x.alloc(n);
-  y.alloc(n);
-  st.alloc(2);
   const int g = level == 0 ? slice_grid(c.nslices) : grid1(n);
   const int gn = std::min(kPart, grid1(n));
-  part.alloc(std::max(g, gn));
-  HIP_CHECK(hipMemsetAsync(st.get(), 0, 2 * sizeof(double), c.stream));
-  hipLaunchKernelGGL(k_power_init, dim3(grid1(n)), dim3(kT), 0, c.stream, n, x.get());
-  hipLaunchKernelGGL(k_sumsq, dim3(gn), dim3(kT), 0, c.stream, n, (const double*)x.get(), part.get());
-  hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x.get(),
-                     (const double*)part.get(), gn, st.get(), 0);
+  const size_t np = std::max(g, gn);
+  // x[n], y[n], partials, the two scalars of k_power_scale
+  double* x = grow(c.mg.dwork, 2 * (size_t)n + np + 2);
+  double* y = x + n;
+  double* part = y + n;
+  double* st = part + np;
+  HIP_CHECK(hipMemsetAsync(st, 0, 2 * sizeof(double), c.stream));
+  hipLaunchKernelGGL(k_power_init, dim3(grid1(n)), dim3(kT), 0, c.stream, n, x);
+  hipLaunchKernelGGL(k_sumsq, dim3(gn), dim3(kT), 0, c.stream, n, (const double*)x, part);
+  hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x, (const double*)part,
+                     gn, st, 0);
   for (int it = 0; it < 20; ++it) {
     if (level == 0)
       hipLaunchKernelGGL(k0_power, dim3(g), dim3(kT), 0, c.stream, mat0(c),
-                         (const double2*)c.inv_diag.get(), (const double2*)x.get(),
-                         (double2*)y.get(), part.get());
+                         (const double2*)c.inv_diag.get(), (const double2*)x, (double2*)y, part);
     else
       hipLaunchKernelGGL(ks_power, dim3(g), dim3(kT), 0, c.stream, mats(L),
-                         (const double*)L.dinv.get(), (const double*)x.get(), y.get(), part.get());
-    hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, y.get(),
-                       (const double*)part.get(), g, st.get(), 1);
-    std::swap(x.p, y.p);
+                         (const double*)L.dinv.get(), (const double*)x, y, part);
+    hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, y, (const double*)part,
+                       g, st, 1);
+    std::swap(x, y);
   }
   HIP_CHECK(hipGetLastError());
   double lam = 0.0;
-  HIP_CHECK(hipMemcpyAsync(&lam, st.get(), sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipMemcpyAsync(&lam, st, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   return lam;
 }
@@ -1151,16 +1191,15 @@ double power_max(Context& c, int level) {
 double gersh_max(Context& c, int level) {
   MgLevel& L = *c.mg.lv[level];
   const int g = level == 0 ? slice_grid(c.nslices) : std::min(1024, grid1(L.n));
-  DBuf<double> part;
-  part.alloc(g);
+  double* part = grow(c.mg.dwork, (size_t)g);
   if (level == 0)
     hipLaunchKernelGGL(k0_gersh, dim3(g), dim3(kT), 0, c.stream, mat0(c),
-                       (const double2*)c.inv_diag.get(), part.get());
+                       (const double2*)c.inv_diag.get(), part);
   else
     hipLaunchKernelGGL(ks_gersh, dim3(g), dim3(kT), 0, c.stream, mats(L), (const double*)L.dinv.get(),
-                       part.get());
+                       part);
   std::vector<double> h(g);
-  HIP_CHECK(hipMemcpyAsync(h.data(), part.get(), sizeof(double) * g, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipMemcpyAsync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   double m = 0.0;
   for (double v : h) m = std::max(m, v);
@@ -1181,27 +1220,27 @@ double learned_smoother_bound(Context& c, int iters) {
   const int gn = std::min(kPart, grid1(n));
   const MatM M{c.sell_off.get(), c.sell_col.get(), c.mpnn.gsell.get(), c.sell_row0.get(),
                c.sell_nrow.get(), c.nslices};
-  DBuf<double> x, y, part, st;
-  x.alloc(n);
-  y.alloc(n);
-  st.alloc(2);
-  part.alloc(std::max(gs, gn));
-  HIP_CHECK(hipMemsetAsync(st.get(), 0, 2 * sizeof(double), c.stream));
-  hipLaunchKernelGGL(k_power_init, dim3(grid1(n)), dim3(kT), 0, c.stream, n, x.get());
-  hipLaunchKernelGGL(k_sumsq, dim3(gn), dim3(kT), 0, c.stream, n, (const double*)x.get(), part.get());
-  hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x.get(),
-                     (const double*)part.get(), gn, st.get(), 0);
+  const size_t np = std::max(gs, gn);
+  double* x = grow(c.mg.dwork, 2 * (size_t)n + np + 2);
+  double* y = x + n;
+  double* part = y + n;
+  double* st = part + np;
+  HIP_CHECK(hipMemsetAsync(st, 0, 2 * sizeof(double), c.stream));
+  hipLaunchKernelGGL(k_power_init, dim3(grid1(n)), dim3(kT), 0, c.stream, n, x);
+  hipLaunchKernelGGL(k_sumsq, dim3(gn), dim3(kT), 0, c.stream, n, (const double*)x, part);
+  hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x, (const double*)part,
+                     gn, st, 0);
   for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL(k0_apply, dim3(gs), dim3(kT), 0, c.stream, mat0(c),
-                       (const double2*)x.get(), (double2*)y.get());
-    hipLaunchKernelGGL(k0_mapply_sq, dim3(gs), dim3(kT), 0, c.stream, M, (const double2*)y.get(),
-                       (double2*)x.get(), part.get());
-    hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x.get(),
-                       (const double*)part.get(), gs, st.get(), 1);
+    hipLaunchKernelGGL(k0_apply, dim3(gs), dim3(kT), 0, c.stream, mat0(c), (const double2*)x,
+                       (double2*)y);
+    hipLaunchKernelGGL(k0_mapply_sq, dim3(gs), dim3(kT), 0, c.stream, M, (const double2*)y,
+                       (double2*)x, part);
+    hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x, (const double*)part,
+                       gs, st, 1);
   }
   HIP_CHECK(hipGetLastError());
   double lam = 0.0;
-  HIP_CHECK(hipMemcpyAsync(&lam, st.get(), sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipMemcpyAsync(&lam, st, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   return lam;
 }
@@ -1560,28 +1599,6 @@ void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* 
   HIP_CHECK(hipGetLastError());
 }
 
-namespace {
-
-// member lists of the aggregates into page-locked staging: mptr[na + 1] (aggregate offsets),
-// mem[nodes] (member nodes, ascending within an aggregate), agg[nodes] (a copy of the map)
-void stage_members(const std::vector<int>& h_agg, int nodes, int na, int bs, int* stage) {
-  int* mptr = stage;
-  int* mem = stage + na + 1;
-  int* agg = mem + nodes;
-  std::fill(mptr, mptr + na + 1, 0);
-  for (int i = 0; i < nodes; ++i) ++mptr[h_agg[i] + 1];
-  for (int a = 0; a < na; ++a) mptr[a + 1] += mptr[a];
-  for (int a = 0; a < na; ++a)
-    if (bs * (mptr[a + 1] - mptr[a]) > 192) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: aggregate too large");
-  {
-    std::vector<int> fill(mptr, mptr + na);
-    for (int i = 0; i < nodes; ++i) mem[fill[h_agg[i]]++] = i;
-  }
-  std::copy(h_agg.begin(), h_agg.end(), agg);
-}
-
-}  // namespace
-
 void mg_drop_prepared(Context& c) {
   c.mg.join_prepared();
   c.mg.pre_error = nullptr;
@@ -1616,9 +1633,9 @@ void mg_prepare(Context& c) {
     try {
       HIP_CHECK(hipEventSynchronize(done));
       mg.pre_nagg = aggregate(pin, pin + nb + 1, nb, mg.pre_agg);
-      // the member lists too, over the node graph (no longer read) when they fit there
-      if (2 * (size_t)nb + mg.pre_nagg + 1 <= room) {
-        stage_members(mg.pre_agg, nb, mg.pre_nagg, 2, pin);
+      // the map into the staging area for its upload, over the node graph (no longer read)
+      if ((size_t)nb <= room) {
+        std::copy(mg.pre_agg.begin(), mg.pre_agg.end(), pin);
         mg.pre_staged = true;
       }
     } catch (...) {
@@ -1693,8 +1710,6 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
   }
   const double cx = 0.5 * (xlo + xhi), cy = 0.5 * (ylo + yhi);
   const unsigned char* fixed = c.fixed.n == (size_t)c.n ? c.fixed.get() : nullptr;
-  DBuf<int> flag;
-  flag.alloc(1);
   for (int l = 0;; ++l) {
     MgLevel& L = *mg.lv[l];
     const bool last = l + 1 >= max_levels - 1;
@@ -1707,26 +1722,38 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     const int na = L.nagg;
     // members and the aggregate map go up from page-locked memory; the staging area follows
     // the node graph, which the host no longer reads once the aggregation is done
-    const size_t need = 2 * (size_t)L.nodes + na + 1;
+    // the aggregate map goes up from page-locked staging; the member lists are built on the
+    // device from it
     int* stage;
     if (l == 0 && prepared && mg.pre_staged) {
-      stage = mg.pinned;  // the worker staged them
+      stage = mg.pinned;  // the worker copied the map there
     } else {
-      // the uploads of the level above read the staging area: let them finish (on level 0
-      // only the node-graph download used it, and that was waited for)
-      if (l > 0 || need > mg.pinned_n) HIP_CHECK(hipStreamSynchronize(c.stream));
-      stage = pinned_ints(mg, std::max(mg.pinned_n, need));
-      stage_members(L.h_agg, L.nodes, na, L.bs, stage);
+      // the upload of the level above reads the staging area: let it finish (on level 0 only
+      // the node-graph download used it, and that was waited for)
+      if (l > 0 || (size_t)L.nodes > mg.pinned_n) HIP_CHECK(hipStreamSynchronize(c.stream));
+      stage = pinned_ints(mg, std::max(mg.pinned_n, (size_t)L.nodes));
+      std::copy(L.h_agg.begin(), L.h_agg.end(), stage);
     }
-    int* mptr = stage;
-    int* mem = stage + na + 1;
-    int* agg = mem + L.nodes;
     L.agg.alloc(L.nodes);
     L.mptr.alloc(na + 1);
     L.mem.alloc(L.nodes);
-    HIP_CHECK(hipMemcpyAsync(L.agg.get(), agg, sizeof(int) * L.nodes, hipMemcpyHostToDevice, c.stream));
-    HIP_CHECK(hipMemcpyAsync(L.mptr.get(), mptr, sizeof(int) * (na + 1), hipMemcpyHostToDevice, c.stream));
-    HIP_CHECK(hipMemcpyAsync(L.mem.get(), mem, sizeof(int) * L.nodes, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(L.agg.get(), stage, sizeof(int) * L.nodes, hipMemcpyHostToDevice, c.stream));
+    // iwork: flags[2] (coarse degree overflow, aggregate too large), counts[na + 1] (later the
+    // coarse degrees), cursors[na]
+    const size_t niw = 2 + 2 * (size_t)na + 1;
+    int* iw = grow(mg.iwork, niw);
+    int* flags = iw;
+    int* cnt = iw + 2;
+    int* cursor = cnt + na + 1;
+    HIP_CHECK(hipMemsetAsync(iw, 0, sizeof(int) * niw, c.stream));
+    hipLaunchKernelGGL(k_member_count, dim3(grid1(L.nodes)), dim3(kT), 0, c.stream, L.nodes,
+                       (const int*)L.agg.get(), cnt);
+    exclusive_scan_int(c, cnt, L.mptr.get(), na + 1, nullptr);
+    hipLaunchKernelGGL(k_member_fill, dim3(grid1(L.nodes)), dim3(kT), 0, c.stream, L.nodes,
+                       (const int*)L.agg.get(), (const int*)L.mptr.get(), cursor, L.mem.get());
+    hipLaunchKernelGGL(k_member_sort, dim3(grid1(na)), dim3(kT), 0, c.stream, na, L.bs,
+                       (const int*)L.mptr.get(), L.mem.get(), flags + 1);
+    HIP_CHECK(hipGetLastError());
     tr.mark(c, l, "aggregate, members");
     MgLevel* C = mg.take_level();
     C->nodes = na;
@@ -1744,19 +1771,18 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     // coarse block pattern (device): sorted distinct neighbour aggregates of each aggregate
     const int* dbp = l == 0 ? c.brow_ptr.get() : L.d_bptr.get();
     const int* dbc = l == 0 ? c.bcol.get() : L.d_bcol.get();
-    DBuf<int> deg;
-    deg.alloc(na);
-    HIP_CHECK(hipMemsetAsync(flag.get(), 0, sizeof(int), c.stream));
+    int* deg = cnt;  // the counts were consumed by the scan
     hipLaunchKernelGGL(k_cpat_count, dim3(grid1(na)), dim3(kT), 0, c.stream, na,
                        (const int*)L.mptr.get(), (const int*)L.mem.get(), dbp, dbc,
-                       (const int*)L.agg.get(), deg.get(), flag.get());
+                       (const int*)L.agg.get(), deg, flags);
     HIP_CHECK(hipGetLastError());
     std::vector<int> hdeg(na);
-    int hflag = 0;
-    HIP_CHECK(hipMemcpyAsync(hdeg.data(), deg.get(), sizeof(int) * na, hipMemcpyDeviceToHost, c.stream));
-    HIP_CHECK(hipMemcpyAsync(&hflag, flag.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    int hflag[2] = {0, 0};
+    HIP_CHECK(hipMemcpyAsync(hdeg.data(), deg, sizeof(int) * na, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(hflag, flags, sizeof(hflag), hipMemcpyDeviceToHost, c.stream));
     HIP_CHECK(hipStreamSynchronize(c.stream));
-    if (hflag)
+    if (hflag[1]) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: aggregate too large");
+    if (hflag[0])
       throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: an aggregate touches more than " +
                                            std::to_string(kMaxCoarseDeg) + " aggregates");
     C->bptr.assign(na + 1, 0);
@@ -1764,12 +1790,11 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     const int ncblk = C->bptr[na];
     upload(C->d_bptr, C->bptr);
     C->d_bcol.alloc(std::max(ncblk, 1));
-    DBuf<int> d_crow;
-    d_crow.alloc(std::max(ncblk, 1));
+    int* d_crow = grow(mg.crow, (size_t)std::max(ncblk, 1));
     hipLaunchKernelGGL(k_cpat_fill, dim3(grid1(na)), dim3(kT), 0, c.stream, na,
                        (const int*)L.mptr.get(), (const int*)L.mem.get(), dbp, dbc,
                        (const int*)L.agg.get(), (const int*)C->d_bptr.get(), C->d_bcol.get(),
-                       d_crow.get());
+                       d_crow);
     HIP_CHECK(hipGetLastError());
     C->bcol.resize(ncblk);
     HIP_CHECK(hipMemcpyAsync(C->bcol.data(), C->d_bcol.get(), sizeof(int) * ncblk, hipMemcpyDeviceToHost, c.stream));
@@ -1790,7 +1815,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     if (ncblk > 0) {
       if (l == 0)
         hipLaunchKernelGGL(k_rap0, dim3(grid1(ncblk)), dim3(kT), 0, c.stream, ncblk,
-                           (const int*)d_crow.get(), (const int*)C->d_bptr.get(),
+                           (const int*)d_crow, (const int*)C->d_bptr.get(),
                            (const int*)C->d_bcol.get(), (const int*)L.mptr.get(),
                            (const int*)L.mem.get(), (const int*)L.agg.get(), (const int*)c.brow_ptr.get(),
                            (const int*)c.bcol.get(), (const int64_t*)c.sell_pos.get(),
@@ -1798,7 +1823,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
                            (const int64_t*)C->off.get(), C->val.get());
       else
         hipLaunchKernelGGL(k_rapS, dim3(grid1(ncblk)), dim3(kT), 0, c.stream, ncblk,
-                           (const int*)d_crow.get(), (const int*)C->d_bptr.get(),
+                           (const int*)d_crow, (const int*)C->d_bptr.get(),
                            (const int*)C->d_bcol.get(), (const int*)L.mptr.get(),
                            (const int*)L.mem.get(), (const int*)L.agg.get(), (const int*)L.d_bptr.get(),
                            (const int*)L.d_bcol.get(), (const int64_t*)L.off.get(),
