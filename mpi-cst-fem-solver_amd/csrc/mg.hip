@@ -598,7 +598,19 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars*
       const int i = tt / kTailLanes, j = tt % kTailLanes;
       const double* row = T.cinv + (size_t)i * T.nc;
       double sum = 0.0;
-      for (int k = j; k < T.nc; k += kTailLanes) sum = sum + row[k] * crhs[k];
+      // entries k = j, j + kTailLanes, ... in order, their loads issued 8 at a time
+      for (int k = j; k < T.nc; k += 8 * kTailLanes) {
+        double a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int kk = k + u * kTailLanes < T.nc ? k + u * kTailLanes : k;
+          a[u] = row[kk];
+          b[u] = crhs[kk];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (k + u * kTailLanes < T.nc) sum = sum + a[u] * b[u];
+      }
 #pragma unroll
       for (int m = 1; m < kTailLanes; m <<= 1) sum += __shfl_xor(sum, m, 64);
       if (j == 0) ckx[i] = sum;
