@@ -128,6 +128,8 @@ int cstfem_upload_csr(cstfem_ctx* ctx, int n, const int* row_ptr, const int* col
                       const double* vals);
 
 /* ---- right-hand side and Dirichlet elimination ---------------------------------------- */
+/* The global load vector f (driver.cpp:135-148 builds it as zeros plus the loads).  n must
+ * equal the system size; b == NULL sets f = 0 on the device without a host copy. */
 int cstfem_set_rhs(cstfem_ctx* ctx, const double* b, int n);
 int cstfem_download_rhs(cstfem_ctx* ctx, double* b, int n);
 /* Adds the consistent nodal load of a uniform body force (bx, by) per unit volume,

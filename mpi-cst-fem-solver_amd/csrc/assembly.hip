@@ -60,21 +60,26 @@ __global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ tile_o
     if (base + i < count) out[base + i] += add;
 }
 
-void scan_rec(Context& c, const int* in, int* out, int64_t count) {
+// scratch slots of the context (tmp_ints): the scan takes two per recursion depth
+constexpr int kSlotScan = 0;     // 0 .. 2 * kScanDepth - 1
+constexpr int kScanDepth = 4;    // tiles of tiles: 2048^4 elements at most
+constexpr int kSlotCount = 8;    // per-node / per-block counts
+constexpr int kSlotCursor = 9;   // fill cursors
+
+void scan_rec(Context& c, const int* in, int* out, int64_t count, int depth) {
+  if (depth >= kScanDepth) throw Error(CSTFEM_ERR_ARGUMENT, "scan: too many elements");
   const int64_t tiles = (count + kScanTile - 1) / kScanTile;
-  DBuf<int> sums, sums_off;
-  sums.alloc(tiles);
-  sums_off.alloc(tiles);
+  int* sums = tmp_ints(c, kSlotScan + 2 * depth, tiles);
+  int* sums_off = tmp_ints(c, kSlotScan + 2 * depth + 1, tiles);
   hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kScanThreads), 0, c.stream, in,
-                     out, sums.get(), count);
+                     out, sums, count);
   HIP_CHECK(hipGetLastError());
   if (tiles > 1) {
-    scan_rec(c, sums.get(), sums_off.get(), tiles);
+    scan_rec(c, sums, sums_off, tiles, depth + 1);
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)tiles), dim3(kScanThreads), 0, c.stream, out,
-                       sums_off.get(), count);
+                       sums_off, count);
     HIP_CHECK(hipGetLastError());
   }
-  HIP_CHECK(hipStreamSynchronize(c.stream));  // sums freed on return
 }
 
 }  // namespace
@@ -85,7 +90,7 @@ void exclusive_scan_int(Context& c, const int* in, int* out, int64_t count, int6
     if (total) *total = 0;
     return;
   }
-  scan_rec(c, in, out, count);
+  scan_rec(c, in, out, count, 0);
   if (total) {
     int last_in = 0, last_out = 0;  // read in stream order after the scan kernels
     HIP_CHECK(hipMemcpyAsync(&last_in, in + count - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -93,6 +98,13 @@ void exclusive_scan_int(Context& c, const int* in, int* out, int64_t count, int6
     HIP_CHECK(hipStreamSynchronize(c.stream));
     *total = (int64_t)last_in + last_out;
   }
+}
+
+int* tmp_ints(Context& c, int slot, size_t count) {
+  DBuf<int>& b = c.tmp_i[slot];
+  // growing frees the old buffer; hipFree waits for the kernels still using it
+  if (!b.get() || b.n < count) b.alloc(std::max<size_t>(count, 1));
+  return b.get();
 }
 
 int grid_for(int64_t threads, int block, int max_grid) {
@@ -145,21 +157,19 @@ __global__ void k_sort_node_elems(int nn, const int* __restrict__ ptr, int* list
 void build_node_elements(Context& c) {
   const int nn = c.nn;
   const int64_t ne = c.ne;
-  DBuf<int> cnt, cursor;
-  cnt.alloc(nn + 1);
-  cursor.alloc(nn);
-  HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(int) * (nn + 1), c.stream));
-  HIP_CHECK(hipMemsetAsync(cursor.get(), 0, sizeof(int) * nn, c.stream));
+  int* cnt = tmp_ints(c, kSlotCount, (size_t)nn + 1);
+  int* cursor = tmp_ints(c, kSlotCursor, (size_t)nn);
+  HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * (nn + 1), c.stream));
+  HIP_CHECK(hipMemsetAsync(cursor, 0, sizeof(int) * nn, c.stream));
   const int g = grid_for(ne, 256, 8192);
-  hipLaunchKernelGGL(k_count_node_elems, dim3(g), dim3(256), 0, c.stream, c.tri.get(), ne,
-                     cnt.get());
+  hipLaunchKernelGGL(k_count_node_elems, dim3(g), dim3(256), 0, c.stream, c.tri.get(), ne, cnt);
   HIP_CHECK(hipGetLastError());
   c.ne_ptr.alloc(nn + 1);
   int64_t total = 0;
-  exclusive_scan_int(c, cnt.get(), c.ne_ptr.get(), nn + 1, &total);
+  exclusive_scan_int(c, cnt, c.ne_ptr.get(), nn + 1, &total);
   c.ne_list.alloc(3 * ne);
   hipLaunchKernelGGL(k_fill_node_elems, dim3(g), dim3(256), 0, c.stream, c.tri.get(), ne,
-                     c.ne_ptr.get(), cursor.get(), c.ne_list.get());
+                     c.ne_ptr.get(), cursor, c.ne_list.get());
   HIP_CHECK(hipGetLastError());
   const int big = INT_MAX;
   HIP_CHECK(hipMemcpyAsync(c.err_flag.get(), &big, sizeof(int), hipMemcpyHostToDevice, c.stream));
@@ -237,15 +247,14 @@ void build_pattern(Context& c) {
   mg_drop_prepared(c);  // a level-0 aggregation under way reads the old pattern
   build_node_elements(c);
   const int nn = c.nn;
-  DBuf<int> deg;
-  deg.alloc(nn + 1);
-  HIP_CHECK(hipMemsetAsync(deg.get(), 0, sizeof(int) * (nn + 1), c.stream));
+  int* deg = tmp_ints(c, kSlotCount, (size_t)nn + 1);
+  HIP_CHECK(hipMemsetAsync(deg, 0, sizeof(int) * (nn + 1), c.stream));
   hipLaunchKernelGGL(k_adj_count, dim3(grid_for(nn, 128, 0)), dim3(128), 0, c.stream, nn,
-                     c.tri.get(), c.ne_ptr.get(), c.ne_list.get(), deg.get());
+                     c.tri.get(), c.ne_ptr.get(), c.ne_list.get(), deg);
   HIP_CHECK(hipGetLastError());
   c.brow_ptr.alloc(nn + 1);
   int64_t total = 0;
-  exclusive_scan_int(c, deg.get(), c.brow_ptr.get(), nn + 1, &total);
+  exclusive_scan_int(c, deg, c.brow_ptr.get(), nn + 1, &total);
   if (4 * total > INT_MAX)
     throw Error(CSTFEM_ERR_ARGUMENT, "pattern exceeds 32-bit nnz (" + std::to_string(4 * total) + ")");
   c.nblk = total;
@@ -336,13 +345,12 @@ void build_sell(Context& c) {
                            hipMemcpyHostToDevice, c.stream));
   HIP_CHECK(hipMemcpyAsync(c.sell_nrow.get(), nrow.data(), sizeof(int) * nslices,
                            hipMemcpyHostToDevice, c.stream));
-  DBuf<int> width;
-  width.alloc(nslices);
+  int* width = tmp_ints(c, kSlotCount, (size_t)nslices);
   hipLaunchKernelGGL(k_slice_width, dim3(grid_for(nslices, 256, 0)), dim3(256), 0, c.stream,
-                     nslices, c.sell_row0.get(), c.sell_nrow.get(), c.brow_ptr.get(), width.get());
+                     nslices, c.sell_row0.get(), c.sell_nrow.get(), c.brow_ptr.get(), width);
   HIP_CHECK(hipGetLastError());
   std::vector<int> w(nslices);
-  HIP_CHECK(hipMemcpyAsync(w.data(), width.get(), sizeof(int) * nslices, hipMemcpyDeviceToHost,
+  HIP_CHECK(hipMemcpyAsync(w.data(), width, sizeof(int) * nslices, hipMemcpyDeviceToHost,
                            c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   c.h_sell_off.assign(nslices + 1, 0);
@@ -411,16 +419,15 @@ __global__ void k_slot_map(int nb, const int* __restrict__ tri, const int* __res
 }  // namespace
 
 void build_slot_map(Context& c) {
-  DBuf<int> cnt;
-  cnt.alloc(c.nblk + 1);
-  HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(int) * (c.nblk + 1), c.stream));
+  int* cnt = tmp_ints(c, kSlotCount, (size_t)c.nblk + 1);
+  HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * (c.nblk + 1), c.stream));
   const dim3 g(grid_for(c.nb, 128, 0)), b(128);
   hipLaunchKernelGGL(k_slot_map<false>, g, b, 0, c.stream, c.nb, c.tri.get(), c.ne_ptr.get(),
-                     c.ne_list.get(), c.brow_ptr.get(), c.bcol.get(), cnt.get(), (int*)nullptr);
+                     c.ne_list.get(), c.brow_ptr.get(), c.bcol.get(), cnt, (int*)nullptr);
   HIP_CHECK(hipGetLastError());
   c.slot_ptr.alloc(c.nblk + 1);
   int64_t total = 0;
-  exclusive_scan_int(c, cnt.get(), c.slot_ptr.get(), c.nblk + 1, &total);
+  exclusive_scan_int(c, cnt, c.slot_ptr.get(), c.nblk + 1, &total);
   if (total != 9 * c.ne)
     throw Error(CSTFEM_ERR_MISSING_ENTRY,
                 "assemble_stiffness: slot map holds " + std::to_string(total) +

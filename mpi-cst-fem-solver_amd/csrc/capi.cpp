@@ -421,14 +421,16 @@ int cstfem_upload_csr(cstfem_ctx* ctx, int n, const int* row_ptr, const int* col
 
 int cstfem_set_rhs(cstfem_ctx* ctx, const double* b, int n) {
   return guarded([&] {
-    need(ctx && b, "cstfem_set_rhs: NULL argument");
+    need(ctx != nullptr, "cstfem_set_rhs: NULL argument");
     Context& c = ctx->c;
     use_device(&c);
     need(c.have_pattern, "cstfem_set_rhs: no matrix on the device");
     need(n == c.n, "cstfem_set_rhs: size " + std::to_string(n) + " != " + std::to_string(c.n));
     ensure_vectors(c);
     HIP_CHECK(hipMemsetAsync(c.b.get(), 0, sizeof(double) * c.b.n, c.stream));
-    HIP_CHECK(hipMemcpyAsync(c.b.get(), b, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    // b == NULL: the zero vector the reference starts its load assembly from, set on the
+    // device instead of copied from the host
+    if (b) HIP_CHECK(hipMemcpyAsync(c.b.get(), b, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
     HIP_CHECK(hipStreamSynchronize(c.stream));
     return CSTFEM_OK;
   });

@@ -194,6 +194,11 @@ struct Multigrid {
   // row of every coarse block (crow), power-iteration vectors (dwork)
   DBuf<int> iwork, crow;
   DBuf<double> dwork;
+  DBuf<double> dwork2;             // the learned smoother's check, on the auxiliary stream
+  DBuf<double> dwork3;             // the levels' smoothing bounds, on the auxiliary stream
+  DBuf<double> lam_dev;            // those bounds on the device, one per level
+  hipEvent_t bound_ev = nullptr;   // the check starts after this point of the main stream
+  hipEvent_t lam_ev = nullptr;     // a level's bound starts after its matrix
   int* pinned = nullptr;  // page-locked staging of the fine node graph, kept across set-ups
   size_t pinned_n = 0;
   // level-0 aggregation started ahead of the set-up (mg_prepare): a worker thread waits for
@@ -233,6 +238,8 @@ struct Multigrid {
     if (pinned) (void)hipHostFree(pinned);
     if (pre_queued) (void)hipEventDestroy(pre_queued);
     if (pre_done) (void)hipEventDestroy(pre_done);
+    if (bound_ev) (void)hipEventDestroy(bound_ev);
+    if (lam_ev) (void)hipEventDestroy(lam_ev);
   }
   void clear() {
     spare.insert(spare.begin(), lv.begin(), lv.end());
@@ -307,6 +314,10 @@ struct Context {
   Mpnn mpnn;
   Multigrid mg;
 
+  // grow-only integer scratch of the pattern builders and scans, one buffer per call site
+  // (slots in assembly.hip), kept so that rebuilding a pattern allocates and frees nothing
+  DBuf<int> tmp_i[12];
+
   // reduction scratch
   DBuf<double> partials;   // 4 * max_blocks
   DBuf<PcgScalars> scal;
@@ -321,7 +332,10 @@ struct Context {
 void set_error(const std::string& m);
 
 // helpers implemented in the .hip files
+// exclusive prefix sums on the library stream; with total, waits for them and returns the sum
 void exclusive_scan_int(Context& c, const int* in, int* out, int64_t count, int64_t* total);
+// the context's integer scratch buffer `slot` (0..11), grown to at least count elements
+int* tmp_ints(Context& c, int slot, size_t count);
 int grid_for(int64_t threads, int block, int max_grid);
 
 // kernels-level entry points (assembly.hip, pcg.hip, ...)

@@ -1163,98 +1163,100 @@ void upload(DBuf<T>& d, const std::vector<T>& h) {
   if (!h.empty()) HIP_CHECK(hipMemcpy(d.get(), h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
 }
 
-// largest eigenvalue of D^-1 A by 20 power steps from a fixed vector, all on the device
-// (one host synchronisation at the end)
-double power_max(Context& c, int level) {
+constexpr int kLamSlots = 18;  // lambda estimates on the device: one per level, one spare
+
+// lam = min(Gershgorin bound, 1.3 x power estimate): the Gershgorin partials gp[0..g) are
+// maxed in a fixed order, st[0] holds the power estimate
+__global__ void k_lam_final(const double* __restrict__ gp, int g, const double* __restrict__ st,
+                            double* __restrict__ lam) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double m = 0.0;
+  for (int i = 0; i < g; ++i) m = fmax(m, gp[i]);
+  const double pw = 1.3 * st[0];
+  *lam = pw < m ? pw : m;
+}
+
+// Smoothing interval upper end of level `level` on `stream`, into *lam_dev: the power estimate
+// of lambda_max(D^-1 A) (20 steps from a fixed vector) with a margin, never above the
+// Gershgorin bound (which holds for every eigenvalue).  `ws` is the stream's workspace.
+void smoothing_max_enqueue(Context& c, int level, hipStream_t stream, DBuf<double>& ws,
+                           double* lam_dev) {
   MgLevel& L = *c.mg.lv[level];
   const int n = L.n;
   const int g = level == 0 ? slice_grid(c.nslices) : grid1(n);
   const int gn = std::min(kPart, grid1(n));
+  const int gg = level == 0 ? slice_grid(c.nslices) : std::min(1024, grid1(n));
   const size_t np = std::max(g, gn);
-  // x[n], y[n], partials, the two scalars of k_power_scale
-  double* x = grow(c.mg.dwork, 2 * (size_t)n + np + 2);
+  // x[n], y[n], power partials, the two scalars of k_power_scale, Gershgorin partials
+  double* x = grow(ws, 2 * (size_t)n + np + 2 + gg);
   double* y = x + n;
   double* part = y + n;
   double* st = part + np;
-  HIP_CHECK(hipMemsetAsync(st, 0, 2 * sizeof(double), c.stream));
-  hipLaunchKernelGGL(k_power_init, dim3(grid1(n)), dim3(kT), 0, c.stream, n, x);
-  hipLaunchKernelGGL(k_sumsq, dim3(gn), dim3(kT), 0, c.stream, n, (const double*)x, part);
-  hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x, (const double*)part,
+  double* gpart = st + 2;
+  if (level == 0)
+    hipLaunchKernelGGL(k0_gersh, dim3(gg), dim3(kT), 0, stream, mat0(c),
+                       (const double2*)c.inv_diag.get(), gpart);
+  else
+    hipLaunchKernelGGL(ks_gersh, dim3(gg), dim3(kT), 0, stream, mats(L), (const double*)L.dinv.get(),
+                       gpart);
+  HIP_CHECK(hipMemsetAsync(st, 0, 2 * sizeof(double), stream));
+  hipLaunchKernelGGL(k_power_init, dim3(grid1(n)), dim3(kT), 0, stream, n, x);
+  hipLaunchKernelGGL(k_sumsq, dim3(gn), dim3(kT), 0, stream, n, (const double*)x, part);
+  hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, stream, n, x, (const double*)part,
                      gn, st, 0);
   for (int it = 0; it < 20; ++it) {
     if (level == 0)
-      hipLaunchKernelGGL(k0_power, dim3(g), dim3(kT), 0, c.stream, mat0(c),
+      hipLaunchKernelGGL(k0_power, dim3(g), dim3(kT), 0, stream, mat0(c),
                          (const double2*)c.inv_diag.get(), (const double2*)x, (double2*)y, part);
     else
-      hipLaunchKernelGGL(ks_power, dim3(g), dim3(kT), 0, c.stream, mats(L),
+      hipLaunchKernelGGL(ks_power, dim3(g), dim3(kT), 0, stream, mats(L),
                          (const double*)L.dinv.get(), (const double*)x, y, part);
-    hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, y, (const double*)part,
+    hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, stream, n, y, (const double*)part,
                        g, st, 1);
     std::swap(x, y);
   }
+  hipLaunchKernelGGL(k_lam_final, dim3(1), dim3(64), 0, stream, (const double*)gpart, gg,
+                     (const double*)st, lam_dev);
   HIP_CHECK(hipGetLastError());
+}
+
+// the same, waited for on the library stream
+double smoothing_max(Context& c, int level) {
+  double* lam_dev = grow(c.mg.lam_dev, kLamSlots) + kLamSlots - 1;  // slot of the waited form
+  smoothing_max_enqueue(c, level, c.stream, c.mg.dwork, lam_dev);
   double lam = 0.0;
-  HIP_CHECK(hipMemcpyAsync(&lam, st, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipMemcpyAsync(&lam, lam_dev, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   return lam;
 }
 
-// Gershgorin upper bound of lambda_max(D^-1 A) (max over rows of dinv_i sum_j |a_ij|)
-double gersh_max(Context& c, int level) {
-  MgLevel& L = *c.mg.lv[level];
-  const int g = level == 0 ? slice_grid(c.nslices) : std::min(1024, grid1(L.n));
-  double* part = grow(c.mg.dwork, (size_t)g);
-  if (level == 0)
-    hipLaunchKernelGGL(k0_gersh, dim3(g), dim3(kT), 0, c.stream, mat0(c),
-                       (const double2*)c.inv_diag.get(), part);
-  else
-    hipLaunchKernelGGL(ks_gersh, dim3(g), dim3(kT), 0, c.stream, mats(L), (const double*)L.dinv.get(),
-                       part);
-  std::vector<double> h(g);
-  HIP_CHECK(hipMemcpyAsync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
-  double m = 0.0;
-  for (double v : h) m = std::max(m, v);
-  return m;
-}
-
-// Smoothing interval upper end: the power estimate with a margin, never above the Gershgorin
-// bound (which holds for every eigenvalue)
-double smoothing_max(Context& c, int level) {
-  const double gb = gersh_max(c, level);
-  const double pw = power_max(c, level);
-  return std::min(gb, 1.3 * pw);
-}
-
-// largest eigenvalue of M A (the learned smoother on level 0) by `iters` power steps
-double learned_smoother_bound(Context& c, int iters) {
+// largest eigenvalue of M A (the learned smoother on level 0) by `iters` power steps, enqueued
+// on stream `st` with its own workspace (mg.dwork2); the estimate is left in *lam_dev
+double* learned_smoother_bound_enqueue(Context& c, int iters, hipStream_t stream) {
   const int n = c.n, gs = slice_grid(c.nslices);
   const int gn = std::min(kPart, grid1(n));
   const MatM M{c.sell_off.get(), c.sell_col.get(), c.mpnn.gsell.get(), c.sell_row0.get(),
                c.sell_nrow.get(), c.nslices};
   const size_t np = std::max(gs, gn);
-  double* x = grow(c.mg.dwork, 2 * (size_t)n + np + 2);
+  double* x = grow(c.mg.dwork2, 2 * (size_t)n + np + 2);
   double* y = x + n;
   double* part = y + n;
   double* st = part + np;
-  HIP_CHECK(hipMemsetAsync(st, 0, 2 * sizeof(double), c.stream));
-  hipLaunchKernelGGL(k_power_init, dim3(grid1(n)), dim3(kT), 0, c.stream, n, x);
-  hipLaunchKernelGGL(k_sumsq, dim3(gn), dim3(kT), 0, c.stream, n, (const double*)x, part);
-  hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x, (const double*)part,
+  HIP_CHECK(hipMemsetAsync(st, 0, 2 * sizeof(double), stream));
+  hipLaunchKernelGGL(k_power_init, dim3(grid1(n)), dim3(kT), 0, stream, n, x);
+  hipLaunchKernelGGL(k_sumsq, dim3(gn), dim3(kT), 0, stream, n, (const double*)x, part);
+  hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, stream, n, x, (const double*)part,
                      gn, st, 0);
   for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL(k0_apply, dim3(gs), dim3(kT), 0, c.stream, mat0(c), (const double2*)x,
+    hipLaunchKernelGGL(k0_apply, dim3(gs), dim3(kT), 0, stream, mat0(c), (const double2*)x,
                        (double2*)y);
-    hipLaunchKernelGGL(k0_mapply_sq, dim3(gs), dim3(kT), 0, c.stream, M, (const double2*)y,
+    hipLaunchKernelGGL(k0_mapply_sq, dim3(gs), dim3(kT), 0, stream, M, (const double2*)y,
                        (double2*)x, part);
-    hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, c.stream, n, x, (const double*)part,
+    hipLaunchKernelGGL(k_power_scale, dim3(gn), dim3(kT), 0, stream, n, x, (const double*)part,
                        gs, st, 1);
   }
   HIP_CHECK(hipGetLastError());
-  double lam = 0.0;
-  HIP_CHECK(hipMemcpyAsync(&lam, st, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
-  return lam;
+  return st;
 }
 
 // Chebyshev coefficients of the two smoothing steps on [lam / 30, lam]
@@ -1683,6 +1685,16 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
   // page-locked copy.  The pattern is complete since build_pattern returned, so the copy runs
   // on the auxiliary stream and the host aggregation overlaps whatever the main stream is
   // still doing (building M, typically); the device work below follows it in stream order.
+  // the check of the learned smoother needs K and M only: it runs on the auxiliary stream,
+  // after the work queued so far (M), while this thread and the main stream build the levels
+  double* bound_dev = nullptr;
+  if (!c.aux) HIP_CHECK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
+  if (fine_smoother == 0 && c.mpnn.built) {
+    if (!mg.bound_ev) HIP_CHECK(hipEventCreateWithFlags(&mg.bound_ev, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(mg.bound_ev, c.stream));
+    HIP_CHECK(hipStreamWaitEvent(c.aux, mg.bound_ev, 0));
+    bound_dev = learned_smoother_bound_enqueue(c, kSmootherSteps, c.aux);
+  }
   MgLevel* L0 = mg.take_level();
   L0->n = c.n;
   L0->nodes = c.nb;
@@ -1702,7 +1714,6 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     L0->nagg = mg.pre_nagg;
     tr.mark(c, 0, "join prepared aggregation");
   } else {
-    if (!c.aux) HIP_CHECK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
     pin = pinned_ints(mg, (size_t)c.nb + 1 + (size_t)c.nblk);
     HIP_CHECK(hipMemcpyAsync(pin, c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost, c.aux));
     HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1, c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost, c.aux));
@@ -1710,7 +1721,16 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     tr.mark(c, 0, "download pattern");
   }
   alloc_work(c, *L0);
-  if (fine_smoother == 1) L0->lam = smoothing_max(c, 0);
+  // smoothing bounds of the levels run on the auxiliary stream, each after its level's matrix
+  // is built on the main stream; the set-up collects them at the end
+  double* lam_dev = grow(mg.lam_dev, kLamSlots);
+  auto bound_async = [&](int level) {
+    if (!mg.lam_ev) HIP_CHECK(hipEventCreateWithFlags(&mg.lam_ev, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(mg.lam_ev, c.stream));
+    HIP_CHECK(hipStreamWaitEvent(c.aux, mg.lam_ev, 0));
+    smoothing_max_enqueue(c, level, c.aux, mg.dwork3, lam_dev + level);
+  };
+  if (fine_smoother == 1) bound_async(0);
   tr.mark(c, 0, "work, smoothing");
   // rigid-body rotation about the centre of the mesh's bounding box
   double xlo = c.h_xy[0], xhi = c.h_xy[0], ylo = c.h_xy[1], yhi = c.h_xy[1];
@@ -1863,17 +1883,26 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
       tr.mark(c, l + 1, "coarsest inverse");
       break;
     }
-    C->lam = smoothing_max(c, l + 1);
+    bound_async(l + 1);
     tr.mark(c, l + 1, "smoothing bound");
   }
   HIP_CHECK(hipStreamSynchronize(c.stream));
+  {
+    // the smoothing bounds (levels below the coarsest; level 0 with the Chebyshev smoother)
+    const int nl = (int)mg.lv.size();
+    std::vector<double> lam(nl, 0.0);
+    HIP_CHECK(hipMemcpyAsync(lam.data(), lam_dev, sizeof(double) * nl, hipMemcpyDeviceToHost, c.aux));
+    HIP_CHECK(hipStreamSynchronize(c.aux));
+    for (int l = fine_smoother == 1 ? 0 : 1; l + 1 < nl; ++l) mg.lv[l]->lam = lam[l];
+  }
   mg.tail_from = tail_start(mg);
   if (mg.coarse_fp32) ensure_vals32(c);
-  if (fine_smoother == 0 && c.mpnn.built) {
+  if (bound_dev) {
     // A good learned smoother has lambda_max(M A) close to 1 (1.14-1.16 on the plates it was
     // trained for); one far from its training data overshoots on some modes.  Such an M is
     // replaced by the Chebyshev smoother before any iteration is spent on it.
-    mg.m_bound = learned_smoother_bound(c, kSmootherSteps);
+    HIP_CHECK(hipMemcpyAsync(&mg.m_bound, bound_dev, sizeof(double), hipMemcpyDeviceToHost, c.aux));
+    HIP_CHECK(hipStreamSynchronize(c.aux));
     if (!(mg.m_bound <= kSmootherAccept)) {
       mg.ready = true;
       mg_set_fine_smoother(c, 1);

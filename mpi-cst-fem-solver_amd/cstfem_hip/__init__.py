@@ -274,6 +274,10 @@ class Device:
 
     # ---- loads and Dirichlet (bc.cpp:137-253)
     def set_rhs(self, b):
+        """The load vector; None sets it to zero on the device."""
+        if b is None:
+            _check(lib().cstfem_set_rhs(self._h, None, int(self.n)))
+            return
         b = _f64(b)
         _check(lib().cstfem_set_rhs(self._h, _ptr(b, ctypes.c_double), int(b.shape[0])))
 

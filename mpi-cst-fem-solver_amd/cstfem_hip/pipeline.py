@@ -175,7 +175,7 @@ def assemble_and_constrain(dev: Device, prob: Problem, times: PhaseTimes | None 
     t1 = time.perf_counter()
     dev.assemble_stiffness()
     t2 = time.perf_counter()
-    dev.set_rhs(np.zeros(dev.n))
+    dev.set_rhs(None)  # f = 0 on the device
     for g, kw in prob.tractions:
         dev.add_traction(prob.mesh.groups[g], **kw)
     if prob.body is not None:

@@ -194,8 +194,7 @@ int run_case(const std::string& case_path, std::ostream& out, std::ostream& err,
 
     // ---- boundary conditions
     t0 = Clock::now();
-    std::vector<double> zero(n, 0.0);
-    check_rc(cstfem_set_rhs(ctx, zero.data(), n));
+    check_rc(cstfem_set_rhs(ctx, nullptr, n));  // f = 0, then the loads
     const std::string traction = c.get_string("bc.traction", "");
     std::optional<std::pair<double, double>> kirsch;  // (sigma_inf, a)
     if (traction == "kirsch_exact") {

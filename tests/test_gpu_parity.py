@@ -203,6 +203,12 @@ def test_zero_rhs_not_converged_bad_diagonal(dev):
     dev.setup()
     x, st = dev.solve(rtol=1e-14, max_iter=2)
     assert not st.converged and st.iterations == 2 and "NOT converged" in st.message
+    # a NULL load vector is the zero vector, set on the device over the previous one
+    dev.set_rhs(None)
+    assert np.all(dev.download_rhs() == 0.0)
+    dev.setup()
+    x, st = dev.solve()
+    assert st.converged and st.iterations == 0 and np.all(x == 0.0)
     dev.upload_csr(*dense_to_csr(np.array([[2.0, 1.0], [1.0, -1.0]])))
     with pytest.raises(cstfem_hip.CstfemError, match="non-positive diagonal"):
         dev.setup()
