@@ -144,11 +144,14 @@ struct MgLevel {
   // block pattern over the nodes (host; ascending columns), device copy for levels >= 1
   std::vector<int> bptr, bcol;
   DBuf<int> d_bptr, d_bcol;
-  // scalar SELL-64 matrix (levels >= 1): rows contiguous, slice s = rows 64s..64s+63
+  // scalar SELL-64 matrix (levels >= 1): rows contiguous, slice s = rows 64s..64s+63; a
+  // row's entries come in triples of one 3x3 block (slice widths are multiples of 3), so the
+  // column indices are stored once per block: entry k of a row in slice s has column
+  // 3 scol[off[s] / 3 + (k / 3) * 64 + lane] + k % 3
   int nslices = 0;
   std::vector<int64_t> h_off;
   DBuf<int64_t> off;
-  DBuf<int> col;
+  DBuf<int> scol;
   DBuf<double> val;
   DBuf<float> val32;  // FP32 copy of val for the cycle's row sums (when enabled)
   DBuf<double> dinv;

@@ -215,13 +215,19 @@ __global__ __launch_bounds__(kT) void k0_cheb(Mat0 A, const double2* __restrict_
 // ---- levels >= 1: scalar SELL-64 ---------------------------------------------------------
 
 // Scalar SELL-64 matrix of a level >= 1 with values of type T (the FP64 values, or an FP32
-// copy for the cycle's row sums)
+// copy for the cycle's row sums).  Entries come in triples of one 3x3 block; bcol holds one
+// block column per triple (value index q = off[s] + k * 64 + lane -> block column index
+// off[s] / 3 + (k / 3) * 64 + lane).
 template <class T>
 struct MatST {
   const int64_t* off;
-  const int* col;
+  const int* bcol;
   const T* val;
   int n;
+  // column of entry k of the row at `lane` of the slice starting at value index o
+  __device__ int col(int64_t o, int k, int lane) const {
+    return 3 * bcol[o / 3 + (int64_t)(k / 3) * 64 + lane] + k % 3;
+  }
 };
 using MatS = MatST<double>;
 using MatS32 = MatST<float>;
@@ -246,17 +252,16 @@ struct XCheb1 {
 // values, then all x entries) so that a row costs two memory round trips per group instead
 // of two per entry.
 template <int G, class XF, class T>
-__device__ inline double row_entries(const MatST<T>& A, const XF& xf, int64_t base, int w, int k0,
-                                     int step, double a) {
+__device__ inline double row_entries(const MatST<T>& A, const XF& xf, int64_t o, int lane, int w,
+                                     int k0, int step, double a) {
   for (int k = k0; k < w; k += G * step) {
     int cidx[G];
     double v[G], xv[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
-      const int kk = k + u * step;
-      const int64_t q = base + (int64_t)(kk < w ? kk : k0) * 64;  // in-row index for the tail
-      cidx[u] = A.col[q];
-      v[u] = (double)A.val[q];
+      const int kk = k + u * step < w ? k + u * step : k0;  // in-row index for the tail
+      cidx[u] = A.col(o, kk, lane);
+      v[u] = (double)A.val[o + (int64_t)kk * 64 + lane];
     }
 #pragma unroll
     for (int u = 0; u < G; ++u) xv[u] = xf(cidx[u]);
@@ -272,7 +277,7 @@ __device__ inline double rowS_f(const MatST<T>& A, const XF& xf, int i) {
   const int s = i >> 6, lane = i & 63;
   const int64_t o = A.off[s];
   const int w = (int)((A.off[s + 1] - o) >> 6);
-  return row_entries<8>(A, xf, o + lane, w, 0, 1, 0.0);
+  return row_entries<8>(A, xf, o, lane, w, 0, 1, 0.0);
 }
 
 __device__ inline double rowS(const MatS& A, const double* __restrict__ x, int i) {
@@ -290,7 +295,7 @@ __device__ inline double rowS_lanes(const MatST<T>& A, const XF& xf, int i, int 
   const int w = (int)((A.off[s + 1] - o) >> 6);
   // rows hold about 21 entries: with 4 or more lanes per row, groups of 4 per lane cover a row
   // in one or two rounds
-  double a = j < w ? row_entries<(LPR >= 4 ? 4 : 8)>(A, xf, o + lane, w, j, LPR, 0.0) : 0.0;
+  double a = j < w ? row_entries<(LPR >= 4 ? 4 : 8)>(A, xf, o, lane, w, j, LPR, 0.0) : 0.0;
 #pragma unroll
   for (int m = 1; m < LPR; m <<= 1) a += __shfl_xor(a, m, 64);
   return a;
@@ -741,7 +746,7 @@ __global__ void ks_dinv(MatS A, double* __restrict__ dinv) {
   double d = 0.0;
   for (int k = 0; k < w; ++k) {
     const int64_t q = o + (int64_t)k * 64 + lane;
-    if (A.col[q] == i) d += A.val[q];
+    if (A.col(o, k, lane) == i) d += A.val[q];
   }
   dinv[i] = d > 0.0 ? 1.0 / d : 0.0;
 }
@@ -908,19 +913,22 @@ __global__ void k_cpat_fill(int na, const int* __restrict__ mptr, const int* __r
 // column indices of the scalar SELL-64 layout of a 3x3-block pattern: lane = row, block
 // column t of block row a gives the columns 3 ccol[t] + c'; padding repeats the row's own
 // column (value 0)
+// block columns of the scalar SELL layout: row r = 3a + c of slice s takes the coarse block
+// columns of aggregate a in order; padding slots (zero values) point at the row's own block
+// (rows past n: the last block), so that every column index stays in range
 __global__ void k_sell_cols(int n, int nslices, const int64_t* __restrict__ off,
                             const int* __restrict__ cptr, const int* __restrict__ ccol,
-                            int* __restrict__ col) {
+                            int* __restrict__ scol) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= 64 * nslices) return;
   const int s = r >> 6, lane = r & 63;
-  const int w = (int)((off[s + 1] - off[s]) >> 6);
+  const int wb = (int)((off[s + 1] - off[s]) >> 6) / 3;
   const int a = r / 3;
   const int nbk = r < n ? cptr[a + 1] - cptr[a] : 0;
-  for (int k = 0; k < w; ++k) {
-    int cidx = r < n ? r : n - 1;
-    if (k < 3 * nbk) cidx = 3 * ccol[cptr[a] + k / 3] + k % 3;
-    col[off[s] + (int64_t)k * 64 + lane] = cidx;
+  for (int k = 0; k < wb; ++k) {
+    int b = (r < n ? r : n - 1) / 3;
+    if (k < nbk) b = ccol[cptr[a] + k];
+    scol[off[s] / 3 + (int64_t)k * 64 + lane] = b;
   }
 }
 
@@ -1093,8 +1101,8 @@ Mat0 mat0(Context& c) {
               c.sell_nrow.get(), c.nslices};
 }
 
-MatS mats(MgLevel& L) { return MatS{L.off.get(), L.col.get(), L.val.get(), L.n}; }
-MatS32 mats32(MgLevel& L) { return MatS32{L.off.get(), L.col.get(), L.val32.get(), L.n}; }
+MatS mats(MgLevel& L) { return MatS{L.off.get(), L.scol.get(), L.val.get(), L.n}; }
+MatS32 mats32(MgLevel& L) { return MatS32{L.off.get(), L.scol.get(), L.val32.get(), L.n}; }
 
 int slice_grid(int nslices) { return std::max(1, std::min(2048, (nslices + kW - 1) / kW)); }
 
@@ -1274,9 +1282,9 @@ Cheb cheb(double lam) {
 void coarsest_inverse(Context& c, MgLevel& L) {
   const int n = L.n;
   std::vector<double> val(L.val.n);
-  std::vector<int> col(L.val.n);
+  std::vector<int> scol(L.scol.n);
   HIP_CHECK(hipMemcpy(val.data(), L.val.get(), sizeof(double) * val.size(), hipMemcpyDeviceToHost));
-  HIP_CHECK(hipMemcpy(col.data(), L.col.get(), sizeof(int) * col.size(), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(scol.data(), L.scol.get(), sizeof(int) * scol.size(), hipMemcpyDeviceToHost));
   std::vector<double> A((size_t)n * n, 0.0);
   for (int s = 0; s < L.nslices; ++s) {
     const int w = (int)((L.h_off[s + 1] - L.h_off[s]) / 64);
@@ -1285,7 +1293,8 @@ void coarsest_inverse(Context& c, MgLevel& L) {
       if (r >= n) continue;
       for (int k = 0; k < w; ++k) {
         const int64_t q = L.h_off[s] + (int64_t)k * 64 + lane;
-        A[(size_t)r * n + col[q]] += val[q];
+        const int col = 3 * scol[L.h_off[s] / 3 + (int64_t)(k / 3) * 64 + lane] + k % 3;
+        A[(size_t)r * n + col] += val[q];
       }
     }
   }
@@ -1835,12 +1844,12 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     sell_offsets(*C, C->bptr);
     upload(C->off, C->h_off);
     const int64_t nent = C->h_off[C->nslices];
-    C->col.alloc(std::max<int64_t>(nent, 1));
+    C->scol.alloc(std::max<int64_t>(nent / 3, 1));
     C->val.alloc(std::max<int64_t>(nent, 1));
     HIP_CHECK(hipMemsetAsync(C->val.get(), 0, sizeof(double) * std::max<int64_t>(nent, 1), c.stream));
     hipLaunchKernelGGL(k_sell_cols, dim3(grid1(64 * (int64_t)C->nslices)), dim3(kT), 0, c.stream, C->n,
                        C->nslices, (const int64_t*)C->off.get(), (const int*)C->d_bptr.get(),
-                       (const int*)C->d_bcol.get(), C->col.get());
+                       (const int*)C->d_bcol.get(), C->scol.get());
     HIP_CHECK(hipGetLastError());
     tr.mark(c, l, "sell layout");
     // Galerkin product
