@@ -369,7 +369,10 @@ void mg_prepare(Context& c);
 void mg_drop_prepared(Context& c);
 // z = B r on the library stream (level-0 vectors of 2*nb doubles); kernels return at once
 // when sc is given and its status is no longer running
-void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* sc);
+// With zq_part (and the learned smoother), the last kernel also leaves the block partials of
+// z'zq there and the count is returned; 0 when they were not formed
+int mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* sc,
+                     const double* zq = nullptr, double* zq_part = nullptr);
 int64_t mg_level_info(Context& c, int level, int* n, int* nagg, double* lam);
 int mg_levels(Context& c);
 // switches the level-0 smoother (0 learned M, 1 Chebyshev; its spectral bound is computed on

@@ -165,12 +165,17 @@ __global__ __launch_bounds__(kT) void k0_resid32(MatM A, const double2* __restri
   }
 }
 
-// out = base + M in   (base == nullptr: out = M in)
+// out = base + M in   (base == nullptr: out = M in); with dpart, also the block partials of
+// out'dq (the flexible CG's z'q, formed where z is written)
 __global__ __launch_bounds__(kT) void k0_msmooth(MatM M, const double2* __restrict__ in,
                                                  const double2* base, double2* out,
+                                                 const double2* __restrict__ dq,
+                                                 double* __restrict__ dpart,
                                                  const PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kW];
   if (stopped(sc)) return;
   const int lane = threadIdx.x & 63;
+  double dot = 0.0;
   for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < M.nslices; s += gridDim.x * kW) {
     double y0, y1;
     rowM_spmv(M, in, s, lane, y0, y1);
@@ -183,7 +188,15 @@ __global__ __launch_bounds__(kT) void k0_msmooth(MatM M, const double2* __restri
         o.y = b.y + o.y;
       }
       out[i] = o;
+      if (dpart) {
+        const double2 qi = dq[i];
+        dot += o.x * qi.x + o.y * qi.y;
+      }
     }
+  }
+  if (dpart) {
+    const double t = block_sum(dot, lds);
+    if (threadIdx.x == 0) dpart[blockIdx.x] = t;
   }
 }
 
@@ -306,21 +319,34 @@ __device__ inline double rowS_lanes(const MatST<T>& A, const XF& xf, int i, int 
 // of loads would set the kernel's duration.
 inline int lanes_per_row(int n) { return n >= 200000 ? 1 : n >= 50000 ? 2 : n >= 10000 ? 4 : 8; }
 
+// With dpart, also the block partials of xout'dq (the K-cycle's z'q of the cycle's result).
 template <int LPR, class T>
 __global__ __launch_bounds__(kT) void ks_cheb(MatST<T> A, const double* __restrict__ xin,
                                               double* __restrict__ xout, double* __restrict__ d,
                                               const double* __restrict__ r,
                                               const double* __restrict__ dinv, double cd, double cr,
+                                              const double* __restrict__ dq,
+                                              double* __restrict__ dpart,
                                               const PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kW];
   if (stopped(sc)) return;
   const int t = blockIdx.x * kT + threadIdx.x;
   const int i = t / LPR, j = t % LPR;
-  if (i >= A.n) return;
-  const double y = xin ? rowS_lanes<LPR>(A, XVec{xin}, i, j) : 0.0;
-  if (j != 0) return;
-  const double dd = (cd != 0.0 ? cd * d[i] : 0.0) + cr * (dinv[i] * (r[i] - y));
-  d[i] = dd;
-  xout[i] = (xin ? xin[i] : 0.0) + dd;
+  double dot = 0.0;
+  if (i < A.n) {
+    const double y = xin ? rowS_lanes<LPR>(A, XVec{xin}, i, j) : 0.0;
+    if (j == 0) {
+      const double dd = (cd != 0.0 ? cd * d[i] : 0.0) + cr * (dinv[i] * (r[i] - y));
+      d[i] = dd;
+      const double xo = (xin ? xin[i] : 0.0) + dd;
+      xout[i] = xo;
+      if (dpart) dot = xo * dq[i];
+    }
+  }
+  if (dpart) {
+    const double s = block_sum(dot, lds);
+    if (threadIdx.x == 0) dpart[blockIdx.x] = s;
+  }
 }
 
 // Both pre-smoothing steps from x = 0 in one pass: the first step's x (= c1r D^-1 rhs) is
@@ -366,14 +392,14 @@ __global__ __launch_bounds__(kT) void ks_spmv_dir(MatST<T> A, const double* __re
                                                   double* __restrict__ p, double* __restrict__ q,
                                                   const double* __restrict__ res,
                                                   double* __restrict__ part,
-                                                  const double* __restrict__ part_zq,
+                                                  const double* __restrict__ part_zq, int nzq,
                                                   const double* __restrict__ part_pq, int np,
                                                   int first, const PcgScalars* __restrict__ sc) {
   __shared__ double lds[kW];
   if (stopped(sc)) return;
   double beta = 0.0;
   if (!first) {
-    const double zq = sum_parts(part_zq, np, lds);
+    const double zq = sum_parts(part_zq, nzq, lds);
     const double pq = sum_parts(part_pq, np, lds);
     beta = pq > 0.0 ? -zq / pq : 0.0;
   }
@@ -1369,7 +1395,8 @@ void alloc_work(Context& c, MgLevel& L) {
 
 void kcycle(Context& c, int l, const PcgScalars* sc);
 
-void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars* sc);
+int cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars* sc,
+            const double* dq = nullptr, double* dpart = nullptr);
 
 // ---- launches with the lane count chosen per level ------------------------------------------
 
@@ -1402,11 +1429,15 @@ void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars
     }                                                                                         \
   }
 
+// grid of ks_cheb on a level (one block partial each when it also forms a dot product)
+int cheb_grid(const MgLevel& L) { return grid1((int64_t)L.n * lanes_per_row(L.n)); }
+
 void launch_cheb(Context& c, MgLevel& L, const double* xin, double* xout, const double* r,
-                 double cd, double cr, const PcgScalars* sc) {
+                 double cd, double cr, const PcgScalars* sc, const double* dq = nullptr,
+                 double* dpart = nullptr) {
   const int lpr = lanes_per_row(L.n);
-  CSTFEM_LPR_SWITCH_M(L, lpr, ks_cheb, grid1((int64_t)L.n * lpr), xin, xout, L.d.get(), r,
-                      (const double*)L.dinv.get(), cd, cr, sc);
+  CSTFEM_LPR_SWITCH_M(L, lpr, ks_cheb, cheb_grid(L), xin, xout, L.d.get(), r,
+                      (const double*)L.dinv.get(), cd, cr, dq, dpart, sc);
 }
 
 void launch_cheb_pre(Context& c, MgLevel& L, double* x2, const double* r, const Cheb& ch,
@@ -1514,10 +1545,14 @@ void coarse_correction(Context& c, int l, const PcgScalars* sc) {
 }
 
 // cycle on level l >= 1: out = B_l(rhs)
-void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars* sc) {
+// With dpart, the last smoothing step also leaves the block partials of out'dq there and the
+// return value is their count; 0 when the partials were not formed (the fused tail; a level
+// whose smoothing grid exceeds the 2 kPart partial slots)
+int cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars* sc,
+            const double* dq, double* dpart) {
   if (l == c.mg.tail_from) {
     launch_tail(c, l, rhs, out, sc);
-    return;
+    return 0;
   }
   MgLevel& L = *c.mg.lv[l];
   MgLevel& C = *c.mg.lv[l + 1];
@@ -1530,7 +1565,10 @@ void cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars
                      (const int*)L.agg.get(), (const double*)L.P.get(), (const double*)C.kx.get(),
                      L.x2.get(), sc);
   launch_cheb(c, L, L.x2.get(), L.x.get(), rhs, 0.0, ch.c1r, sc);
-  launch_cheb(c, L, L.x.get(), out, rhs, ch.c2d, ch.c2r, sc);
+  const bool dot = dpart && cheb_grid(L) <= 2 * kPart;
+  launch_cheb(c, L, L.x.get(), out, rhs, ch.c2d, ch.c2r, sc, dot ? dq : nullptr,
+              dot ? dpart : nullptr);
+  return dot ? cheb_grid(L) : 0;
 }
 
 // two flexible CG steps on level l for the right-hand side in lv[l].kres (overwritten by the
@@ -1542,19 +1580,23 @@ void kcycle(Context& c, int l, const PcgScalars* sc) {
   double* s0 = L.part.get();             // p'q, p'r of step 1
   double* s1 = L.part.get() + 2 * kPart;  // z'q_old
   double* s2 = L.part.get() + 4 * kPart;  // p'q, p'r of step 2
-  cycle_s(c, l, L.kres.get(), L.kz.get(), sc);  // the restriction filled kres
+  cycle_s(c, l, L.kres.get(), L.kz.get(), sc, nullptr, nullptr);  // the restriction filled kres
   CSTFEM_LPR_SWITCH_M(L, lpr, ks_spmv_dir, g, (const double*)L.kz.get(), L.kp.get(), L.kq.get(),
-                      (const double*)L.kres.get(), s0, (const double*)nullptr,
+                      (const double*)L.kres.get(), s0, (const double*)nullptr, 0,
                       (const double*)nullptr, g, 1, sc);
   hipLaunchKernelGGL(ks_update, dim3(g), dim3(kT), 0, c.stream, L.n, L.kx.get(), L.kres.get(),
                      (const double*)L.kp.get(), (const double*)L.kq.get(), (const double*)s0, g, 1, 1,
                      sc);
-  cycle_s(c, l, L.kres.get(), L.kz.get(), sc);
-  hipLaunchKernelGGL(ks_dot, dim3(g), dim3(kT), 0, c.stream, L.n, (const double*)L.kz.get(),
-                     (const double*)L.kq.get(), s1, sc);
+  // z'q_old: formed by the cycle's last smoothing step where it can, else by ks_dot
+  int nzq = cycle_s(c, l, L.kres.get(), L.kz.get(), sc, (const double*)L.kq.get(), s1);
+  if (nzq == 0) {
+    hipLaunchKernelGGL(ks_dot, dim3(g), dim3(kT), 0, c.stream, L.n, (const double*)L.kz.get(),
+                       (const double*)L.kq.get(), s1, sc);
+    nzq = g;
+  }
   CSTFEM_LPR_SWITCH_M(L, lpr, ks_spmv_dir, g, (const double*)L.kz.get(), L.kp.get(), L.kq.get(),
-                      (const double*)L.kres.get(), s2, (const double*)s1, (const double*)s0, g, 0,
-                      sc);
+                      (const double*)L.kres.get(), s2, (const double*)s1, nzq, (const double*)s0,
+                      g, 0, sc);
   hipLaunchKernelGGL(ks_update, dim3(g), dim3(kT), 0, c.stream, L.n, L.kx.get(), L.kres.get(),
                      (const double*)L.kp.get(), (const double*)L.kq.get(), (const double*)s2, g, 0, 0,
                      sc);
@@ -1564,7 +1606,8 @@ void kcycle(Context& c, int l, const PcgScalars* sc) {
 
 // ---------------------------------------------------------------------------------------------
 
-void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* sc) {
+int mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* sc,
+                     const double* zq, double* zq_part) {
   Multigrid& mg = c.mg;
   if (!mg.ready) throw Error(CSTFEM_ERR_ORDER, "multilevel: not set up");
   MgLevel& L = *mg.lv[0];
@@ -1581,7 +1624,7 @@ void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* 
   // pre-smoothing
   if (learned) {
     hipLaunchKernelGGL(k0_msmooth, dim3(gs), dim3(kT), 0, c.stream, M, r2, (const double2*)nullptr,
-                       z2, sc);
+                       z2, (const double2*)nullptr, (double*)nullptr, sc);
   } else {
     hipLaunchKernelGGL(k0_cheb, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)nullptr,
                        (double2*)L.x.get(), (double2*)L.d.get(), r2, (const double2*)c.inv_diag.get(),
@@ -1611,7 +1654,7 @@ void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* 
   if (learned) {
     resid0();
     hipLaunchKernelGGL(k0_msmooth, dim3(gs), dim3(kT), 0, c.stream, M, (const double2*)L.res.get(),
-                       (const double2*)z2, z2, sc);
+                       (const double2*)z2, z2, (const double2*)zq, zq_part, sc);
   } else {
     hipLaunchKernelGGL(k0_cheb, dim3(gs), dim3(kT), 0, c.stream, A, (const double2*)z2,
                        (double2*)L.x.get(), (double2*)L.d.get(), r2, (const double2*)c.inv_diag.get(),
@@ -1620,6 +1663,7 @@ void mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* 
                        (double2*)L.d.get(), r2, (const double2*)c.inv_diag.get(), ch.c2d, ch.c2r, sc);
   }
   HIP_CHECK(hipGetLastError());
+  return learned && zq_part ? gs : 0;
 }
 
 void mg_drop_prepared(Context& c) {

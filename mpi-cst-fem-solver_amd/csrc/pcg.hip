@@ -1327,15 +1327,22 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
   double* Ax = fp32 ? c.work.get() : ax64.get();
   // the multilevel preconditioner (FP64 flexible CG only) or the learned M alone
   const bool multilevel = !fp32 && c.mg.ready;
-  const int nC = multilevel ? gv : gs;  // partials of z'q
+  int nC = multilevel ? gv : gs;  // partials of z'q
   auto precond = [&](int cur, int test) {
     if (multilevel) {
       if (test)
         hipLaunchKernelGGL(k_fcg_test, dim3(1), dim3(kThreads), 0, c.stream, (const double*)pB, gv,
                            cur, c.scal.get());
-      mg_enqueue_apply(c, (const double*)r, (double*)z, c.scal.get());
-      hipLaunchKernelGGL(k_fcg_zq<P>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb, (const V*)z,
-                         (const V*)q, pC, (const PcgScalars*)c.scal.get());
+      // the cycle's last kernel forms the z'q partials where it can (learned smoother)
+      const int nzq = mg_enqueue_apply(c, (const double*)r, (double*)z, c.scal.get(),
+                                       (const double*)q, pC);
+      if (nzq > 0) {
+        nC = nzq;
+      } else {
+        hipLaunchKernelGGL(k_fcg_zq<P>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb, (const V*)z,
+                           (const V*)q, pC, (const PcgScalars*)c.scal.get());
+        nC = gv;
+      }
     } else {
       hipLaunchKernelGGL(k_fcg_precond<P>, dim3(gs), dim3(kThreads), 0, c.stream, G, (const V*)r,
                          (const V*)q, z, (const double*)pB, gv, pC, cur, c.scal.get(), test);
