@@ -128,6 +128,22 @@ __device__ inline void rowM_spmv(const MatM& A, const double2* __restrict__ x, i
   y1 = a1;
 }
 
+// Slices of workgroup b: the 8 XCDs receive workgroups round-robin, so with a grid of 8k
+// workgroups (k >= 8) each XCD sweeps one contiguous eighth of the slices and the gathered x
+// entries of neighbouring rows stay in its L2; other grids interleave
+struct Slices {
+  int first, end, step;
+};
+__device__ inline Slices xcd_slices(int nslices) {
+  const int G = gridDim.x, wave = threadIdx.x >> 6;
+  if (G >= 64 && (G & 7) == 0) {
+    const int xcd = blockIdx.x & 7, idx = blockIdx.x >> 3, per = G >> 3;
+    const int lo = (int)((int64_t)nslices * xcd / 8), hi = (int)((int64_t)nslices * (xcd + 1) / 8);
+    return {lo + idx * kW + wave, hi, per * kW};
+  }
+  return {(int)blockIdx.x * kW + wave, nslices, G * kW};
+}
+
 // res = r - A x
 __global__ __launch_bounds__(kT) void k0_resid(Mat0 A, const double2* __restrict__ x,
                                                const double2* __restrict__ r,
@@ -135,7 +151,8 @@ __global__ __launch_bounds__(kT) void k0_resid(Mat0 A, const double2* __restrict
                                                const PcgScalars* __restrict__ sc) {
   if (stopped(sc)) return;
   const int lane = threadIdx.x & 63;
-  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < A.nslices; s += gridDim.x * kW) {
+  const Slices sl = xcd_slices(A.nslices);
+  for (int s = sl.first; s < sl.end; s += sl.step) {
     double y0, y1;
     row0_spmv(A, x, s, lane, y0, y1);
     if (lane < A.nrow[s]) {
@@ -154,7 +171,8 @@ __global__ __launch_bounds__(kT) void k0_resid32(MatM A, const double2* __restri
                                                  const PcgScalars* __restrict__ sc) {
   if (stopped(sc)) return;
   const int lane = threadIdx.x & 63;
-  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < A.nslices; s += gridDim.x * kW) {
+  const Slices sl = xcd_slices(A.nslices);
+  for (int s = sl.first; s < sl.end; s += sl.step) {
     double y0, y1;
     rowM_spmv(A, x, s, lane, y0, y1);
     if (lane < A.nrow[s]) {
@@ -176,7 +194,8 @@ __global__ __launch_bounds__(kT) void k0_msmooth(MatM M, const double2* __restri
   if (stopped(sc)) return;
   const int lane = threadIdx.x & 63;
   double dot = 0.0;
-  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < M.nslices; s += gridDim.x * kW) {
+  const Slices sl = xcd_slices(M.nslices);
+  for (int s = sl.first; s < sl.end; s += sl.step) {
     double y0, y1;
     rowM_spmv(M, in, s, lane, y0, y1);
     if (lane < M.nrow[s]) {
@@ -209,7 +228,8 @@ __global__ __launch_bounds__(kT) void k0_cheb(Mat0 A, const double2* __restrict_
                                               double cr, const PcgScalars* __restrict__ sc) {
   if (stopped(sc)) return;
   const int lane = threadIdx.x & 63;
-  for (int s = blockIdx.x * kW + (threadIdx.x >> 6); s < A.nslices; s += gridDim.x * kW) {
+  const Slices sl = xcd_slices(A.nslices);
+  for (int s = sl.first; s < sl.end; s += sl.step) {
     double y0 = 0.0, y1 = 0.0;
     if (xin) row0_spmv(A, xin, s, lane, y0, y1);
     if (lane < A.nrow[s]) {
@@ -1130,7 +1150,11 @@ Mat0 mat0(Context& c) {
 MatS mats(MgLevel& L) { return MatS{L.off.get(), L.scol.get(), L.val.get(), L.n}; }
 MatS32 mats32(MgLevel& L) { return MatS32{L.off.get(), L.scol.get(), L.val32.get(), L.n}; }
 
-int slice_grid(int nslices) { return std::max(1, std::min(2048, (nslices + kW - 1) / kW)); }
+int slice_grid(int nslices) {
+  int g = std::min(2048, (nslices + kW - 1) / kW);
+  if (g >= 64) g &= ~7;  // a multiple of 8: one contiguous slice range per XCD
+  return std::max(1, g);
+}
 
 // Greedy aggregation of a node graph (see the file header).
 int aggregate(const int* ptr, const int* col, int n, std::vector<int>& agg) {
