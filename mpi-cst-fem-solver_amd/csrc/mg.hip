@@ -576,6 +576,7 @@ constexpr int kTailLevels = 4;       // cycle levels in the fused tail (plus the
 constexpr int kTailThreads = 1024;
 constexpr int kTailLds = 64 * 1024;  // bytes of LDS for the tail's vectors
 constexpr int kTailLanes = 8;        // lanes per coarse DOF (restriction) and per dense row
+constexpr int kTailRegW = 21;        // row entries a tail thread keeps in registers (7 blocks)
 
 struct TailLevel {
   MatS A;
@@ -603,10 +604,43 @@ struct Tail {
 // prolongation, two steps) on each tail level in turn, with the dense solve at the bottom, in
 // one workgroup: the vectors live in LDS, the matrices are read from L2, and barriers separate
 // the phases.  The restriction and the dense rows use kTailLanes lanes each (butterfly sums).
+// Row i of the tail's top level, loaded once into registers (kReg: that level has at most
+// kTailThreads rows of at most kTailRegW entries): its four row sums then read only LDS.  The
+// sums run over the entries in the order of row_entries, so they are bit-identical to rowS_f.
+struct RegRow {
+  double v[kTailRegW];
+  int b[kTailRegW / 3];
+  int w = 0;
+  __device__ void load(const MatS& A, int i) {
+    const int s = i >> 6, lane = i & 63;
+    const int64_t o = A.off[s];
+    w = (int)((A.off[s + 1] - o) >> 6);
+#pragma unroll
+    for (int k = 0; k < kTailRegW; ++k) v[k] = k < w ? A.val[o + (int64_t)k * 64 + lane] : 0.0;
+#pragma unroll
+    for (int k = 0; k < kTailRegW / 3; ++k)
+      b[k] = 3 * k < w ? A.bcol[o / 3 + (int64_t)k * 64 + lane] : 0;
+  }
+  __device__ double sum(const double* __restrict__ x) const {
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < kTailRegW; ++k)
+      if (k < w) a = a + v[k] * x[3 * b[k / 3] + k % 3];
+    return a;
+  }
+};
+
+template <bool kReg>
 __global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars* __restrict__ sc) {
   extern __shared__ double lds[];
   if (stopped(sc)) return;
   const int tid = threadIdx.x;
+  RegRow rr;
+  if (kReg && tid < T.L[0].A.n) rr.load(T.L[0].A, tid);
+  // row sum of level l's row i: from registers on the top level (kReg), else from L2
+  auto row = [&](int l, const TailLevel& L, const double* x, int i) {
+    return (kReg && l == 0) ? rr.sum(x) : rowS_f(L.A, XVec{x}, i);
+  };
   {
     double* rhs0 = lds + T.L[0].lds;
     for (int i = tid; i < T.L[0].A.n; i += kTailThreads) rhs0[i] = T.rhs[i];
@@ -624,13 +658,13 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars*
     }
     __syncthreads();
     for (int i = tid; i < n; i += kTailThreads) {
-      const double y = rowS_f(L.A, XVec{x}, i);
+      const double y = row(l, L, x, i);
       const double dd = (L.c2d != 0.0 ? L.c2d * d[i] : 0.0) + L.c2r * (L.dinv[i] * (rhs[i] - y));
       d[i] = dd;
       x2[i] = x[i] + dd;
     }
     __syncthreads();
-    for (int i = tid; i < n; i += kTailThreads) res[i] = rhs[i] - rowS_f(L.A, XVec{x2}, i);
+    for (int i = tid; i < n; i += kTailThreads) res[i] = rhs[i] - row(l, L, x2, i);
     __syncthreads();
     double* rc = lds + (l + 1 < T.nlev ? T.L[l + 1].lds : T.lds_c);
     for (int tt = tid; tt < 3 * L.nagg * kTailLanes; tt += kTailThreads) {
@@ -684,7 +718,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars*
     }
     __syncthreads();
     for (int i = tid; i < n; i += kTailThreads) {
-      const double y = rowS_f(L.A, XVec{x2}, i);
+      const double y = row(l, L, x2, i);
       const double dd = 0.0 + L.c1r * (L.dinv[i] * (rhs[i] - y));
       d[i] = dd;
       x[i] = x2[i] + dd;
@@ -692,7 +726,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars*
     __syncthreads();
     double* out = l == 0 ? T.out : res;
     for (int i = tid; i < n; i += kTailThreads) {
-      const double y = rowS_f(L.A, XVec{x}, i);
+      const double y = row(l, L, x, i);
       const double dd = (L.c2d != 0.0 ? L.c2d * d[i] : 0.0) + L.c2r * (L.dinv[i] * (rhs[i] - y));
       d[i] = dd;
       out[i] = x[i] + dd;
@@ -1543,7 +1577,14 @@ void launch_tail(Context& c, int t, const double* rhs, double* out, const PcgSca
   T.rhs = rhs;
   T.out = out;
   const size_t bytes = sizeof(double) * (size_t)tail_lds_doubles(mg, t);
-  hipLaunchKernelGGL(k_tail, dim3(1), dim3(kTailThreads), bytes, c.stream, T, sc);
+  // the top level's rows in registers when they fit (one row per thread, <= kTailRegW entries)
+  const MgLevel& top = *mg.lv[t];
+  bool reg = top.n <= kTailThreads;
+  for (int s = 0; reg && s < top.nslices; ++s) reg = top.h_off[s + 1] - top.h_off[s] <= 64 * kTailRegW;
+  if (reg)
+    hipLaunchKernelGGL(k_tail<true>, dim3(1), dim3(kTailThreads), bytes, c.stream, T, sc);
+  else
+    hipLaunchKernelGGL(k_tail<false>, dim3(1), dim3(kTailThreads), bytes, c.stream, T, sc);
 }
 
 // where the restriction of level l's residual goes: straight into the K-cycle's residual when
