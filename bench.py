@@ -26,6 +26,11 @@ cpu_baseline is the oracle (the CPU restatement of the reference's element loop 
 bitwise equal to the reference on uniform-E inputs) timed on a bounded sample, its PCG on
 all host threads (OMP_NUM_THREADS, else every core up to 16); the one-core figures are kept
 beside it.
+
+--config 3 times the batched workload instead: 512 independent plates (578 to 51,842 DOFs)
+solved by one persistent launch (FP64, and FP32 under FP64 refinement), the launch shape
+picked by the autotuner; its CPU baseline runs the oracle on every 8th plate, plates in
+parallel on the host threads.
 """
 import argparse
 import json
@@ -105,6 +110,112 @@ def cpu_baseline(prob, iterations, sample_iters):
                               value=(t1 - t0) + per_it1 * iterations))
 
 
+def oracle_plate_solve(oc, p, rtol):
+    """One plate through the oracle's restatement of the reference path (pattern, assembly,
+    loads, Dirichlet, Jacobi-PCG), as the reference solves one case; returns iterations."""
+    m = p.mesh
+    rp, c = oc.build_pattern(m.num_nodes, m.tri)
+    v = oc.assemble(m.xy, m.tri, rp, c, E=p.E, nu=p.nu, strain=0, t=p.thickness)
+    f = np.zeros(m.num_dofs)
+    for g, kw in p.tractions:
+        oc.add_traction(m.xy, m.tri, m.groups[g], f, t=p.thickness, **kw)
+    d, val = p.dirichlet_dofs()
+    K, b = oc.apply_dirichlet(rp, c, v, f, d, val)
+    _, st = oc.pcg_jacobi(rp, c, K, b, rtol, 200000)
+    return st["iterations"], len(K)
+
+
+def run_batch(args):
+    """The batched workload (512 independent plates of 578 to 51,842 DOFs): one persistent
+    launch solves every plate to an FP64 true relative residual below 1e-10 (FP64 Jacobi-CG,
+    and FP32 Jacobi-CG under FP64 refinement), with the launch shape picked by the autotuner.
+    The plates are assembled and constrained on the device before the timed region; a step
+    is one solve of the whole batch (device time between HIP events)."""
+    import cstfem_hip
+    from cstfem_hip import pipeline
+    t = time.perf_counter()
+    bp = pipeline.batch_problem()
+    log(f"batch: {bp.count} plates, {bp.mesh.num_dofs} DOFs (generated in "
+        f"{time.perf_counter() - t:.1f} s)")
+    dev = cstfem_hip.Device(0)
+    device_name, cus = dev.info()
+    pipeline.load_batch(dev, bp)
+    best, t_tune = dev.batch_autotune()
+    log(f"autotuned launch shape {best} ({t_tune * 1e3:.1f} ms)")
+    methods = {"fp64": cstfem_hip.METHOD_CG_FP64_REFINED, "fp32": cstfem_hip.METHOD_IR_FP32}
+    results = {}
+    for name, meth in methods.items():
+        for _ in range(args.warmup):
+            dev.batch_solve(method=meth, config=best, download=False)
+        times, res = [], None
+        for _ in range(args.steps):
+            _, res, secs = dev.batch_solve(method=meth, config=best, download=False)
+            times.append(secs)
+        its = [r.iterations for r in res]
+        ok = all(r.converged and r.true_rel_residual < 1e-10 for r in res)
+        results[name] = dict(s=sum(times) / len(times), verified=bool(ok),
+                             iterations_min=min(its), iterations_max=max(its),
+                             iterations_total=int(sum(its)),
+                             worst_true_rel_residual=max(r.true_rel_residual for r in res))
+        log(f"batch {name}: {results[name]['s']:.3f} s, iterations {min(its)}..{max(its)}, "
+            f"verified {ok}")
+    fp64 = results["fp64"]
+    # effective bandwidth of the FP64 solve: per plate and iteration, the SpMV bytes of its
+    # 2x2-block matrix (8 nnz values, nnz / 4 block columns of 4 bytes, x read and y written:
+    # 16 n) plus 11 vector passes of 8 n, summed over the plates' own iteration counts, over
+    # the makespan
+    _, res64, _ = dev.batch_solve(method=methods["fp64"], config=best, download=False)
+    rp, _ = dev.download_pattern()
+    moved = 0.0
+    for k, p in enumerate(bp.plates):
+        n = p.mesh.num_dofs
+        lo = 2 * int(bp.node_off[k])
+        nnz = int(rp[lo + n] - rp[lo])
+        moved += res64[k].iterations * (9.0 * nnz + 104.0 * n)
+    achieved = moved / fp64["s"] / 1e9
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        from concurrent.futures import ThreadPoolExecutor
+        oc = oracle_lib.Oracle(oracle_lib.ORACLE_PATH, "oc")
+        sample = list(range(0, bp.count, 8))
+        threads = cpu_threads()
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as pool:
+            out = list(pool.map(lambda k: oracle_plate_solve(oc, bp.plates[k], 1e-10), sample))
+        t_cpu = time.perf_counter() - t0
+        # scale by the GPU's iterations x DOFs (the work per plate) of all plates over the sample
+        w_all = sum(res64[k].iterations * bp.plates[k].mesh.num_dofs for k in range(bp.count))
+        w_smp = sum(res64[k].iterations * bp.plates[k].mesh.num_dofs for k in sample)
+        cpu = dict(value=t_cpu * w_all / w_smp, unit="s", cores=threads, kind="port",
+                   sample=(f"oracle restatement of the reference path (pattern, assembly, "
+                           f"loads, Dirichlet, Jacobi-PCG to 1e-10) on every 8th plate "
+                           f"({len(sample)} plates, {t_cpu:.1f} s on {threads} threads, plates "
+                           f"in parallel), scaled by the GPU iterations x DOFs of all plates"))
+        log(f"cpu baseline: {len(sample)} plates in {t_cpu:.1f} s -> {cpu['value']:.1f} s")
+    line = {
+        "metric": "batched solve: 512 plates to FP64 true rel residual 1e-10 each (s)",
+        "value": fp64["s"], "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": fp64["s"] * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batch512: {bp.count} plates, {bp.mesh.num_dofs} DOFs, one "
+                               f"persistent launch", "dofs": int(bp.mesh.num_dofs),
+                   "plates": bp.count, "launch_shape": list(best),
+                   "device": device_name, "cus": cus},
+        "methods": results,
+        "roofline": {"bound": "hbm", "kernel": "k_batch_cg (persistent, FP64)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "note": "effective: per plate and iteration 9 nnz + 104 n bytes "
+                             "(SpMV and 11 vector passes), summed over the plates, over the "
+                             "makespan; the largest plate on one workgroup sets the makespan"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    dev.close()
+
+
 def measured_traffic(dofs):
     """HBM bytes per launch of k_cg_spmv<F64> from the committed rocprofv3 --pmc measurement
     (FETCH_SIZE and WRITE_SIZE runs, scaled by the copy-kernel calibration) when it was taken
@@ -152,6 +263,10 @@ def main():
         print(f"bench.py measures one GPU: the whole plate sits in one MI355X's HBM and this "
               f"build has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == 3:
+        run_batch(args)
+        return
 
     import cstfem_hip
     from cstfem_hip import pipeline

@@ -129,7 +129,7 @@ def load_batch(dev: Device, bp: BatchProblem, times: "PhaseTimes | None" = None)
     t2 = time.perf_counter()
     dev.assemble_stiffness()
     t3 = time.perf_counter()
-    dev.set_rhs(np.zeros(dev.n))
+    dev.set_rhs(None)  # f = 0 on the device
     dofs, vals = [], []
     for k, p in enumerate(bp.plates):
         off = int(bp.node_off[k])
