@@ -27,6 +27,7 @@
 //              flexible CG steps preconditioned by the cycle of level l+1) while l+1 <= kdepth,
 //              and one cycle of level l+1 (V-cycle) below that.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -909,6 +910,15 @@ __global__ void k_mgs(int na, int bs, const int* __restrict__ mptr, const int* _
     for (int cc = 0; cc < 3; ++cc) Rn[9 * (size_t)a + 3 * c + cc] = alive[c] ? R[c][cc] : 0.0;
 }
 
+// bandwidth of a node graph with ascending columns: max over rows of i - first and last - i
+__global__ void k_graph_bandwidth(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                                  int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || ptr[i + 1] == ptr[i]) return;
+  const int b = max(i - col[ptr[i]], col[ptr[i + 1] - 1] - i);
+  atomicMax(out, b);
+}
+
 // member lists of the aggregates: counts, offsets (a scan), members in arrival order, then
 // every list sorted ascending (the order of the Gram-Schmidt step, the restriction and the
 // Galerkin sums); err = 1 if an aggregate holds more than 192 DOFs
@@ -1217,6 +1227,63 @@ int aggregate(const int* ptr, const int* col, int n, std::vector<int>& agg) {
   }
   for (int i = 0; i < n; ++i)
     if (agg[i] < 0) agg[i] = na++;
+  return na;
+}
+
+// The same aggregation with the leftover pass (phase 2) on a second thread.  Given the
+// graph's bandwidth B (|i - j| <= B for every edge), phase 1 settles node k once it has
+// passed node k + B (only roots within B of k can claim it), so the leftover pass may treat
+// node i (reading i and its neighbours, all <= i + B) once phase 1 has passed i + 2B; phase 1
+// never reads or writes below its own position minus B again.  The leftover pass runs in node
+// order as in aggregate(), and the new aggregates of phase 3 are numbered in node order after
+// both, so the result is aggregate()'s exactly.
+int aggregate_pipelined(const int* ptr, const int* col, int n, int bandwidth,
+                        std::vector<int>& agg) {
+  if (n < 100000 || bandwidth < 0 || bandwidth > n) return aggregate(ptr, col, n, agg);
+  agg.assign(n, -1);
+  std::atomic<int> progress{0};  // phase 1 has finished the nodes below this
+  std::vector<int> fail;         // leftovers with no aggregated neighbour (phase 3)
+  int* a = agg.data();
+  const int64_t lag = 2 * (int64_t)bandwidth + 1;
+  std::exception_ptr error;
+  std::thread leftovers([&]() {
+    try {
+      int seen = 0;
+      for (int i = 0; i < n; ++i) {
+        while ((int64_t)seen < (int64_t)i + lag && seen < n)
+          seen = progress.load(std::memory_order_acquire);
+        if (a[i] >= 0) continue;
+        int g = -1;
+        for (int q = ptr[i]; q < ptr[i + 1]; ++q)
+          if (a[col[q]] >= 0) {
+            g = a[col[q]];
+            break;
+          }
+        if (g >= 0)
+          a[i] = g;
+        else
+          fail.push_back(i);
+      }
+    } catch (...) {
+      error = std::current_exception();
+    }
+  });
+  int na = 0;
+  for (int i = 0; i < n; ++i) {
+    if ((i & 1023) == 0) progress.store(i, std::memory_order_release);
+    if (a[i] >= 0) continue;
+    bool free_nb = true;
+    for (int q = ptr[i]; q < ptr[i + 1] && free_nb; ++q) free_nb = a[col[q]] < 0;
+    if (!free_nb) continue;
+    for (int q = ptr[i]; q < ptr[i + 1]; ++q) a[col[q]] = na;
+    a[i] = na;
+    ++na;
+  }
+  progress.store(n, std::memory_order_release);
+  leftovers.join();
+  if (error) std::rethrow_exception(error);
+  for (int i : fail)
+    if (a[i] < 0) a[i] = na++;
   return na;
 }
 
@@ -1749,30 +1816,48 @@ void mg_prepare(Context& c) {
   if (!c.aux) HIP_CHECK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
   if (!mg.pre_queued) HIP_CHECK(hipEventCreateWithFlags(&mg.pre_queued, hipEventDisableTiming));
   if (!mg.pre_done) HIP_CHECK(hipEventCreateWithFlags(&mg.pre_done, hipEventDisableTiming));
-  const size_t need = (size_t)c.nb + 1 + (size_t)c.nblk;
+  // staging: row pointers, columns, then the graph's bandwidth
+  const size_t need = (size_t)c.nb + 1 + (size_t)c.nblk + 1;
   // growing the staging frees the old one, which uploads of an earlier set-up may still read
   if (need > mg.pinned_n) HIP_CHECK(hipStreamSynchronize(c.stream));
   int* pin = pinned_ints(mg, need);
+  int* bw = grow(mg.iwork, 1);
+  HIP_CHECK(hipMemsetAsync(bw, 0, sizeof(int), c.stream));
+  hipLaunchKernelGGL(k_graph_bandwidth, dim3(grid1(c.nb)), dim3(kT), 0, c.stream, c.nb,
+                     (const int*)c.brow_ptr.get(), (const int*)c.bcol.get(), bw);
+  HIP_CHECK(hipGetLastError());
   // the download follows the work queued so far on the main stream (the pattern's kernels,
-  // the uploads of an earlier set-up from the same staging)
+  // the bandwidth, the uploads of an earlier set-up from the same staging)
   HIP_CHECK(hipEventRecord(mg.pre_queued, c.stream));
   HIP_CHECK(hipStreamWaitEvent(c.aux, mg.pre_queued, 0));
   HIP_CHECK(hipMemcpyAsync(pin, c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost, c.aux));
   HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1, c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost, c.aux));
+  HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1 + c.nblk, bw, sizeof(int), hipMemcpyDeviceToHost, c.aux));
   HIP_CHECK(hipEventRecord(mg.pre_done, c.aux));
+  const int64_t nblk = c.nblk;
   const int nb = c.nb;
   const hipEvent_t done = mg.pre_done;
   mg.pre_error = nullptr;
   mg.pre_staged = false;
   const size_t room = mg.pinned_n;
-  mg.pre_worker = std::thread([&mg, pin, nb, done, room]() {
+  mg.pre_worker = std::thread([&mg, pin, nb, nblk, done, room]() {
     try {
+      const bool trace = std::getenv("CSTFEM_MG_TRACE") != nullptr;
+      const auto t0 = std::chrono::steady_clock::now();
       HIP_CHECK(hipEventSynchronize(done));
-      mg.pre_nagg = aggregate(pin, pin + nb + 1, nb, mg.pre_agg);
+      const auto t1 = std::chrono::steady_clock::now();
+      mg.pre_nagg = aggregate_pipelined(pin, pin + nb + 1, nb, pin[nb + 1 + nblk], mg.pre_agg);
+      const auto t2 = std::chrono::steady_clock::now();
       // the map into the staging area for its upload, over the node graph (no longer read)
       if ((size_t)nb <= room) {
         std::copy(mg.pre_agg.begin(), mg.pre_agg.end(), pin);
         mg.pre_staged = true;
+      }
+      if (trace) {
+        const auto t3 = std::chrono::steady_clock::now();
+        auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        std::fprintf(stderr, "[mg prepare] download %.3f ms, aggregation %.3f ms, staging %.3f ms\n",
+                     ms(t0, t1), ms(t1, t2), ms(t2, t3));
       }
     } catch (...) {
       mg.pre_error = std::current_exception();
