@@ -144,8 +144,11 @@ def test_prepared_aggregation_is_the_inline_one(dev):
     download on a second stream, greedy pass on a host thread) while assembly and the loads go
     on.  The set-up that takes it must build the same hierarchy, bit for bit, as one that
     aggregates inline.  A new pattern or mesh drops a started aggregation: the set-up after
-    one must not use the stale result."""
-    big, small = hetero(300, e_seed=4, seed=6), hetero(90, e_seed=5, seed=7)
+    one must not use the stale result.  The big plate has more than 100,000 nodes, so the
+    prepared pass runs its leftover phase on a second thread behind the root phase (bounded
+    by the graph's bandwidth), against the sequential pass of the inline set-up."""
+    big, small = hetero(340, e_seed=4, seed=6), hetero(90, e_seed=5, seed=7)
+    assert big.mesh.num_nodes > 100_000
     ref = {}
     for name, prob in (("big", big), ("small", small)):
         pipeline.load_problem(dev, prob)
