@@ -14,6 +14,8 @@
 - a good trained smoother passes the set-up check.
 No reference counterpart exists for this component.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -188,6 +190,38 @@ def test_prepared_aggregation_is_the_inline_one(dev):
         d2.mg_prepare()
     finally:
         d2.close()
+
+
+def renumbered(prob, seed):
+    """The same plate with its nodes in random order: the node graph's bandwidth is then
+    close to the node count."""
+    m = prob.mesh
+    order = np.random.default_rng(seed).permutation(m.num_nodes)  # new node k = old order[k]
+    new_id = np.empty_like(order)
+    new_id[order] = np.arange(order.size)
+    groups = {g: new_id[e].astype(np.int32) for g, e in m.groups.items()}
+    mesh = dataclasses.replace(m, xy=m.xy[order].copy(), tri=new_id[m.tri].astype(np.int32),
+                               groups=groups)
+    return dataclasses.replace(prob, mesh=mesh)
+
+
+@pytest.mark.parametrize("case", ["kirsch", "renumbered"])
+def test_prepared_aggregation_on_other_graphs(dev, case):
+    """The two-thread prepared pass on a graph of another shape (the Kirsch quarter plate,
+    graded around the hole) and on one whose bandwidth is close to its node count (the
+    leftover phase then trails the root phase by nearly the whole graph): the same hierarchy
+    as the sequential inline pass."""
+    prob = (pipeline.config_problem(1, 0.5) if case == "kirsch"
+            else renumbered(hetero(340, e_seed=4, seed=6), 3))
+    assert prob.mesh.num_nodes > 100_000
+    hs = []
+    for prepared in (False, True):
+        pipeline.load_problem(dev, prob)
+        pipeline.assemble_and_constrain(dev, prob, multilevel=prepared)
+        dev.setup()
+        dev.mg_setup(10, 200, 1)
+        hs.append(hierarchy(dev))
+    assert_same_hierarchy(hs[1], hs[0])
 
 
 def test_coarse_level_above_one_k_cycle_grid(dev):
