@@ -111,6 +111,15 @@ int cstfem_set_material(cstfem_ctx* ctx, double E, double nu, int plane, double 
  * constitutive matrix D[9] (assembly.hpp:22); clears any per-element modulus. */
 int cstfem_set_constitutive(cstfem_ctx* ctx, const double* D, double thickness);
 
+/* ---- element stiffness ---------------------------------------------------------------- */
+/* Replaces cst_stiffness(xy, D, thickness) (element.hpp:27, element.cpp:37-55) for `count`
+ * triangles at once: xy[6*count] holds (x1, y1, x2, y2, x3, y3) per triangle (TriCoords),
+ * D[9] the row-major constitutive matrix (constitutive_matrix, element.cpp:7-15), and
+ * Ke[36*count] receives each row-major 6x6 Ke = t |A| B^T D B, upper triangle computed as
+ * the reference computes it and mirrored, so equal to it bit for bit. */
+int cstfem_element_stiffness(cstfem_ctx* ctx, int64_t count, const double* xy, const double* D,
+                             double thickness, double* Ke);
+
 /* ---- pattern and assembly ------------------------------------------------------------ */
 /* Replaces build_stiffness_pattern (assembly.cpp:9-42): builds the pattern on the device,
  * plus the element-to-slot map used by assembly.  *nnz_out = scalar nnz. */
@@ -135,12 +144,23 @@ int cstfem_download_rhs(cstfem_ctx* ctx, double* b, int n);
 /* Adds the consistent nodal load of a uniform body force (bx, by) per unit volume,
  * t*|A|/3 per node of every element, in element order (no reference counterpart). */
 int cstfem_add_body_force(cstfem_ctx* ctx, double bx, double by);
-/* Replaces add_traction (bc.cpp:190-253) for one line group: 2-point Gauss per edge,
- * outward normal from the adjacent triangle.  mode 0: constant (p0, p1) = (tx, ty);
- * mode 1: Kirsch traction with (p0, p1) = (sigma_inf, a) (kirsch.cpp:40-45).
- * Computed on the host from the host mesh, added to the device RHS. */
+/* Replaces add_traction(mesh, group, thickness, tx, ty, f) (bc.cpp:190-253, bc.hpp:73) for
+ * one line group given as nedge node pairs edges[2*nedge]: 2-point Gauss per edge, outward
+ * normal from the adjacent triangle, accumulated into the device RHS in the reference's edge
+ * order.  mode 0: constant (p0, p1) = (tx, ty), entirely on the device; mode 1: the Kirsch
+ * traction with (p0, p1) = (sigma_inf, a) (kirsch.cpp:40-45), evaluated on the host at the
+ * Gauss points the device computed.  An edge not adjacent to exactly one triangle leaves the
+ * RHS unchanged and returns CSTFEM_ERR_TRACTION_EDGE. */
 int cstfem_add_traction(cstfem_ctx* ctx, int nedge, const int* edges, int mode, double p0,
                         double p1);
+/* The reference's TractionFn (bc.hpp:65): traction (tx, ty) at point (x, y) with outward unit
+ * normal (nx, ny), called once per add_traction with all 2*nedge Gauss points:
+ * pts[4*count] = (x, y, nx, ny) per point, t[2*count] = (tx, ty) to be written. */
+typedef void (*cstfem_traction_fn)(void* user, int count, const double* pts, double* t);
+/* Replaces add_traction(mesh, group, thickness, const TractionFn&, f) (bc.hpp:71-72): the
+ * geometry, quadrature and accumulation run on the device, fn on the host. */
+int cstfem_add_traction_fn(cstfem_ctx* ctx, int nedge, const int* edges, cstfem_traction_fn fn,
+                           void* user);
 /* Replaces apply_dirichlet (bc.cpp:137-166) on the device matrix and RHS.  dofs/values:
  * m prescribed DOFs; a repeated DOF must repeat its value (DirichletDofs::add, bc.cpp:107). */
 int cstfem_apply_dirichlet(cstfem_ctx* ctx, int m, const int* dofs, const double* values);

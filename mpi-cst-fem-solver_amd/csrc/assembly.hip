@@ -6,6 +6,7 @@
 // elements that share its two nodes in ascending element order starting from 0.0 -- the
 // order in which assemble_stiffness's element loop adds them -- with no FMA contraction
 // (this file is compiled with -ffp-contract=off), so every value equals the reference's.
+#include <algorithm>
 #include <climits>
 
 #include "internal.hpp"
@@ -355,6 +356,8 @@ void build_sell(Context& c) {
   HIP_CHECK(hipStreamSynchronize(c.stream));
   c.h_sell_off.assign(nslices + 1, 0);
   for (int s = 0; s < nslices; ++s) c.h_sell_off[s + 1] = c.h_sell_off[s] + (int64_t)w[s] * kSlice;
+  c.max_row_blocks = nslices ? *std::max_element(w.begin(), w.end()) : 0;
+  c.pattern_symmetric = true;  // node adjacency through triangles is symmetric
   c.nslices = nslices;
   c.sell_blocks = c.h_sell_off[nslices];
   c.sell_off.alloc(nslices + 1);
@@ -553,6 +556,26 @@ __global__ void k_assemble(const int* __restrict__ row0, const int* __restrict__
   }
 }
 
+// cst_stiffness (element.cpp:37-55) of a batch of triangles: one thread per 2x2 block (a, b)
+// of one element; Ke is row-major 6x6 per element
+__global__ void k_element_stiffness(int64_t count, const double* __restrict__ xy, Mat3 Dm,
+                                    double t, double* __restrict__ Ke) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= 9 * count) return;
+  const int64_t e = g / 9;
+  const int ab = (int)(g - 9 * e), a = ab / 3, b = ab - 3 * a;
+  double exy[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) exy[i] = xy[6 * e + i];
+  double kb[4];
+  ke_block(exy, Dm.d, t, a, b, kb);
+  double* o = Ke + 36 * e;
+  o[(2 * a) * 6 + 2 * b] = kb[0];
+  o[(2 * a) * 6 + 2 * b + 1] = kb[1];
+  o[(2 * a + 1) * 6 + 2 * b] = kb[2];
+  o[(2 * a + 1) * 6 + 2 * b + 1] = kb[3];
+}
+
 // SELL values -> the reference's CSR order
 __global__ void k_export_csr(int nb, const int* __restrict__ brow_ptr,
                              const int64_t* __restrict__ spos, const double* __restrict__ val,
@@ -586,8 +609,25 @@ void assemble_values(Context& c) {
   HIP_CHECK(hipGetLastError());
   c.setup_done = false;
   c.vals32_valid = false;
-  c.fixed.release();  // a fresh matrix has no constrained DOFs
+  c.fixed_valid = false;  // a fresh matrix has no constrained DOFs
   c.mpnn.built = false;
+}
+
+void element_stiffness(Context& c, int64_t count, const double* xy, const double* D, double t,
+                       double* Ke) {
+  if (count == 0) return;
+  Mat3 Dm;
+  for (int i = 0; i < 9; ++i) Dm.d[i] = D[i];
+  // staging in the context's buffers, grown on demand and kept
+  if (c.ke_io.n < (size_t)(42 * count)) c.ke_io.alloc(42 * (size_t)count);
+  double* d_xy = c.ke_io.get();
+  double* d_ke = d_xy + 6 * count;
+  HIP_CHECK(hipMemcpyAsync(d_xy, xy, sizeof(double) * 6 * count, hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL(k_element_stiffness, dim3(grid_for(9 * count, 256, 0)), dim3(256), 0,
+                     c.stream, count, (const double*)d_xy, Dm, t, d_ke);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(Ke, d_ke, sizeof(double) * 36 * count, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
 }
 
 void download_values(Context& c, double* vals) {

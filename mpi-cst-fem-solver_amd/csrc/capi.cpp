@@ -55,41 +55,6 @@ void need(bool ok, const std::string& msg) {
 
 void use_device(Context* c) { HIP_CHECK(hipSetDevice(c->device)); }
 
-std::uint64_t edge_key(int a, int b) {
-  if (a > b) std::swap(a, b);
-  return (static_cast<std::uint64_t>(a) << 32) | static_cast<std::uint32_t>(b);
-}
-
-// kirsch.cpp:7-45: Kirsch stress at (x, y) and the traction sigma.n
-void kirsch_traction(double sigma_inf, double a, double x, double y, double nx, double ny,
-                     double t[2]) {
-  const double r = std::hypot(x, y), theta = std::atan2(y, x);
-  const double q2 = (a * a) / (r * r), q4 = q2 * q2;
-  const double c2 = std::cos(2.0 * theta), s2 = std::sin(2.0 * theta);
-  const double s = sigma_inf;
-  const double rr = 0.5 * s * (1.0 - q2) + 0.5 * s * (1.0 - 4.0 * q2 + 3.0 * q4) * c2;
-  const double tt = 0.5 * s * (1.0 + q2) - 0.5 * s * (1.0 + 3.0 * q4) * c2;
-  const double rt = -0.5 * s * (1.0 + 2.0 * q2 - 3.0 * q4) * s2;
-  const double c = std::cos(theta), n = std::sin(theta);
-  const double sxx = rr * c * c + tt * n * n - 2.0 * rt * n * c;
-  const double syy = rr * n * n + tt * c * c + 2.0 * rt * n * c;
-  const double txy = (rr - tt) * n * c + rt * (c * c - n * n);
-  t[0] = sxx * nx + txy * ny;
-  t[1] = txy * nx + syy * ny;
-}
-
-__global__ void k_gather(int m, const int* __restrict__ idx, const double* __restrict__ src,
-                         double* __restrict__ dst) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < m) dst[k] = src[idx[k]];
-}
-
-__global__ void k_scatter(int m, const int* __restrict__ idx, const double* __restrict__ src,
-                          double* __restrict__ dst) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < m) dst[idx[k]] = src[k];
-}
-
 }  // namespace
 }  // namespace cstfem_hip
 
@@ -177,10 +142,8 @@ int cstfem_upload_mesh(cstfem_ctx* ctx, int nn, const double* xy, int64_t ne, co
     c.have_pattern = c.mesh_pattern = c.setup_done = c.vals32_valid = false;
     c.has_E_elem = c.has_nu_elem = false;
     c.h_segments.clear();
-    c.h_ne_ptr.clear();
-    c.h_ne_list.clear();
     c.batch_count = 0;
-    c.fixed.release();
+    c.fixed_valid = false;
     c.mpnn.built = false;
     c.ne_ptr.release();
     c.ne_list.release();
@@ -230,6 +193,18 @@ int cstfem_set_constitutive(cstfem_ctx* ctx, const double* D, double thickness) 
     }
     c.thick = thickness;
     c.has_E_elem = c.has_nu_elem = false;
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_element_stiffness(cstfem_ctx* ctx, int64_t count, const double* xy, const double* D,
+                             double thickness, double* Ke) {
+  return guarded([&] {
+    need(ctx && D && (count == 0 || (xy && Ke)), "cstfem_element_stiffness: NULL argument");
+    need(count >= 0 && count <= (int64_t(1) << 40) / 42, "cstfem_element_stiffness: bad count");
+    Context& c = ctx->c;
+    use_device(&c);
+    element_stiffness(c, count, xy, D, thickness, Ke);
     return CSTFEM_OK;
   });
 }
@@ -329,6 +304,20 @@ int cstfem_upload_csr(cstfem_ctx* ctx, int n, const int* row_ptr, const int* col
       }
     }
     const int64_t nblk = bc.size();
+    // structural symmetry of the block pattern: Dirichlet elimination then visits only the
+    // rows named in the constrained nodes' rows, otherwise every row
+    bool symmetric = true;
+    int max_row = 0;
+    for (int I = 0; I < nb && symmetric; ++I) {
+      for (int q = bp[I]; q < bp[I + 1]; ++q) {
+        const int J = bc[q];
+        if (!std::binary_search(bc.begin() + bp[J], bc.begin() + bp[J + 1], I)) {
+          symmetric = false;
+          break;
+        }
+      }
+    }
+    for (int I = 0; I < nb; ++I) max_row = std::max(max_row, bp[I + 1] - bp[I]);
     // SELL-64 layout
     const int nslices = (nb + kSlice - 1) / kSlice;
     std::vector<int64_t> off(nslices + 1, 0);
@@ -411,6 +400,9 @@ int cstfem_upload_csr(cstfem_ctx* ctx, int n, const int* row_ptr, const int* col
     c.have_pattern = true;
     c.mesh_pattern = false;
     c.generic = true;
+    c.pattern_symmetric = symmetric;
+    c.max_row_blocks = max_row;
+    c.fixed_valid = false;
     c.setup_done = false;
     c.vals32_valid = false;
     c.b.release();
@@ -465,102 +457,35 @@ int cstfem_add_traction(cstfem_ctx* ctx, int nedge, const int* edges, int mode, 
                         double p1) {
   return guarded([&] {
     need(ctx && (nedge == 0 || edges), "cstfem_add_traction: NULL argument");
+    need(nedge >= 0, "cstfem_add_traction: negative edge count");
     Context& c = ctx->c;
     use_device(&c);
     need(c.nn > 0 && c.have_pattern && c.mesh_pattern, "cstfem_add_traction: build the mesh pattern first");
     need(mode == 0 || mode == 1, "cstfem_add_traction: mode must be 0 (constant) or 1 (kirsch)");
     ensure_vectors(c);
-    const std::vector<double>& xy = c.h_xy;
-    const std::vector<int>& tri = c.h_tri;
-    // triangles adjacent to each group edge (bc.cpp:194-211): the triangles holding both
-    // nodes, found through a host node -> elements map built once per mesh; the opposite node
-    // is taken from the last such triangle in element order, as the reference's scan does
-    if (c.h_ne_ptr.empty()) {
-      c.h_ne_ptr.assign(c.nn + 1, 0);
-      for (int64_t k = 0; k < 3 * c.ne; ++k) ++c.h_ne_ptr[tri[k] + 1];
-      for (int i = 0; i < c.nn; ++i) c.h_ne_ptr[i + 1] += c.h_ne_ptr[i];
-      c.h_ne_list.assign(3 * c.ne, 0);
-      std::vector<int> fill(c.h_ne_ptr.begin(), c.h_ne_ptr.end() - 1);
-      for (int64_t e = 0; e < c.ne; ++e)
-        for (int a = 0; a < 3; ++a) c.h_ne_list[fill[tri[3 * e + a]]++] = (int)e;
-    }
-    std::unordered_map<std::uint64_t, int> opposite, count;
-    for (int k = 0; k < nedge; ++k) {
-      const int a = edges[2 * k], b = edges[2 * k + 1];
-      need(a >= 0 && a < c.nn && b >= 0 && b < c.nn, "cstfem_add_traction: edge node out of range");
-      const auto key = edge_key(a, b);
-      if (count.count(key)) continue;
-      int cnt = 0, opp = -1;
-      for (int q = c.h_ne_ptr[a]; q < c.h_ne_ptr[a + 1]; ++q) {
-        const int* nd = &tri[3 * (int64_t)c.h_ne_list[q]];
-        if (nd[0] != b && nd[1] != b && nd[2] != b) continue;
-        if (a == b) continue;
-        ++cnt;
-        for (int i = 0; i < 3; ++i)
-          if (nd[i] != a && nd[i] != b) opp = nd[i];
-      }
-      opposite[key] = opp;
-      count[key] = cnt;
-    }
-    // touched DOFs in first-touch order; accumulation continues from the device values
-    std::vector<int> dofs;
-    std::unordered_map<int, int> slot;
-    auto touch = [&](int dof) {
-      if (slot.emplace(dof, (int)dofs.size()).second) dofs.push_back(dof);
-    };
-    for (int k = 0; k < nedge; ++k) {
-      touch(2 * edges[2 * k]);
-      touch(2 * edges[2 * k] + 1);
-      touch(2 * edges[2 * k + 1]);
-      touch(2 * edges[2 * k + 1] + 1);
-    }
-    const int m = (int)dofs.size();
-    if (m == 0) return CSTFEM_OK;
-    std::vector<double> f(m);
-    DBuf<int> d_idx;
-    DBuf<double> d_val;
-    d_idx.alloc(m);
-    d_val.alloc(m);
-    HIP_CHECK(hipMemcpyAsync(d_idx.get(), dofs.data(), sizeof(int) * m, hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_gather, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, d_idx.get(),
-                       (const double*)c.b.get(), d_val.get());
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(f.data(), d_val.get(), sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
-    HIP_CHECK(hipStreamSynchronize(c.stream));
-    const double g = 1.0 / std::sqrt(3.0);
-    const double xi[2] = {-g, g};
-    for (int k = 0; k < nedge; ++k) {
-      const int a = edges[2 * k], b = edges[2 * k + 1];
-      const auto key = edge_key(a, b);
-      if (count[key] != 1)
-        throw Error(CSTFEM_ERR_TRACTION_EDGE,
-                    "traction group: edge (" + std::to_string(a) + ", " + std::to_string(b) +
-                        ") is adjacent to " + std::to_string(count[key]) + " triangles (expected 1)");
-      const double xa = xy[2 * a], ya = xy[2 * a + 1], xb = xy[2 * b], yb = xy[2 * b + 1];
-      const double len = std::hypot(xb - xa, yb - ya);
-      double nx = (yb - ya) / len, ny = -(xb - xa) / len;
-      const int cn = opposite[key];
-      if (nx * (xy[2 * cn] - xa) + ny * (xy[2 * cn + 1] - ya) > 0.0) {
-        nx = -nx;
-        ny = -ny;
-      }
-      for (double q : xi) {
-        const double Na = 0.5 * (1.0 - q), Nb = 0.5 * (1.0 + q);
-        const double x = Na * xa + Nb * xb, y = Na * ya + Nb * yb;
-        double tr[2] = {p0, p1};
-        if (mode == 1) kirsch_traction(p0, p1, x, y, nx, ny, tr);
-        const double w = c.thick * 0.5 * len;
-        f[slot[2 * a]] += w * Na * tr[0];
-        f[slot[2 * a + 1]] += w * Na * tr[1];
-        f[slot[2 * b]] += w * Nb * tr[0];
-        f[slot[2 * b + 1]] += w * Nb * tr[1];
-      }
-    }
-    HIP_CHECK(hipMemcpyAsync(d_val.get(), f.data(), sizeof(double) * m, hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_scatter, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, d_idx.get(),
-                       (const double*)d_val.get(), c.b.get());
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(c.stream));
+    TractionSource src;
+    src.kind = mode == 0 ? TractionSource::kConstant : TractionSource::kKirsch;
+    src.p0 = p0;
+    src.p1 = p1;
+    add_traction_device(c, nedge, edges, src);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_add_traction_fn(cstfem_ctx* ctx, int nedge, const int* edges, cstfem_traction_fn fn,
+                           void* user) {
+  return guarded([&] {
+    need(ctx && fn && (nedge == 0 || edges), "cstfem_add_traction_fn: NULL argument");
+    need(nedge >= 0, "cstfem_add_traction_fn: negative edge count");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.nn > 0 && c.have_pattern && c.mesh_pattern, "cstfem_add_traction_fn: build the mesh pattern first");
+    ensure_vectors(c);
+    TractionSource src;
+    src.kind = TractionSource::kFunction;
+    src.fn = fn;
+    src.user = user;
+    add_traction_device(c, nedge, edges, src);
     return CSTFEM_OK;
   });
 }

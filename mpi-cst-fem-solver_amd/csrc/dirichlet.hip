@@ -1,11 +1,21 @@
-// dirichlet.hip -- Dirichlet elimination and body-force load on the device.
+// dirichlet.hip -- Dirichlet elimination on the device.
 //
-// Reference: src/bc.cpp:137-166 (apply_dirichlet).  One thread owns a block row (the two
-// scalar rows 2n, 2n+1) and visits its blocks in ascending column order, so the
-// subtractions b_i -= K_ik u_k happen in the reference's column order and every value of
-// K and b equals the reference's.  The pattern is kept (eliminated entries stay stored as
-// zeros).
+// Reference: src/bc.cpp:137-166 (apply_dirichlet).  Only the rows the elimination can change
+// are visited: the rows of constrained DOFs and the rows holding a constrained column.  On a
+// structurally symmetric pattern those are the block rows listed in the constrained nodes'
+// own rows, so the work is proportional to the constraints, not to the matrix:
+//   k_dir_mark   one thread per constrained DOF sets its bit in a DOF bitmap and appends the
+//                block rows of its node's row (itself included) to a compact list, each once
+//                (a block-row bitmap decides who appends);
+//   k_dir_rows   one thread per scalar row of the list visits its blocks in ascending column
+//                order, so the subtractions b_i -= K_ik u_k happen in the reference's column
+//                order and every value of K and b equals the reference's; the pattern is kept
+//                (eliminated entries stay stored as zeros);
+//   k_dir_clear  clears the bits it set, so the bitmaps are zero again for the next call.
+// A pattern that is not structurally symmetric (a generic CSR upload) visits every row.
+#include <algorithm>
 #include <climits>
+#include <mutex>
 
 #include "internal.hpp"
 
@@ -13,30 +23,89 @@ namespace cstfem_hip {
 
 namespace {
 
-__global__ void k_mark_fixed(int m, const int* __restrict__ dofs, const double* __restrict__ val,
-                             unsigned char* __restrict__ fixed, double* __restrict__ ubar,
-                             unsigned char* __restrict__ all_fixed) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  fixed[dofs[k]] = 1;
-  all_fixed[dofs[k]] = 1;
-  ubar[dofs[k]] = val[k];
+__global__ void k_dir_begin(int* __restrict__ ctr) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctr[0] = 0;        // block rows listed
+    ctr[1] = 0;        // list overflow
+    ctr[2] = INT_MAX;  // smallest constrained DOF with a non-positive diagonal
+  }
 }
 
-__global__ void k_dirichlet(int nb, int n, const int* __restrict__ brow_ptr,
-                            const int* __restrict__ bcol, const int* __restrict__ bdiag,
-                            const int64_t* __restrict__ spos, double* __restrict__ val,
-                            const unsigned char* __restrict__ fixed,
-                            const double* __restrict__ ubar, double* __restrict__ b,
-                            int* __restrict__ err) {
-  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= nb) return;
-  const int q0 = brow_ptr[nd], q1 = brow_ptr[nd + 1];
-  for (int c = 0; c < 2; ++c) {
-    const int i = 2 * nd + c;
-    if (i >= n) break;
-    if (fixed[i]) {
-      const int qd = bdiag[nd];
+__global__ void k_zero_bytes(unsigned char* __restrict__ p, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+__device__ inline bool test_bit(const unsigned* __restrict__ bits, int i) {
+  return (bits[i >> 5] >> (i & 31)) & 1u;
+}
+
+// appends block row r to the list unless it is there already
+__device__ inline void list_row(int r, unsigned* __restrict__ row_bits, int* __restrict__ rows,
+                                int* __restrict__ ctr, int cap) {
+  const unsigned bit = 1u << (r & 31);
+  const unsigned old = atomicOr(&row_bits[r >> 5], bit);
+  if (!(old & bit)) {
+    const int idx = atomicAdd(&ctr[0], 1);
+    if (idx < cap)
+      rows[idx] = r;
+    else
+      ctr[1] = 1;
+  }
+}
+
+__global__ void k_dir_mark(int m, const int* __restrict__ dofs, const int* __restrict__ brow_ptr,
+                           const int* __restrict__ bcol, int all_rows,
+                           unsigned* __restrict__ dof_bits, unsigned* __restrict__ row_bits,
+                           int* __restrict__ rows, int* __restrict__ ctr, int cap,
+                           unsigned char* __restrict__ fixed_union) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const int dof = dofs[k];
+  atomicOr(&dof_bits[dof >> 5], 1u << (dof & 31));
+  fixed_union[dof] = 1;
+  if (all_rows) return;
+  const int v = dof >> 1;
+  list_row(v, row_bits, rows, ctr, cap);  // its own row, even without a stored diagonal
+  for (int q = brow_ptr[v]; q < brow_ptr[v + 1]; ++q) list_row(bcol[q], row_bits, rows, ctr, cap);
+}
+
+// prescribed value of constrained DOF j: binary search in the sorted list
+__device__ inline double ubar(int j, const int* __restrict__ dofs, const double* __restrict__ vals,
+                              int m) {
+  int lo = 0, hi = m;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (dofs[mid] < j)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return vals[lo];
+}
+
+constexpr int kDirChunk = 8;  // blocks of a row whose column and bitmap loads are issued together
+
+__global__ void k_dir_rows(int nb, int n, int all_rows, const int* __restrict__ rows,
+                           const int* __restrict__ ctr, int cap, const int* __restrict__ brow_ptr,
+                           const int* __restrict__ bcol, const int* __restrict__ bdiag,
+                           const int64_t* __restrict__ spos, double* __restrict__ val,
+                           const unsigned* __restrict__ dof_bits, const int* __restrict__ dofs,
+                           const double* __restrict__ vals, int m, double* __restrict__ b,
+                           int* __restrict__ err) {
+  if (!all_rows && ctr[1]) return;  // the list overflowed: change nothing
+  const int64_t cnt = all_rows ? nb : min(ctr[0], cap);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < 2 * cnt;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int r = all_rows ? (int)(t >> 1) : rows[t >> 1];
+    const int c = (int)(t & 1);
+    const int i = 2 * r + c;
+    if (i >= n) continue;
+    const int q0 = brow_ptr[r], q1 = brow_ptr[r + 1];
+    if (test_bit(dof_bits, i)) {
+      // constrained row: off-diagonal entries to zero, b_i = K_ii u_i
+      const int qd = bdiag[r];
       const double diag = qd >= 0 ? val[sell_vidx(spos[qd], 3 * c)] : 0.0;
       if (!(diag > 0.0)) {
         atomicMin(err, i);
@@ -44,22 +113,35 @@ __global__ void k_dirichlet(int nb, int n, const int* __restrict__ brow_ptr,
       }
       for (int q = q0; q < q1; ++q) {
         double* v = val + sell_vidx(spos[q], 2 * c);  // row c: v[0], v[1]
-        const int m = bcol[q];
-        if (2 * m != i) v[0] = 0.0;
-        if (2 * m + 1 != i) v[1] = 0.0;
+        const int mc = bcol[q];
+        if (2 * mc != i) v[0] = 0.0;
+        if (2 * mc + 1 != i) v[1] = 0.0;
       }
-      b[i] = diag * ubar[i];
+      b[i] = diag * ubar(i, dofs, vals, m);
     } else {
+      // free row: b_i -= K_ik u_k and K_ik = 0 for every constrained column, in column order
       double bi = b[i];
-      for (int q = q0; q < q1; ++q) {
-        const int m = bcol[q];
-        double* v = val + sell_vidx(spos[q], 2 * c);  // row c: v[0], v[1]
+      for (int q = q0; q < q1; q += kDirChunk) {
+        int mc[kDirChunk];
+        unsigned w[kDirChunk];
 #pragma unroll
-        for (int d = 0; d < 2; ++d) {
-          const int j = 2 * m + d;
-          if (j < n && fixed[j]) {
-            bi -= v[d] * ubar[j];
-            v[d] = 0.0;
+        for (int u = 0; u < kDirChunk; ++u) mc[u] = q + u < q1 ? bcol[q + u] : -1;
+#pragma unroll
+        for (int u = 0; u < kDirChunk; ++u) w[u] = mc[u] >= 0 ? dof_bits[mc[u] >> 4] : 0u;
+#pragma unroll
+        for (int u = 0; u < kDirChunk; ++u) {
+          if (mc[u] < 0) continue;
+          const int j0 = 2 * mc[u];  // even: j0 and j0 + 1 share a bitmap word
+          const unsigned f = (w[u] >> (j0 & 31)) & 3u;
+          if (!f) continue;
+          double* v = val + sell_vidx(spos[q + u], 2 * c);
+          if (f & 1u) {
+            bi -= v[0] * ubar(j0, dofs, vals, m);
+            v[0] = 0.0;
+          }
+          if (f & 2u) {
+            bi -= v[1] * ubar(j0 + 1, dofs, vals, m);
+            v[1] = 0.0;
           }
         }
       }
@@ -68,97 +150,156 @@ __global__ void k_dirichlet(int nb, int n, const int* __restrict__ brow_ptr,
   }
 }
 
-// t*|A|/3*(bx, by) to each node of every element, per node in ascending element order
-__global__ void k_body_force(int nn, const int* __restrict__ ptr, const int* __restrict__ list,
-                             const int* __restrict__ tri, const double* __restrict__ xy,
-                             double t, double bx, double by, double* __restrict__ f) {
-  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= nn) return;
-  double fx = f[2 * nd], fy = f[2 * nd + 1];
-  for (int k = ptr[nd]; k < ptr[nd + 1]; ++k) {
-    const int e = list[k];
-    const int n0 = tri[3 * e], n1 = tri[3 * e + 1], n2 = tri[3 * e + 2];
-    const double x1 = xy[2 * n0], y1 = xy[2 * n0 + 1], x2 = xy[2 * n1], y2 = xy[2 * n1 + 1];
-    const double x3 = xy[2 * n2], y3 = xy[2 * n2 + 1];
-    const double area = 0.5 * ((x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1));
-    const double w = t * fabs(area) / 3.0;
-    fx += w * bx;
-    fy += w * by;
+__global__ void k_dir_clear(int m, const int* __restrict__ dofs, const int* __restrict__ ctr,
+                            int cap, const int* __restrict__ rows, int64_t row_words,
+                            unsigned* __restrict__ dof_bits, unsigned* __restrict__ row_bits) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (int64_t k = t0; k < m; k += stride) dof_bits[dofs[k] >> 5] = 0u;
+  const int listed = min(ctr[0], cap);
+  for (int64_t k = t0; k < listed; k += stride) row_bits[rows[k] >> 5] = 0u;
+  if (ctr[1])  // rows beyond the list's capacity set bits it does not name
+    for (int64_t k = t0; k < row_words; k += stride) row_bits[k] = 0u;
+}
+
+// Device pointers of one elimination's workspace: the sorted constrained DOFs and values, the
+// two bitmaps (zero between calls), the block-row list and three counters.
+struct DirWork {
+  int* ctr;
+  int* dofs;
+  double* vals;
+  unsigned* dof_bits;
+  unsigned* row_bits;
+  int* rows;
+  int cap;
+};
+
+// DirichletDofs::add (bc.cpp:107-116) over the caller's pairs: the first pair in input order
+// whose DOF was given before with another value is a conflict; then the range check of
+// apply_dirichlet (bc.cpp:141-146) over the DOFs in ascending order.  Returns the sorted
+// distinct DOFs and their values.
+void collect_dofs(int n, int m, const int* dofs, const double* values, std::vector<int>& d,
+                  std::vector<double>& v) {
+  std::vector<int> order(m);
+  for (int k = 0; k < m; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return dofs[a] < dofs[b]; });
+  int conflict = -1, first_of_conflict = -1;
+  d.clear();
+  v.clear();
+  for (int g = 0; g < m;) {
+    int e = g + 1;
+    while (e < m && dofs[order[e]] == dofs[order[g]]) ++e;
+    const int k0 = order[g];  // the group's first pair in input order (stable sort)
+    for (int h = g + 1; h < e; ++h)
+      if (values[order[h]] != values[k0] && (conflict < 0 || order[h] < conflict)) {
+        conflict = order[h];
+        first_of_conflict = k0;
+      }
+    d.push_back(dofs[k0]);
+    v.push_back(values[k0]);
+    g = e;
   }
-  f[2 * nd] = fx;
-  f[2 * nd + 1] = fy;
+  if (conflict >= 0) {
+    char buf[256];
+    const int dof = dofs[conflict];
+    snprintf(buf, sizeof buf,
+             "conflicting Dirichlet values for node %d (0-based) component %s: %.17g vs %.17g",
+             dof / 2, dof % 2 == 0 ? "ux" : "uy", values[first_of_conflict], values[conflict]);
+    throw Error(CSTFEM_ERR_CONFLICT, buf);
+  }
+  for (int dof : d)
+    if (dof < 0 || dof >= n)
+      throw Error(CSTFEM_ERR_OUT_OF_RANGE,
+                  "apply_dirichlet: DOF " + std::to_string(dof) + " out of range");
+}
+
+// enqueues the elimination of the sorted constrained DOFs d (values v), whose copies are
+// already in w.dofs / w.vals on the library stream, and returns the counters
+void run_elimination(Context& c, int m, const DirWork& w, int out_ctr[3]) {
+  const int n = c.n;
+  const int all_rows = c.pattern_symmetric ? 0 : 1;
+  if (!c.fixed_valid) {  // union of the constrained DOFs since the last assembly
+    hipLaunchKernelGGL(k_zero_bytes, dim3(grid_for((int64_t)c.fixed.n, 256, 1024)), dim3(256), 0,
+                       c.stream, c.fixed.get(), (int64_t)c.fixed.n);
+    HIP_CHECK(hipGetLastError());
+    c.fixed_valid = true;
+  }
+  hipLaunchKernelGGL(k_dir_begin, dim3(1), dim3(64), 0, c.stream, w.ctr);
+  HIP_CHECK(hipGetLastError());
+  if (m > 0) {
+    hipLaunchKernelGGL(k_dir_mark, dim3(grid_for(m, 256, 0)), dim3(256), 0, c.stream, m, w.dofs,
+                       c.brow_ptr.get(), c.bcol.get(), all_rows, w.dof_bits, w.row_bits, w.rows,
+                       w.ctr, w.cap, c.fixed.get());
+    HIP_CHECK(hipGetLastError());
+    // rows to visit: at most the distinct constrained nodes times (1 + the widest row)
+    const int64_t bound = all_rows ? (int64_t)c.nb
+                                   : std::min<int64_t>((int64_t)w.cap,
+                                                       (int64_t)m * (1 + c.max_row_blocks));
+    hipLaunchKernelGGL(k_dir_rows, dim3(grid_for(2 * bound, 256, 4096)), dim3(256), 0, c.stream,
+                       c.nb, n, all_rows, w.rows, w.ctr, w.cap, c.brow_ptr.get(), c.bcol.get(),
+                       c.bdiag.get(), c.sell_pos.get(), c.sell_val.get(), w.dof_bits, w.dofs,
+                       w.vals, m, c.b.get(), w.ctr + 2);
+    HIP_CHECK(hipGetLastError());
+    const int64_t row_words = (c.nb + 31) / 32;
+    hipLaunchKernelGGL(k_dir_clear, dim3(grid_for(std::max<int64_t>(m, 1), 256, 1024)), dim3(256), 0,
+                       c.stream, m, w.dofs, w.ctr, w.cap, w.rows, row_words, w.dof_bits,
+                       w.row_bits);
+    HIP_CHECK(hipGetLastError());
+  }
+  HIP_CHECK(hipMemcpyAsync(out_ctr, w.ctr, sizeof(int) * 3, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+void check_preconditions(Context& c) {
+  if (!c.have_pattern) throw Error(CSTFEM_ERR_ORDER, "apply_dirichlet: no matrix on the device");
+  if (c.b.n == 0) throw Error(CSTFEM_ERR_ORDER, "apply_dirichlet: set the right-hand side first");
+}
+
+void finish(Context& c, const int ctr[3]) {
+  c.setup_done = false;
+  c.vals32_valid = false;
+  if (ctr[2] != INT_MAX)
+    throw Error(CSTFEM_ERR_BAD_DIAGONAL,
+                "apply_dirichlet: non-positive diagonal at DOF " + std::to_string(ctr[2]));
+}
+
+template <class T>
+T* grow(DBuf<T>& b, size_t count) {
+  if (!b.get() || b.n < count) b.alloc(std::max<size_t>(count, 1));
+  return b.get();
 }
 
 }  // namespace
 
 void apply_dirichlet_device(Context& c, int m, const int* dofs, const double* values) {
-  if (!c.have_pattern) throw Error(CSTFEM_ERR_ORDER, "apply_dirichlet: no matrix on the device");
-  if (c.b.n == 0) throw Error(CSTFEM_ERR_ORDER, "apply_dirichlet: set the right-hand side first");
-  const int n = c.n;
-  // DirichletDofs::add (bc.cpp:107-116): a repeated DOF must repeat its value
-  std::vector<std::pair<int, double>> seen;
-  seen.reserve(m);
-  for (int k = 0; k < m; ++k) {
-    if (dofs[k] < 0 || dofs[k] >= n)
-      throw Error(CSTFEM_ERR_OUT_OF_RANGE,
-                  "apply_dirichlet: DOF " + std::to_string(dofs[k]) + " out of range");
-    seen.emplace_back(dofs[k], values[k]);
+  check_preconditions(c);
+  std::vector<int> d;
+  std::vector<double> v;
+  collect_dofs(c.n, m, dofs, values, d, v);
+  const int mu = (int)d.size();
+  // workspace of the context, grown on demand and kept: repeated eliminations allocate
+  // nothing; the bitmaps start at zero and every call leaves them at zero
+  DirWork w{};
+  w.ctr = c.err_flag.get();
+  w.dofs = grow(c.dir_dofs, mu);
+  w.vals = grow(c.dir_vals, mu);
+  const size_t dof_words = ((size_t)c.n + 31) / 32, row_words = ((size_t)c.nb + 31) / 32;
+  if (c.dir_bits.n < dof_words + row_words) {
+    grow(c.dir_bits, dof_words + row_words);
+    HIP_CHECK(hipMemsetAsync(c.dir_bits.get(), 0, sizeof(unsigned) * c.dir_bits.n, c.stream));
   }
-  std::stable_sort(seen.begin(), seen.end(),
-                   [](const auto& a, const auto& b) { return a.first < b.first; });
-  for (size_t k = 1; k < seen.size(); ++k)
-    if (seen[k].first == seen[k - 1].first && seen[k].second != seen[k - 1].second) {
-      char buf[256];
-      const int dof = seen[k].first;
-      snprintf(buf, sizeof buf,
-               "conflicting Dirichlet values for node %d (0-based) component %s: %.17g vs %.17g",
-               dof / 2, dof % 2 == 0 ? "ux" : "uy", seen[k - 1].second, seen[k].second);
-      throw Error(CSTFEM_ERR_CONFLICT, buf);
-    }
-  DBuf<int> d_dofs;
-  DBuf<double> d_vals, ubar;
-  DBuf<unsigned char> fixed;
-  fixed.alloc(n);
-  ubar.alloc(n);
-  HIP_CHECK(hipMemsetAsync(fixed.get(), 0, n, c.stream));
-  HIP_CHECK(hipMemsetAsync(ubar.get(), 0, sizeof(double) * n, c.stream));
-  if (c.fixed.n != (size_t)n) {  // union of all constrained DOFs since the last assembly
-    c.fixed.alloc(n);
-    HIP_CHECK(hipMemsetAsync(c.fixed.get(), 0, n, c.stream));
+  w.dof_bits = c.dir_bits.get();
+  w.row_bits = c.dir_bits.get() + dof_words;
+  w.rows = grow(c.dir_rows, (size_t)c.nb);
+  w.cap = c.nb;  // every block row fits: the list never overflows
+  if (mu > 0) {
+    HIP_CHECK(hipMemcpyAsync(w.dofs, d.data(), sizeof(int) * mu, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(w.vals, v.data(), sizeof(double) * mu, hipMemcpyHostToDevice, c.stream));
   }
-  if (m > 0) {
-    d_dofs.alloc(m);
-    d_vals.alloc(m);
-    HIP_CHECK(hipMemcpyAsync(d_dofs.get(), dofs, sizeof(int) * m, hipMemcpyHostToDevice, c.stream));
-    HIP_CHECK(hipMemcpyAsync(d_vals.get(), values, sizeof(double) * m, hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_mark_fixed, dim3(grid_for(m, 256, 0)), dim3(256), 0, c.stream, m,
-                       d_dofs.get(), d_vals.get(), fixed.get(), ubar.get(), c.fixed.get());
-    HIP_CHECK(hipGetLastError());
-  }
-  const int big = INT_MAX;
-  HIP_CHECK(hipMemcpyAsync(c.err_flag.get(), &big, sizeof(int), hipMemcpyHostToDevice, c.stream));
-  hipLaunchKernelGGL(k_dirichlet, dim3(grid_for(c.nb, 128, 0)), dim3(128), 0, c.stream, c.nb, n,
-                     c.brow_ptr.get(), c.bcol.get(), c.bdiag.get(), c.sell_pos.get(),
-                     c.sell_val.get(), fixed.get(), ubar.get(), c.b.get(), c.err_flag.get());
-  HIP_CHECK(hipGetLastError());
-  int bad = INT_MAX;
-  HIP_CHECK(hipMemcpyAsync(&bad, c.err_flag.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
-  c.setup_done = false;
-  c.vals32_valid = false;
-  if (bad != INT_MAX)
-    throw Error(CSTFEM_ERR_BAD_DIAGONAL,
-                "apply_dirichlet: non-positive diagonal at DOF " + std::to_string(bad));
-}
-
-void add_body_force_device(Context& c, double bx, double by) {
-  if (c.ne_ptr.n == 0) build_node_elements(c);
-  if (c.b.n == 0) throw Error(CSTFEM_ERR_ORDER, "add_body_force: set the right-hand side first");
-  hipLaunchKernelGGL(k_body_force, dim3(grid_for(c.nn, 256, 0)), dim3(256), 0, c.stream, c.nn,
-                     c.ne_ptr.get(), c.ne_list.get(), c.tri.get(), c.xy.get(), c.thick, bx, by,
-                     c.b.get());
-  HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  int ctr[3];
+  run_elimination(c, mu, w, ctr);
+  finish(c, ctr);
 }
 
 }  // namespace cstfem_hip

@@ -273,7 +273,6 @@ struct Context {
   Mat3 D{};  // uniform constitutive matrix (used when there is no per-element modulus)
 
   DBuf<int> ne_ptr, ne_list;
-  std::vector<int> h_ne_ptr, h_ne_list;  // host node -> elements (boundary loads)
 
   // pattern (block CSR) and slot map
   bool have_pattern = false;
@@ -312,14 +311,32 @@ struct Context {
   DBuf<int> batch_seg_slice, batch_order, batch_queue;
   DBuf<cstfem_batch_result> batch_results;
 
-  // constrained DOFs (union over apply_dirichlet calls since the last assembly)
+  // constrained DOFs (union over apply_dirichlet calls since the last assembly): one byte
+  // per DOF, allocated with the vectors; fixed_valid is false until the first elimination
+  // after an assembly or upload has cleared it
   DBuf<unsigned char> fixed;
+  bool fixed_valid = false;
+  bool pattern_symmetric = true;  // structurally symmetric block pattern (mesh patterns are)
+  int max_row_blocks = 0;         // blocks of the longest block row
+  // Dirichlet elimination workspace, grown on demand and kept: sorted constrained DOFs and
+  // values, the DOF and block-row bitmaps (zero between calls) and the affected-row list
+  DBuf<int> dir_dofs, dir_rows;
+  DBuf<double> dir_vals;
+  DBuf<unsigned> dir_bits;
   Mpnn mpnn;
   Multigrid mg;
 
   // grow-only integer scratch of the pattern builders and scans, one buffer per call site
   // (slots in assembly.hip), kept so that rebuilding a pattern allocates and frees nothing
   DBuf<int> tmp_i[12];
+
+  // staging of cstfem_element_stiffness: 6 coordinates and 36 entries per triangle
+  DBuf<double> ke_io;
+
+  // traction workspace (loads.hip): packed edge/node index lists, edge geometry and the
+  // tractions at the Gauss points, grown on demand and kept
+  DBuf<int> trac_i;
+  DBuf<double> trac_d;
 
   // reduction scratch
   DBuf<double> partials;   // 4 * max_blocks
@@ -347,9 +364,22 @@ void build_pattern(Context& c);
 void build_sell(Context& c);
 void build_slot_map(Context& c);
 void assemble_values(Context& c);
+void element_stiffness(Context& c, int64_t count, const double* xy, const double* D, double t,
+                       double* Ke);
 void download_values(Context& c, double* vals);
 void apply_dirichlet_device(Context& c, int m, const int* dofs, const double* values);
 void add_body_force_device(Context& c, double bx, double by);
+// where the traction at a Gauss point comes from: a constant (p0, p1), the Kirsch field with
+// (sigma_inf, a) = (p0, p1), or a caller's function
+struct TractionSource {
+  enum Kind { kConstant, kKirsch, kFunction } kind = kConstant;
+  double p0 = 0.0, p1 = 0.0;
+  cstfem_traction_fn fn = nullptr;
+  void* user = nullptr;
+};
+void add_traction_device(Context& c, int nedge, const int* edges, const TractionSource& src);
+void kirsch_traction(double sigma_inf, double a, double x, double y, double nx, double ny,
+                     double t[2]);
 void setup_solver(Context& c);
 void spmv_fp64(Context& c, const double* x, double* y);
 int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,

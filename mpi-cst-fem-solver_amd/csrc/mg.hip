@@ -1949,7 +1949,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     yhi = std::max(yhi, c.h_xy[2 * (size_t)i + 1]);
   }
   const double cx = 0.5 * (xlo + xhi), cy = 0.5 * (ylo + yhi);
-  const unsigned char* fixed = c.fixed.n == (size_t)c.n ? c.fixed.get() : nullptr;
+  const unsigned char* fixed = c.fixed_valid && c.fixed.n >= (size_t)c.n ? c.fixed.get() : nullptr;
   for (int l = 0;; ++l) {
     MgLevel& L = *mg.lv[l];
     const bool last = l + 1 >= max_levels - 1;

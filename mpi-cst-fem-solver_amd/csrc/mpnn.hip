@@ -658,7 +658,7 @@ void mpnn_forward(Context& c) {
   hipLaunchKernelGGL(k_features, dim3((N + 127) / 128), dim3(128), 0, c.stream, N, c.brow_ptr.get(),
                      c.bcol.get(), c.bdiag.get(), c.sell_pos.get(), c.sell_val.get(), c.ne_ptr.get(),
                      c.ne_list.get(), c.has_E_elem ? c.E_elem.get() : (const double*)nullptr,
-                     c.fixed.n ? c.fixed.get() : (const unsigned char*)nullptr, m.feat.get());
+                     c.fixed_valid ? c.fixed.get() : (const unsigned char*)nullptr, m.feat.get());
   weights_bf16(c);
   for (int l = 0; l <= kRounds; ++l) m.H[l].alloc((size_t)N * kH);
   for (int l = 1; l <= kRounds; ++l) m.X[l].alloc((size_t)N * 2 * kH);

@@ -572,6 +572,10 @@ void ensure_vectors(Context& c) {
     HIP_CHECK(hipMemsetAsync(c.b.get(), 0, sizeof(double) * nv, c.stream));
   }
   c.partials.alloc(6 * kMaxGrid + 64);
+  if (c.fixed.n != nv) {
+    c.fixed.alloc(nv);
+    c.fixed_valid = false;
+  }
 }
 
 void setup_solver(Context& c) {

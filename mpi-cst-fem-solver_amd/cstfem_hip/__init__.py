@@ -91,6 +91,10 @@ class SolverStats:
     breakdown: dict = field(default_factory=dict)
 
 
+# cstfem_traction_fn: (user, count, pts[4*count] = (x, y, nx, ny), t[2*count]) -> None
+TRACTION_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -118,6 +122,10 @@ SIGNATURES = {
     "cstfem_add_body_force": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double]),
     "cstfem_add_traction": (ctypes.c_int, [_P, ctypes.c_int, _I, ctypes.c_int, ctypes.c_double,
                                            ctypes.c_double]),
+    "cstfem_add_traction_fn": (ctypes.c_int, [_P, ctypes.c_int, _I, ctypes.c_void_p,
+                                              ctypes.c_void_p]),
+    "cstfem_element_stiffness": (ctypes.c_int, [_P, ctypes.c_int64, _D, _D, ctypes.c_double,
+                                                _D]),
     "cstfem_apply_dirichlet": (ctypes.c_int, [_P, ctypes.c_int, _I, _D]),
     "cstfem_setup": (ctypes.c_int, [_P]),
     "cstfem_solve": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions),
@@ -294,6 +302,32 @@ class Device:
             mode, p0, p1 = 0, float(tx), float(ty)
         _check(lib().cstfem_add_traction(self._h, int(e.size // 2), _ptr(e, ctypes.c_int), mode,
                                          p0, p1))
+
+    def add_traction_fn(self, edges, fn):
+        """The reference's TractionFn form (bc.hpp:71-72): fn(x, y, nx, ny) -> (tx, ty) with
+        numpy arrays of the Gauss points' coordinates and outward normals."""
+        e = _i32(edges).reshape(-1)
+
+        def cb(_user, count, pts, t):
+            p = np.ctypeslib.as_array(pts, shape=(count, 4))
+            tx, ty = fn(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), p[:, 3].copy())
+            out = np.ctypeslib.as_array(t, shape=(count, 2))
+            out[:, 0] = tx
+            out[:, 1] = ty
+
+        cfn = TRACTION_FN(cb)
+        _check(lib().cstfem_add_traction_fn(self._h, int(e.size // 2), _ptr(e, ctypes.c_int),
+                                            ctypes.cast(cfn, ctypes.c_void_p), None))
+
+    def element_stiffness(self, xy, D, thickness=1.0):
+        """cst_stiffness (element.cpp:37-55) for a batch: xy [count x 6] -> Ke [count x 6 x 6]."""
+        xy = _f64(xy).reshape(-1, 6)
+        D = _f64(D).reshape(-1)
+        Ke = np.empty((xy.shape[0], 6, 6), dtype=np.float64)
+        _check(lib().cstfem_element_stiffness(self._h, int(xy.shape[0]), _ptr(xy, ctypes.c_double),
+                                              _ptr(D, ctypes.c_double), float(thickness),
+                                              _ptr(Ke, ctypes.c_double)))
+        return Ke
 
     def add_body_force(self, bx, by):
         _check(lib().cstfem_add_body_force(self._h, float(bx), float(by)))
