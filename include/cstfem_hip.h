@@ -164,6 +164,16 @@ int cstfem_add_traction_fn(cstfem_ctx* ctx, int nedge, const int* edges, cstfem_
 /* Replaces apply_dirichlet (bc.cpp:137-166) on the device matrix and RHS.  dofs/values:
  * m prescribed DOFs; a repeated DOF must repeat its value (DirichletDofs::add, bc.cpp:107). */
 int cstfem_apply_dirichlet(cstfem_ctx* ctx, int m, const int* dofs, const double* values);
+/* The same elimination without any device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Arguments, results and errors are those of
+ * cstfem_apply_dirichlet.  Its workspace is one fixed device array per GPU (13 MiB, held
+ * until the process ends) instead of context buffers, which limits a call to systems of at
+ * most 2^25 DOFs, at most 2^18 distinct constrained DOFs and at most 2^20 block rows visited
+ * (the constrained nodes and their neighbours); a larger call returns CSTFEM_ERR_ARGUMENT and
+ * leaves K and b unchanged.  Calls on one GPU share the array and run one at a time. */
+int cstfem_apply_dirichlet_noalloc(cstfem_ctx* ctx, int m, const int* dofs,
+                                   const double* values);
 
 /* ---- solve (replaces CgJacobiSolver::setup + solve / pcg_jacobi, solver.cpp:109-157) --- */
 int cstfem_setup(cstfem_ctx* ctx);

@@ -87,6 +87,7 @@ int cstfem_ctx_create(int device, cstfem_ctx** out) {
       HIP_CHECK(hipHostMalloc((void**)&c.h_scal, 2 * sizeof(PcgScalars), hipHostMallocDefault));
       c.scal.alloc(1);
       c.err_flag.alloc(4);
+      noalloc_attach(c);
     } catch (...) {
       delete x;
       throw;
@@ -493,8 +494,19 @@ int cstfem_add_traction_fn(cstfem_ctx* ctx, int nedge, const int* edges, cstfem_
 int cstfem_apply_dirichlet(cstfem_ctx* ctx, int m, const int* dofs, const double* values) {
   return guarded([&] {
     need(ctx && (m == 0 || (dofs && values)), "cstfem_apply_dirichlet: NULL argument");
+    need(m >= 0, "cstfem_apply_dirichlet: negative count");
     use_device(&ctx->c);
     apply_dirichlet_device(ctx->c, m, dofs, values);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_apply_dirichlet_noalloc(cstfem_ctx* ctx, int m, const int* dofs, const double* values) {
+  return guarded([&] {
+    need(ctx && (m == 0 || (dofs && values)), "cstfem_apply_dirichlet_noalloc: NULL argument");
+    need(m >= 0, "cstfem_apply_dirichlet_noalloc: negative count");
+    use_device(&ctx->c);
+    apply_dirichlet_noalloc(ctx->c, m, dofs, values);
     return CSTFEM_OK;
   });
 }

@@ -13,11 +13,36 @@
 //                (eliminated entries stay stored as zeros);
 //   k_dir_clear  clears the bits it set, so the bitmaps are zero again for the next call.
 // A pattern that is not structurally symmetric (a generic CSR upload) visits every row.
+//
+// Two entry points share these kernels: apply_dirichlet_device keeps its workspace in the
+// context (grown on demand), apply_dirichlet_noalloc in `scratch`, a fixed device array per
+// GPU, for callers that cannot allocate device memory.
 #include <algorithm>
 #include <climits>
 #include <mutex>
 
 #include "internal.hpp"
+
+// Workspace of the allocation-free entry point: counters, the sorted constrained DOFs and
+// their values, the DOF and block-row bitmaps and the block-row list, at fixed offsets.
+// Limits: 2^25 scalar rows, 2^18 distinct constrained DOFs per call, 2^20 listed block rows.
+namespace cstfem_hip {
+namespace na {
+constexpr int64_t kMaxDofs = int64_t(1) << 25;
+constexpr int kMaxFixed = 1 << 18;
+constexpr int kMaxRows = 1 << 20;
+constexpr size_t kCtr = 0;                                        // 64 ints
+constexpr size_t kDofs = 256;                                     // kMaxFixed ints
+constexpr size_t kVals = kDofs + 4 * (size_t)kMaxFixed;           // kMaxFixed doubles
+constexpr size_t kDofBits = kVals + 8 * (size_t)kMaxFixed;        // kMaxDofs / 32 words
+constexpr size_t kRowBits = kDofBits + 4 * (size_t)(kMaxDofs / 32);  // kMaxDofs / 64 words
+constexpr size_t kRows = kRowBits + 4 * (size_t)(kMaxDofs / 64);  // kMaxRows ints
+constexpr size_t kBytes = kRows + 4 * (size_t)kMaxRows;           // 13 MiB + 256 B
+}  // namespace na
+}  // namespace cstfem_hip
+
+// zero at load; every call leaves the bitmaps at zero again
+static __device__ __align__(256) unsigned char scratch[cstfem_hip::na::kBytes];
 
 namespace cstfem_hip {
 
@@ -58,13 +83,11 @@ __device__ inline void list_row(int r, unsigned* __restrict__ row_bits, int* __r
 __global__ void k_dir_mark(int m, const int* __restrict__ dofs, const int* __restrict__ brow_ptr,
                            const int* __restrict__ bcol, int all_rows,
                            unsigned* __restrict__ dof_bits, unsigned* __restrict__ row_bits,
-                           int* __restrict__ rows, int* __restrict__ ctr, int cap,
-                           unsigned char* __restrict__ fixed_union) {
+                           int* __restrict__ rows, int* __restrict__ ctr, int cap) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   const int dof = dofs[k];
   atomicOr(&dof_bits[dof >> 5], 1u << (dof & 31));
-  fixed_union[dof] = 1;
   if (all_rows) return;
   const int v = dof >> 1;
   list_row(v, row_bits, rows, ctr, cap);  // its own row, even without a stored diagonal
@@ -93,7 +116,7 @@ __global__ void k_dir_rows(int nb, int n, int all_rows, const int* __restrict__ 
                            const int64_t* __restrict__ spos, double* __restrict__ val,
                            const unsigned* __restrict__ dof_bits, const int* __restrict__ dofs,
                            const double* __restrict__ vals, int m, double* __restrict__ b,
-                           int* __restrict__ err) {
+                           unsigned char* __restrict__ fixed_union, int* __restrict__ err) {
   if (!all_rows && ctr[1]) return;  // the list overflowed: change nothing
   const int64_t cnt = all_rows ? nb : min(ctr[0], cap);
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < 2 * cnt;
@@ -105,6 +128,7 @@ __global__ void k_dir_rows(int nb, int n, int all_rows, const int* __restrict__ 
     const int q0 = brow_ptr[r], q1 = brow_ptr[r + 1];
     if (test_bit(dof_bits, i)) {
       // constrained row: off-diagonal entries to zero, b_i = K_ii u_i
+      fixed_union[i] = 1;
       const int qd = bdiag[r];
       const double diag = qd >= 0 ? val[sell_vidx(spos[qd], 3 * c)] : 0.0;
       if (!(diag > 0.0)) {
@@ -230,7 +254,7 @@ void run_elimination(Context& c, int m, const DirWork& w, int out_ctr[3]) {
   if (m > 0) {
     hipLaunchKernelGGL(k_dir_mark, dim3(grid_for(m, 256, 0)), dim3(256), 0, c.stream, m, w.dofs,
                        c.brow_ptr.get(), c.bcol.get(), all_rows, w.dof_bits, w.row_bits, w.rows,
-                       w.ctr, w.cap, c.fixed.get());
+                       w.ctr, w.cap);
     HIP_CHECK(hipGetLastError());
     // rows to visit: at most the distinct constrained nodes times (1 + the widest row)
     const int64_t bound = all_rows ? (int64_t)c.nb
@@ -239,7 +263,7 @@ void run_elimination(Context& c, int m, const DirWork& w, int out_ctr[3]) {
     hipLaunchKernelGGL(k_dir_rows, dim3(grid_for(2 * bound, 256, 4096)), dim3(256), 0, c.stream,
                        c.nb, n, all_rows, w.rows, w.ctr, w.cap, c.brow_ptr.get(), c.bcol.get(),
                        c.bdiag.get(), c.sell_pos.get(), c.sell_val.get(), w.dof_bits, w.dofs,
-                       w.vals, m, c.b.get(), w.ctr + 2);
+                       w.vals, m, c.b.get(), c.fixed.get(), w.ctr + 2);
     HIP_CHECK(hipGetLastError());
     const int64_t row_words = (c.nb + 31) / 32;
     hipLaunchKernelGGL(k_dir_clear, dim3(grid_for(std::max<int64_t>(m, 1), 256, 1024)), dim3(256), 0,
@@ -299,6 +323,73 @@ void apply_dirichlet_device(Context& c, int m, const int* dofs, const double* va
   }
   int ctr[3];
   run_elimination(c, mu, w, ctr);
+  finish(c, ctr);
+}
+
+// ---- the allocation-free entry point ---------------------------------------------------
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu[kMaxDevices];  // calls on one GPU run one at a time
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_init_mu;
+}  // namespace
+
+void noalloc_attach(Context& c) {
+  if (c.device < 0 || c.device >= kMaxDevices)
+    throw Error(CSTFEM_ERR_ARGUMENT, "device index beyond the library's table of " +
+                                         std::to_string(kMaxDevices));
+  std::lock_guard<std::mutex> lk(g_scratch_init_mu);
+  if (!g_scratch_ptr[c.device]) {
+    // the array's device address on the current device; its host-side name is not one
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error(CSTFEM_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                         hipGetErrorString(e));
+    g_scratch_ptr[c.device] = p;
+  }
+  c.na_scratch = static_cast<unsigned char*>(g_scratch_ptr[c.device]);
+}
+
+void apply_dirichlet_noalloc(Context& c, int m, const int* dofs, const double* values) {
+  check_preconditions(c);
+  if (!c.na_scratch) throw Error(CSTFEM_ERR_ORDER, "apply_dirichlet_noalloc: no scratch on this device");
+  if ((int64_t)c.n > na::kMaxDofs)
+    throw Error(CSTFEM_ERR_ARGUMENT, "apply_dirichlet_noalloc: " + std::to_string(c.n) +
+                                         " DOFs exceed the scratch limit of " +
+                                         std::to_string(na::kMaxDofs));
+  if (c.fixed.n < (size_t)c.n)
+    throw Error(CSTFEM_ERR_ORDER, "apply_dirichlet_noalloc: build the pattern first");
+  std::vector<int> d;
+  std::vector<double> v;
+  collect_dofs(c.n, m, dofs, values, d, v);
+  const int mu = (int)d.size();
+  if (mu > na::kMaxFixed)
+    throw Error(CSTFEM_ERR_ARGUMENT, "apply_dirichlet_noalloc: " + std::to_string(mu) +
+                                         " constrained DOFs exceed the scratch limit of " +
+                                         std::to_string(na::kMaxFixed));
+  std::lock_guard<std::mutex> lk(g_scratch_mu[c.device]);
+  unsigned char* base = c.na_scratch;
+  DirWork w{};
+  w.ctr = reinterpret_cast<int*>(base + na::kCtr);
+  w.dofs = reinterpret_cast<int*>(base + na::kDofs);
+  w.vals = reinterpret_cast<double*>(base + na::kVals);
+  w.dof_bits = reinterpret_cast<unsigned*>(base + na::kDofBits);
+  w.row_bits = reinterpret_cast<unsigned*>(base + na::kRowBits);
+  w.rows = reinterpret_cast<int*>(base + na::kRows);
+  w.cap = na::kMaxRows;
+  if (mu > 0) {
+    HIP_CHECK(hipMemcpyAsync(w.dofs, d.data(), sizeof(int) * mu, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(w.vals, v.data(), sizeof(double) * mu, hipMemcpyHostToDevice, c.stream));
+  }
+  int ctr[3];
+  run_elimination(c, mu, w, ctr);
+  if (ctr[1])
+    throw Error(CSTFEM_ERR_ARGUMENT,
+                "apply_dirichlet_noalloc: the elimination visits more than " +
+                    std::to_string(na::kMaxRows) +
+                    " block rows (the scratch limit); K and b are unchanged");
   finish(c, ctr);
 }
 

@@ -323,6 +323,8 @@ struct Context {
   DBuf<int> dir_dofs, dir_rows;
   DBuf<double> dir_vals;
   DBuf<unsigned> dir_bits;
+  // device address of the allocation-free elimination's scratch array on this GPU
+  unsigned char* na_scratch = nullptr;
   Mpnn mpnn;
   Multigrid mg;
 
@@ -368,6 +370,10 @@ void element_stiffness(Context& c, int64_t count, const double* xy, const double
                        double* Ke);
 void download_values(Context& c, double* vals);
 void apply_dirichlet_device(Context& c, int m, const int* dofs, const double* values);
+// the same elimination with the fixed per-GPU scratch array instead of context buffers
+void apply_dirichlet_noalloc(Context& c, int m, const int* dofs, const double* values);
+// looks up (once per device) the scratch array's device address for this context's GPU
+void noalloc_attach(Context& c);
 void add_body_force_device(Context& c, double bx, double by);
 // where the traction at a Gauss point comes from: a constant (p0, p1), the Kirsch field with
 // (sigma_inf, a) = (p0, p1), or a caller's function

@@ -127,6 +127,7 @@ SIGNATURES = {
     "cstfem_element_stiffness": (ctypes.c_int, [_P, ctypes.c_int64, _D, _D, ctypes.c_double,
                                                 _D]),
     "cstfem_apply_dirichlet": (ctypes.c_int, [_P, ctypes.c_int, _I, _D]),
+    "cstfem_apply_dirichlet_noalloc": (ctypes.c_int, [_P, ctypes.c_int, _I, _D]),
     "cstfem_setup": (ctypes.c_int, [_P]),
     "cstfem_solve": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions),
                                     ctypes.POINTER(SolverStatsC), _D, _D]),
@@ -336,6 +337,13 @@ class Device:
         d, v = _i32(dofs), _f64(values)
         _check(lib().cstfem_apply_dirichlet(self._h, int(d.size), _ptr(d, ctypes.c_int),
                                             _ptr(v, ctypes.c_double)))
+
+    def apply_dirichlet_noalloc(self, dofs, values):
+        """apply_dirichlet without device allocation (fixed per-GPU workspace, limits in
+        include/cstfem_hip.h)."""
+        d, v = _i32(dofs), _f64(values)
+        _check(lib().cstfem_apply_dirichlet_noalloc(self._h, int(d.size), _ptr(d, ctypes.c_int),
+                                                    _ptr(v, ctypes.c_double)))
 
     # ---- solve (solver.cpp:109-157)
     def setup(self):
