@@ -3,7 +3,7 @@
 //
 //   reference (file:line)                         here
 //   read_gmsh            mesh.hpp:33-37           cstfem_hip::read_gmsh (Gmsh 4.1 ASCII)
-//   CaseFile             case.hpp:13-44           cstfem_hip::CaseFile (same key = value format)
+//   CaseFile             case.hpp:13-44           cstfem_hip::CaseKeys (same key = value format)
 //   run_serial_case      driver.hpp:60-62         cstfem_hip::run_case (one GPU; writes
 //                        driver.cpp:150-310       timings.json with the reference's fields)
 //
@@ -11,7 +11,6 @@
 #pragma once
 
 #include <istream>
-#include <map>
 #include <ostream>
 #include <string>
 #include <vector>
@@ -20,9 +19,10 @@
 
 namespace cstfem_hip {
 
-// Reads a Gmsh .msh 4.1 ASCII file (mesh_io.cpp:313-356): nodes in file order (0-based
-// index = position), 3-node triangles reoriented counter-clockwise, 2-node lines collected per
-// physical group name.
+// Reads a Gmsh .msh 4.1 ASCII file (replaces read_gmsh, mesh.hpp:33-37): nodes in file order
+// (0-based index = position), 3-node triangles reoriented counter-clockwise, 2-node lines
+// collected per physical group name; the same files are accepted and rejected as by the
+// reference's reader (mesh_io.cpp:114-356), with its messages.
 Mesh read_gmsh(const std::string& path);
 Mesh read_gmsh(std::istream& in, const std::string& source_name);
 
@@ -31,32 +31,33 @@ Mesh read_gmsh(std::istream& in, const std::string& source_name);
 // triangle, one curve entity per boundary group.
 void write_gmsh(const Mesh& mesh, std::ostream& out);
 
-// key = value lines, '#' comments (case.cpp:25-57)
-class CaseFile {
+// The settings of a case file (the reference's cases/*.case format): "key = value" lines,
+// '#' starts a comment, each key at most once.  Typed lookups name the file and line of a
+// value that does not parse.
+class CaseKeys {
  public:
-  static CaseFile load(const std::string& path);
-  static CaseFile parse(std::istream& in, const std::string& source_name);
-  bool has(const std::string& key) const;
-  std::string get_string(const std::string& key) const;
-  std::string get_string(const std::string& key, const std::string& fallback) const;
-  double get_double(const std::string& key) const;
-  double get_double(const std::string& key, double fallback) const;
-  int get_int(const std::string& key) const;
-  int get_int(const std::string& key, int fallback) const;
-  std::vector<std::string> unknown_keys(const std::vector<std::string>& known) const;
-  int warn_unknown_keys(const std::vector<std::string>& known, std::ostream& os) const;
+  static CaseKeys read(const std::string& path);
+  static CaseKeys read(std::istream& in, const std::string& source_name);
+  bool has(const std::string& key) const { return find(key) != nullptr; }
+  std::string text(const std::string& key) const;  // required key
+  std::string text(const std::string& key, const std::string& fallback) const;
+  double number(const std::string& key) const;  // required key
+  double number(const std::string& key, double fallback) const;
+  int integer(const std::string& key, int fallback) const;
+  // writes "warning: file:line: unknown key 'k'" for each key not in `known`, in file order
+  int report_unknown(const std::vector<std::string>& known, std::ostream& os) const;
   const std::string& source() const { return source_; }
 
  private:
-  struct Entry {
-    std::string value;
+  struct Item {
+    std::string key, value;
     int line = 0;
   };
-  std::string where(int line) const;
-  const Entry& entry(const std::string& key) const;
+  const Item* find(const std::string& key) const;
+  const Item& need(const std::string& key) const;
+  double parse_number(const Item& it) const;
   std::string source_;
-  std::map<std::string, Entry> entries_;
-  std::vector<std::string> order_;
+  std::vector<Item> items_;  // file order
 };
 
 // Runs one case on GPU `device` (run_serial_case, driver.cpp:150-310): read the mesh, assemble,

@@ -123,11 +123,11 @@ int run_case(const std::string& case_path, std::ostream& out, std::ostream& err,
     const auto t_all0 = Clock::now();
     // ---- read
     auto t0 = Clock::now();
-    const CaseFile c = CaseFile::load(case_path);
-    c.warn_unknown_keys(known_keys(), err);
-    const std::string mesh_path = c.get_string("mesh");
-    const std::string out_dir = c.get_string("output_dir", "results");
-    const std::string plane_name = c.get_string("plane");
+    const CaseKeys c = CaseKeys::read(case_path);
+    c.report_unknown(known_keys(), err);
+    const std::string mesh_path = c.text("mesh");
+    const std::string out_dir = c.text("output_dir", "results");
+    const std::string plane_name = c.text("plane");
     int plane;
     if (plane_name == "plane_stress")
       plane = CSTFEM_PLANE_STRESS;
@@ -136,22 +136,22 @@ int run_case(const std::string& case_path, std::ostream& out, std::ostream& err,
     else
       throw std::runtime_error(c.source() + ": plane must be plane_stress or plane_strain, got '" +
                                plane_name + "'");
-    const double E = c.get_double("E"), nu = c.get_double("nu"), t = c.get_double("thickness");
+    const double E = c.number("E"), nu = c.number("nu"), t = c.number("thickness");
     if (!(E > 0)) throw std::runtime_error(c.source() + ": E must be positive");
     if (!(nu > -1.0 && nu < 0.5)) throw std::runtime_error(c.source() + ": nu must be in (-1, 0.5)");
     if (!(t > 0)) throw std::runtime_error(c.source() + ": thickness must be positive");
-    const std::string solver = c.get_string("solver", "cg_jacobi");
+    const std::string solver = c.text("solver", "cg_jacobi");
     if (solver != "cg_jacobi")
       throw std::runtime_error(c.source() + ": solver must be cg_jacobi on the GPU path");
-    const std::string path = c.get_string("path", "auto");
+    const std::string path = c.text("path", "auto");
     if (path != "auto" && path != "serial")
       throw std::runtime_error(c.source() + ": path must be auto or serial on the GPU path");
-    const std::string instrument = c.get_string("instrument", "true");
+    const std::string instrument = c.text("instrument", "true");
     if (instrument != "true" && instrument != "false")
       throw std::runtime_error(c.source() + ": instrument must be true or false");
-    const double rtol = c.get_double("rtol", 1e-10);
-    const int max_iter = c.get_int("max_iter", 20000);
-    const std::string gmethod = c.get_string("gpu.method", "cg_jacobi");
+    const double rtol = c.number("rtol", 1e-10);
+    const int max_iter = c.integer("max_iter", 20000);
+    const std::string gmethod = c.text("gpu.method", "cg_jacobi");
     const std::map<std::string, int> methods = {
         {"cg_jacobi", CSTFEM_METHOD_CG_FP64}, {"cg_fp64_refined", CSTFEM_METHOD_CG_FP64_REFINED},
         {"ir_fp32", CSTFEM_METHOD_IR_FP32}, {"fcg_multilevel", CSTFEM_METHOD_FCG_LEARNED},
@@ -159,7 +159,7 @@ int run_case(const std::string& case_path, std::ostream& out, std::ostream& err,
     if (!methods.count(gmethod)) throw std::runtime_error(c.source() + ": unknown gpu.method '" + gmethod + "'");
     std::vector<float> theta;
     if (gmethod == "fcg_learned_multilevel") {
-      const std::string pp = c.get_string("gpu.mpnn_params");
+      const std::string pp = c.text("gpu.mpnn_params");
       std::ifstream pf(pp, std::ios::binary);
       if (!pf) throw std::runtime_error(pp + ": cannot open file");
       theta.resize(cstfem_mpnn_num_params());
@@ -195,10 +195,10 @@ int run_case(const std::string& case_path, std::ostream& out, std::ostream& err,
     // ---- boundary conditions
     t0 = Clock::now();
     check_rc(cstfem_set_rhs(ctx, nullptr, n));  // f = 0, then the loads
-    const std::string traction = c.get_string("bc.traction", "");
+    const std::string traction = c.text("bc.traction", "");
     std::optional<std::pair<double, double>> kirsch;  // (sigma_inf, a)
     if (traction == "kirsch_exact") {
-      const double s_inf = c.get_double("kirsch.sigma_inf"), a = c.get_double("kirsch.a");
+      const double s_inf = c.number("kirsch.sigma_inf"), a = c.number("kirsch.a");
       if (!(a > 0)) throw std::runtime_error(c.source() + ": kirsch.a must be positive");
       kirsch = std::make_pair(s_inf, a);
       for (const char* g : {"right", "top"}) {
@@ -222,7 +222,7 @@ int run_case(const std::string& case_path, std::ostream& out, std::ostream& err,
       }
     }
     DirichletDofs dofs;
-    for (const auto& [group, kv] : group_list(c.get_string("bc.dirichlet", ""), "bc.dirichlet")) {
+    for (const auto& [group, kv] : group_list(c.text("bc.dirichlet", ""), "bc.dirichlet")) {
       std::set<std::string> seen;
       for (const auto& [key, value] : kv) {
         if (key != "ux" && key != "uy")

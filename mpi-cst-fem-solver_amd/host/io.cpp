@@ -1,19 +1,37 @@
-// io.cpp -- Gmsh 4.1 ASCII reader / writer and the case-file parser (host side).
+// io.cpp -- host-side input: the Gmsh 4.1 ASCII reader and writer, and the case-file keys.
 //
-// Reference behaviour followed: src/mesh_io.cpp:20-356 (sections, node numbering by file
-// order, element types 1/2/15, degenerate-triangle threshold 1e-14 * bbox diagonal^2,
-// clockwise triangles flipped, line elements collected per physical name, "file:line:"
-// errors), src/case.cpp:25-128 (key = value, '#' comments, duplicate keys rejected).
+// Gmsh reader.  The file is read whole and cut into non-blank lines that keep their line
+// numbers.  A first pass splits it into sections ($Name ... $EndName) and checks their order;
+// a second pass parses each known section from its own range of lines, so a parser can
+// neither run past its $End marker nor into the next section; a last check lists the
+// sections a mesh needs.  Node tags are looked up in a sorted (tag, index) table.
+// The format facts kept from the reference's reader (src/mesh_io.cpp:114-356), so that both
+// number a file's nodes and triangles identically and reject the same files:
+//   - version 4.1, ASCII (file-type 0); $MeshFormat first; $PhysicalNames, $Entities, $Nodes
+//     and $Elements required, $Nodes and $Entities before $Elements; other sections skipped;
+//   - node index = position in the file; element types 1 (2-node line), 2 (3-node
+//     triangle), 15 (point);
+//   - a triangle is degenerate when its signed area is 0 or below 1e-14 times the squared
+//     diagonal of the nodes' bounding box, and is reoriented counter-clockwise (second and
+//     third node swapped) when its signed area is negative;
+//   - line elements are collected under every physical name of their entity (the tag itself
+//     when it has no name);
+//   - errors are std::runtime_error "file:line: message" with the reference's messages, the
+//     line being the last one read (the file's line count at its end).
+//
+// Case keys.  "key = value" per line, '#' starts a comment, a key may appear once; typed
+// lookups name the file and line of a bad value.
 #include <algorithm>
 #include <cerrno>
 #include <climits>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <iomanip>
+#include <iterator>
 #include <limits>
 #include <sstream>
-#include <unordered_map>
 
 #include "cstfem_hip_io.hpp"
 
@@ -21,292 +39,414 @@ namespace cstfem_hip {
 
 namespace {
 
-std::string trim(const std::string& s, const char* ws = " \t\r") {
-  const auto a = s.find_first_not_of(ws);
-  if (a == std::string::npos) return {};
-  const auto b = s.find_last_not_of(ws);
-  return s.substr(a, b - a + 1);
-}
+bool is_space(char ch) { return ch == ' ' || ch == '\t' || ch == '\r'; }
 
-// Non-blank, trimmed lines with their line numbers.
-class Lines {
+// ---- a Gmsh file as numbered lines -----------------------------------------------------
+
+struct MshLine {
+  int no;            // 1-based line number in the file
+  std::string text;  // without leading and trailing blanks
+};
+
+// [begin, end) indices into the line list of one section's body
+struct Section {
+  std::string name;
+  int header;  // index of the $Name line
+  int begin, end;
+};
+
+class MshFile {
  public:
-  Lines(std::istream& in, std::string src) : in_(in), src_(std::move(src)) {}
-  bool next(std::string& out) {
+  MshFile(std::istream& in, std::string source) : source_(std::move(source)) {
     std::string raw;
-    while (std::getline(in_, raw)) {
-      ++no_;
-      out = trim(raw);
-      if (!out.empty()) return true;
+    while (std::getline(in, raw)) {
+      ++physical_lines_;
+      size_t a = 0, b = raw.size();
+      while (a < b && is_space(raw[a])) ++a;
+      while (b > a && is_space(raw[b - 1])) --b;
+      if (a < b) lines_.push_back({physical_lines_, raw.substr(a, b - a)});
     }
-    return false;
   }
-  std::string need(const std::string& what) {
-    std::string l;
-    if (!next(l)) fail("unexpected end of file while reading " + what);
-    return l;
-  }
-  [[noreturn]] void fail(const std::string& msg) const {
-    throw std::runtime_error(src_ + ":" + std::to_string(no_) + ": " + msg);
+  const std::vector<MshLine>& lines() const { return lines_; }
+  int physical_lines() const { return physical_lines_; }
+  [[noreturn]] void fail(int line_no, const std::string& msg) const {
+    throw std::runtime_error(source_ + ":" + std::to_string(line_no) + ": " + msg);
   }
 
  private:
-  std::istream& in_;
-  std::string src_;
-  int no_ = 0;
+  std::string source_;
+  std::vector<MshLine> lines_;
+  int physical_lines_ = 0;
 };
 
+// The values of one line, read left to right like a C++ stream: a number is the longest
+// numeric prefix after the blanks (what follows it is left for the next read), names and
+// special values (nan, inf) are not numbers.
 class Fields {
  public:
-  Fields(Lines& l, const std::string& line) : l_(l), ss_(line) {}
-  long long i(const char* what) {
-    long long v = 0;
-    if (!(ss_ >> v)) l_.fail(std::string("expected integer (") + what + ")");
+  Fields(const MshFile& f, const MshLine& l) : file_(f), line_(l) {}
+
+  long long integer(const char* what) {
+    const char* p = start();
+    char* end = nullptr;
+    errno = 0;
+    const long long v = numeric(p) ? std::strtoll(p, &end, 10) : 0;
+    if (!numeric(p) || end == p || errno == ERANGE)
+      file_.fail(line_.no, std::string("expected integer (") + what + ")");
+    pos_ = static_cast<size_t>(end - line_.text.c_str());
     return v;
   }
-  double d(const char* what) {
-    double v = 0.0;
-    if (!(ss_ >> v)) l_.fail(std::string("expected number (") + what + ")");
+  double real(const char* what) {
+    const char* p = start();
+    char* end = nullptr;
+    errno = 0;
+    const double v = numeric(p) ? std::strtod(p, &end) : 0.0;
+    if (!numeric(p) || end == p || (errno == ERANGE && std::isinf(v)))
+      file_.fail(line_.no, std::string("expected number (") + what + ")");
+    pos_ = static_cast<size_t>(end - line_.text.c_str());
     return v;
   }
+  int bounded(const char* what) {
+    const long long v = integer(what);
+    if (v < INT_MIN || v > INT_MAX) file_.fail(line_.no, std::string(what) + " out of range");
+    return static_cast<int>(v);
+  }
+  size_t count(const char* what) {
+    const long long v = integer(what);
+    if (v < 0) file_.fail(line_.no, std::string(what) + " must be non-negative");
+    return static_cast<size_t>(v);
+  }
+  // the rest of the line after the values read so far, leading blanks removed
   std::string rest() {
-    std::string s;
-    std::getline(ss_, s);
-    return trim(s, " \t");
+    const char* p = start();
+    pos_ = line_.text.size();
+    return std::string(p);
   }
 
  private:
-  Lines& l_;
-  std::istringstream ss_;
+  const char* start() {
+    while (pos_ < line_.text.size() && is_space(line_.text[pos_])) ++pos_;
+    return line_.text.c_str() + pos_;
+  }
+  static bool numeric(const char* p) {
+    if (*p == '+' || *p == '-') ++p;
+    return (*p >= '0' && *p <= '9') || *p == '.';
+  }
+  const MshFile& file_;
+  const MshLine& line_;
+  size_t pos_ = 0;
 };
 
-int as_int(Lines& l, long long v, const char* what) {
-  if (v < INT_MIN || v > INT_MAX) l.fail(std::string(what) + " out of range");
-  return (int)v;
+// Reads the lines of one section body in order; running out of lines or leaving lines
+// unread are errors at the reference's line numbers.
+class Body {
+ public:
+  Body(const MshFile& f, const Section& s) : file_(f), sec_(s), at_(s.begin) {}
+  const MshLine& take(const std::string& what) {
+    if (at_ >= sec_.end) {
+      // the reference's sequential reader would read the $End line as data here
+      const auto& l = file_.lines()[sec_.end];
+      file_.fail(l.no, "expected " + what + ", got '" + l.text + "'");
+    }
+    last_ = at_;
+    return file_.lines()[at_++];
+  }
+  int last_no() const { return file_.lines()[last_ >= 0 ? last_ : sec_.header].no; }
+  void finish() const {
+    if (at_ < sec_.end) {
+      const auto& l = file_.lines()[at_];
+      file_.fail(l.no, "expected $End" + sec_.name + ", got '" + l.text + "'");
+    }
+  }
+
+ private:
+  const MshFile& file_;
+  const Section& sec_;
+  int at_;
+  int last_ = -1;
+};
+
+// cst_signed_area (element.cpp:17-19)
+double signed_area(const double* xy, const int c[3]) {
+  const double x1 = xy[2 * c[0]], y1 = xy[2 * c[0] + 1];
+  const double x2 = xy[2 * c[1]], y2 = xy[2 * c[1] + 1];
+  const double x3 = xy[2 * c[2]], y3 = xy[2 * c[2] + 1];
+  return 0.5 * ((x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1));
 }
 
-std::size_t as_count(Lines& l, long long v, const char* what) {
-  if (v < 0) l.fail(std::string(what) + " must be non-negative");
-  return (std::size_t)v;
-}
+class MshParser {
+ public:
+  explicit MshParser(const MshFile& f) : f_(f) {}
 
-void end_of(Lines& l, const std::string& section) {
-  const std::string line = l.need("$End" + section);
-  if (line != "$End" + section) l.fail("expected $End" + section + ", got '" + line + "'");
-}
-
-struct Reader {
-  Lines& l;
-  Mesh& m;
-  bool fmt = false, names = false, ents = false, nodes = false, elems = false;
-  std::map<std::pair<int, int>, std::string> phys_name;          // (dim, tag) -> name
-  std::map<std::pair<int, int>, std::vector<int>> ent_phys;      // entity -> physical tags
-  std::unordered_map<long long, int> index_of;                    // node tag -> index
-
-  void format() {
-    const std::string line = l.need("$MeshFormat");
-    const std::string version = line.substr(0, line.find_first_of(" \t"));
-    if (version != "4.1") l.fail("unsupported .msh version '" + version + "' (need 4.1)");
-    Fields f(l, line);
-    f.d("version");
-    if (f.i("file-type") != 0) l.fail("binary .msh files are not supported (need ASCII, file-type 0)");
-    f.i("data-size");
-    end_of(l, "MeshFormat");
-    fmt = true;
-  }
-
-  void physical_names() {
-    Fields h(l, l.need("$PhysicalNames"));
-    const std::size_t n = as_count(l, h.i("numPhysicalNames"), "numPhysicalNames");
-    for (std::size_t k = 0; k < n; ++k) {
-      Fields f(l, l.need("physical name"));
-      const int dim = as_int(l, f.i("dimension"), "dimension");
-      const int tag = as_int(l, f.i("physical tag"), "physical tag");
-      std::string name = f.rest();
-      if (name.size() < 2 || name.front() != '"' || name.back() != '"')
-        l.fail("physical name must be quoted");
-      name = name.substr(1, name.size() - 2);
-      if (!phys_name.emplace(std::make_pair(dim, tag), name).second)
-        l.fail("duplicate physical group (" + std::to_string(dim) + ", " + std::to_string(tag) + ")");
-      m.physical_groups.push_back({dim, tag, name});
-    }
-    end_of(l, "PhysicalNames");
-    names = true;
-  }
-
-  void entities() {
-    Fields h(l, l.need("$Entities"));
-    std::size_t count[4];
-    const char* what[4] = {"numPoints", "numCurves", "numSurfaces", "numVolumes"};
-    for (int d = 0; d < 4; ++d) count[d] = as_count(l, h.i(what[d]), what[d]);
-    for (int dim = 0; dim < 4; ++dim)
-      for (std::size_t k = 0; k < count[dim]; ++k) {
-        Fields f(l, l.need("entity"));
-        const int tag = as_int(l, f.i("entity tag"), "entity tag");
-        for (int c = 0; c < (dim == 0 ? 3 : 6); ++c) f.d("entity coordinate");
-        const std::size_t np = as_count(l, f.i("numPhysicalTags"), "numPhysicalTags");
-        std::vector<int> tags(np);
-        for (auto& t : tags) t = as_int(l, f.i("physical tag"), "physical tag");
-        ent_phys[{dim, tag}] = std::move(tags);
+  Mesh run() {
+    const std::vector<Section> secs = split();
+    bool format = false, names = false, entities = false, nodes = false, elements = false;
+    for (const Section& s : secs) {
+      Body body(f_, s);
+      if (s.name == "MeshFormat") {
+        mesh_format(body);
+        format = true;
+      } else if (s.name == "PhysicalNames") {
+        physical_names(body);
+        names = true;
+      } else if (s.name == "Entities") {
+        entities_section(body);
+        entities = true;
+      } else if (s.name == "Nodes") {
+        nodes_section(body);
+        nodes = true;
+      } else if (s.name == "Elements") {
+        const int at = f_.lines()[s.header].no;
+        if (!nodes) f_.fail(at, "$Elements must come after $Nodes");
+        if (!entities) f_.fail(at, "$Elements must come after $Entities");
+        elements_section(body);
+        elements = true;
+      } else {
+        continue;  // a section the mesh does not need
       }
-    end_of(l, "Entities");
-    ents = true;
+      body.finish();
+    }
+    const int end = f_.physical_lines();
+    if (!format) f_.fail(end, "missing $MeshFormat section");
+    if (!names) f_.fail(end, "missing $PhysicalNames section");
+    if (!entities) f_.fail(end, "missing $Entities section");
+    if (!nodes) f_.fail(end, "missing $Nodes section");
+    if (!elements) f_.fail(end, "missing $Elements section");
+    return std::move(mesh_);
   }
 
-  void node_section() {
-    Fields h(l, l.need("$Nodes"));
-    const std::size_t nblocks = as_count(l, h.i("numEntityBlocks"), "numEntityBlocks");
-    const std::size_t total = as_count(l, h.i("numNodes"), "numNodes");
-    m.xy.reserve(2 * total);
-    m.node_tags.reserve(total);
-    for (std::size_t b = 0; b < nblocks; ++b) {
-      Fields bh(l, l.need("node block header"));
-      bh.i("entityDim");
-      bh.i("entityTag");
-      bh.i("parametric");
-      const std::size_t n = as_count(l, bh.i("numNodesInBlock"), "numNodesInBlock");
-      for (std::size_t k = 0; k < n; ++k) {
-        Fields f(l, l.need("node tag"));
-        const long long tag = f.i("node tag");
-        if (tag <= 0) l.fail("node tag must be positive");
-        if (!index_of.emplace(tag, (int)m.node_tags.size()).second)
-          l.fail("duplicate node tag " + std::to_string(tag));
-        m.node_tags.push_back((std::size_t)tag);
+ private:
+  // $Name ... $EndName ranges, in file order; $MeshFormat first and only once
+  std::vector<Section> split() const {
+    const auto& L = f_.lines();
+    std::vector<Section> out;
+    const int n = static_cast<int>(L.size());
+    for (int i = 0; i < n;) {
+      const std::string& h = L[i].text;
+      if (h.empty() || h[0] != '$' || h.compare(0, 4, "$End") == 0)
+        f_.fail(L[i].no, "expected a section header, got '" + h + "'");
+      const std::string name = h.substr(1);
+      if (out.empty() && name != "MeshFormat") f_.fail(L[i].no, "file must start with $MeshFormat");
+      if (name == "MeshFormat" && !out.empty()) f_.fail(L[i].no, "duplicate $MeshFormat section");
+      const std::string close = "$End" + name;
+      int j = i + 1;
+      while (j < n && L[j].text != close) ++j;
+      if (j == n) {
+        if (name == "MeshFormat" || name == "PhysicalNames" || name == "Entities" ||
+            name == "Nodes" || name == "Elements")
+          f_.fail(f_.physical_lines(), "unexpected end of file while reading " + close);
+        f_.fail(f_.physical_lines(), "unexpected end of file inside $" + name);
       }
-      for (std::size_t k = 0; k < n; ++k) {
-        Fields f(l, l.need("node coordinates"));
-        const double x = f.d("x"), y = f.d("y");
-        f.d("z");
-        m.xy.push_back(x);
-        m.xy.push_back(y);
-      }
+      out.push_back({name, i, i + 1, j});
+      i = j + 1;
     }
-    if (m.node_tags.size() != total)
-      l.fail("node count mismatch: header says " + std::to_string(total) + ", blocks contain " +
-             std::to_string(m.node_tags.size()));
-    end_of(l, "Nodes");
-    nodes = true;
+    return out;
   }
 
-  void element_section() {
-    if (!nodes) l.fail("$Elements must come after $Nodes");
-    if (!ents) l.fail("$Elements must come after $Entities");
-    double x0 = std::numeric_limits<double>::max(), x1 = std::numeric_limits<double>::lowest();
-    double y0 = x0, y1 = x1;
-    for (std::size_t i = 0; i < m.num_nodes(); ++i) {
-      x0 = std::min(x0, m.xy[2 * i]);
-      x1 = std::max(x1, m.xy[2 * i]);
-      y0 = std::min(y0, m.xy[2 * i + 1]);
-      y1 = std::max(y1, m.xy[2 * i + 1]);
+  void mesh_format(Body& b) {
+    const MshLine& l = b.take("$MeshFormat");
+    const std::string version = l.text.substr(0, l.text.find_first_of(" \t"));
+    if (version != "4.1") f_.fail(l.no, "unsupported .msh version '" + version + "' (need 4.1)");
+    Fields t(f_, l);
+    t.real("version");
+    if (t.integer("file-type") != 0)
+      f_.fail(l.no, "binary .msh files are not supported (need ASCII, file-type 0)");
+    t.integer("data-size");
+  }
+
+  void physical_names(Body& b) {
+    Fields head(f_, b.take("$PhysicalNames"));
+    const size_t n = head.count("numPhysicalNames");
+    for (size_t i = 0; i < n; ++i) {
+      const MshLine& l = b.take("physical name");
+      Fields t(f_, l);
+      const int dim = t.bounded("dimension");
+      const int tag = t.bounded("physical tag");
+      const std::string q = t.rest();
+      if (q.size() < 2 || q.front() != '"' || q.back() != '"')
+        f_.fail(l.no, "physical name must be quoted");
+      PhysicalGroup g{dim, tag, q.substr(1, q.size() - 2)};
+      for (const PhysicalGroup& o : mesh_.physical_groups)
+        if (o.dim == dim && o.tag == tag)
+          f_.fail(l.no, "duplicate physical group (" + std::to_string(dim) + ", " +
+                            std::to_string(tag) + ")");
+      mesh_.physical_groups.push_back(std::move(g));
     }
-    const double diag2 = m.num_nodes() ? (x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0) : 0.0;
-    const double min_area = 1e-14 * diag2;
-    Fields h(l, l.need("$Elements"));
-    const std::size_t nblocks = as_count(l, h.i("numEntityBlocks"), "numEntityBlocks");
-    const std::size_t total = as_count(l, h.i("numElements"), "numElements");
-    std::size_t seen = 0;
-    for (std::size_t b = 0; b < nblocks; ++b) {
-      Fields bh(l, l.need("element block header"));
-      const int edim = as_int(l, bh.i("entityDim"), "entityDim");
-      const int etag = as_int(l, bh.i("entityTag"), "entityTag");
-      const int type = as_int(l, bh.i("elementType"), "elementType");
-      const std::size_t n = as_count(l, bh.i("numElementsInBlock"), "numElementsInBlock");
-      seen += n;
-      const int npe = type == 1 ? 2 : type == 2 ? 3 : type == 15 ? 1 : 0;
-      if (npe == 0)
-        l.fail("unsupported element type " + std::to_string(type) +
-               " (supported: 1 = 2-node line, 2 = 3-node triangle, 15 = point)");
-      std::vector<std::string> groups;
-      if (type == 1) {
-        const auto it = ent_phys.find({edim, etag});
-        if (it != ent_phys.end())
-          for (int p : it->second) {
-            const auto nm = phys_name.find({edim, p});
-            groups.push_back(nm != phys_name.end() ? nm->second : std::to_string(p));
+  }
+
+  void entities_section(Body& b) {
+    Fields head(f_, b.take("$Entities"));
+    static const char* const kCount[4] = {"numPoints", "numCurves", "numSurfaces", "numVolumes"};
+    size_t per_dim[4];
+    for (int d = 0; d < 4; ++d) per_dim[d] = head.count(kCount[d]);
+    for (int d = 0; d < 4; ++d)
+      for (size_t i = 0; i < per_dim[d]; ++i) {
+        Fields t(f_, b.take("entity"));
+        EntityPhys e;
+        e.dim = d;
+        e.tag = t.bounded("entity tag");
+        for (int k = 0; k < (d == 0 ? 3 : 6); ++k) t.real("entity coordinate");  // point / box
+        const size_t np = t.count("numPhysicalTags");
+        for (size_t k = 0; k < np; ++k) e.phys.push_back(t.bounded("physical tag"));
+        // the bounding entities of curves and surfaces follow and are not needed
+        entity_phys_.push_back(std::move(e));
+      }
+  }
+
+  void nodes_section(Body& b) {
+    Fields head(f_, b.take("$Nodes"));
+    const size_t nblocks = head.count("numEntityBlocks");
+    const size_t nnodes = head.count("numNodes");
+    mesh_.xy.reserve(2 * nnodes);
+    mesh_.node_tags.reserve(nnodes);
+    std::vector<int> tag_line;
+    tag_line.reserve(nnodes);
+    for (size_t blk = 0; blk < nblocks; ++blk) {
+      Fields h(f_, b.take("node block header"));
+      h.integer("entityDim");
+      h.integer("entityTag");
+      h.integer("parametric");
+      const size_t n = h.count("numNodesInBlock");
+      for (size_t i = 0; i < n; ++i) {
+        const MshLine& l = b.take("node tag");
+        const long long tag = Fields(f_, l).integer("node tag");
+        if (tag <= 0) f_.fail(l.no, "node tag must be positive");
+        mesh_.node_tags.push_back(static_cast<size_t>(tag));
+        tag_line.push_back(l.no);
+      }
+      for (size_t i = 0; i < n; ++i) {
+        Fields t(f_, b.take("node coordinates"));
+        const double x = t.real("x"), y = t.real("y");
+        t.real("z");  // planar mesh: z (and any parametric coordinates) are not used
+        mesh_.xy.push_back(x);
+        mesh_.xy.push_back(y);
+      }
+    }
+    // tag -> index table; a repeated tag is reported where it first repeats in the file
+    by_tag_.resize(mesh_.node_tags.size());
+    for (size_t i = 0; i < by_tag_.size(); ++i) by_tag_[i] = {mesh_.node_tags[i], (int)i};
+    std::sort(by_tag_.begin(), by_tag_.end());
+    int dup = -1;
+    for (size_t i = 1; i < by_tag_.size(); ++i)
+      if (by_tag_[i].first == by_tag_[i - 1].first) {
+        const int second = by_tag_[i].second;  // the later of the two in file order
+        if (dup < 0 || second < dup) dup = second;
+      }
+    if (dup >= 0)
+      f_.fail(tag_line[dup], "duplicate node tag " + std::to_string(mesh_.node_tags[dup]));
+    if (mesh_.node_tags.size() != nnodes)
+      f_.fail(b.last_no(), "node count mismatch: header says " + std::to_string(nnodes) +
+                               ", blocks contain " + std::to_string(mesh_.node_tags.size()));
+  }
+
+  int node_index(long long tag, int line_no) const {
+    const auto it = std::lower_bound(by_tag_.begin(), by_tag_.end(),
+                                     std::pair<size_t, int>(static_cast<size_t>(tag), INT_MIN));
+    if (tag <= 0 || it == by_tag_.end() || it->first != static_cast<size_t>(tag))
+      f_.fail(line_no, "unknown node tag " + std::to_string(tag));
+    return it->second;
+  }
+
+  // physical names of the line entity (dim, tag): names, or the tag number when unnamed
+  std::vector<std::string> line_groups(int dim, int tag) const {
+    std::vector<std::string> out;
+    for (const EntityPhys& e : entity_phys_) {
+      if (e.dim != dim || e.tag != tag) continue;
+      out.clear();  // a repeated entity: the last definition holds
+      for (int p : e.phys) {
+        std::string name = std::to_string(p);
+        for (const PhysicalGroup& g : mesh_.physical_groups)
+          if (g.dim == dim && g.tag == p) {
+            name = g.name;
+            break;
           }
+        out.push_back(std::move(name));
       }
-      for (std::size_t k = 0; k < n; ++k) {
-        Fields f(l, l.need("element"));
-        f.i("element tag");
-        int conn[3] = {0, 0, 0};
-        for (int c = 0; c < npe; ++c) {
-          const long long tag = f.i("element node tag");
-          const auto it = index_of.find(tag);
-          if (tag <= 0 || it == index_of.end()) l.fail("unknown node tag " + std::to_string(tag));
-          conn[c] = it->second;
-        }
+    }
+    return out;
+  }
+
+  void elements_section(Body& b) {
+    double lo[2] = {std::numeric_limits<double>::max(), std::numeric_limits<double>::max()};
+    double hi[2] = {std::numeric_limits<double>::lowest(), std::numeric_limits<double>::lowest()};
+    const size_t nn = mesh_.num_nodes();
+    for (size_t i = 0; i < 2 * nn; ++i) {
+      lo[i & 1] = std::min(lo[i & 1], mesh_.xy[i]);
+      hi[i & 1] = std::max(hi[i & 1], mesh_.xy[i]);
+    }
+    const double dx = hi[0] - lo[0], dy = hi[1] - lo[1];
+    const double threshold = 1e-14 * (nn ? dx * dx + dy * dy : 0.0);
+
+    Fields head(f_, b.take("$Elements"));
+    const size_t nblocks = head.count("numEntityBlocks");
+    const size_t nelems = head.count("numElements");
+    size_t total = 0;
+    for (size_t blk = 0; blk < nblocks; ++blk) {
+      const MshLine& hl = b.take("element block header");
+      Fields h(f_, hl);
+      const int edim = h.bounded("entityDim");
+      const int etag = h.bounded("entityTag");
+      const int type = h.bounded("elementType");
+      const size_t n = h.count("numElementsInBlock");
+      total += n;
+      const int nodes = type == 1 ? 2 : type == 2 ? 3 : type == 15 ? 1 : 0;
+      if (!nodes)
+        f_.fail(hl.no, "unsupported element type " + std::to_string(type) +
+                           " (supported: 1 = 2-node line, 2 = 3-node triangle, 15 = point)");
+      const std::vector<std::string> groups =
+          type == 1 ? line_groups(edim, etag) : std::vector<std::string>();
+      for (size_t i = 0; i < n; ++i) {
+        const MshLine& l = b.take("element");
+        Fields t(f_, l);
+        t.integer("element tag");
+        int c[3] = {0, 0, 0};
+        for (int k = 0; k < nodes; ++k) c[k] = node_index(t.integer("element node tag"), l.no);
         if (type == 2) {
-          const double* p0 = &m.xy[2 * conn[0]];
-          const double* p1 = &m.xy[2 * conn[1]];
-          const double* p2 = &m.xy[2 * conn[2]];
-          // element.cpp:17-19
-          const double area =
-              0.5 * ((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
-          if (area == 0.0 || std::abs(area) < min_area) {
-            std::ostringstream msg;
-            msg << "degenerate triangle (area " << area << ", threshold " << min_area << ")";
-            l.fail(msg.str());
+          const double area = signed_area(mesh_.xy.data(), c);
+          if (area == 0.0 || std::fabs(area) < threshold) {
+            char msg[128];
+            std::snprintf(msg, sizeof msg, "degenerate triangle (area %g, threshold %g)", area,
+                          threshold);
+            f_.fail(l.no, msg);
           }
           if (area < 0.0) {
-            std::swap(conn[1], conn[2]);
-            ++m.flipped_triangles;
+            std::swap(c[1], c[2]);
+            ++mesh_.flipped_triangles;
           }
-          m.triangles.insert(m.triangles.end(), conn, conn + 3);
+          mesh_.triangles.insert(mesh_.triangles.end(), c, c + 3);
         } else if (type == 1) {
-          for (const auto& g : groups) m.boundary_edges[g].push_back({conn[0], conn[1]});
+          for (const std::string& g : groups) mesh_.boundary_edges[g].push_back({c[0], c[1]});
         }
       }
     }
-    if (seen != total)
-      l.fail("element count mismatch: header says " + std::to_string(total) +
-             ", blocks contain " + std::to_string(seen));
-    end_of(l, "Elements");
-    elems = true;
+    if (total != nelems)
+      f_.fail(b.last_no(), "element count mismatch: header says " + std::to_string(nelems) +
+                               ", blocks contain " + std::to_string(total));
   }
 
-  void skip(const std::string& section) {
-    std::string line;
-    while (l.next(line))
-      if (line == "$End" + section) return;
-    l.fail("unexpected end of file inside $" + section);
-  }
+  struct EntityPhys {
+    int dim = 0, tag = 0;
+    std::vector<int> phys;
+  };
+
+  const MshFile& f_;
+  Mesh mesh_;
+  std::vector<EntityPhys> entity_phys_;
+  std::vector<std::pair<size_t, int>> by_tag_;  // (node tag, index), ascending
 };
+
+std::string trimmed(const std::string& s) {
+  size_t a = 0, b = s.size();
+  while (a < b && is_space(s[a])) ++a;
+  while (b > a && is_space(s[b - 1])) --b;
+  return s.substr(a, b - a);
+}
 
 }  // namespace
 
 Mesh read_gmsh(std::istream& in, const std::string& source_name) {
-  Lines l(in, source_name);
-  Mesh m;
-  Reader r{l, m};
-  std::string line;
-  while (l.next(line)) {
-    if (line.front() != '$' || line.rfind("$End", 0) == 0)
-      l.fail("expected a section header, got '" + line + "'");
-    const std::string sec = line.substr(1);
-    if (!r.fmt && sec != "MeshFormat") l.fail("file must start with $MeshFormat");
-    if (sec == "MeshFormat") {
-      if (r.fmt) l.fail("duplicate $MeshFormat section");
-      r.format();
-    } else if (sec == "PhysicalNames") {
-      r.physical_names();
-    } else if (sec == "Entities") {
-      r.entities();
-    } else if (sec == "Nodes") {
-      r.node_section();
-    } else if (sec == "Elements") {
-      r.element_section();
-    } else {
-      r.skip(sec);
-    }
-  }
-  const std::pair<bool, const char*> need[] = {{r.fmt, "$MeshFormat"},
-                                               {r.names, "$PhysicalNames"},
-                                               {r.ents, "$Entities"},
-                                               {r.nodes, "$Nodes"},
-                                               {r.elems, "$Elements"}};
-  for (const auto& [ok, name] : need)
-    if (!ok) l.fail(std::string("missing ") + name + " section");
-  return m;
+  const MshFile file(in, source_name);
+  return MshParser(file).run();
 }
 
 Mesh read_gmsh(const std::string& path) {
@@ -315,11 +455,13 @@ Mesh read_gmsh(const std::string& path) {
   return read_gmsh(in, path);
 }
 
+// ---- writer ------------------------------------------------------------------------------
+
 void write_gmsh(const Mesh& m, std::ostream& out) {
-  const std::size_t nn = m.num_nodes(), ne = m.num_triangles();
-  const int ng = (int)m.boundary_edges.size();
+  const size_t nn = m.num_nodes(), ne = m.num_triangles();
+  const int ng = static_cast<int>(m.boundary_edges.size());
   double x0 = 0, x1 = 0, y0 = 0, y1 = 0;
-  for (std::size_t i = 0; i < nn; ++i) {
+  for (size_t i = 0; i < nn; ++i) {
     const double x = m.xy[2 * i], y = m.xy[2 * i + 1];
     if (i == 0 || x < x0) x0 = x;
     if (i == 0 || x > x1) x1 = x;
@@ -339,121 +481,111 @@ void write_gmsh(const Mesh& m, std::ostream& out) {
   out << "1 " << x0 << " " << y0 << " 0 " << x1 << " " << y1 << " 0 1 " << ng + 1 << " 0\n";
   out << "$EndEntities\n";
   out << "$Nodes\n1 " << nn << " 1 " << nn << "\n2 1 0 " << nn << "\n";
-  for (std::size_t i = 0; i < nn; ++i) out << i + 1 << "\n";
-  for (std::size_t i = 0; i < nn; ++i) out << m.xy[2 * i] << " " << m.xy[2 * i + 1] << " 0\n";
+  for (size_t i = 0; i < nn; ++i) out << i + 1 << "\n";
+  for (size_t i = 0; i < nn; ++i) out << m.xy[2 * i] << " " << m.xy[2 * i + 1] << " 0\n";
   out << "$EndNodes\n";
-  std::size_t nedges = 0;
+  size_t nedges = 0;
   for (const auto& g : m.boundary_edges) nedges += g.second.size();
   out << "$Elements\n" << ng + 1 << " " << ne + nedges << " 1 " << ne + nedges << "\n";
-  std::size_t etag = 1;
+  size_t etag = 1;
   int c = 1;
   for (const auto& g : m.boundary_edges) {
     out << "1 " << c++ << " 1 " << g.second.size() << "\n";
     for (const auto& e : g.second) out << etag++ << " " << e[0] + 1 << " " << e[1] + 1 << "\n";
   }
   out << "2 1 2 " << ne << "\n";
-  for (std::size_t e = 0; e < ne; ++e)
+  for (size_t e = 0; e < ne; ++e)
     out << etag++ << " " << m.triangles[3 * e] + 1 << " " << m.triangles[3 * e + 1] + 1 << " "
         << m.triangles[3 * e + 2] + 1 << "\n";
   out << "$EndElements\n";
 }
 
-// ---- case file ------------------------------------------------------------------------------
+// ---- case keys ---------------------------------------------------------------------------
 
-CaseFile CaseFile::load(const std::string& path) {
+CaseKeys CaseKeys::read(const std::string& path) {
   std::ifstream in(path);
   if (!in) throw std::runtime_error(path + ": cannot open file");
-  return parse(in, path);
+  return read(in, path);
 }
 
-CaseFile CaseFile::parse(std::istream& in, const std::string& source_name) {
-  CaseFile cf;
-  cf.source_ = source_name;
+CaseKeys CaseKeys::read(std::istream& in, const std::string& source_name) {
+  CaseKeys ck;
+  ck.source_ = source_name;
   std::string raw;
-  int no = 0;
-  while (std::getline(in, raw)) {
-    ++no;
-    const std::string line = trim(raw.substr(0, raw.find('#')));
+  for (int no = 1; std::getline(in, raw); ++no) {
+    const std::string line = trimmed(raw.substr(0, raw.find('#')));
     if (line.empty()) continue;
-    const auto eq = line.find('=');
+    const size_t eq = line.find('=');
+    const std::string at = source_name + ":" + std::to_string(no) + ": ";
     if (eq == std::string::npos)
-      throw std::runtime_error(cf.where(no) + ": expected 'key = value', got '" + line + "'");
-    const std::string key = trim(line.substr(0, eq)), value = trim(line.substr(eq + 1));
-    if (key.empty()) throw std::runtime_error(cf.where(no) + ": empty key");
-    if (key.find_first_of(" \t") != std::string::npos)
-      throw std::runtime_error(cf.where(no) + ": key '" + key + "' contains whitespace");
-    const auto [it, fresh] = cf.entries_.emplace(key, Entry{value, no});
-    if (!fresh)
-      throw std::runtime_error(cf.where(no) + ": duplicate key '" + key +
-                               "' (first defined on line " + std::to_string(it->second.line) + ")");
-    cf.order_.push_back(key);
+      throw std::runtime_error(at + "expected 'key = value', got '" + line + "'");
+    Item it{trimmed(line.substr(0, eq)), trimmed(line.substr(eq + 1)), no};
+    if (it.key.empty() || it.key.find_first_of(" \t") != std::string::npos)
+      throw std::runtime_error(at + "bad key '" + it.key + "'");
+    if (const Item* prev = ck.find(it.key))
+      throw std::runtime_error(at + "key '" + it.key + "' repeats line " +
+                               std::to_string(prev->line));
+    ck.items_.push_back(std::move(it));
   }
-  return cf;
+  return ck;
 }
 
-bool CaseFile::has(const std::string& key) const { return entries_.count(key) != 0; }
-
-std::string CaseFile::where(int line) const { return source_ + ":" + std::to_string(line); }
-
-const CaseFile::Entry& CaseFile::entry(const std::string& key) const {
-  const auto it = entries_.find(key);
-  if (it == entries_.end()) throw std::runtime_error(source_ + ": missing required key '" + key + "'");
-  return it->second;
+const CaseKeys::Item* CaseKeys::find(const std::string& key) const {
+  for (const Item& it : items_)
+    if (it.key == key) return &it;
+  return nullptr;
 }
 
-std::string CaseFile::get_string(const std::string& key) const { return entry(key).value; }
-
-std::string CaseFile::get_string(const std::string& key, const std::string& fallback) const {
-  return has(key) ? get_string(key) : fallback;
+const CaseKeys::Item& CaseKeys::need(const std::string& key) const {
+  const Item* it = find(key);
+  if (!it) throw std::runtime_error(source_ + ": key '" + key + "' is required");
+  return *it;
 }
 
-double CaseFile::get_double(const std::string& key) const {
-  const Entry& e = entry(key);
-  const char* b = e.value.c_str();
+std::string CaseKeys::text(const std::string& key) const { return need(key).value; }
+
+std::string CaseKeys::text(const std::string& key, const std::string& fallback) const {
+  const Item* it = find(key);
+  return it ? it->value : fallback;
+}
+
+double CaseKeys::parse_number(const Item& it) const {
   char* end = nullptr;
   errno = 0;
-  const double v = std::strtod(b, &end);
-  if (e.value.empty() || end != b + e.value.size())
-    throw std::runtime_error(where(e.line) + ": key '" + key + "': '" + e.value + "' is not a number");
-  if (errno == ERANGE || !std::isfinite(v))
-    throw std::runtime_error(where(e.line) + ": key '" + key + "': '" + e.value +
-                             "' is out of range or not finite");
+  const double v = std::strtod(it.value.c_str(), &end);
+  if (it.value.empty() || *end != '\0' || errno == ERANGE || !std::isfinite(v))
+    throw std::runtime_error(source_ + ":" + std::to_string(it.line) + ": '" + it.key +
+                             "' needs a finite number, got '" + it.value + "'");
   return v;
 }
 
-double CaseFile::get_double(const std::string& key, double fallback) const {
-  return has(key) ? get_double(key) : fallback;
+double CaseKeys::number(const std::string& key) const { return parse_number(need(key)); }
+
+double CaseKeys::number(const std::string& key, double fallback) const {
+  const Item* it = find(key);
+  return it ? parse_number(*it) : fallback;
 }
 
-int CaseFile::get_int(const std::string& key) const {
-  const Entry& e = entry(key);
-  const char* b = e.value.c_str();
+int CaseKeys::integer(const std::string& key, int fallback) const {
+  const Item* it = find(key);
+  if (!it) return fallback;
   char* end = nullptr;
   errno = 0;
-  const long v = std::strtol(b, &end, 10);
-  if (e.value.empty() || end != b + e.value.size())
-    throw std::runtime_error(where(e.line) + ": key '" + key + "': '" + e.value + "' is not an integer");
-  if (errno == ERANGE || v < INT_MIN || v > INT_MAX)
-    throw std::runtime_error(where(e.line) + ": key '" + key + "': '" + e.value + "' is out of range");
-  return (int)v;
+  const long long v = std::strtoll(it->value.c_str(), &end, 10);
+  if (it->value.empty() || *end != '\0' || errno == ERANGE || v < INT_MIN || v > INT_MAX)
+    throw std::runtime_error(source_ + ":" + std::to_string(it->line) + ": '" + key +
+                             "' needs an integer, got '" + it->value + "'");
+  return static_cast<int>(v);
 }
 
-int CaseFile::get_int(const std::string& key, int fallback) const {
-  return has(key) ? get_int(key) : fallback;
-}
-
-std::vector<std::string> CaseFile::unknown_keys(const std::vector<std::string>& known) const {
-  std::vector<std::string> out;
-  for (const auto& k : order_)
-    if (std::find(known.begin(), known.end(), k) == known.end()) out.push_back(k);
-  return out;
-}
-
-int CaseFile::warn_unknown_keys(const std::vector<std::string>& known, std::ostream& os) const {
-  const auto unknown = unknown_keys(known);
-  for (const auto& k : unknown)
-    os << "warning: " << where(entries_.at(k).line) << ": unknown key '" << k << "'\n";
-  return (int)unknown.size();
+int CaseKeys::report_unknown(const std::vector<std::string>& known, std::ostream& os) const {
+  int count = 0;
+  for (const Item& it : items_)
+    if (std::find(known.begin(), known.end(), it.key) == known.end()) {
+      os << "warning: " << source_ << ":" << it.line << ": unknown key '" << it.key << "'\n";
+      ++count;
+    }
+  return count;
 }
 
 }  // namespace cstfem_hip

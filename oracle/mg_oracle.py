@@ -22,13 +22,15 @@ preconditioner; this module restates the device algorithm step by step:
                 levels l+1 <= kdepth, the K-cycle of level l+1: two flexible CG steps (x = 0,
                 alpha = p'r/p'q with 0 when p'q <= 0, beta = -z'q/p'q) preconditioned by the
                 cycle of level l+1, and below that one cycle of level l+1 (V-cycle).
+  fcg           the flexible CG the preconditioner runs in (FCG(1), the device's recurrence).
 
 Pinning (tests/test_oracle_mg.py): known answers for the aggregation and the Gram-Schmidt
 step; the prolongator is orthonormal per aggregate and reproduces the near-null space
 (P R = B); coarse matrices are symmetric and the global rigid-body modes of the unconstrained
 operator stay in the range of every prolongator and in the null space of every coarse matrix;
-the two-level cycle is symmetric positive; and CG with the cycle needs a small fraction of
-Jacobi-CG's iterations.
+the two-level cycle is symmetric positive; CG with the cycle needs a small fraction of
+Jacobi-CG's iterations; and fcg with the Jacobi preconditioner reproduces the C oracle's
+Jacobi-PCG (the same Krylov space: iterations and solution).
 """
 from __future__ import annotations
 
@@ -184,11 +186,19 @@ def cheb_coeffs(lam):
     return 1.0 / theta, rho1 * rho0, 2.0 * rho1 / delta
 
 
+def rowsum_matrix(L):
+    """The matrix the cycle's row sums read on a level: L.As when a test sets it (values
+    rounded as the device stores them), else L.A.  Galerkin products, diagonals and the
+    coarsest inverse always come from L.A."""
+    return getattr(L, "As", None) if getattr(L, "As", None) is not None else L.A
+
+
 def cheb2(L, x, r, lam):
     c1r, c2d, c2r = cheb_coeffs(lam)
-    d = c1r * (L.dinv * (r - (L.A @ x if x is not None else 0.0)))
+    A = rowsum_matrix(L)
+    d = c1r * (L.dinv * (r - (A @ x if x is not None else 0.0)))
     x = (x if x is not None else 0.0) + d
-    d = c2d * d + c2r * (L.dinv * (r - L.A @ x))
+    d = c2d * d + c2r * (L.dinv * (r - A @ x))
     return x + d
 
 
@@ -202,19 +212,20 @@ def coarse_correction(levels, l, rc, lams, kdepth=99):
 
 def cycle(levels, l, r, lams, M=None, kdepth=99):
     L = levels[l]
+    A = rowsum_matrix(L)
     if l == 0 and M is not None:
         x = M @ r
     else:
         x = cheb2(L, None, r, lams[l])
-    rc = L.P.T @ (r - L.A @ x)
+    rc = L.P.T @ (r - A @ x)
     x = x + L.P @ coarse_correction(levels, l, rc, lams, kdepth)
     if l == 0 and M is not None:
-        return x + M @ (r - L.A @ x)
+        return x + M @ (r - A @ x)
     return cheb2(L, x, r, lams[l])
 
 
 def kcycle(levels, l, rhs, lams, kdepth=99):
-    A = levels[l].A
+    A = rowsum_matrix(levels[l])
     res = rhs.copy()
     z = cycle(levels, l, res, lams, kdepth=kdepth)
     p = z.copy()
@@ -231,3 +242,30 @@ def kcycle(levels, l, rhs, lams, kdepth=99):
     pq, pr = p @ q, p @ res
     alpha = pr / pq if pq > 0 else 0.0
     return x + alpha * p
+
+
+def fcg(A, b, precond, rtol=1e-10, max_iter=1000):
+    """Flexible CG with one kept direction (Notay's FCG(1)), the device's recurrence
+    (csrc/pcg.hip fcg_run): x = 0, r = b, z = B r, p = z; per iteration q = A p,
+    alpha = p'r / p'q, x += alpha p, r -= alpha q, stop when ||r|| / ||b|| < rtol,
+    z = B r, beta = -z'q / p'q, p = z + beta p.  Returns (x, iterations, residual history)."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    bnorm = np.linalg.norm(b)
+    hist = [1.0]
+    if bnorm == 0.0:
+        return x, 0, hist
+    p = precond(r)
+    for k in range(1, max_iter + 1):
+        q = A @ p
+        pq = p @ q
+        alpha = (p @ r) / pq
+        x = x + alpha * p
+        r = r - alpha * q
+        hist.append(np.linalg.norm(r) / bnorm)
+        if hist[-1] < rtol:
+            return x, k, hist
+        z = precond(r)
+        beta = -(z @ q) / pq
+        p = z + beta * p
+    return x, max_iter, hist

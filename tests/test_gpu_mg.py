@@ -4,6 +4,9 @@
 - coarse matrices (Galerkin products on the device) equal the oracle's to 1e-12 relative;
 - one application of the cycle equals the oracle's cycle (FP64; reductions in a different
   order) to 1e-9 relative, for hierarchies of 2 to 5 levels and K-cycle depths 1 to all;
+- the headline cycle with the learned fine smoother equals the oracle's cycle given the
+  device's M and stored values (FP32 copies included) to 1e-9, and its flexible CG (z'q
+  from the smoothing kernel) matches the oracle's FCG in iterations and solution;
 - FCG with the multilevel preconditioner reaches the FP64 true residual in a small fraction
   of Jacobi-CG's iterations, with the Chebyshev and with the learned fine smoother;
 - the trained smoother needs fewer iterations than the untrained (polynomial) one and than
@@ -23,6 +26,7 @@ import scipy.sparse as sp
 import cstfem_hip
 from cstfem_hip import learned, pipeline
 from oracle import mg_oracle as mgo
+from oracle import mpnn_oracle as mo
 
 pytestmark = pytest.mark.gpu
 
@@ -84,6 +88,105 @@ def test_hierarchy_and_cycle_match_oracle(dev, n, max_levels, coarse_max, kdepth
         assert np.linalg.norm(z - zo) <= 1e-9 * np.linalg.norm(zo)
     # the cycle is deterministic
     np.testing.assert_array_equal(dev.mg_apply(r), z)
+
+
+@pytest.fixture(scope="module")
+def theta40(dev):
+    """A network trained briefly (40 Adam steps from the polynomial start) on a small plate of
+    the family: its decoder uses all of a, b, d, c."""
+    theta, _ = learned.train(dev, learned.training_plate(24), steps=40)
+    return theta
+
+
+def device_m(dev, prob, A):
+    """The learned M as the device stores it (csrc/mpnn.hip, k_decode): FP32 values formed
+    from the decoder outputs [a, b, d, c] of every stored block in the device's operation
+    order, g = (s_i a) s_j + c (float)(d_i K_ij d_j) with s = (float)sqrt(D^-1)."""
+    g = mo.Graph(A.indptr, A.indices, A.data, prob.mesh.tri, dev.n // 2)
+    out = dev.mpnn_download(2, len(g.bcol) * mo.DEC).reshape(-1, mo.DEC)
+    a, b, d, c = (out[:, u] for u in range(4))
+    s = np.sqrt(g.invd).astype(np.float32)
+    i, j = g.brow, g.bcol
+    si, sj = (s[2 * i], s[2 * i + 1]), (s[2 * j], s[2 * j + 1])
+    di, dj = (g.invd[2 * i], g.invd[2 * i + 1]), (g.invd[2 * j], g.invd[2 * j + 1])
+    blk = g.blocks
+    vals = []
+    for (u, v), w in zip(((0, 0), (0, 1), (1, 0), (1, 1)), (a, b, b, d)):
+        t = (di[u] * blk[:, u, v] * dj[v]).astype(np.float32)
+        vals.append((si[u] * w * sj[v] + c * t).astype(np.float64))
+    rows = np.concatenate([2 * i, 2 * i, 2 * i + 1, 2 * i + 1])
+    cols = np.concatenate([2 * j, 2 * j + 1, 2 * j, 2 * j + 1])
+    return sp.csr_matrix((np.concatenate(vals), (rows, cols)), shape=A.shape)
+
+
+def rounded32(A):
+    return sp.csr_matrix((A.data.astype(np.float32).astype(np.float64), A.indices, A.indptr),
+                         shape=A.shape)
+
+
+def learned_hierarchy(dev, theta, monkeypatch, coarse32, tail, n=40, kdepth=3):
+    """The device's learned-smoother hierarchy on a plate and the oracle's, with the oracle's
+    row sums reading the values the device's cycle reads: FP32 copies of K (level-0
+    residuals) and of the coarse matrices when CSTFEM_MG_COARSE32 is on."""
+    prob = hetero(n, e_seed=3, seed=4)
+    A, xy, fixed, bptr, bcol = system(dev, prob)
+    dev.mpnn_set_params(theta)
+    dev.mpnn_build()
+    monkeypatch.setenv("CSTFEM_MG_COARSE32", coarse32)
+    monkeypatch.setenv("CSTFEM_MG_TAIL", tail)
+    dev.mg_setup(6, 40, 0, kdepth)
+    levels = mgo.setup(A, xy, fixed, max_levels=6, coarse_max=40, ptr0=bptr, col0=bcol)
+    assert dev.mg_levels() == len(levels) >= 4
+    lams = [dev.mg_level_info(l)[3] for l in range(len(levels))]
+    assert 0.5 < lams[0] <= 1.5  # the set-up kept the learned smoother
+    if coarse32 == "1":
+        for L in levels[:-1]:
+            L.As = rounded32(L.A)
+    return prob, A, levels, lams, device_m(dev, prob, A)
+
+
+@pytest.mark.parametrize("coarse32,tail,kdepth", [("0", "0", 3), ("1", "0", 3), ("1", "0", 1),
+                                                  ("1", "1", 1)])
+def test_learned_cycle_matches_oracle(dev, theta40, monkeypatch, coarse32, tail, kdepth):
+    """One application of the headline cycle (learned fine smoother: k0_msmooth, k0_resid or
+    k0_resid32, restriction, K-cycles or V-cycles, prolongation) against the oracle's cycle
+    with the same M and the same stored values: 1e-12 relative (summation orders differ;
+    measured ~1e-14, while M's values taken in FP64 instead move the cycle by ~3e-9).  With
+    the fused tail, whose levels keep FP64 values where the oracle rounds them, 1e-5."""
+    prob, A, levels, lams, M = learned_hierarchy(dev, theta40, monkeypatch, coarse32, tail,
+                                                 kdepth=kdepth)
+    rng = np.random.default_rng(17)
+    tol = 1e-12 if tail == "0" else 1e-5
+    for _ in range(2):
+        r = rng.standard_normal(dev.n)
+        z = dev.mg_apply(r)
+        zo = mgo.cycle(levels, 0, r, lams, M=M, kdepth=kdepth)
+        err = np.linalg.norm(z - zo) / np.linalg.norm(zo)
+        print(f"coarse32 {coarse32} tail {tail}: cycle vs oracle {err:.3e}")
+        assert err <= tol
+    # a wrong smoother value is seen: the oracle with M's values in FP64 differs by more
+    if tail == "0":
+        M64 = mo.m_matrix(dev.mpnn_download(2, M.nnz // 4 * mo.DEC).reshape(-1, mo.DEC),
+                          mo.Graph(A.indptr, A.indices, A.data, prob.mesh.tri, dev.n // 2))
+        z64 = mgo.cycle(levels, 0, r, lams, M=M64, kdepth=kdepth)
+        err64 = np.linalg.norm(z - z64) / np.linalg.norm(zo)
+        print(f"  with M's values in FP64: {err64:.3e}")
+        assert err64 > 100 * tol
+
+
+def test_learned_fcg_matches_oracle_fcg(dev, theta40, monkeypatch):
+    """The flexible CG with the headline cycle, whose z'q comes from the last smoothing
+    kernel's partials (k0_msmooth's epilogue), against the oracle's FCG(1) with the oracle's
+    cycle: iterations within 2, solution within 1e-8."""
+    prob, A, levels, lams, M = learned_hierarchy(dev, theta40, monkeypatch, "1", "0")
+    b = dev.download_rhs()
+    x, st = dev.solve(rtol=1e-10, max_iter=500, method=FCG)
+    xo, its, hist = mgo.fcg(A, b, lambda r: mgo.cycle(levels, 0, r, lams, M=M, kdepth=3),
+                            rtol=1e-10, max_iter=500)
+    print(f"device {st.iterations} iterations ({st.outer_iterations} refreshes), oracle {its}")
+    assert st.converged and not st.fallback and st.outer_iterations <= 2
+    assert abs(st.iterations - its) <= 2, (st.iterations, its)
+    assert np.max(np.abs(x - xo)) <= 1e-8 * np.max(np.abs(xo))
 
 
 @pytest.mark.parametrize("smoother", [1, 0])

@@ -138,6 +138,46 @@ def test_malformed_files(tool, tmp_path, text, line, message):
     assert message in err, err
 
 
+OK_ELEMS = elements([(1, 2, 3)])
+MORE = [
+    (HEAD.replace('1 1 "edge"', "1 1 edge") + NODES + OK_ELEMS, 6, "physical name must be quoted"),
+    (HEAD.replace('2 2 "domain"', '1 1 "dom"') + NODES + OK_ELEMS, 7,
+     "duplicate physical group (1, 1)"),
+    (HEAD + NODES.replace("1 4 1 4", "1 5 1 4") + OK_ELEMS, 24,
+     "node count mismatch: header says 5, blocks contain 4"),
+    (HEAD + NODES + OK_ELEMS.replace("2 2 1 9", "2 3 1 9"), 31,
+     "element count mismatch: header says 3, blocks contain 2"),
+    (HEAD + NODES.replace("\n2\n3\n", "\n-2\n3\n") + OK_ELEMS, 18, "node tag must be positive"),
+    (HEAD + NODES.replace("1 0 0\n", "1 zz 0\n") + OK_ELEMS, 22, "expected number (y)"),
+    (HEAD + OK_ELEMS + NODES, 14, "$Elements must come after $Nodes"),
+    (HEAD + "junk\n" + NODES, 14, "expected a section header, got 'junk'"),
+    (HEAD + "$EndNodes\n" + NODES, 14, "expected a section header, got '$EndNodes'"),
+    (HEAD + "$MeshFormat\n4.1 0 8\n$EndMeshFormat\n" + NODES, 14, "duplicate $MeshFormat section"),
+    (HEAD + NODES + OK_ELEMS + "$Comments\nhello\n", 34, "unexpected end of file inside $Comments"),
+    (HEAD + NODES.replace("$EndNodes", "0 0 0\n$EndNodes") + OK_ELEMS, 25,
+     "expected $EndNodes, got '0 0 0'"),
+    (HEAD.replace("0 1 1 0", "0 -1 1 0") + NODES, 10, "numCurves must be non-negative"),
+]
+
+
+@pytest.mark.parametrize("text,line,message", MORE)
+def test_more_malformed_files(tool, tmp_path, text, line, message):
+    path = write(tmp_path, text)
+    m, err = read(tool, path, tmp_path)
+    assert m is None and err == f"{path}:{line}: {message}", err
+
+
+def test_lenient_like_a_stream_and_skips_unknown_sections(tool, tmp_path):
+    """A number's trailing characters are left unread, as the reference's stream reads do;
+    unknown sections are skipped; point elements are accepted and ignored."""
+    text = (HEAD + "$Comments\nhello\n$EndComments\n" + NODES.replace("\n2\n3\n", "\n2x\n3\n")
+            + OK_ELEMS.replace("2 2 1 9", "3 3 1 9").replace("$EndElements\n", "")
+            + "0 1 15 1\n9 1\n$EndElements\n")
+    m, err = read(tool, write(tmp_path, text), tmp_path)
+    assert err is None, err
+    np.testing.assert_array_equal(m["tri"], [[0, 1, 2]])
+
+
 def test_errors_match_the_compiled_reference(tool, tmp_path, ref):
     """Where the reference's own reader is built (oracle/_ref), its message for each malformed
     file equals the host reader's."""
@@ -149,7 +189,7 @@ def test_errors_match_the_compiled_reference(tool, tmp_path, ref):
     cases = [HEAD.replace("4.1 0 8", "2.2 0 8") + NODES, HEAD + NODES,
              HEAD + NODES.replace("\n2\n3\n", "\n2\n2\n"),
              HEAD + NODES + elements([(1, 2, 9)]), HEAD + NODES + elements([(1, 2, 2)]),
-             HEAD + NODES + elements([(1, 2, 3, 4)], etype=3)]
+             HEAD + NODES + elements([(1, 2, 3, 4)], etype=3)] + [t for t, _, _ in MORE]
     for k, text in enumerate(cases):
         path = str(tmp_path / f"c{k}.msh")
         open(path, "w").write(text)

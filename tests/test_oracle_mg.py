@@ -133,3 +133,28 @@ def test_multilevel_fcg_beats_jacobi(oracle):
         assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-9
         counts[name] = it[0]
     assert counts["multilevel"] * 8 < counts["jacobi"], counts
+
+
+def test_fcg_with_jacobi_reproduces_pcg(oracle):
+    """FCG(1) with a fixed SPD preconditioner spans the same Krylov space as PCG, so with
+    B = D^-1 it reproduces the C oracle's Jacobi-PCG (solver.cpp:35-105): iterations within
+    one, solution to 1e-8; a dropped z'q term (beta = 0, steepest descent) does not."""
+    A, b, _, _ = plate_system(oracle, n=24)
+    dinv = 1.0 / A.diagonal()
+    x, its, hist = mg.fcg(A, b, lambda r: dinv * r, rtol=1e-10, max_iter=5000)
+    rp, c, v = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    x_ref, st = oracle.pcg_jacobi(rp, c, v, b, 1e-10, 5000)
+    assert abs(its - st["iterations"]) <= 1, (its, st["iterations"])
+    assert np.max(np.abs(x - x_ref)) <= 1e-8 * np.max(np.abs(x_ref))
+    assert len(hist) == its + 1 and hist[-1] < 1e-10
+
+    def steepest(A, b, precond, rtol, max_iter):  # FCG with the z'q term planted as zero
+        x, r = np.zeros_like(b), b.copy()
+        for k in range(1, max_iter + 1):
+            p = precond(r)
+            q = A @ p
+            x, r = x + (p @ r) / (p @ q) * p, r - (p @ r) / (p @ q) * q
+            if np.linalg.norm(r) < rtol * np.linalg.norm(b):
+                return k
+        return max_iter
+    assert steepest(A, b, lambda r: dinv * r, 1e-10, 5000) > 3 * its
