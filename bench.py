@@ -216,6 +216,148 @@ def run_batch(args):
     dev.close()
 
 
+
+def method_codes():
+    import cstfem_hip
+    return {"fp64": cstfem_hip.METHOD_CG_FP64_REFINED, "ir32": cstfem_hip.METHOD_IR_FP32,
+            "learned1": cstfem_hip.METHOD_FCG_LEARNED,
+            "learned32": cstfem_hip.METHOD_FCG_LEARNED_FP32,
+            "learned_mg": cstfem_hip.METHOD_FCG_LEARNED,
+            "mg_cheb": cstfem_hip.METHOD_FCG_LEARNED}
+
+
+def time_methods(dev, prob, args, chosen, thetas, kind=None):
+    """Times `args.steps` steps (after `args.warmup`) of every method in `chosen` on the
+    problem loaded in `dev`.  thetas: network parameters per method name (methods using the
+    network); kind: the solver of each name when it is not the name itself (e.g. two
+    learned_mg variants with different networks)."""
+    from cstfem_hip import pipeline
+    codes = method_codes()
+    kind = kind or {}
+
+    def step(name):
+        method = kind.get(name, name)
+        times = pipeline.PhaseTimes()
+        pipeline.assemble_and_constrain(dev, prob, times,
+                                        multilevel=method in ("learned_mg", "mg_cheb"))
+        t0 = time.perf_counter()
+        dev.setup()
+        if name in thetas:
+            dev.mpnn_set_params(thetas[name])
+            dev.mpnn_build()
+        if method in ("learned_mg", "mg_cheb"):
+            learned_fine = method == "learned_mg"
+            dev.mg_setup(args.mg_levels, args.mg_coarse, 0 if learned_fine else 1,
+                         args.mg_kcycle if learned_fine else args.mg_kcycle_cheb)
+        t1 = time.perf_counter()
+        _, st = dev.solve(rtol=1e-10, max_iter=args.max_iter, method=codes[method],
+                          download=False)
+        times.solve += time.perf_counter() - t0
+        st.precond_s = t1 - t0
+        return st, times
+
+    results = {}
+    for meth in chosen:
+        for i in range(args.warmup):
+            st, times = step(meth)
+            log(f"{meth} warmup {i}: {st.message}, {st.outer_iterations} refinements, "
+                f"true rel {st.true_rel_residual:.3e}")
+        step_times = []
+        for i in range(args.steps):
+            t0 = time.perf_counter()
+            st, times = step(meth)
+            dt = time.perf_counter() - t0
+            step_times.append(dt)
+            log(f"{meth} step {i}: {dt:.3f} s, {st.message}, {st.outer_iterations} refinements, "
+                f"true rel {st.true_rel_residual:.3e}")
+        verified = st.converged and st.true_rel_residual < 1e-10
+        if not verified:
+            log(f"WARNING: {meth} not verified: {st.message}; true rel {st.true_rel_residual:.3e}")
+        results[meth] = dict(s=sum(step_times) / len(step_times), iterations=st.iterations,
+                             refinements=st.outer_iterations, verified=bool(verified),
+                             true_rel_residual=st.true_rel_residual,
+                             fallback=int(st.fallback),
+                             s_per_iteration=(times.solve - st.precond_s) / max(1, st.iterations),
+                             phases_s={"pattern": times.pattern, "assembly": times.assembly,
+                                       "bcs": times.bcs, "solve": times.solve,
+                                       "precond": st.precond_s})
+    return results
+
+
+def run_protocol(args):
+    """The learned preconditioner's training protocol and its deployment: pairs harvested at
+    plain Jacobi-CG iterations 1..50 of 2,000 plates of 128x128 cells (ln E ~ N(0,1) per
+    element, nu = 0.3, plane strain, unit traction of random direction), solved in batches on
+    the device, plus white-noise pairs (training on the harvested pairs alone gives a
+    smoother the set-up check rejects); the network then runs as the fine smoother of the
+    multilevel flexible CG (FP32 values in the cycle, FP64 outer iteration) on the 4M-DOF
+    heterogeneous plate.  Beside it: the same hierarchy with the default network (white
+    noise on one 64x64-node plate of the family) and FP64 Jacobi-CG."""
+    import cstfem_hip
+    from cstfem_hip import learned, pipeline
+    t0 = time.perf_counter()
+    with cstfem_hip.Device(0) as tdev:
+        theta_h, losses_h, info = learned.train_harvested(
+            tdev, plates=args.h_plates, chunk=args.h_chunk, cells=args.h_cells, k_first=1,
+            k_last=50, steps_per_chunk=args.h_steps, project=False, white=args.h_white, log=log)
+    protocol_s = time.perf_counter() - t0
+    log(f"protocol: {args.h_plates} plates in {protocol_s:.1f} s ({info})")
+    t0 = time.perf_counter()
+    with cstfem_hip.Device(0) as tdev:
+        theta_w, losses_w = learned.train(tdev, learned.training_problem(2, args.train_n),
+                                          steps=args.train_steps)
+    white_s = time.perf_counter() - t0
+    prob = pipeline.config_problem(2, args.scale)
+    dev = cstfem_hip.Device(0)
+    device_name, cus = dev.info()
+    pipeline.load_problem(dev, prob)
+    chosen = ["learned_mg_protocol", "learned_mg_white"] + (["fp64"] if args.with_fp64 else [])
+    results = time_methods(dev, prob, args, chosen,
+                           {"learned_mg_protocol": theta_h, "learned_mg_white": theta_w},
+                           kind={"learned_mg_protocol": "learned_mg",
+                                 "learned_mg_white": "learned_mg"})
+    dev.setup()
+    dev.mpnn_set_params(theta_h)
+    dev.mpnn_build()
+    lam = None
+    dev.mg_setup(args.mg_levels, args.mg_coarse, 0, args.mg_kcycle)
+    lam = dev.mg_level_info(0)[3]
+    n, nnz = dev.n, dev.nnz
+    t_fcg = dev.profile_kernel(6, reps=50)
+    b_fcg = spmv_bytes(n, nnz) + 8 * n
+    res = results["learned_mg_protocol"]
+    line = {
+        "metric": METRIC, "value": res["s"], "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["s"] * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 outer + f32 cycle + bf16 network",
+        "data": "synthetic",
+        "config": {"workload": (f"learned-preconditioner protocol: {args.h_plates} plates of "
+                                f"{args.h_cells}x{args.h_cells} cells, pairs at Jacobi-CG "
+                                f"iterations 1..50 (+{args.h_white} white-noise rows per batch), "
+                                f"deployed on {prob.name} ({prob.mesh.num_dofs} DOFs) to FP64 "
+                                f"true rel residual 1e-10"),
+                   "dofs": prob.mesh.num_dofs, "method": "learned_mg (protocol network)",
+                   "device": device_name, "cus": cus},
+        "protocol": dict(info, plates=args.h_plates, chunk=args.h_chunk, cells=args.h_cells,
+                         steps=len(losses_h), protocol_s=protocol_s,
+                         loss_first=losses_h[0], loss_last=losses_h[-1],
+                         lambda_max_MK=lam),
+        "white_network": dict(train_s=white_s, steps=len(losses_w), loss_first=losses_w[0],
+                              loss_last=losses_w[-1]),
+        "methods": results,
+        "speedup_vs_fp64": (results["fp64"]["s"] / res["s"]) if "fp64" in results else None,
+        "roofline": {"bound": "hbm", "kernel": "k_fcg_spmv<F64>", "bytes_per_launch": b_fcg,
+                     "avg_launch_s": t_fcg, "achieved": b_fcg / t_fcg / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": b_fcg / t_fcg / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    dev.close()
+
+
+
 def measured_traffic(dofs):
     """HBM bytes per launch of k_cg_spmv<F64> from the committed rocprofv3 --pmc measurement
     (FETCH_SIZE and WRITE_SIZE runs, scaled by the copy-kernel calibration) when it was taken
@@ -258,6 +400,14 @@ def main():
                     help="coarse levels on the K-cycle (Chebyshev fine smoother)")
     ap.add_argument("--train-n", type=int, default=64, help="training plate nodes per side")
     ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--h-plates", type=int, default=2000, help="--config 4: training plates")
+    ap.add_argument("--h-chunk", type=int, default=250, help="--config 4: plates per batch")
+    ap.add_argument("--h-cells", type=int, default=128, help="--config 4: cells per side")
+    ap.add_argument("--h-steps", type=int, default=20, help="--config 4: Adam steps per batch")
+    ap.add_argument("--h-white", type=int, default=16,
+                    help="--config 4: white-noise pair rows added per batch")
+    ap.add_argument("--with-fp64", action="store_true",
+                    help="--config 4: also time FP64 Jacobi-CG on the deployment plate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole plate sits in one MI355X's HBM and this "
@@ -266,6 +416,9 @@ def main():
 
     if args.config == 3:
         run_batch(args)
+        return
+    if args.config == 4:
+        run_protocol(args)
         return
 
     import cstfem_hip
@@ -303,11 +456,6 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
-    methods = {"fp64": cstfem_hip.METHOD_CG_FP64_REFINED, "ir32": cstfem_hip.METHOD_IR_FP32,
-               "learned1": cstfem_hip.METHOD_FCG_LEARNED,
-               "learned32": cstfem_hip.METHOD_FCG_LEARNED_FP32,
-               "learned_mg": cstfem_hip.METHOD_FCG_LEARNED,
-               "mg_cheb": cstfem_hip.METHOD_FCG_LEARNED}
     uses_net = ("learned1", "learned32", "learned_mg")
     chosen = [m for m in args.methods.split(",") if m]
     theta, train_info = None, None
@@ -323,51 +471,8 @@ def main():
         log(f"trained the preconditioner in {train_info['train_s']:.2f} s "
             f"(loss {losses[0]:.4f} -> {losses[-1]:.4f})")
 
-    def step(method):
-        times = pipeline.PhaseTimes()
-        pipeline.assemble_and_constrain(dev, prob, times,
-                                        multilevel=method in ("learned_mg", "mg_cheb"))
-        t0 = time.perf_counter()
-        dev.setup()
-        if method in uses_net:
-            dev.mpnn_set_params(theta)
-            dev.mpnn_build()
-        if method in ("learned_mg", "mg_cheb"):
-            learned_fine = method == "learned_mg"
-            dev.mg_setup(args.mg_levels, args.mg_coarse, 0 if learned_fine else 1,
-                         args.mg_kcycle if learned_fine else args.mg_kcycle_cheb)
-        t1 = time.perf_counter()
-        _, st = dev.solve(rtol=1e-10, max_iter=args.max_iter, method=methods[method],
-                          download=False)
-        times.solve += time.perf_counter() - t0
-        st.precond_s = t1 - t0
-        return st, times
-
-    results = {}
-    for meth in chosen:
-        for i in range(args.warmup):
-            st, times = step(meth)
-            log(f"{meth} warmup {i}: {st.message}, {st.outer_iterations} refinements, "
-                f"true rel {st.true_rel_residual:.3e}")
-        step_times = []
-        for i in range(args.steps):
-            t0 = time.perf_counter()
-            st, times = step(meth)
-            dt = time.perf_counter() - t0
-            step_times.append(dt)
-            log(f"{meth} step {i}: {dt:.3f} s, {st.message}, {st.outer_iterations} refinements, "
-                f"true rel {st.true_rel_residual:.3e}")
-        verified = st.converged and st.true_rel_residual < 1e-10
-        if not verified:
-            log(f"WARNING: {meth} not verified: {st.message}; true rel {st.true_rel_residual:.3e}")
-        results[meth] = dict(s=sum(step_times) / len(step_times), iterations=st.iterations,
-                             refinements=st.outer_iterations, verified=bool(verified),
-                             true_rel_residual=st.true_rel_residual,
-                             fallback=int(st.fallback),
-                             s_per_iteration=(times.solve - st.precond_s) / max(1, st.iterations),
-                             phases_s={"pattern": times.pattern, "assembly": times.assembly,
-                                       "bcs": times.bcs, "solve": times.solve,
-                                       "precond": st.precond_s})
+    results = time_methods(dev, prob, args, chosen,
+                           {m_: theta for m_ in chosen if m_ in uses_net})
     ok = {k: v for k, v in results.items() if v["verified"]} or results
     best = min(ok, key=lambda k: ok[k]["s"])
     per_step = results[best]["s"]

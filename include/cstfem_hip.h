@@ -261,6 +261,20 @@ int cstfem_mpnn_build(cstfem_ctx* ctx);
  * r = K e after plain FP64 Jacobi-CG iterations k = k_first, k_first + stride, ... <= k_last.
  * *npairs = number of pairs kept on the device. */
 int cstfem_mpnn_harvest(cstfem_ctx* ctx, int k_first, int k_last, int stride, int* npairs);
+/* Training pairs from a batch of plates (cstfem_batch_define): x* of every plate by the
+ * batched solve with opt (an FP64 method), then every plate's plain Jacobi-CG from x = 0 (the
+ * reference recurrence) for k_last iterations, recording e_k = x* - x_k and r_k = K e_k after
+ * iterations k = k_first..k_last; pair row k - k_first holds iteration k of every plate.
+ * Leaves x = x_{k_last}.  *npairs = k_last - k_first + 1. */
+int cstfem_batch_harvest(cstfem_ctx* ctx, const cstfem_solver_options* opt, int k_first,
+                         int k_last, int* npairs);
+/* Appends `count` white-noise training pairs drawn on the device: e ~ N(0, I) from `seed`,
+ * r = K e (FP64 SpMV); existing pairs are kept. */
+int cstfem_mpnn_add_white_pairs(cstfem_ctx* ctx, int count, int seed);
+/* Replaces every training pair (r, e) by (K e', e') with e' = e - P C(P^T K e): the part of
+ * the error that the multilevel hierarchy's coarse correction C (cstfem_mg_setup, K-cycles
+ * included, no level-0 smoothing) leaves, which is what the fine-level smoother must reduce. */
+int cstfem_mpnn_project_pairs(cstfem_ctx* ctx);
 /* Caller-supplied pairs: R, E are K rows of n floats (either may be NULL = zeros). */
 int cstfem_mpnn_set_pairs(cstfem_ctx* ctx, int K, const float* R, const float* E);
 /* Forward, loss (1/K) sum_k |M r_k - e_k|_K^2 / |e_k|_K^2 and its gradient w.r.t. the
@@ -269,7 +283,8 @@ int cstfem_mpnn_loss_grad(cstfem_ctx* ctx, double* loss);
 /* cstfem_mpnn_loss_grad followed by one Adam step (beta 0.9 / 0.999, eps 1e-8). */
 int cstfem_mpnn_train_step(cstfem_ctx* ctx, float lr, double* loss);
 /* what: 0 node features (nb x 16), 1 final hidden state (nb x 64), 2 decoder outputs
- * [a, b, d, c] per stored block (nblk x 4), 3 gradient (num_params).  count must match. */
+ * [a, b, d, c] per stored block (nblk x 4), 3 gradient (num_params), 4 / 5 the training
+ * pairs' residuals / errors (npairs x n).  count must match. */
 int cstfem_mpnn_download(cstfem_ctx* ctx, int what, float* out, int64_t count);
 
 /* ---- multilevel preconditioner (no reference counterpart) -------------------------------- */

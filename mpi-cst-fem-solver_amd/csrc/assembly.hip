@@ -611,6 +611,7 @@ void assemble_values(Context& c) {
   c.vals32_valid = false;
   c.fixed_valid = false;  // a fresh matrix has no constrained DOFs
   c.mpnn.built = false;
+  c.mpnn.npairs = 0;  // training pairs belong to the matrix they were formed with
 }
 
 void element_stiffness(Context& c, int64_t count, const double* xy, const double* D, double t,

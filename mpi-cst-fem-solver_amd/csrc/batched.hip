@@ -59,6 +59,11 @@ struct BatchArgs {
   cstfem_batch_result* out;
   double rtol, delta;
   int max_iter, max_refresh;
+  // iterates recorded for training pairs: x after iteration snap_first + j goes to
+  // snap + j * snap_stride (FP64, the row layout of x); NULL records nothing
+  double* snap;
+  int snap_first, snap_count;
+  int64_t snap_stride;
 };
 
 __device__ inline double wave_sum(double v) {
@@ -255,6 +260,11 @@ __device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* pa
         xi.x = xi.x + alpha * (double)pi.x;
         xi.y = xi.y + alpha * (double)pi.y;
         A.x[row] = xi;
+        if (A.snap) {  // iteration it + 1 completes here
+          const int j = it + 1 - A.snap_first;
+          if (j >= 0 && j < A.snap_count)
+            reinterpret_cast<double2*>(A.snap + (int64_t)j * A.snap_stride)[row] = xi;
+        }
         V ri = r[row];
         if (kF32) {
           ri.x = (S)((double)ri.x - alpha * (double)api.x);
@@ -374,6 +384,32 @@ __global__ void k_reset_queue(int* q) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
 }
 
+// every snapshot row starts as x* (a plate that stops early leaves e = 0 in later rows)
+__global__ void k_fill_snapshots(int64_t n, int count, const double* __restrict__ xs,
+                                 double* __restrict__ snap) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    snap[i] = xs[i % n];
+}
+
+// w = x* - x_k (FP64) and e_k = (float) w
+__global__ void k_pair_error(int64_t n, const double* __restrict__ xs,
+                             const double* __restrict__ xk, double* __restrict__ w,
+                             float* __restrict__ e) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double d = xs[i] - xk[i];
+    w[i] = d;
+    e[i] = (float)d;
+  }
+}
+
+__global__ void k_to_float(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
 }  // namespace
 
 void batch_define(Context& c, int count, const int* node_off) {
@@ -456,7 +492,7 @@ BatchArgs batch_args(Context& c, const cstfem_solver_options& o) {
 }  // namespace
 
 double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batch_config& cfg,
-                   cstfem_batch_result* results) {
+                   cstfem_batch_result* results, const BatchSnapshots* snaps) {
   if (c.batch_count <= 0) throw Error(CSTFEM_ERR_ORDER, "cstfem_batch_solve: define the batch first");
   if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "cg_jacobi: solve() called before setup()");
   if (o.method != CSTFEM_METHOD_CG_FP64_REFINED && o.method != CSTFEM_METHOD_CG_FP64 &&
@@ -475,7 +511,13 @@ double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batc
     c.Ap32.alloc(nv);
     ensure_vals32(c);
   }
-  const BatchArgs args = batch_args(c, o);
+  BatchArgs args = batch_args(c, o);
+  if (snaps) {
+    args.snap = snaps->x;
+    args.snap_first = snaps->first;
+    args.snap_count = snaps->count;
+    args.snap_stride = snaps->stride;
+  }
   int grid = cfg.groups_per_cu * (c.cu_count > 0 ? c.cu_count : 256);
   if (grid > c.batch_count) grid = c.batch_count;
   // one captured graph: queue reset (a kernel, not a memset node) + the persistent solve
@@ -571,6 +613,60 @@ cstfem_batch_config batch_autotune(Context& c, const cstfem_solver_options& o,
   }
   if (best_seconds) *best_seconds = best_t;
   return best;
+}
+
+// Training pairs of the learned preconditioner from a batch: x* of every plate by the batched
+// solve, then every plate's plain Jacobi-CG from x = 0 (the reference recurrence, no
+// refreshes) for k_last iterations recording x_k, and e_k = x* - x_k, r_k = K e_k (FP64
+// SpMV) stored as FP32 pair rows: row k - k_first holds iteration k of every plate.
+int batch_harvest(Context& c, const cstfem_solver_options& o, int k_first, int k_last) {
+  if (c.batch_count <= 0) throw Error(CSTFEM_ERR_ORDER, "cstfem_batch_harvest: define the batch first");
+  if (!c.mpnn.initialized) throw Error(CSTFEM_ERR_ORDER, "mpnn: call cstfem_mpnn_init first");
+  if (k_first < 1 || k_last < k_first)
+    throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_harvest: need 1 <= k_first <= k_last");
+  if (o.method != CSTFEM_METHOD_CG_FP64 && o.method != CSTFEM_METHOD_CG_FP64_REFINED)
+    throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_harvest: x* needs an FP64 method");
+  const int K = k_last - k_first + 1;
+  const int64_t n = 2 * (int64_t)c.nb;
+  const cstfem_batch_config cfg{8, 1};
+  std::vector<cstfem_batch_result> res(c.batch_count);
+  batch_solve(c, o, cfg, res.data());
+  // x* needs to be far more accurate than the errors e_k of the first iterations, not to
+  // reach a tolerance near the FP64 floor: a plate whose true residual stalls above rtol but
+  // below 1e-8 is accepted
+  for (int k = 0; k < c.batch_count; ++k)
+    if (!res[k].converged && !(res[k].true_rel_residual < 1e-8))
+      throw Error(CSTFEM_ERR_DEVICE, "cstfem_batch_harvest: plate " + std::to_string(k) +
+                                         " did not converge for x* (true relative residual " +
+                                         std::to_string(res[k].true_rel_residual) + ")");
+  DBuf<double> xs, snap;
+  xs.alloc(n);
+  snap.alloc((size_t)n * K);
+  HIP_CHECK(hipMemcpyAsync(xs.get(), c.x.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+  hipLaunchKernelGGL(k_fill_snapshots, dim3(grid_for(n * K, 256, 4096)), dim3(256), 0, c.stream, n,
+                     K, (const double*)xs.get(), snap.get());
+  HIP_CHECK(hipGetLastError());
+  cstfem_solver_options p{};
+  p.rtol = 1e-300;  // never met: no refresh, every plate runs k_last iterations
+  p.max_iter = k_last;
+  p.method = CSTFEM_METHOD_CG_FP64;
+  const BatchSnapshots sn{snap.get(), k_first, K, n};
+  batch_solve(c, p, cfg, nullptr, &sn);
+  Mpnn& m = c.mpnn;
+  m.R.alloc((size_t)n * K);
+  m.E.alloc((size_t)n * K);
+  for (int k = 0; k < K; ++k) {
+    hipLaunchKernelGGL(k_pair_error, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c.stream, n,
+                       (const double*)xs.get(), (const double*)snap.get() + (size_t)k * n,
+                       c.work.get(), m.E.get() + (size_t)k * n);
+    spmv_fp64(c, c.work.get(), c.Ap.get());
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c.stream, n,
+                       (const double*)c.Ap.get(), m.R.get() + (size_t)k * n);
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  m.npairs = K;
+  return K;
 }
 
 }  // namespace cstfem_hip

@@ -146,6 +146,7 @@ int cstfem_upload_mesh(cstfem_ctx* ctx, int nn, const double* xy, int64_t ne, co
     c.batch_count = 0;
     c.fixed_valid = false;
     c.mpnn.built = false;
+    c.mpnn.npairs = 0;
     c.ne_ptr.release();
     c.ne_list.release();
     c.b.release();
@@ -404,6 +405,8 @@ int cstfem_upload_csr(cstfem_ctx* ctx, int n, const int* row_ptr, const int* col
     c.pattern_symmetric = symmetric;
     c.max_row_blocks = max_row;
     c.fixed_valid = false;
+    c.mpnn.built = false;
+    c.mpnn.npairs = 0;
     c.setup_done = false;
     c.vals32_valid = false;
     c.b.release();
@@ -647,6 +650,18 @@ int cstfem_batch_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt,
   });
 }
 
+int cstfem_batch_harvest(cstfem_ctx* ctx, const cstfem_solver_options* opt, int k_first,
+                         int k_last, int* npairs) {
+  return guarded([&] {
+    need(ctx && opt, "cstfem_batch_harvest: NULL argument");
+    need(opt->rtol > 0 && opt->max_iter > 0, "cstfem_batch_harvest: bad rtol or max_iter");
+    use_device(&ctx->c);
+    const int got = batch_harvest(ctx->c, *opt, k_first, k_last);
+    if (npairs) *npairs = got;
+    return CSTFEM_OK;
+  });
+}
+
 int cstfem_batch_autotune(cstfem_ctx* ctx, const cstfem_solver_options* opt,
                           const char* cache_path, cstfem_batch_config* best,
                           double* best_seconds) {
@@ -721,6 +736,24 @@ int cstfem_mpnn_harvest(cstfem_ctx* ctx, int k_first, int k_last, int stride, in
     use_device(&ctx->c);
     const int got = mpnn_harvest(ctx->c, k_first, k_last, stride);
     if (npairs) *npairs = got;
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mpnn_add_white_pairs(cstfem_ctx* ctx, int count, int seed) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_mpnn_add_white_pairs: NULL context");
+    use_device(&ctx->c);
+    mpnn_add_white_pairs(ctx->c, count, seed);
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_mpnn_project_pairs(cstfem_ctx* ctx) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_mpnn_project_pairs: NULL context");
+    use_device(&ctx->c);
+    mpnn_project_pairs(ctx->c);
     return CSTFEM_OK;
   });
 }

@@ -132,6 +132,10 @@ struct Mpnn {
   DBuf<float> ghat, dghat, gsell;
   DBuf<int> btrans;
   DBuf<float> R, E;
+  // workspace of a training step (loss and backward), grown on demand and kept
+  DBuf<float> wsU, wsV, wsdH, wsdX, wsCP, wsP;
+  DBuf<uint16_t> wsdZ;
+  DBuf<double> wsPart, wsW;
 };
 
 // One level of the aggregation multilevel preconditioner (mg.hip).  Level 0 is the context's
@@ -409,6 +413,8 @@ void mg_drop_prepared(Context& c);
 // z'zq there and the count is returned; 0 when they were not formed
 int mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* sc,
                      const double* zq = nullptr, double* zq_part = nullptr);
+// z = P C(P^T r) with C the hierarchy's coarse correction (no level-0 smoothing)
+void mg_enqueue_coarse(Context& c, const double* r, double* z);
 int64_t mg_level_info(Context& c, int level, int* n, int* nagg, double* lam);
 int mg_levels(Context& c);
 // switches the level-0 smoother (0 learned M, 1 Chebyshev; its spectral bound is computed on
@@ -423,6 +429,8 @@ void mpnn_get_params(Context& c, float* theta, int n);
 void mpnn_forward(Context& c);
 void mpnn_set_pairs(Context& c, int K, const float* R, const float* E);
 int mpnn_harvest(Context& c, int k_first, int k_last, int stride);
+void mpnn_project_pairs(Context& c);
+void mpnn_add_white_pairs(Context& c, int count, int seed);
 double mpnn_loss_grad(Context& c);
 double mpnn_train_step(Context& c, float lr);
 void mpnn_download(Context& c, int what, float* out, int64_t count);
@@ -430,8 +438,15 @@ double profile_kernel(Context& c, int kind, int reps, int64_t bytes);
 void element_stresses(Context& c, double* sig, double* vm, double* szz, double* cxy, double* area);
 void kirsch_metrics(Context& c, double sigma_inf, double a, cstfem_kirsch_result* out);
 void batch_define(Context& c, int count, const int* node_off);
+// iterates a batched solve records: x after iteration first + j at x + j * stride
+struct BatchSnapshots {
+  double* x;
+  int first, count;
+  int64_t stride;
+};
 double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batch_config& cfg,
-                   cstfem_batch_result* results);
+                   cstfem_batch_result* results, const BatchSnapshots* snaps = nullptr);
+int batch_harvest(Context& c, const cstfem_solver_options& o, int k_first, int k_last);
 cstfem_batch_config batch_autotune(Context& c, const cstfem_solver_options& o,
                                    const char* cache_path, double* best_seconds);
 

@@ -1803,6 +1803,21 @@ int mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* s
   return learned && zq_part ? gs : 0;
 }
 
+// z = P C(P^T r): the cycle's coarse correction alone, without level-0 smoothing
+void mg_enqueue_coarse(Context& c, const double* r, double* z) {
+  Multigrid& mg = c.mg;
+  if (!mg.ready) throw Error(CSTFEM_ERR_ORDER, "multilevel: not set up");
+  MgLevel& L = *mg.lv[0];
+  MgLevel& C = *mg.lv[1];
+  HIP_CHECK(hipMemsetAsync(z, 0, sizeof(double) * (size_t)L.n, c.stream));
+  launch_restrict(c, L, r, coarse_rhs(mg, 0), nullptr);
+  coarse_correction(c, 0, nullptr);
+  hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n, 2,
+                     (const int*)L.agg.get(), (const double*)L.P.get(), (const double*)C.kx.get(), z,
+                     (const PcgScalars*)nullptr);
+  HIP_CHECK(hipGetLastError());
+}
+
 void mg_drop_prepared(Context& c) {
   c.mg.join_prepared();
   c.mg.pre_error = nullptr;

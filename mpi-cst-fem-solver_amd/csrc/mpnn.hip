@@ -619,6 +619,12 @@ void mpnn_get_params(Context& c, float* th, int n) {
 
 namespace {
 
+template <class T>
+T* grow_ws(DBuf<T>& b, size_t count) {
+  if (!b.get() || b.n < count) b.alloc(std::max<size_t>(count, 1));
+  return b.get();
+}
+
 void check_ready(Context& c) {
   if (!c.mpnn.initialized) throw Error(CSTFEM_ERR_ORDER, "mpnn: call cstfem_mpnn_init first");
   if (!c.have_pattern || !c.mesh_pattern)
@@ -748,6 +754,109 @@ int mpnn_harvest(Context& c, int k_first, int k_last, int stride) {
   return got;
 }
 
+namespace {
+// e = e - z (FP64) and its FP32 copy
+__global__ void k_sub_to_f32(int64_t n, double* __restrict__ e, const double* __restrict__ z,
+                             float* __restrict__ e32) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = e[i] - z[i];
+    e[i] = v;
+    e32[i] = (float)v;
+  }
+}
+}  // namespace
+
+namespace {
+// splitmix64 of a counter: independent uniform 64-bit words per (seed, index)
+__device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// e[i] ~ N(0, 1) by the Box-Muller transform of two uniforms drawn from (seed, row, i)
+__global__ void k_white(int64_t n, uint64_t key, double* __restrict__ e64, float* __restrict__ e32) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t w = mix64(key ^ mix64((uint64_t)i));
+    const double u1 = ((double)(w >> 11) + 0.5) * 0x1p-53;                // (0, 1)
+    const double u2 = (double)(mix64(w) >> 11) * 0x1p-53;                 // [0, 1)
+    const double v = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    e64[i] = v;
+    e32[i] = (float)v;
+  }
+}
+}  // namespace
+
+// Appends `count` white-noise pairs: e ~ N(0, I) drawn on the device from `seed`, r = K e
+// (FP64 SpMV); pairs already present are kept.
+void mpnn_add_white_pairs(Context& c, int count, int seed) {
+  check_ready(c);
+  if (count <= 0) throw Error(CSTFEM_ERR_ARGUMENT, "mpnn_add_white_pairs: count must be positive");
+  Mpnn& m = c.mpnn;
+  const size_t n = 2 * (size_t)c.nb;
+  const int old = m.npairs > 0 && m.R.n >= (size_t)m.npairs * n ? m.npairs : 0;
+  const int K = old + count;
+  DBuf<float> R2, E2;
+  R2.alloc((size_t)K * n);
+  E2.alloc((size_t)K * n);
+  if (old) {
+    HIP_CHECK(hipMemcpyAsync(R2.get(), m.R.get(), sizeof(float) * old * n, hipMemcpyDeviceToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(E2.get(), m.E.get(), sizeof(float) * old * n, hipMemcpyDeviceToDevice, c.stream));
+  }
+  DBuf<double> e64, r64;
+  e64.alloc(n);
+  r64.alloc(n);
+  const int g = std::max(1, std::min(4096, (int)((n + 255) / 256)));
+  for (int k = 0; k < count; ++k) {
+    const uint64_t key = ((uint64_t)(uint32_t)seed << 32) ^ (uint64_t)(uint32_t)(old + k);
+    hipLaunchKernelGGL(k_white, dim3(g), dim3(256), 0, c.stream, (int64_t)n, key, e64.get(),
+                       E2.get() + (size_t)(old + k) * n);
+    spmv_fp64(c, e64.get(), r64.get());
+    hipLaunchKernelGGL(k_to_f32, dim3((c.nb + 255) / 256), dim3(256), 0, c.stream, c.nb,
+                       (const double2*)r64.get(), (float2*)(R2.get() + (size_t)(old + k) * n));
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  std::swap(m.R.p, R2.p);
+  std::swap(m.R.n, R2.n);
+  std::swap(m.E.p, E2.p);
+  std::swap(m.E.n, E2.n);
+  m.npairs = K;
+}
+
+// Every pair (r, e) becomes (K e', e') with e' = e - P C(P^T K e): the part of the error the
+// hierarchy's coarse correction leaves, which is all a fine-level smoother has to reduce.
+void mpnn_project_pairs(Context& c) {
+  Mpnn& m = c.mpnn;
+  if (m.npairs <= 0) throw Error(CSTFEM_ERR_ORDER, "mpnn: no training pairs");
+  if (!c.mg.ready) throw Error(CSTFEM_ERR_ORDER, "mpnn: projecting pairs needs the multilevel hierarchy (cstfem_mg_setup)");
+  const int n2 = c.nb;
+  const size_t n = 2 * (size_t)c.nb;
+  DBuf<double> e64, r64, z64;
+  e64.alloc(n);
+  r64.alloc(n);
+  z64.alloc(n);
+  const int g = std::max(1, std::min(4096, (n2 + 255) / 256));
+  for (int k = 0; k < m.npairs; ++k) {
+    float* E = m.E.get() + k * n;
+    float* R = m.R.get() + k * n;
+    hipLaunchKernelGGL(k_to_f64, dim3((n2 + 255) / 256), dim3(256), 0, c.stream, n2,
+                       (const float2*)E, (double2*)e64.get());
+    spmv_fp64(c, e64.get(), r64.get());
+    mg_enqueue_coarse(c, r64.get(), z64.get());
+    hipLaunchKernelGGL(k_sub_to_f32, dim3(g), dim3(256), 0, c.stream, (int64_t)n, e64.get(),
+                       (const double*)z64.get(), E);
+    spmv_fp64(c, e64.get(), r64.get());
+    hipLaunchKernelGGL(k_to_f32, dim3((n2 + 255) / 256), dim3(256), 0, c.stream, n2,
+                       (const double2*)r64.get(), (float2*)R);
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
 // forward + loss over the pairs + backward into c.mpnn.grad (zeroed first); returns the loss
 double mpnn_loss_grad(Context& c) {
   Mpnn& m = c.mpnn;
@@ -756,20 +865,20 @@ double mpnn_loss_grad(Context& c) {
   const int N = c.nb, K = m.npairs, n2 = c.nb;
   const size_t n = 2 * (size_t)N;
   ensure_vals32(c);
-  DBuf<float> U, V;
-  U.alloc(K * n);
-  V.alloc(K * n);
-  DBuf<double> part;
+  // the step's workspace is kept in the context (grown on demand): at millions of nodes and
+  // tens of pairs it is gigabytes, which a step must not allocate and free
+  float* const Ug = grow_ws(m.wsU, (size_t)K * n);
+  float* const Vg = grow_ws(m.wsV, (size_t)K * n);
   const int gd = std::max(1, std::min(1024, (n2 + kT - 1) / kT));
-  part.alloc(4 * (size_t)gd * K + 8);
+  double* const partg = grow_ws(m.wsPart, 4 * (size_t)gd * K + 8);
   std::vector<double> h_part(4 * (size_t)gd * K);
   // u_k = M r_k ; d_k = u_k - e_k ; v_k = A d_k (FP32 values of A)
   const int gs = std::max(1, std::min(2048, (c.nslices + 3) / 4));
   for (int k = 0; k < K; ++k) {
     const float2* r = (const float2*)(m.R.get() + k * n);
     const float2* e = (const float2*)(m.E.get() + k * n);
-    float2* u = (float2*)(U.get() + k * n);
-    float2* v = (float2*)(V.get() + k * n);
+    float2* u = (float2*)(Ug + k * n);
+    float2* v = (float2*)(Vg + k * n);
     hipLaunchKernelGGL(k_gspmv, dim3(gs), dim3(kT), 0, c.stream, c.nslices, c.sell_row0.get(),
                        c.sell_nrow.get(), c.sell_off.get(), c.sell_col.get(), (const float*)m.gsell.get(), r, u);
     hipLaunchKernelGGL(k_sub32, dim3((n2 + 255) / 256), dim3(256), 0, c.stream, n2, (const float2*)u, e, u);
@@ -778,11 +887,11 @@ double mpnn_loss_grad(Context& c) {
                        (const float2*)u, v);
     // d'Ad and e'Ae (= e'r)
     hipLaunchKernelGGL(k_dot32, dim3(gd), dim3(kT), 0, c.stream, n2, (const float2*)u, (const float2*)v,
-                       part.get() + (size_t)(2 * k) * gd);
-    hipLaunchKernelGGL(k_dot32, dim3(gd), dim3(kT), 0, c.stream, n2, e, r, part.get() + (size_t)(2 * k + 1) * gd);
+                       partg + (size_t)(2 * k) * gd);
+    hipLaunchKernelGGL(k_dot32, dim3(gd), dim3(kT), 0, c.stream, n2, e, r, partg + (size_t)(2 * k + 1) * gd);
   }
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipMemcpyAsync(h_part.data(), part.get(), sizeof(double) * 2 * gd * K, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipMemcpyAsync(h_part.data(), partg, sizeof(double) * 2 * gd * K, hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   std::vector<double> wk(K);
   double loss = 0.0;
@@ -796,70 +905,65 @@ double mpnn_loss_grad(Context& c) {
     loss += dad * ce / K;
     wk[k] = 2.0 * ce / K;  // dL/dM = sum_k (2 / (K e'Ae)) A d_k r_k^T
   }
-  DBuf<double> d_wk;
-  d_wk.alloc(K);
-  HIP_CHECK(hipMemcpyAsync(d_wk.get(), wk.data(), sizeof(double) * K, hipMemcpyHostToDevice, c.stream));
+  double* const d_wk = grow_ws(m.wsW, (size_t)K);
+  HIP_CHECK(hipMemcpyAsync(d_wk, wk.data(), sizeof(double) * K, hipMemcpyHostToDevice, c.stream));
   // backward
   HIP_CHECK(hipMemsetAsync(m.grad.get(), 0, sizeof(float) * kParams, c.stream));
   m.dghat.alloc(kDec * (size_t)c.nblk);
   hipLaunchKernelGGL(k_dghat, dim3((N + 127) / 128), dim3(128), 0, c.stream, N, K, n2, c.brow_ptr.get(),
                      c.bcol.get(), c.sell_pos.get(), (const double*)c.inv_diag.get(),
-                     (const double*)c.sell_val.get(), (const float2*)V.get(),
-                     (const float2*)m.R.get(), (const double*)d_wk.get(), m.dghat.get());
+                     (const double*)c.sell_val.get(), (const float2*)Vg,
+                     (const float2*)m.R.get(), (const double*)d_wk, m.dghat.get());
   if (m.btrans.n != (size_t)c.nblk) {
     m.btrans.alloc(c.nblk);
     hipLaunchKernelGGL(k_btrans, dim3((N + 127) / 128), dim3(128), 0, c.stream, N, c.brow_ptr.get(),
                        c.bcol.get(), m.btrans.get());
   }
-  DBuf<float> dH, dX;
-  dH.alloc((size_t)N * kH);
-  dX.alloc((size_t)N * 2 * kH);
-  DBuf<uint16_t> dZ;
-  dZ.alloc((size_t)N * kH);
+  float* const dH = grow_ws(m.wsdH, (size_t)N * kH);
+  float* const dX = grow_ws(m.wsdX, (size_t)N * 2 * kH);
+  uint16_t* const dZ = grow_ws(m.wsdZ, (size_t)N * kH);
   hipLaunchKernelGGL(k_decode_bwd_h, dim3((N + 3) / 4), dim3(4 * kH), 0, c.stream, N, c.brow_ptr.get(),
                      c.bcol.get(), (const int*)m.btrans.get(), (const uint16_t*)m.H[kRounds].get(),
-                     (const float*)m.theta.get(), (const float*)m.dghat.get(), dH.get());
+                     (const float*)m.theta.get(), (const float*)m.dghat.get(), dH);
   hipLaunchKernelGGL(k_decode_bwd_w, dim3(kH + 1), dim3(kT), 0, c.stream, N, c.brow_ptr.get(),
                      c.bcol.get(), (const uint16_t*)m.H[kRounds].get(), (const float*)m.dghat.get(),
                      m.grad.get());
   const int chunks = std::min((N + 63) / 64, 256);  // workgroups of the dW reductions
   const int crows = (N + chunks - 1) / chunks;
-  DBuf<float> CP;
-  CP.alloc((size_t)chunks * kH);
+  float* const CP = grow_ws(m.wsCP, (size_t)chunks * kH);
   auto colsum = [&](const uint16_t* G, float* out) {
-    hipLaunchKernelGGL(k_colsum, dim3(chunks), dim3(kH), 0, c.stream, N, kH, crows, G, CP.get());
+    hipLaunchKernelGGL(k_colsum, dim3(chunks), dim3(kH), 0, c.stream, N, kH, crows, G, CP);
     hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(kH), 0, c.stream, chunks, kH,
-                       (const float*)CP.get(), out);
+                       (const float*)CP, out);
   };
-  DBuf<float> P;
-  P.alloc((size_t)chunks * 2 * kH * kH);
+  float* const P = grow_ws(m.wsP, (size_t)chunks * 2 * kH * kH);
   for (int l = kRounds; l >= 1; --l) {
     // dZ = dH * relu'(H_l)
     hipLaunchKernelGGL(k_relu_bwd, dim3((N * kH + 255) / 256), dim3(256), 0, c.stream, (int64_t)N * kH,
-                       (const float*)dH.get(), (const uint16_t*)m.H[l].get(), dZ.get());
+                       (const float*)dH, (const uint16_t*)m.H[l].get(), dZ);
     // dW_l = X_l^T dZ, db_l = colsum dZ
     hipLaunchKernelGGL((k_dw_partial<2 * kH, kH>), dim3(chunks), dim3(kT), 0, c.stream, N,
-                       (const uint16_t*)m.X[l].get(), (const uint16_t*)dZ.get(), P.get());
+                       (const uint16_t*)m.X[l].get(), (const uint16_t*)dZ, P);
     hipLaunchKernelGGL(k_sum_chunks, dim3((2 * kH * kH + 255) / 256), dim3(256), 0, c.stream, chunks,
-                       2 * kH * kH, (const float*)P.get(), m.grad.get() + kWl + (l - 1) * 2 * kH * kH);
-    colsum(dZ.get(), m.grad.get() + kBl + (l - 1) * kH);
+                       2 * kH * kH, (const float*)P, m.grad.get() + kWl + (l - 1) * 2 * kH * kH);
+    colsum(dZ, m.grad.get() + kBl + (l - 1) * kH);
     // dX_l = dZ W_l^T  ([N x kH] x [kH x 2kH]): B = W_l as [in x out] = [2kH rows x kH]
     hipLaunchKernelGGL((k_dense<kH, 2 * kH, false, 1>), dim3(rows_grid(N)), dim3(kT), 0, c.stream, N,
-                       (const uint16_t*)dZ.get(), (const uint16_t*)m.wn[l].get(), (const float*)nullptr,
-                       (void*)dX.get());
+                       (const uint16_t*)dZ, (const uint16_t*)m.wn[l].get(), (const float*)nullptr,
+                       (void*)dX);
     // dH_{l-1} = dX[:, :kH] + aggregation backward of dX[:, kH:]
-    HIP_CHECK(hipMemsetAsync(dH.get(), 0, sizeof(float) * (size_t)N * kH, c.stream));
+    HIP_CHECK(hipMemsetAsync(dH, 0, sizeof(float) * (size_t)N * kH, c.stream));
     hipLaunchKernelGGL(k_aggregate_bwd, dim3((N + 3) / 4), dim3(4 * kH), 0, c.stream, N,
-                       c.brow_ptr.get(), c.bcol.get(), (const float*)dX.get(), dH.get());
+                       c.brow_ptr.get(), c.bcol.get(), (const float*)dX, dH);
   }
   // encoder: dW0 = F^T dZ0, db0
   hipLaunchKernelGGL(k_relu_bwd, dim3((N * kH + 255) / 256), dim3(256), 0, c.stream, (int64_t)N * kH,
-                     (const float*)dH.get(), (const uint16_t*)m.H[0].get(), dZ.get());
+                     (const float*)dH, (const uint16_t*)m.H[0].get(), dZ);
   hipLaunchKernelGGL((k_dw_partial<kF, kH>), dim3(chunks), dim3(kT), 0, c.stream, N,
-                     (const uint16_t*)m.feat.get(), (const uint16_t*)dZ.get(), P.get());
+                     (const uint16_t*)m.feat.get(), (const uint16_t*)dZ, P);
   hipLaunchKernelGGL(k_sum_chunks, dim3((kF * kH + 255) / 256), dim3(256), 0, c.stream, chunks, kF * kH,
-                     (const float*)P.get(), m.grad.get() + kW0);
-  colsum(dZ.get(), m.grad.get() + kB0);
+                     (const float*)P, m.grad.get() + kW0);
+  colsum(dZ, m.grad.get() + kB0);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c.stream));
   return loss;
@@ -903,6 +1007,11 @@ void mpnn_download(Context& c, int what, float* out, int64_t count) {
     case 3:
       src = m.grad.get();
       n = kParams;
+      break;
+    case 4:  // training residuals r_k, npairs x n
+    case 5:  // training errors e_k, npairs x n
+      src = what == 4 ? m.R.get() : m.E.get();
+      n = (int64_t)m.npairs * 2 * c.nb;
       break;
     default:
       throw Error(CSTFEM_ERR_ARGUMENT, "mpnn_download: unknown array");

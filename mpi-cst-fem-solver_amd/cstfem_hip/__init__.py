@@ -148,6 +148,10 @@ SIGNATURES = {
     "cstfem_mpnn_build": (ctypes.c_int, [_P]),
     "cstfem_mpnn_harvest": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I]),
     "cstfem_mpnn_set_pairs": (ctypes.c_int, [_P, ctypes.c_int, _F, _F]),
+    "cstfem_mpnn_project_pairs": (ctypes.c_int, [_P]),
+    "cstfem_mpnn_add_white_pairs": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
+    "cstfem_batch_harvest": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), ctypes.c_int,
+                                            ctypes.c_int, _I]),
     "cstfem_mpnn_loss_grad": (ctypes.c_int, [_P, _D]),
     "cstfem_mpnn_train_step": (ctypes.c_int, [_P, ctypes.c_float, _D]),
     "cstfem_mpnn_download": (ctypes.c_int, [_P, ctypes.c_int, _F, ctypes.c_int64]),
@@ -450,6 +454,23 @@ class Device:
         _check(lib().cstfem_mpnn_harvest(self._h, int(k_first), int(k_last), int(stride),
                                          ctypes.byref(got)))
         return got.value
+
+    def batch_harvest(self, k_first=1, k_last=50, rtol=1e-10, max_iter=200000):
+        """Pairs (r_k, e_k) of every plate of the batch after plain Jacobi-CG iterations
+        k_first..k_last (x* by the batched FP64 solve to rtol); returns the pair count."""
+        o = SolverOptions(rtol, max_iter, 0, 0, METHOD_CG_FP64_REFINED, 0.0, 0, 0)
+        got = ctypes.c_int()
+        _check(lib().cstfem_batch_harvest(self._h, ctypes.byref(o), int(k_first), int(k_last),
+                                          ctypes.byref(got)))
+        return got.value
+
+    def mpnn_add_white_pairs(self, count, seed=0):
+        """Appends `count` white-noise pairs (e ~ N(0, I) drawn on the device, r = K e)."""
+        _check(lib().cstfem_mpnn_add_white_pairs(self._h, int(count), int(seed)))
+
+    def mpnn_project_pairs(self):
+        """Pairs -> (K e', e'), e' = the error the multilevel coarse correction leaves."""
+        _check(lib().cstfem_mpnn_project_pairs(self._h))
 
     def mpnn_set_pairs(self, R, E):
         R = None if R is None else np.ascontiguousarray(R, dtype=np.float32).reshape(-1, self.n)

@@ -70,6 +70,58 @@ def train(dev, problem, steps: int = 300, lr: float = 1e-3, pairs: int = 32, see
     return dev.mpnn_get_params(), losses
 
 
+def train_harvested(dev, plates: int = 2000, chunk: int = 250, cells: int = 128,
+                    k_first: int = 1, k_last: int = 50, steps_per_chunk: int = 40,
+                    lr: float = 1e-3, seed: int = 0, project: bool = True, white: int = 0,
+                    theta0=None, log=None):
+    """Trains on residual/error pairs that plain Jacobi-CG generates (the protocol of the
+    learned preconditioner's training data): `plates` plates of cells x cells
+    (pipeline.harvest_batch) in batches of `chunk`; per batch, x* of every plate by the
+    batched FP64 solve, then the pairs (r_k = K e_k, e_k = x* - x_k) after plain Jacobi-CG
+    iterations k_first..k_last of every plate (cstfem_batch_harvest: one pair row holds
+    iteration k of all the batch's plates), with project=True each pair reduced to the error
+    the multilevel coarse correction leaves (cstfem_mpnn_project_pairs: the network is the
+    fine smoother of that hierarchy, so this is the part it must reduce), `white` white-noise
+    pairs drawn on the device added to them when > 0, and `steps_per_chunk` Adam steps on
+    the device.  Adam's state carries over between batches.
+    Returns (theta, losses, info)."""
+    import time
+    from . import pipeline
+    losses, info = [], dict(gen_s=0.0, load_s=0.0, harvest_s=0.0, train_s=0.0, pairs=0)
+    nchunks = (plates + chunk - 1) // chunk
+    for ci in range(nchunks):
+        t0 = time.perf_counter()
+        bp = pipeline.harvest_batch(min(chunk, plates - ci * chunk), seed * 1000 + ci, cells)
+        t1 = time.perf_counter()
+        pipeline.load_batch(dev, bp)
+        if ci == 0:
+            dev.mpnn_init(seed)
+            dev.mpnn_set_params(polynomial_start(dev.mpnn_get_params())
+                                if theta0 is None else theta0)
+        t2 = time.perf_counter()
+        got = dev.batch_harvest(k_first, k_last)
+        if project:
+            dev.mg_setup(12, 500, 1, 3)  # the hierarchy alone: Chebyshev smoother, no network
+            dev.mpnn_project_pairs()
+            dev.setup()  # drops the hierarchy
+        if white > 0:
+            dev.mpnn_add_white_pairs(white, seed * 7919 + ci + 1)
+        t3 = time.perf_counter()
+        for it in range(steps_per_chunk):
+            losses.append(dev.mpnn_train_step(lr))
+        t4 = time.perf_counter()
+        info["gen_s"] += t1 - t0
+        info["load_s"] += t2 - t1
+        info["harvest_s"] += t3 - t2
+        info["train_s"] += t4 - t3
+        info["pairs"] += got * bp.count
+        if log is not None:
+            log(f"harvest batch {ci + 1}/{nchunks}: {bp.count} plates, {dev.n} DOFs, {got} pair "
+                f"rows in {t3 - t2:.2f} s; loss {losses[-steps_per_chunk]:.4f} -> "
+                f"{losses[-1]:.4f} in {t4 - t3:.2f} s")
+    return dev.mpnn_get_params(), losses, info
+
+
 def training_plate(n: int = 64, seed: int = 5, e_seed: int = 101):
     """A small member of the heterogeneous plate family (ln E ~ N(0,1) per element,
     plane strain, jitter 0.2h, body force) used for training."""

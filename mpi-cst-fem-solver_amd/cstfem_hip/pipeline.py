@@ -118,6 +118,34 @@ def batch_problem(count=512, seed=3, nmin=16, nmax=160) -> BatchProblem:
                         nu_elem=np.concatenate(nu), plates=plates)
 
 
+def harvest_batch(count=250, seed=0, cells=128, Lx=2.0, Ly=1.0) -> BatchProblem:
+    """Plates of the learned preconditioner's training protocol: `count` rectangles Lx x Ly
+    of cells x cells (jitter 0.2h), per-element ln E ~ N(0, 1), nu = 0.3, plane strain,
+    ux = 0 on x = 0, uy = 0 on y = 0, a unit traction of uniformly random direction on
+    x = Lx; the seed fixes meshes, moduli and directions."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    plates, xy, tri, E, nu = [], [], [], [], []
+    node_off = [0]
+    for k in range(count):
+        s_mesh, s_e = (int(v) for v in rng.integers(0, 2 ** 31, 2))
+        th = float(rng.uniform(0.0, 2.0 * np.pi))
+        m = meshes.rect_plate(cells, cells, Lx, Ly, 0.2, s_mesh)
+        Ee = meshes.lognormal_modulus(m.num_elements, 1.0, s_e)
+        p = Problem(mesh=m, E=1.0, nu=0.3, plane=PLANE_STRAIN, thickness=1.0, E_elem=Ee,
+                    tractions=[("right", dict(tx=float(np.cos(th)), ty=float(np.sin(th))))],
+                    dirichlet=[("left", 0, 0.0), ("bottom", 1, 0.0)], name="train")
+        plates.append(p)
+        xy.append(m.xy)
+        tri.append(m.tri + node_off[-1])
+        E.append(Ee)
+        nu.append(np.full(m.num_elements, 0.3))
+        node_off.append(node_off[-1] + m.num_nodes)
+    big = meshes.Mesh(xy=np.concatenate(xy), tri=np.concatenate(tri).astype(np.int32), groups={},
+                      meta=dict(kind="harvest", count=count, seed=seed, cells=cells))
+    return BatchProblem(mesh=big, node_off=np.array(node_off, np.int32), E_elem=np.concatenate(E),
+                        nu_elem=np.concatenate(nu), plates=plates, plane=PLANE_STRAIN)
+
+
 def load_batch(dev: Device, bp: BatchProblem, times: "PhaseTimes | None" = None):
     """Uploads the concatenated plates and assembles/constrains them on the device."""
     t0 = time.perf_counter()
