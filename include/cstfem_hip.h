@@ -182,8 +182,12 @@ int cstfem_setup(cstfem_ctx* ctx);
 int cstfem_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt, cstfem_solver_stats* st,
                  double* x_out, double* history);
 int cstfem_download_solution(cstfem_ctx* ctx, double* x, int n);
-/* y = A x with the device matrix (FP64, per-row summation in column order). */
+/* y = A x with the device matrix (FP64, per-row summation in column order); replaces
+ * CsrMatrix::multiply (csr.cpp:19-26) and CsrOperator::apply (solver.cpp:15-22). */
 int cstfem_spmv(cstfem_ctx* ctx, const double* x, double* y, int n);
+/* *out = a'b over n entries (n = the system size), summed on the device in a fixed order;
+ * replaces SerialReduction::dot (solver.cpp:24-31). */
+int cstfem_dot(cstfem_ctx* ctx, const double* a, const double* b, int n, double* out);
 
 /* ---- stresses and Kirsch metrics -------------------------------------------------------- */
 /* Replaces compute_element_stresses (postprocess.cpp:35-60) for the last solution on the

@@ -3,7 +3,8 @@
 //
 //   reference (file:line)                         here
 //   constitutive_matrix      element.hpp:18       cstfem_hip::constitutive_matrix
-//   cst_stiffness            element.hpp:27       (device only; assembled through the slot map)
+//   cst_stiffness            element.hpp:27       cstfem_hip::cst_stiffness(Device&, xy, D, t) (and
+//                                                 a batch form); assembly uses it through the slot map
 //   CsrMatrix                csr.hpp:14-32        cstfem_hip::CsrMatrix (same fields)
 //   build_stiffness_pattern  assembly.hpp:15      cstfem_hip::build_stiffness_pattern(Device&, Mesh)
 //   assemble_stiffness       assembly.hpp:22      cstfem_hip::assemble_stiffness(Device&, Mesh, D, t, K)
@@ -143,6 +144,25 @@ class Device {
  private:
   cstfem_ctx* ctx_ = nullptr;
 };
+
+using TriCoords = std::array<double, 6>;  // element.hpp:15: (x1, y1, x2, y2, x3, y3)
+using Mat6 = std::array<double, 36>;      // element.hpp:13, row-major
+
+// cst_stiffness (element.cpp:37-55) of a batch of triangles on the device, bit for bit the
+// reference's Ke; the single-triangle form below keeps the reference's signature plus the
+// device.
+inline std::vector<Mat6> cst_stiffness(Device& dev, const std::vector<TriCoords>& xy,
+                                       const Mat3& D, double thickness) {
+  std::vector<Mat6> K(xy.size());
+  check(cstfem_element_stiffness(dev.get(), static_cast<int64_t>(xy.size()),
+                                 xy.empty() ? nullptr : xy[0].data(), D.data(), thickness,
+                                 K.empty() ? nullptr : K[0].data()));
+  return K;
+}
+
+inline Mat6 cst_stiffness(Device& dev, const TriCoords& xy, const Mat3& D, double thickness) {
+  return cst_stiffness(dev, std::vector<TriCoords>{xy}, D, thickness)[0];
+}
 
 // assembly.cpp:9-42 on the device; returns the host CSR with zeroed values.  The mesh,
 // its pattern and the element-to-slot map stay resident on `dev`.

@@ -579,14 +579,36 @@ int cstfem_spmv(cstfem_ctx* ctx, const double* x, double* y, int n) {
     Context& c = ctx->c;
     use_device(&c);
     need(c.have_pattern && n == c.n, "cstfem_spmv: size mismatch or no matrix");
-    DBuf<double> dx, dy;
-    dx.alloc(2 * (size_t)c.nb);
-    dy.alloc(2 * (size_t)c.nb);
-    HIP_CHECK(hipMemsetAsync(dx.get(), 0, sizeof(double) * dx.n, c.stream));
-    HIP_CHECK(hipMemcpyAsync(dx.get(), x, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
-    spmv_fp64(c, dx.get(), dy.get());
-    HIP_CHECK(hipMemcpyAsync(y, dy.get(), sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    ensure_vectors(c);
+    // staged in the context's work vectors (no allocation per call); the padding row of an
+    // odd-sized generic matrix reads zero
+    double* dx = c.work.get();
+    double* dy = c.Ap.get();
+    if (n != 2 * c.nb) HIP_CHECK(hipMemsetAsync(dx, 0, sizeof(double) * 2 * c.nb, c.stream));
+    HIP_CHECK(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    spmv_fp64(c, dx, dy);
+    HIP_CHECK(hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
     HIP_CHECK(hipStreamSynchronize(c.stream));
+    return CSTFEM_OK;
+  });
+}
+
+int cstfem_dot(cstfem_ctx* ctx, const double* a, const double* b, int n, double* out) {
+  return guarded([&] {
+    need(ctx && a && b && out, "cstfem_dot: NULL argument");
+    Context& c = ctx->c;
+    use_device(&c);
+    need(c.have_pattern && n == c.n, "cstfem_dot: size mismatch or no matrix");
+    ensure_vectors(c);
+    double* da = c.work.get();
+    double* db = c.Ap.get();
+    if (n != 2 * c.nb) {
+      HIP_CHECK(hipMemsetAsync(da, 0, sizeof(double) * 2 * c.nb, c.stream));
+      HIP_CHECK(hipMemsetAsync(db, 0, sizeof(double) * 2 * c.nb, c.stream));
+    }
+    HIP_CHECK(hipMemcpyAsync(da, a, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(db, b, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    *out = dot_fp64(c, da, db);
     return CSTFEM_OK;
   });
 }

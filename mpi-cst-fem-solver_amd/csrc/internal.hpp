@@ -392,6 +392,8 @@ void kirsch_traction(double sigma_inf, double a, double x, double y, double nx, 
                      double t[2]);
 void setup_solver(Context& c);
 void spmv_fp64(Context& c, const double* x, double* y);
+// a'b over the 2*nb entries of two device vectors (fixed-order reduction), synchronous
+double dot_fp64(Context& c, const double* a, const double* b);
 int solve_cg_fp64(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
                   double* history);
 int solve_refined(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st, bool fp32,

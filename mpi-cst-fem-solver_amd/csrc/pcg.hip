@@ -609,6 +609,19 @@ void ensure_vals32(Context& c) {
   c.vals32_valid = true;
 }
 
+double dot_fp64(Context& c, const double* a, const double* b) {
+  const int gv = vec_grid(c);
+  hipLaunchKernelGGL(k_dot_rp<F64>, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                     (const double2*)a, (const double2*)b, partX(c));
+  hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(kThreads), 0, c.stream, partX(c), gv,
+                     &c.scal.get()->rr);
+  HIP_CHECK(hipGetLastError());
+  double out = 0.0;
+  HIP_CHECK(hipMemcpyAsync(&out, &c.scal.get()->rr, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  return out;
+}
+
 void spmv_fp64(Context& c, const double* x, double* y) {
   hipLaunchKernelGGL(k_spmv<F64>, dim3(spmv_grid(c)), dim3(kThreads), 0, c.stream, mat64(c),
                      (const double2*)x, (double2*)y);

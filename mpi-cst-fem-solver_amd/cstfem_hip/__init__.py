@@ -133,6 +133,7 @@ SIGNATURES = {
                                     ctypes.POINTER(SolverStatsC), _D, _D]),
     "cstfem_download_solution": (ctypes.c_int, [_P, _D, ctypes.c_int]),
     "cstfem_spmv": (ctypes.c_int, [_P, _D, _D, ctypes.c_int]),
+    "cstfem_dot": (ctypes.c_int, [_P, _D, _D, ctypes.c_int, _D]),
     "cstfem_profile_kernel": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _D]),
     "cstfem_set_material_fields": (ctypes.c_int, [_P, _D, _D, ctypes.c_int, ctypes.c_double]),
     "cstfem_batch_define": (ctypes.c_int, [_P, ctypes.c_int, _I]),
@@ -543,6 +544,13 @@ class Device:
         _check(lib().cstfem_mg_apply(self._h, _ptr(r, ctypes.c_double), _ptr(z, ctypes.c_double),
                                      self.n))
         return z
+
+    def dot(self, a, b):
+        a, b = _f64(a), _f64(b)
+        out = ctypes.c_double()
+        _check(lib().cstfem_dot(self._h, _ptr(a, ctypes.c_double), _ptr(b, ctypes.c_double),
+                                int(a.size), ctypes.byref(out)))
+        return out.value
 
     def spmv(self, x):
         x = _f64(x)
