@@ -167,7 +167,10 @@ struct MgLevel {
   DBuf<double> P;            // n x 3: rows of the tentative prolongator
   DBuf<double> nns;          // n x 3 near-null space of levels >= 1 (R factors of the level above)
   // work vectors
-  DBuf<double> rhs, x, x2, d, res, kx, kres, kp, kq, kz;
+  DBuf<double> rhs, x, x2, d, res, kx, kres, kres2, kp, kq, kz;
+  // fused residual + restriction on this level: aggregates per workgroup and the most member
+  // rows one workgroup holds (0: separate kernels)
+  int rr_ag = 0, rr_cap = 0;
   DBuf<double> part;  // reduction partials of the K-cycle steps
   // before reuse by a new hierarchy: sizes and bounds cleared, and the buffer that only some
   // hierarchies fill (val32, read whenever present) released; the others are reallocated by
@@ -195,6 +198,14 @@ struct Multigrid {
   int tail_from = -1;     // first level of the fused single-workgroup tail cycle (-1: none)
   double m_bound = 0.0;   // lambda_max(M A) of the learned level-0 smoother
   bool coarse_fp32 = false;  // levels >= 1 hold FP32 copies of their values (learned smoother)
+  // fused cycle kernels (CSTFEM_MG_FUSE=0: separate kernels): on levels >= 1 of fewer than
+  // fuse_n DOFs, residual + restriction in one kernel, and where their row sums also take 4 or
+  // more lanes the prolongation inside the first post-smoothing step; a K-cycle update left to
+  // the kernel that reads it next where that kernel's workgroups read at most fold_max
+  // partials in all
+  bool fuse = true;
+  int fuse_n = 200000;
+  int64_t fold_max = 200000;
   DBuf<double> cinv;
   // device workspaces of the set-up, grown on demand and kept, so that repeated set-ups
   // allocate nothing: member-list scratch, coarse degrees and error flags (iwork), the coarse

@@ -51,6 +51,32 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 __device__ inline bool stopped(const PcgScalars* sc) { return sc && sc->status != kRunning; }
 
+// The coarse levels' kernels are latency-bound: each dependent round trip to memory costs about
+// a microsecond of a few-microsecond kernel.  They therefore read the status word together
+// with the row's own operands (status_of, then the loads, then hold, then halted), so
+// that both share one round trip, instead of testing the status before issuing anything.
+__device__ int kNoStatus = kRunning;  // read when a launch has no status word (never written)
+__device__ inline int status_of(const PcgScalars* sc) {
+  // a per-lane zero offset makes this a vector load, whose wait counts in order with the row's
+  // loads (a scalar load's wait would also hold back every later kernel-argument load); no
+  // branch, so that nothing waits for it before the row's loads are issued
+  const int* p = sc ? &sc->status : &kNoStatus;
+  int z = 0;
+  asm volatile("" : "+v"(z));
+  return p[z];
+}
+__device__ inline bool halted(int status) { return status != kRunning; }
+// The values are needed here: their loads, issued above together, complete here, and none of
+// them may be moved below this point (past the status test that follows)
+template <class X>
+__device__ inline void hold1(X& v) {
+  asm volatile("" : "+v"(v));
+}
+template <class... V>
+__device__ inline void hold(V&... v) {
+  (hold1(v), ...);
+}
+
 __device__ inline double block_sum(double v, double* lds) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -277,13 +303,82 @@ struct XVec {
 };
 
 // x = the first Chebyshev step from zero, c1r D^-1 rhs, evaluated where it is needed (the
-// same expression as the separate step, so the same bits)
+// same expression as the separate step, so the same bits); RF gives the right-hand side
+template <class RF>
 struct XCheb1 {
   const double* __restrict__ dinv;
-  const double* __restrict__ rhs;
+  RF rhs;
   double c1r;
-  __device__ double operator()(int j) const { return 0.0 + c1r * (dinv[j] * (rhs[j] - 0.0)); }
+  __device__ double operator()(int j) const { return 0.0 + c1r * (dinv[j] * (rhs(j) - 0.0)); }
 };
+
+// rhs = res - alpha q: the K-cycle residual after its first step (ks_update's expression),
+// evaluated where it is needed
+struct RhsFold {
+  const double* __restrict__ res;
+  const double* __restrict__ q;
+  double alpha;
+  __device__ double operator()(int j) const { return res[j] - alpha * q[j]; }
+};
+
+// x2 + P e after the coarse correction (k_prolong's expression), evaluated where it is
+// needed; e = the coarse level's kx, plus alpha kp when that level's last K-cycle update
+// (ks_update's expression) is left to this reader
+struct XProlong {
+  const double* __restrict__ x2;
+  const int* __restrict__ agg;
+  const double* __restrict__ P;
+  const double* __restrict__ ec;
+  const double* __restrict__ kp;  // null: no update left
+  double alpha;
+  int bs;
+  __device__ double e(int k) const { return kp ? ec[k] + alpha * kp[k] : ec[k]; }
+  // from x2[f], the aggregate a of f's node and P's row f, already loaded
+  __device__ double at(double x2f, int a, double p0, double p1, double p2) const {
+    return x2f + (p0 * e(3 * a) + p1 * e(3 * a + 1) + p2 * e(3 * a + 2));
+  }
+  __device__ double operator()(int f) const {
+    const int a = agg[f / bs];
+    const double* p = P + (size_t)f * 3;
+    return at(x2[f], a, p[0], p[1], p[2]);
+  }
+};
+
+// A reduction's partials, this thread's first kSlots slots tid, tid + kT, ... loaded ahead of
+// their sum, so that the loads share a round trip with the kernel's others.  sum() adds them
+// in slot order and then any slots beyond kSlots kT, which it reads itself: sum_parts's value
+// bit for bit, for any count.  kSlots covers the counts that occur (kPart for the K-cycle
+// steps, 2 kPart for the z'q partials of a smoothing grid).
+template <int kSlots>
+struct PartLoad {
+  double v[kSlots];
+  const double* p = nullptr;
+  __device__ void load(const double* __restrict__ part, int n) {
+    p = part;
+#pragma unroll
+    for (int u = 0; u < kSlots; ++u) {
+      const int i = threadIdx.x + u * kT;
+      v[u] = i < n ? part[i] : 0.0;
+    }
+  }
+  __device__ void hold() {
+#pragma unroll
+    for (int u = 0; u < kSlots; ++u) hold1(v[u]);
+  }
+  __device__ double sum(int n, double* lds) const {
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < kSlots; ++u)
+      if (threadIdx.x + u * kT < n) s += v[u];
+    for (int i = threadIdx.x + kSlots * kT; i < n; i += kT) s += p[i];
+    return block_sum(s, lds);
+  }
+};
+using StepParts = PartLoad<kPart / kT>;     // a K-cycle step's p'q or p'r (<= kPart)
+using DotParts = PartLoad<2 * kPart / kT>;  // z'q of a smoothing step (<= 2 kPart)
+
+// alpha = p'res / p'q of a K-cycle step (0 unless p'q > 0)
+__device__ inline double alpha_of(double pq, double pr) { return pq > 0.0 ? pr / pq : 0.0; }
 
 // Entries k = k0, k0 + step, ... (k < w) of one SELL row in ascending order, accumulated
 // into a in that order (FP64).  Loads are issued in groups of G (all column indices and
@@ -310,12 +405,28 @@ __device__ inline double row_entries(const MatST<T>& A, const XF& xf, int64_t o,
   return a;
 }
 
+// Where row i of a scalar level sits in its slice: value index o of the slice's first entry,
+// entries per row w, lane
+struct RowAt {
+  int64_t o, o1;  // value index of the slice's first entry, and of the next slice's
+  int lane;
+  __device__ int w() const { return (int)((o1 - o) >> 6); }
+};
+
+template <class T>
+__device__ inline RowAt row_at(const MatST<T>& A, int i) {
+  const int s = i >> 6;
+  return RowAt{A.off[s], A.off[s + 1], i & 63};
+}
+
+template <class XF, class T>
+__device__ inline double rowS_f(const MatST<T>& A, const XF& xf, const RowAt& ra) {
+  return row_entries<8>(A, xf, ra.o, ra.lane, ra.w(), 0, 1, 0.0);
+}
+
 template <class XF, class T>
 __device__ inline double rowS_f(const MatST<T>& A, const XF& xf, int i) {
-  const int s = i >> 6, lane = i & 63;
-  const int64_t o = A.off[s];
-  const int w = (int)((A.off[s + 1] - o) >> 6);
-  return row_entries<8>(A, xf, o, lane, w, 0, 1, 0.0);
+  return rowS_f(A, xf, row_at(A, i));
 }
 
 __device__ inline double rowS(const MatS& A, const double* __restrict__ x, int i) {
@@ -326,17 +437,20 @@ __device__ inline double rowS(const MatS& A, const double* __restrict__ x, int i
 // the partial sums are combined by a butterfly, so every lane of the group returns the sum).
 // The LPR lanes of a group are always active together.
 template <int LPR, class XF, class T>
-__device__ inline double rowS_lanes(const MatST<T>& A, const XF& xf, int i, int j) {
-  if (LPR == 1) return rowS_f(A, xf, i);
-  const int s = i >> 6, lane = i & 63;
-  const int64_t o = A.off[s];
-  const int w = (int)((A.off[s + 1] - o) >> 6);
+__device__ inline double rowS_lanes(const MatST<T>& A, const XF& xf, const RowAt& ra, int j) {
+  if (LPR == 1) return rowS_f(A, xf, ra);
   // rows hold about 21 entries: with 4 or more lanes per row, groups of 4 per lane cover a row
   // in one or two rounds
-  double a = j < w ? row_entries<(LPR >= 4 ? 4 : 8)>(A, xf, o, lane, w, j, LPR, 0.0) : 0.0;
+  const int w = ra.w();
+  double a = j < w ? row_entries<(LPR >= 4 ? 4 : 8)>(A, xf, ra.o, ra.lane, w, j, LPR, 0.0) : 0.0;
 #pragma unroll
   for (int m = 1; m < LPR; m <<= 1) a += __shfl_xor(a, m, 64);
   return a;
+}
+
+template <int LPR, class XF, class T>
+__device__ inline double rowS_lanes(const MatST<T>& A, const XF& xf, int i, int j) {
+  return rowS_lanes<LPR>(A, xf, row_at(A, i), j);
 }
 
 // Lanes per row on a scalar level of n rows (about 21 entries per row): one lane per row while
@@ -354,18 +468,31 @@ __global__ __launch_bounds__(kT) void ks_cheb(MatST<T> A, const double* __restri
                                               double* __restrict__ dpart,
                                               const PcgScalars* __restrict__ sc) {
   __shared__ double lds[kW];
-  if (stopped(sc)) return;
   const int t = blockIdx.x * kT + threadIdx.x;
   const int i = t / LPR, j = t % LPR;
+  const bool live = i < A.n;
+  int st = status_of(sc);
+  RowAt ra{0, 0, 0};
+  double di = 0.0, ri = 0.0, dold = 0.0, xi = 0.0, qi = 0.0;
+  if (live) {  // every pointer read here is valid (r stands in for the absent ones)
+    ra = row_at(A, i);
+    di = dinv[i];
+    ri = r[i];
+    dold = d[i];
+    xi = (xin ? xin : r)[i];
+    qi = (dpart ? dq : r)[i];
+  }
+  hold(st, ra.o, ra.o1, di, ri, dold, xi, qi);
+  if (halted(st)) return;
   double dot = 0.0;
-  if (i < A.n) {
-    const double y = xin ? rowS_lanes<LPR>(A, XVec{xin}, i, j) : 0.0;
+  if (live) {
+    const double y = xin ? rowS_lanes<LPR>(A, XVec{xin}, ra, j) : 0.0;
     if (j == 0) {
-      const double dd = (cd != 0.0 ? cd * d[i] : 0.0) + cr * (dinv[i] * (r[i] - y));
+      const double dd = (cd != 0.0 ? cd * dold : 0.0) + cr * (di * (ri - y));
       d[i] = dd;
-      const double xo = (xin ? xin[i] : 0.0) + dd;
+      const double xo = (xin ? xi : 0.0) + dd;
       xout[i] = xo;
-      if (dpart) dot = xo * dq[i];
+      if (dpart) dot = xo * qi;
     }
   }
   if (dpart) {
@@ -384,17 +511,121 @@ __global__ __launch_bounds__(kT) void ks_cheb_pre(MatST<T> A, double* __restrict
                                                   const double* __restrict__ dinv, double c1r,
                                                   double c2d, double c2r,
                                                   const PcgScalars* __restrict__ sc) {
-  if (stopped(sc)) return;
   const int t = blockIdx.x * kT + threadIdx.x;
   const int i = t / LPR, j = t % LPR;
   if (i >= A.n) return;
-  const XCheb1 x1{dinv, r, c1r};
-  const double y = rowS_lanes<LPR>(A, x1, i, j);
+  int st = status_of(sc);
+  RowAt ra = row_at(A, i);
+  double di = dinv[i], ri = r[i];
+  hold(st, ra.o, ra.o1, di, ri);
+  if (halted(st)) return;
+  const XCheb1<XVec> x1{dinv, XVec{r}, c1r};
+  const double y = rowS_lanes<LPR>(A, x1, ra, j);
   if (j != 0) return;
-  const double d1 = 0.0 + c1r * (dinv[i] * (r[i] - 0.0));
-  const double dd = (c2d != 0.0 ? c2d * d1 : 0.0) + c2r * (dinv[i] * (r[i] - y));
+  const double d1 = 0.0 + c1r * (di * (ri - 0.0));
+  const double dd = (c2d != 0.0 ? c2d * d1 : 0.0) + c2r * (di * (ri - y));
   d[i] = dd;
   x2[i] = (0.0 + d1) + dd;
+}
+
+// ks_cheb_pre of the K-cycle's second cycle with the first step's update folded in: the
+// right-hand side res - alpha q (RhsFold, alpha from the first step's partials) is formed
+// where it is needed and also stored into res2, and kx = alpha p is written: what ks_update
+// of the first step would have left, bit for bit.
+template <int LPR, class T>
+__global__ __launch_bounds__(kT) void ks_cheb_pre_fold(MatST<T> A, double* __restrict__ x2,
+                                                       double* __restrict__ d,
+                                                       const double* __restrict__ res,
+                                                       const double* __restrict__ q,
+                                                       const double* __restrict__ p,
+                                                       const double* __restrict__ part, int np,
+                                                       double* __restrict__ res2,
+                                                       double* __restrict__ kx,
+                                                       const double* __restrict__ dinv,
+                                                       double c1r, double c2d, double c2r,
+                                                       const PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kW];
+  const int t = blockIdx.x * kT + threadIdx.x;
+  const int i = t / LPR, j = t % LPR;
+  const bool live = i < A.n;
+  int st = status_of(sc);
+  RowAt ra{0, 0, 0};
+  double di = 0.0, resi = 0.0, qi = 0.0, pi = 0.0;
+  if (live) {
+    ra = row_at(A, i);
+    di = dinv[i];
+    resi = res[i];
+    qi = q[i];
+    pi = p[i];
+  }
+  StepParts pq, pr;
+  pq.load(part, np);
+  pr.load(part + kPart, np);
+  hold(st, ra.o, ra.o1, di, resi, qi, pi);
+  pq.hold();
+  pr.hold();
+  const RhsFold rf{res, q, alpha_of(pq.sum(np, lds), pr.sum(np, lds))};
+  if (halted(st) || !live) return;
+  const XCheb1<RhsFold> x1{dinv, rf, c1r};
+  const double y = rowS_lanes<LPR>(A, x1, ra, j);
+  if (j != 0) return;
+  const double ri = resi - rf.alpha * qi;
+  res2[i] = ri;
+  kx[i] = 0.0 + rf.alpha * pi;
+  const double d1 = 0.0 + c1r * (di * (ri - 0.0));
+  const double dd = (c2d != 0.0 ? c2d * d1 : 0.0) + c2r * (di * (ri - y));
+  d[i] = dd;
+  x2[i] = (0.0 + d1) + dd;
+}
+
+// The first post-smoothing step (ks_cheb with cd = 0) with the prolongation folded in: its
+// input x2 + P e (XProlong) is formed where the row sums need it, so k_prolong and its pass
+// over x2 go away; with xp.kp, alpha of the coarse K-cycle's last step comes from its partials.
+template <int LPR, class T>
+__global__ __launch_bounds__(kT) void ks_cheb_prolong(MatST<T> A, XProlong xp,
+                                                      const double* __restrict__ part, int np,
+                                                      double* __restrict__ xout,
+                                                      double* __restrict__ d,
+                                                      const double* __restrict__ r,
+                                                      const double* __restrict__ dinv, double cr,
+                                                      const PcgScalars* __restrict__ sc) {
+  __shared__ double lds[kW];
+  const int t = blockIdx.x * kT + threadIdx.x;
+  const int i = t / LPR, j = t % LPR;
+  const bool live = i < A.n;
+  int st = status_of(sc);
+  RowAt ra{0, 0, 0};
+  double di = 0.0, ri = 0.0, x2i = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0;
+  int ai = 0;
+  if (live) {
+    ra = row_at(A, i);
+    di = dinv[i];
+    ri = r[i];
+    x2i = xp.x2[i];
+    ai = xp.agg[i / xp.bs];
+    p0 = xp.P[(size_t)i * 3];
+    p1 = xp.P[(size_t)i * 3 + 1];
+    p2 = xp.P[(size_t)i * 3 + 2];
+  }
+  StepParts pq, pr;
+  if (xp.kp) {
+    pq.load(part, np);
+    pr.load(part + kPart, np);
+  }
+  hold(st, ra.o, ra.o1, di, ri, x2i, ai, p0, p1, p2);
+  if (xp.kp) {
+    pq.hold();
+    pr.hold();
+    xp.alpha = alpha_of(pq.sum(np, lds), pr.sum(np, lds));
+  }
+  if (halted(st) || !live) return;
+  const double xi = xp.at(x2i, ai, p0, p1, p2);
+  const double y = rowS_lanes<LPR>(A, xp, ra, j);
+  if (j == 0) {
+    const double dd = 0.0 + cr * (di * (ri - y));
+    d[i] = dd;
+    xout[i] = xi + dd;
+  }
 }
 
 template <int LPR, class T>
@@ -402,12 +633,16 @@ __global__ __launch_bounds__(kT) void ks_resid(MatST<T> A, const double* __restr
                                                const double* __restrict__ r,
                                                double* __restrict__ res,
                                                const PcgScalars* __restrict__ sc) {
-  if (stopped(sc)) return;
   const int t = blockIdx.x * kT + threadIdx.x;
   const int i = t / LPR, j = t % LPR;
   if (i >= A.n) return;
-  const double y = rowS_lanes<LPR>(A, XVec{x}, i, j);
-  if (j == 0) res[i] = r[i] - y;
+  int st = status_of(sc);
+  RowAt ra = row_at(A, i);
+  double ri = r[i];
+  hold(st, ra.o, ra.o1, ri);
+  if (halted(st)) return;
+  const double y = rowS_lanes<LPR>(A, XVec{x}, ra, j);
+  if (j == 0) res[i] = ri - y;
 }
 
 // K-cycle step: beta = first ? 0 : -z'q_old / p'q_old; q = A z + beta q; p = z + beta p;
@@ -421,26 +656,50 @@ __global__ __launch_bounds__(kT) void ks_spmv_dir(MatST<T> A, const double* __re
                                                   const double* __restrict__ part_pq, int np,
                                                   int first, const PcgScalars* __restrict__ sc) {
   __shared__ double lds[kW];
-  if (stopped(sc)) return;
-  double beta = 0.0;
-  if (!first) {
-    const double zq = sum_parts(part_zq, nzq, lds);
-    const double pq = sum_parts(part_pq, np, lds);
-    beta = pq > 0.0 ? -zq / pq : 0.0;
-  }
-  double a = 0.0, b = 0.0;
   const int j = threadIdx.x % LPR;
   const int stride = gridDim.x * (kT / LPR);
-  for (int i = (blockIdx.x * kT + threadIdx.x) / LPR; i < A.n; i += stride) {
-    const double y = rowS_lanes<LPR>(A, XVec{z}, i, j);
+  int i = (blockIdx.x * kT + threadIdx.x) / LPR;
+  int st = status_of(sc);
+  // the first row's operands are loaded with the status word and the partials
+  RowAt ra{0, 0, 0};
+  double zi = 0.0, qo = 0.0, po = 0.0, ri = 0.0;
+  auto load_row = [&]() {
+    ra = row_at(A, i);
+    zi = z[i];
+    qo = q[i];
+    po = p[i];
+    ri = res[i];
+  };
+  if (i < A.n) load_row();
+  DotParts zq;
+  StepParts pq;
+  if (!first) {
+    zq.load(part_zq, nzq);
+    pq.load(part_pq, np);
+  }
+  hold(st, ra.o, ra.o1, zi, qo, po, ri);
+  double beta = 0.0;
+  if (!first) {
+    zq.hold();
+    pq.hold();
+    const double szq = zq.sum(nzq, lds);
+    const double spq = pq.sum(np, lds);
+    beta = spq > 0.0 ? -szq / spq : 0.0;
+  }
+  if (halted(st)) return;
+  double a = 0.0, b = 0.0;
+  while (i < A.n) {
+    const double y = rowS_lanes<LPR>(A, XVec{z}, ra, j);
     if (j == 0) {
-      const double qi = first ? y : y + beta * q[i];
-      const double pi = first ? z[i] : z[i] + beta * p[i];
+      const double qi = first ? y : y + beta * qo;
+      const double pi = first ? zi : zi + beta * po;
       q[i] = qi;
       p[i] = pi;
       a += pi * qi;
-      b += pi * res[i];
+      b += pi * ri;
     }
+    i += stride;
+    if (i < A.n) load_row();
   }
   const double sa = block_sum(a, lds);
   const double sb = block_sum(b, lds);
@@ -459,13 +718,30 @@ __global__ __launch_bounds__(kT) void ks_update(int n, double* __restrict__ x,
                                                 int first, int upd_res,
                                                 const PcgScalars* __restrict__ sc) {
   __shared__ double lds[kW];
-  if (stopped(sc)) return;
-  const double pq = sum_parts(part, np, lds);
-  const double pr = sum_parts(part + kPart, np, lds);
-  const double alpha = pq > 0.0 ? pr / pq : 0.0;
-  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
-    x[i] = (first ? 0.0 : x[i]) + alpha * p[i];
-    if (upd_res) res[i] = res[i] - alpha * q[i];
+  int st = status_of(sc);
+  int i = blockIdx.x * kT + threadIdx.x;
+  // the first element's operands are loaded with the status word and the partials
+  double xo = 0.0, pi = 0.0, ro = 0.0, qi = 0.0;
+  auto load = [&]() {
+    xo = x[i];
+    pi = p[i];
+    ro = res[i];
+    qi = q[i];
+  };
+  if (i < n) load();
+  StepParts pq, pr;
+  pq.load(part, np);
+  pr.load(part + kPart, np);
+  hold(st, xo, pi, ro, qi);
+  pq.hold();
+  pr.hold();
+  const double alpha = alpha_of(pq.sum(np, lds), pr.sum(np, lds));
+  if (halted(st)) return;
+  while (i < n) {
+    x[i] = (first ? 0.0 : xo) + alpha * pi;
+    if (upd_res) res[i] = ro - alpha * qi;
+    i += gridDim.x * kT;
+    if (i < n) load();
   }
 }
 
@@ -481,12 +757,12 @@ __global__ __launch_bounds__(kT) void ks_dot(int n, const double* __restrict__ a
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// sum over the member rows k = k0, k0 + step, ... of aggregate a (rows in member order, then
-// DOF order) of P[f * 3 + c] res[f], in that order; loads in groups of 8
-__device__ inline double restrict_sum(int a, int c, int bs, const int* __restrict__ mptr,
+// sum over the member rows k = k0, k0 + step, ... (k < rows) of the aggregate whose member
+// list starts at mem[m0] (rows in member order, then DOF order) of P[f * 3 + c] res[f], in
+// that order; loads in groups of 8
+__device__ inline double restrict_sum(int m0, int rows, int c, int bs,
                                       const int* __restrict__ mem, const double* __restrict__ P,
                                       const double* __restrict__ res, int k0, int step) {
-  const int m0 = mptr[a], rows = bs * (mptr[a + 1] - m0);
   double s = 0.0;
   for (int k = k0; k < rows; k += 8 * step) {
     int f[8];
@@ -517,30 +793,106 @@ __global__ __launch_bounds__(kT) void k_restrict(int nagg, int bs, const int* __
                                                  const double* __restrict__ res,
                                                  double* __restrict__ rc,
                                                  const PcgScalars* __restrict__ sc) {
-  if (stopped(sc)) return;
   const int tt = blockIdx.x * kT + threadIdx.x;
   const int t = tt / LPR, j = tt % LPR;
   if (t >= 3 * nagg) return;
   const int a = t / 3, c = t % 3;
-  double s = restrict_sum(a, c, bs, mptr, mem, P, res, j, LPR);
+  int st = status_of(sc);
+  int m0 = mptr[a], m1 = mptr[a + 1];
+  hold(st, m0, m1);
+  if (halted(st)) return;
+  double s = restrict_sum(m0, bs * (m1 - m0), c, bs, mem, P, res, j, LPR);
 #pragma unroll
   for (int m = 1; m < LPR; m <<= 1) s += __shfl_xor(s, m, 64);
   if (j == 0) rc[t] = s;
 }
 
-// x[f] += sum_c P[f * 3 + c] * ec[3 agg(node of f) + c]
-__global__ __launch_bounds__(kT) void k_prolong(int n, int bs, const int* __restrict__ agg,
-                                                const double* __restrict__ P,
-                                                const double* __restrict__ ec,
-                                                double* __restrict__ x,
+// Residual rhs - A x of the member rows of aggregates [ag b, ag b + ag) (LPR lanes per row),
+// kept in LDS with the rows' indices, then their restriction into rc with lr lanes per coarse
+// DOF: each lane sums the rows k = j, j + lr, ... of its aggregate in member order and the
+// lanes combine by butterfly, which is k_restrict<lr>'s arithmetic on ks_resid's values, so
+// rc is bit-identical to the two separate kernels.  Dynamic LDS: cap doubles, then cap ints.
+template <int LPR, class T>
+__global__ __launch_bounds__(kT) void ks_resid_restrict(MatST<T> A, const double* __restrict__ x,
+                                                        const double* __restrict__ r, int nagg,
+                                                        int ag, int cap, int bs, int lr,
+                                                        const int* __restrict__ mptr,
+                                                        const int* __restrict__ mem,
+                                                        const double* __restrict__ P,
+                                                        double* __restrict__ rc,
+                                                        const PcgScalars* __restrict__ sc) {
+  extern __shared__ double rr_lds[];
+  double* res = rr_lds;
+  int* row = (int*)(rr_lds + cap);
+  const int a0 = blockIdx.x * ag;
+  const int a1 = min(a0 + ag, nagg);
+  int st = status_of(sc);
+  int m0 = mptr[a0], m1 = mptr[a1];
+  hold(st, m0, m1);
+  if (halted(st)) return;
+  const int rows = bs * (m1 - m0);  // <= cap (planned on the host)
+  const int j = threadIdx.x % LPR;
+  for (int t = threadIdx.x / LPR; t < rows; t += kT / LPR) {
+    const int f = bs * mem[m0 + t / bs] + t % bs;
+    const double y = rowS_lanes<LPR>(A, XVec{x}, f, j);
+    if (j == 0) {
+      res[t] = r[f] - y;
+      row[t] = f;
+    }
+  }
+  __syncthreads();
+  for (int tt = threadIdx.x; tt < 3 * (a1 - a0) * lr; tt += kT) {
+    const int u = tt / lr, jj = tt % lr;
+    const int a = a0 + u / 3, c = u % 3;
+    const int k0 = bs * (mptr[a] - m0), k1 = bs * (mptr[a + 1] - m0);
+    double s = 0.0;
+    for (int k = k0 + jj; k < k1; k += 8 * lr) {
+      double pv[8], rv[8];
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const int kk = k + v * lr < k1 ? k + v * lr : k;
+        pv[v] = P[(size_t)row[kk] * 3 + c];
+        rv[v] = res[kk];
+      }
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+        if (k + v * lr < k1) s = s + pv[v] * rv[v];
+    }
+    for (int m = 1; m < lr; m <<= 1) s += __shfl_xor(s, m, 64);
+    if (jj == 0) rc[3 * a + c] = s;
+  }
+}
+
+// x[f] += sum_c P[f * 3 + c] * e[3 agg(node of f) + c] (XProlong with xp.x2 = x); with xp.kp,
+// e = kx + alpha kp, alpha from the coarse K-cycle's last partials
+__global__ __launch_bounds__(kT) void k_prolong(int n, XProlong xp, const double* __restrict__ part,
+                                                int np, double* __restrict__ x,
                                                 const PcgScalars* __restrict__ sc) {
-  if (stopped(sc)) return;
+  __shared__ double lds[kW];
   const int f = blockIdx.x * kT + threadIdx.x;
-  if (f >= n) return;
-  const int a = agg[f / bs];
-  const double* p = P + (size_t)f * 3;
-  const double* e = ec + 3 * (size_t)a;
-  x[f] = x[f] + (p[0] * e[0] + p[1] * e[1] + p[2] * e[2]);
+  int st = status_of(sc);
+  double x2f = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0;
+  int a = 0;
+  if (f < n) {
+    x2f = xp.x2[f];
+    a = xp.agg[f / xp.bs];
+    p0 = xp.P[(size_t)f * 3];
+    p1 = xp.P[(size_t)f * 3 + 1];
+    p2 = xp.P[(size_t)f * 3 + 2];
+  }
+  StepParts pq, pr;
+  if (xp.kp) {
+    pq.load(part, np);
+    pr.load(part + kPart, np);
+  }
+  hold(st, x2f, a, p0, p1, p2);
+  if (xp.kp) {
+    pq.hold();
+    pr.hold();
+    xp.alpha = alpha_of(pq.sum(np, lds), pr.sum(np, lds));
+  }
+  if (halted(st)) return;
+  if (f < n) x[f] = xp.at(x2f, a, p0, p1, p2);
 }
 
 // sum_j row[j] r[j] in ascending j, loads issued in groups of 8
@@ -638,17 +990,23 @@ struct RegRow {
 template <bool kReg>
 __global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars* __restrict__ sc) {
   extern __shared__ double lds[];
-  if (stopped(sc)) return;
   const int tid = threadIdx.x;
+  int st = status_of(sc);
+  // the right-hand side (one entry per thread while the top level fits) and the top level's
+  // rows are loaded with the status word
+  double rhs_first = tid < T.L[0].A.n ? T.rhs[tid] : 0.0;
   RegRow rr;
   if (kReg && tid < T.L[0].A.n) rr.load(T.L[0].A, tid);
+  hold(st, rhs_first);
+  if (halted(st)) return;
   // row sum of level l's row i: from registers on the top level (kReg), else from L2
   auto row = [&](int l, const TailLevel& L, const double* x, int i) {
     return (kReg && l == 0) ? rr.sum(x) : rowS_f(L.A, XVec{x}, i);
   };
   {
     double* rhs0 = lds + T.L[0].lds;
-    for (int i = tid; i < T.L[0].A.n; i += kTailThreads) rhs0[i] = T.rhs[i];
+    if (tid < T.L[0].A.n) rhs0[tid] = rhs_first;
+    for (int i = tid + kTailThreads; i < T.L[0].A.n; i += kTailThreads) rhs0[i] = T.rhs[i];
   }
   __syncthreads();
   for (int l = 0; l < T.nlev; ++l) {
@@ -674,7 +1032,9 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars*
     double* rc = lds + (l + 1 < T.nlev ? T.L[l + 1].lds : T.lds_c);
     for (int tt = tid; tt < 3 * L.nagg * kTailLanes; tt += kTailThreads) {
       const int t = tt / kTailLanes, j = tt % kTailLanes;
-      double sum = restrict_sum(t / 3, t % 3, L.bs, L.mptr, L.mem, L.P, res, j, kTailLanes);
+      const int m0 = L.mptr[t / 3];
+      double sum = restrict_sum(m0, L.bs * (L.mptr[t / 3 + 1] - m0), t % 3, L.bs, L.mem, L.P, res, j,
+                                kTailLanes);
 #pragma unroll
       for (int m = 1; m < kTailLanes; m <<= 1) sum += __shfl_xor(sum, m, 64);
       if (j == 0) rc[t] = sum;
@@ -1513,8 +1873,33 @@ struct SetupTrace {
   }
 };
 
+constexpr size_t kRRLds = 48 * 1024;  // LDS bytes of ks_resid_restrict at most
+
+// Work split of the fused residual + restriction (ks_resid_restrict) on level L, from the
+// host's aggregate map: rr_ag aggregates per workgroup, about one member row per lane group,
+// and rr_cap = the most member rows one workgroup holds.  Level 0 (2x2 blocks, its own row
+// sums) keeps the separate kernels, as does a level whose rows would not fit kRRLds.
+void plan_resid_restrict(MgLevel& L) {
+  L.rr_ag = L.rr_cap = 0;
+  if (L.nagg <= 0 || L.bs != 3) return;
+  std::vector<int> cnt(L.nagg, 0);
+  for (int i = 0; i < L.nodes; ++i) ++cnt[L.h_agg[i]];
+  const double rows_per_agg = (double)L.n / L.nagg;
+  const int ag = std::max(1, (int)((kT / lanes_per_row(L.n)) / rows_per_agg));
+  int cap = 0;
+  for (int a0 = 0; a0 < L.nagg; a0 += ag) {
+    int rows = 0;
+    for (int a = a0; a < std::min(a0 + ag, L.nagg); ++a) rows += L.bs * cnt[a];
+    cap = std::max(cap, rows);
+  }
+  if ((size_t)cap * (sizeof(double) + sizeof(int)) > kRRLds) return;
+  L.rr_ag = ag;
+  L.rr_cap = cap;
+}
+
 void alloc_work(Context& c, MgLevel& L) {
-  for (DBuf<double>* v : {&L.rhs, &L.x, &L.x2, &L.d, &L.res, &L.kx, &L.kres, &L.kp, &L.kq, &L.kz}) {
+  for (DBuf<double>* v :
+       {&L.rhs, &L.x, &L.x2, &L.d, &L.res, &L.kx, &L.kres, &L.kres2, &L.kp, &L.kq, &L.kz}) {
     v->alloc(std::max(1, L.n));
     HIP_CHECK(hipMemsetAsync(v->get(), 0, sizeof(double) * std::max(1, L.n), c.stream));
   }
@@ -1523,10 +1908,9 @@ void alloc_work(Context& c, MgLevel& L) {
 
 // ---- the cycle -----------------------------------------------------------------------------
 
-void kcycle(Context& c, int l, const PcgScalars* sc);
-
+struct PreFold;
 int cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars* sc,
-            const double* dq = nullptr, double* dpart = nullptr);
+            const double* dq = nullptr, double* dpart = nullptr, const PreFold* pf = nullptr);
 
 // ---- launches with the lane count chosen per level ------------------------------------------
 
@@ -1539,25 +1923,28 @@ int cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars*
   }
 
 // the same with the matrix value type as the kernel's second template argument: FP32 when
-// the level holds an FP32 copy of its values and the learned smoother is in use
-#define CSTFEM_LPR_SWITCH_M(L, lpr, KERNEL, grid, ...)                                           \
+// the level holds an FP32 copy of its values and the learned smoother is in use; shm bytes
+// of dynamic LDS
+#define CSTFEM_LPR_SWITCH_MS(L, lpr, KERNEL, grid, shm, ...)                                      \
   if ((L).val32.n && c.mg.fine_smoother == 0) {                                               \
     const MatS32 A_ = mats32(L);                                                              \
     switch (lpr) {                                                                            \
-      case 1: hipLaunchKernelGGL((KERNEL<1, float>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
-      case 2: hipLaunchKernelGGL((KERNEL<2, float>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
-      case 4: hipLaunchKernelGGL((KERNEL<4, float>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
-      default: hipLaunchKernelGGL((KERNEL<8, float>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
+      case 1: hipLaunchKernelGGL((KERNEL<1, float>), dim3(grid), dim3(kT), shm, c.stream, A_, __VA_ARGS__); break; \
+      case 2: hipLaunchKernelGGL((KERNEL<2, float>), dim3(grid), dim3(kT), shm, c.stream, A_, __VA_ARGS__); break; \
+      case 4: hipLaunchKernelGGL((KERNEL<4, float>), dim3(grid), dim3(kT), shm, c.stream, A_, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL((KERNEL<8, float>), dim3(grid), dim3(kT), shm, c.stream, A_, __VA_ARGS__); break; \
     }                                                                                         \
   } else {                                                                                    \
     const MatS A_ = mats(L);                                                                  \
     switch (lpr) {                                                                            \
-      case 1: hipLaunchKernelGGL((KERNEL<1, double>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
-      case 2: hipLaunchKernelGGL((KERNEL<2, double>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
-      case 4: hipLaunchKernelGGL((KERNEL<4, double>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
-      default: hipLaunchKernelGGL((KERNEL<8, double>), dim3(grid), dim3(kT), 0, c.stream, A_, __VA_ARGS__); break; \
+      case 1: hipLaunchKernelGGL((KERNEL<1, double>), dim3(grid), dim3(kT), shm, c.stream, A_, __VA_ARGS__); break; \
+      case 2: hipLaunchKernelGGL((KERNEL<2, double>), dim3(grid), dim3(kT), shm, c.stream, A_, __VA_ARGS__); break; \
+      case 4: hipLaunchKernelGGL((KERNEL<4, double>), dim3(grid), dim3(kT), shm, c.stream, A_, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL((KERNEL<8, double>), dim3(grid), dim3(kT), shm, c.stream, A_, __VA_ARGS__); break; \
     }                                                                                         \
   }
+#define CSTFEM_LPR_SWITCH_M(L, lpr, KERNEL, grid, ...) \
+  CSTFEM_LPR_SWITCH_MS(L, lpr, KERNEL, grid, 0, __VA_ARGS__)
 
 // grid of ks_cheb on a level (one block partial each when it also forms a dot product)
 int cheb_grid(const MgLevel& L) { return grid1((int64_t)L.n * lanes_per_row(L.n)); }
@@ -1583,13 +1970,54 @@ void launch_resid(Context& c, MgLevel& L, const double* x, const double* r, doub
   CSTFEM_LPR_SWITCH_M(L, lpr, ks_resid, grid1((int64_t)L.n * lpr), x, r, res, sc);
 }
 
+// lanes per coarse DOF of the restriction from level L
+int restrict_lanes(const MgLevel& L) {
+  const int nc = 3 * L.nagg;
+  return nc >= 100000 ? 1 : nc >= 20000 ? 2 : 8;
+}
+
 // restriction of level L's residual res into rc (3 per aggregate)
 void launch_restrict(Context& c, MgLevel& L, const double* res, double* rc, const PcgScalars* sc) {
-  const int nc = 3 * L.nagg;
-  const int lpr = nc >= 100000 ? 1 : nc >= 20000 ? 2 : 8;
-  CSTFEM_LPR_SWITCH(lpr, k_restrict, grid1((int64_t)nc * lpr), L.nagg, L.bs,
+  const int lpr = restrict_lanes(L);
+  CSTFEM_LPR_SWITCH(lpr, k_restrict, grid1((int64_t)3 * L.nagg * lpr), L.nagg, L.bs,
                     (const int*)L.mptr.get(), (const int*)L.mem.get(), (const double*)L.P.get(),
                     res, rc, sc);
+}
+
+// levels >= 1 of fewer than fuse_n DOFs run the fused residual + restriction
+bool fused_level(const Multigrid& mg, int l) { return mg.fuse && l >= 1 && mg.lv[l]->n < mg.fuse_n; }
+
+// ... and, where a row sum takes 4 or more lanes (few entries per lane), the prolongation inside
+// the first post-smoothing step: its gathers of x2 + P e cost 8 or 11 loads per entry instead
+// of 1, which on a level with 2 lanes per row (about 11 entries each) takes longer than the
+// separate k_prolong (22 against 10 us on a level of 74k DOFs)
+bool fused_prolong(const Multigrid& mg, int l) {
+  return fused_level(mg, l) && lanes_per_row(mg.lv[l]->n) >= 4;
+}
+
+// A K-cycle update is left to the kernel that reads its result next when that kernel's
+// workgroups, each summing the step's np partials, read fewer than fold_max partials in all
+// (the separate ks_update is cheaper where they would read more)
+bool fold_ok(const Multigrid& mg, int64_t reader_grid, int np) {
+  return mg.fuse && reader_grid * np <= mg.fold_max;
+}
+
+// residual rhs - A x of level l and its restriction into rc: one fused kernel where the level
+// is small and planned for it (rr_ag > 0), else ks_resid + k_restrict
+void launch_resid_restrict(Context& c, int l, const double* x, const double* rhs, double* rc,
+                           const PcgScalars* sc) {
+  MgLevel& L = *c.mg.lv[l];
+  if (!(fused_level(c.mg, l) && L.rr_ag > 0)) {
+    launch_resid(c, L, x, rhs, L.res.get(), sc);
+    launch_restrict(c, L, L.res.get(), rc, sc);
+    return;
+  }
+  const int lpr = lanes_per_row(L.n);
+  const int grid = (L.nagg + L.rr_ag - 1) / L.rr_ag;
+  const size_t shm = (size_t)L.rr_cap * (sizeof(double) + sizeof(int));
+  CSTFEM_LPR_SWITCH_MS(L, lpr, ks_resid_restrict, grid, shm, x, rhs, L.nagg, L.rr_ag, L.rr_cap,
+                       L.bs, restrict_lanes(L), (const int*)L.mptr.get(),
+                       (const int*)L.mem.get(), (const double*)L.P.get(), rc, sc);
 }
 
 // grid of the K-cycle kernels of a level (all of them share it: their partials are summed
@@ -1667,26 +2095,64 @@ double* coarse_rhs(Multigrid& mg, int l) {
   return kc ? C.kres.get() : C.rhs.get();
 }
 
+// The last update of a K-cycle, kx += alpha p with alpha from the step's partials, when it is
+// left to the kernel that reads kx next (the prolongation into the level above); p null: kx
+// is final
+struct Fold {
+  const double* p = nullptr;
+  const double* part = nullptr;
+  int np = 0;
+};
+
+// The K-cycle's first update folded into the second cycle's pre-smoothing (ks_cheb_pre_fold):
+// the residual res - alpha q goes to res2 and kx = alpha p
+struct PreFold {
+  const double* res;
+  const double* q;
+  const double* p;
+  const double* part;
+  int np;
+  double* res2;
+  double* kx;
+};
+
+Fold kcycle(Context& c, int l, const PcgScalars* sc);
+
 // coarse correction of level l: lv[l+1].kx from lv[l+1].rhs -- the dense solve on the
-// coarsest level, the K-cycle on levels <= kdepth, one cycle (V) below
-void coarse_correction(Context& c, int l, const PcgScalars* sc) {
+// coarsest level, the K-cycle on levels <= kdepth, one cycle (V) below.  Returns the update
+// the K-cycle left to the prolongation, if any.
+Fold coarse_correction(Context& c, int l, const PcgScalars* sc) {
   Multigrid& mg = c.mg;
   MgLevel& C = *mg.lv[l + 1];
-  if (l + 1 == (int)mg.lv.size() - 1)
+  if (l + 1 == (int)mg.lv.size() - 1) {
     hipLaunchKernelGGL(k_dense, dim3(grid1(C.n)), dim3(kT), 0, c.stream, C.n,
                        (const double*)mg.cinv.get(), (const double*)C.rhs.get(), C.kx.get(), sc);
-  else if (l + 1 <= mg.kdepth)
-    kcycle(c, l + 1, sc);
-  else
-    cycle_s(c, l + 1, C.rhs.get(), C.kx.get(), sc);
+    return Fold{};
+  }
+  if (l + 1 <= mg.kdepth) return kcycle(c, l + 1, sc);
+  cycle_s(c, l + 1, C.rhs.get(), C.kx.get(), sc);
+  return Fold{};
+}
+
+// XProlong of level L from the coarse level C's kx and the update f left on it
+XProlong prolong_of(const MgLevel& L, const MgLevel& C, double* x2, const Fold& f) {
+  return XProlong{x2, L.agg.get(), L.P.get(), C.kx.get(), f.p, 0.0, L.bs};
+}
+
+// x2 += P e (k_prolong) on level l after its coarse correction
+void launch_prolong(Context& c, int l, double* x2, const Fold& f, const PcgScalars* sc) {
+  MgLevel& L = *c.mg.lv[l];
+  hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n,
+                     prolong_of(L, *c.mg.lv[l + 1], x2, f), f.part, f.np, x2, sc);
 }
 
 // cycle on level l >= 1: out = B_l(rhs)
 // With dpart, the last smoothing step also leaves the block partials of out'dq there and the
 // return value is their count; 0 when the partials were not formed (the fused tail; a level
-// whose smoothing grid exceeds the 2 kPart partial slots)
+// whose smoothing grid exceeds the 2 kPart partial slots).  With pf (the K-cycle's second
+// cycle), rhs is pf->res2, which the pre-smoothing writes first.
 int cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars* sc,
-            const double* dq, double* dpart) {
+            const double* dq, double* dpart, const PreFold* pf) {
   if (l == c.mg.tail_from) {
     launch_tail(c, l, rhs, out, sc);
     return 0;
@@ -1694,49 +2160,75 @@ int cycle_s(Context& c, int l, const double* rhs, double* out, const PcgScalars*
   MgLevel& L = *c.mg.lv[l];
   MgLevel& C = *c.mg.lv[l + 1];
   const Cheb ch = cheb(L.lam);
-  launch_cheb_pre(c, L, L.x2.get(), rhs, ch, sc);
-  launch_resid(c, L, L.x2.get(), rhs, L.res.get(), sc);
-  launch_restrict(c, L, L.res.get(), coarse_rhs(c.mg, l), sc);
-  coarse_correction(c, l, sc);
-  hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n, L.bs,
-                     (const int*)L.agg.get(), (const double*)L.P.get(), (const double*)C.kx.get(),
-                     L.x2.get(), sc);
-  launch_cheb(c, L, L.x2.get(), L.x.get(), rhs, 0.0, ch.c1r, sc);
+  const int lpr = lanes_per_row(L.n);
+  if (pf) {
+    CSTFEM_LPR_SWITCH_M(L, lpr, ks_cheb_pre_fold, grid1((int64_t)L.n * lpr), L.x2.get(),
+                        L.d.get(), pf->res, pf->q, pf->p, pf->part, pf->np, pf->res2, pf->kx,
+                        (const double*)L.dinv.get(), ch.c1r, ch.c2d, ch.c2r, sc);
+  } else {
+    launch_cheb_pre(c, L, L.x2.get(), rhs, ch, sc);
+  }
+  launch_resid_restrict(c, l, L.x2.get(), rhs, coarse_rhs(c.mg, l), sc);
+  const Fold f = coarse_correction(c, l, sc);
+  if (fused_prolong(c.mg, l)) {
+    CSTFEM_LPR_SWITCH_M(L, lpr, ks_cheb_prolong, grid1((int64_t)L.n * lpr),
+                        prolong_of(L, C, L.x2.get(), f), f.part, f.np, L.x.get(), L.d.get(), rhs,
+                        (const double*)L.dinv.get(), ch.c1r, sc);
+  } else {
+    launch_prolong(c, l, L.x2.get(), f, sc);
+    launch_cheb(c, L, L.x2.get(), L.x.get(), rhs, 0.0, ch.c1r, sc);
+  }
   const bool dot = dpart && cheb_grid(L) <= 2 * kPart;
   launch_cheb(c, L, L.x.get(), out, rhs, ch.c2d, ch.c2r, sc, dot ? dq : nullptr,
               dot ? dpart : nullptr);
   return dot ? cheb_grid(L) : 0;
 }
 
-// two flexible CG steps on level l for the right-hand side in lv[l].kres (overwritten by the
-// residual), result in lv[l].kx
-void kcycle(Context& c, int l, const PcgScalars* sc) {
-  MgLevel& L = *c.mg.lv[l];
+// two flexible CG steps on level l for the right-hand side in lv[l].kres, result in lv[l].kx.
+// Where fold_ok allows, the first step's update of kx and the residual happens inside the
+// second cycle's pre-smoothing (the residual goes to kres2), and the second step's update of
+// kx is returned for the prolongation into level l - 1 to apply; otherwise ks_update does them.
+Fold kcycle(Context& c, int l, const PcgScalars* sc) {
+  Multigrid& mg = c.mg;
+  MgLevel& L = *mg.lv[l];
   const int lpr = lanes_per_row(L.n);
   const int g = kcycle_grid(L);           // grid-stride kernels with <= kPart partials
   double* s0 = L.part.get();             // p'q, p'r of step 1
   double* s1 = L.part.get() + 2 * kPart;  // z'q_old
   double* s2 = L.part.get() + 4 * kPart;  // p'q, p'r of step 2
-  cycle_s(c, l, L.kres.get(), L.kz.get(), sc, nullptr, nullptr);  // the restriction filled kres
+  const bool fuse = fold_ok(mg, grid1((int64_t)L.n * lpr), g);
+  // the reader of kx: ks_cheb_prolong on a level above that fuses the prolongation, else k_prolong
+  const MgLevel& U = *mg.lv[l - 1];
+  const int64_t ug = fused_prolong(mg, l - 1) ? grid1((int64_t)U.n * lanes_per_row(U.n)) : grid1(U.n);
+  const bool fold_out = fold_ok(mg, ug, g);
+  cycle_s(c, l, L.kres.get(), L.kz.get(), sc);  // the restriction filled kres
   CSTFEM_LPR_SWITCH_M(L, lpr, ks_spmv_dir, g, (const double*)L.kz.get(), L.kp.get(), L.kq.get(),
                       (const double*)L.kres.get(), s0, (const double*)nullptr, 0,
                       (const double*)nullptr, g, 1, sc);
-  hipLaunchKernelGGL(ks_update, dim3(g), dim3(kT), 0, c.stream, L.n, L.kx.get(), L.kres.get(),
-                     (const double*)L.kp.get(), (const double*)L.kq.get(), (const double*)s0, g, 1, 1,
-                     sc);
+  double* res = L.kres.get();
+  PreFold pf{};
+  if (fuse) {
+    pf = PreFold{L.kres.get(), L.kq.get(), L.kp.get(), s0, g, L.kres2.get(), L.kx.get()};
+    res = L.kres2.get();
+  } else {
+    hipLaunchKernelGGL(ks_update, dim3(g), dim3(kT), 0, c.stream, L.n, L.kx.get(), L.kres.get(),
+                       (const double*)L.kp.get(), (const double*)L.kq.get(), (const double*)s0, g, 1,
+                       1, sc);
+  }
   // z'q_old: formed by the cycle's last smoothing step where it can, else by ks_dot
-  int nzq = cycle_s(c, l, L.kres.get(), L.kz.get(), sc, (const double*)L.kq.get(), s1);
+  int nzq = cycle_s(c, l, res, L.kz.get(), sc, (const double*)L.kq.get(), s1, fuse ? &pf : nullptr);
   if (nzq == 0) {
     hipLaunchKernelGGL(ks_dot, dim3(g), dim3(kT), 0, c.stream, L.n, (const double*)L.kz.get(),
                        (const double*)L.kq.get(), s1, sc);
     nzq = g;
   }
   CSTFEM_LPR_SWITCH_M(L, lpr, ks_spmv_dir, g, (const double*)L.kz.get(), L.kp.get(), L.kq.get(),
-                      (const double*)L.kres.get(), s2, (const double*)s1, nzq, (const double*)s0,
-                      g, 0, sc);
+                      (const double*)res, s2, (const double*)s1, nzq, (const double*)s0, g, 0, sc);
+  if (fold_out) return Fold{L.kp.get(), s2, g};
   hipLaunchKernelGGL(ks_update, dim3(g), dim3(kT), 0, c.stream, L.n, L.kx.get(), L.kres.get(),
                      (const double*)L.kp.get(), (const double*)L.kq.get(), (const double*)s2, g, 0, 0,
                      sc);
+  return Fold{};
 }
 
 }  // namespace
@@ -1748,7 +2240,6 @@ int mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* s
   Multigrid& mg = c.mg;
   if (!mg.ready) throw Error(CSTFEM_ERR_ORDER, "multilevel: not set up");
   MgLevel& L = *mg.lv[0];
-  MgLevel& C = *mg.lv[1];
   const Mat0 A = mat0(c);
   const int gs = slice_grid(c.nslices);
   const double2* r2 = (const double2*)r;
@@ -1783,10 +2274,7 @@ int mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* s
   };
   resid0();
   launch_restrict(c, L, L.res.get(), coarse_rhs(mg, 0), sc);
-  coarse_correction(c, 0, sc);
-  hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n, 2,
-                     (const int*)L.agg.get(), (const double*)L.P.get(), (const double*)C.kx.get(), z,
-                     sc);
+  launch_prolong(c, 0, z, coarse_correction(c, 0, sc), sc);
   // post-smoothing
   if (learned) {
     resid0();
@@ -1808,13 +2296,9 @@ void mg_enqueue_coarse(Context& c, const double* r, double* z) {
   Multigrid& mg = c.mg;
   if (!mg.ready) throw Error(CSTFEM_ERR_ORDER, "multilevel: not set up");
   MgLevel& L = *mg.lv[0];
-  MgLevel& C = *mg.lv[1];
   HIP_CHECK(hipMemsetAsync(z, 0, sizeof(double) * (size_t)L.n, c.stream));
   launch_restrict(c, L, r, coarse_rhs(mg, 0), nullptr);
-  coarse_correction(c, 0, nullptr);
-  hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n, 2,
-                     (const int*)L.agg.get(), (const double*)L.P.get(), (const double*)C.kx.get(), z,
-                     (const PcgScalars*)nullptr);
+  launch_prolong(c, 0, z, coarse_correction(c, 0, nullptr), nullptr);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -1903,6 +2387,12 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     // FP64 residual floor moves with such perturbations (72 -> 90 on the 4M plate)
     const char* env = std::getenv("CSTFEM_MG_COARSE32");
     mg.coarse_fp32 = fine_smoother == 0 && !(env && env[0] == '0');
+    const char* fuse = std::getenv("CSTFEM_MG_FUSE");
+    mg.fuse = !(fuse && fuse[0] == '0');
+    const char* fuse_n = std::getenv("CSTFEM_MG_FUSE_N");
+    mg.fuse_n = fuse_n ? std::atoi(fuse_n) : 200000;
+    const char* fold = std::getenv("CSTFEM_MG_FOLD");
+    mg.fold_max = fold ? std::atoll(fold) : 200000;
   }
   // level 0: the context's matrix; its node graph is read on the host (aggregation) from a
   // page-locked copy.  The pattern is complete since build_pattern returned, so the copy runs
@@ -1975,6 +2465,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
       L.nagg = aggregate(hp, hc, L.nodes, L.h_agg);
     }
     const int na = L.nagg;
+    plan_resid_restrict(L);
     // members and the aggregate map go up from page-locked memory; the staging area follows
     // the node graph, which the host no longer reads once the aggregation is done
     // the aggregate map goes up from page-locked staging; the member lists are built on the

@@ -14,7 +14,8 @@
 - a learned smoother that overshoots (lambda_max(M A) > 1.5) is replaced by Chebyshev at
   set-up; one that breaks down during the solve hands over to the same hierarchy with the
   Chebyshev smoother; either way the solve converges;
-- a good trained smoother passes the set-up check.
+- a good trained smoother passes the set-up check;
+- the fused cycle kernels give bit-identical cycles and solves to the separate kernels.
 No reference counterpart exists for this component.
 """
 import dataclasses
@@ -353,6 +354,44 @@ def test_fused_tail_matches_separate_kernels(dev, monkeypatch, kdepth):
     z_kern = dev.mg_apply(r)
     assert np.linalg.norm(z_tail - z_kern) <= 1e-12 * np.linalg.norm(z_kern)
     assert np.linalg.norm(z_kern) > 0
+
+
+@pytest.mark.parametrize("smoother,kdepth", [(0, 3), (0, 1), (1, 3)])
+def test_fused_cycle_kernels_are_bit_identical(dev, theta40, monkeypatch, smoother, kdepth):
+    """The fused cycle kernels (K-cycle updates folded into their readers, CSTFEM_MG_FOLD;
+    residual + restriction in one kernel and the prolongation inside the first post-smoothing
+    step on levels below CSTFEM_MG_FUSE_N DOFs) perform the separate kernels' arithmetic in the same
+    order: one cycle, and a whole flexible CG solve, are bit-identical to CSTFEM_MG_FUSE=0,
+    with FP32 coarse values (learned smoother) and FP64 ones (Chebyshev), with and without
+    the fused tail below the K-cycle levels."""
+    system(dev, hetero(300, e_seed=3, seed=4))
+    if smoother == 0:
+        dev.mpnn_set_params(theta40)
+        dev.mpnn_build()
+    r = np.random.default_rng(11).standard_normal(dev.n)
+    out = {}
+    for fuse, fuse_n, fold in (("0", "200000", "200000"), ("1", "0", "1000000000"),
+                               ("1", "200000", "200000"), ("1", "100000000", "1000000000"),
+                               ("1", "100000000", "0")):
+        monkeypatch.setenv("CSTFEM_MG_FUSE", fuse)
+        monkeypatch.setenv("CSTFEM_MG_FUSE_N", fuse_n)
+        monkeypatch.setenv("CSTFEM_MG_FOLD", fold)
+        dev.mg_setup(12, 500, smoother, kdepth)
+        if smoother == 0:
+            assert 0.5 < dev.mg_level_info(0)[3] <= 1.5  # the set-up kept the learned smoother
+        sizes = [dev.mg_level_info(l)[0] for l in range(dev.mg_levels())]
+        z = dev.mg_apply(r)
+        x, st = dev.solve(rtol=1e-10, max_iter=2000, method=FCG)
+        assert st.converged and not st.fallback and st.true_rel_residual < 1e-10
+        out[(fuse, fuse_n, fold)] = (z, x, st.iterations)
+        print(f"fuse {fuse} below {fuse_n} DOFs, fold {fold}: levels {sizes}, "
+              f"{st.iterations} iterations")
+    z0, x0, it0 = out[("0", "200000", "200000")]
+    assert np.linalg.norm(z0) > 0
+    for key, (z, x, it) in out.items():
+        np.testing.assert_array_equal(z, z0, err_msg=str(key))
+        np.testing.assert_array_equal(x, x0, err_msg=str(key))
+        assert it == it0, key
 
 
 def test_trained_smoother_beats_untrained_and_chebyshev(dev):
