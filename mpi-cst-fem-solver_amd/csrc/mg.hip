@@ -35,6 +35,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "internal.hpp"
 
@@ -935,8 +936,9 @@ constexpr int kTailLds = 64 * 1024;  // bytes of LDS for the tail's vectors
 constexpr int kTailLanes = 8;        // lanes per coarse DOF (restriction) and per dense row
 constexpr int kTailRegW = 21;        // row entries a tail thread keeps in registers (7 blocks)
 
+template <class T>
 struct TailLevel {
-  MatS A;
+  MatST<T> A;
   const double* dinv;
   const int* mptr;
   const int* mem;
@@ -947,9 +949,10 @@ struct TailLevel {
   double c1r, c2d, c2r;
 };
 
+template <class T>
 struct Tail {
   int nlev;
-  TailLevel L[kTailLevels];
+  TailLevel<T> L[kTailLevels];
   const double* cinv;
   int nc;
   int lds_c;          // offset of the coarsest rhs and solution in LDS
@@ -964,16 +967,17 @@ struct Tail {
 // Row i of the tail's top level, loaded once into registers (kReg: that level has at most
 // kTailThreads rows of at most kTailRegW entries): its four row sums then read only LDS.  The
 // sums run over the entries in the order of row_entries, so they are bit-identical to rowS_f.
+template <class T>
 struct RegRow {
-  double v[kTailRegW];
+  T v[kTailRegW];
   int b[kTailRegW / 3];
   int w = 0;
-  __device__ void load(const MatS& A, int i) {
+  __device__ void load(const MatST<T>& A, int i) {
     const int s = i >> 6, lane = i & 63;
     const int64_t o = A.off[s];
     w = (int)((A.off[s + 1] - o) >> 6);
 #pragma unroll
-    for (int k = 0; k < kTailRegW; ++k) v[k] = k < w ? A.val[o + (int64_t)k * 64 + lane] : 0.0;
+    for (int k = 0; k < kTailRegW; ++k) v[k] = k < w ? A.val[o + (int64_t)k * 64 + lane] : T(0);
 #pragma unroll
     for (int k = 0; k < kTailRegW / 3; ++k)
       b[k] = 3 * k < w ? A.bcol[o / 3 + (int64_t)k * 64 + lane] : 0;
@@ -982,59 +986,86 @@ struct RegRow {
     double a = 0.0;
 #pragma unroll
     for (int k = 0; k < kTailRegW; ++k)
-      if (k < w) a = a + v[k] * x[3 * b[k / 3] + k % 3];
+      if (k < w) a = a + (double)v[k] * x[3 * b[k / 3] + k % 3];
     return a;
   }
 };
 
-template <bool kReg>
-__global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars* __restrict__ sc) {
+// What the top level (kReg) keeps in registers besides its row: its diagonal scaling and the
+// aggregate of its DOF, loaded at the kernel's start with the row (one round trip; a deeper
+// chain of loads there, such as the restriction's member rows and P entries, costs more at
+// the start than it saves later: the tail's data comes from HBM or the MALL at every call)
+struct TopRegs {
+  double dinv = 0.0;
+  int agg = 0;
+};
+
+template <bool kReg, class T>
+__global__ __launch_bounds__(kTailThreads) void k_tail(Tail<T> T_, const PcgScalars* __restrict__ sc) {
   extern __shared__ double lds[];
+  const Tail<T>& TT = T_;
   const int tid = threadIdx.x;
+  const TailLevel<T>& L0 = TT.L[0];
+  const int n0 = L0.A.n;
   int st = status_of(sc);
-  // the right-hand side (one entry per thread while the top level fits) and the top level's
-  // rows are loaded with the status word
-  double rhs_first = tid < T.L[0].A.n ? T.rhs[tid] : 0.0;
-  RegRow rr;
-  if (kReg && tid < T.L[0].A.n) rr.load(T.L[0].A, tid);
+  // the right-hand side (one entry per thread while the top level fits), the top level's rows
+  // and operands and the dense inverse's staging are loaded with the status word
+  double rhs_first = tid < n0 ? TT.rhs[tid] : 0.0;
+  RegRow<T> rr;
+  TopRegs top;
+  if (kReg) {
+    if (tid < n0) {
+      rr.load(L0.A, tid);
+      top.dinv = L0.dinv[tid];
+      top.agg = L0.agg[tid / L0.bs];
+    }
+  }
   hold(st, rhs_first);
   if (halted(st)) return;
-  // row sum of level l's row i: from registers on the top level (kReg), else from L2
-  auto row = [&](int l, const TailLevel& L, const double* x, int i) {
-    return (kReg && l == 0) ? rr.sum(x) : rowS_f(L.A, XVec{x}, i);
+  // row sum of level l's row i: from registers on the top level (kReg), else from L2 (with
+  // the top level's registers taken, in groups of 4 loads)
+  auto row = [&](int l, const TailLevel<T>& L, const double* x, int i) {
+    if (kReg && l == 0) return rr.sum(x);
+    const RowAt ra = row_at(L.A, i);
+    return row_entries<kReg ? 4 : 8>(L.A, XVec{x}, ra.o, ra.lane, ra.w(), 0, 1, 0.0);
+  };
+  // diagonal scaling of level l's row i
+  auto dinv_of = [&](int l, const TailLevel<T>& L, int i) {
+    return (kReg && l == 0) ? top.dinv : L.dinv[i];
   };
   {
-    double* rhs0 = lds + T.L[0].lds;
-    if (tid < T.L[0].A.n) rhs0[tid] = rhs_first;
-    for (int i = tid + kTailThreads; i < T.L[0].A.n; i += kTailThreads) rhs0[i] = T.rhs[i];
+    double* rhs0 = lds + L0.lds;
+    if (tid < n0) rhs0[tid] = rhs_first;
+    for (int i = tid + kTailThreads; i < n0; i += kTailThreads) rhs0[i] = TT.rhs[i];
   }
   __syncthreads();
-  for (int l = 0; l < T.nlev; ++l) {
-    const TailLevel& L = T.L[l];
+  for (int l = 0; l < TT.nlev; ++l) {
+    const TailLevel<T>& L = TT.L[l];
     const int n = L.A.n;
     double* rhs = lds + L.lds;
     double *x = rhs + n, *x2 = x + n, *d = x2 + n, *res = d + n;
     for (int i = tid; i < n; i += kTailThreads) {
-      const double dd = 0.0 + L.c1r * (L.dinv[i] * (rhs[i] - 0.0));
+      const double dd = 0.0 + L.c1r * (dinv_of(l, L, i) * (rhs[i] - 0.0));
       d[i] = dd;
       x[i] = 0.0 + dd;
     }
     __syncthreads();
     for (int i = tid; i < n; i += kTailThreads) {
       const double y = row(l, L, x, i);
-      const double dd = (L.c2d != 0.0 ? L.c2d * d[i] : 0.0) + L.c2r * (L.dinv[i] * (rhs[i] - y));
+      const double dd =
+          (L.c2d != 0.0 ? L.c2d * d[i] : 0.0) + L.c2r * (dinv_of(l, L, i) * (rhs[i] - y));
       d[i] = dd;
       x2[i] = x[i] + dd;
     }
     __syncthreads();
     for (int i = tid; i < n; i += kTailThreads) res[i] = rhs[i] - row(l, L, x2, i);
     __syncthreads();
-    double* rc = lds + (l + 1 < T.nlev ? T.L[l + 1].lds : T.lds_c);
+    double* rc = lds + (l + 1 < TT.nlev ? TT.L[l + 1].lds : TT.lds_c);
     for (int tt = tid; tt < 3 * L.nagg * kTailLanes; tt += kTailThreads) {
       const int t = tt / kTailLanes, j = tt % kTailLanes;
       const int m0 = L.mptr[t / 3];
-      double sum = restrict_sum(m0, L.bs * (L.mptr[t / 3 + 1] - m0), t % 3, L.bs, L.mem, L.P, res, j,
-                                kTailLanes);
+      double sum = restrict_sum(m0, L.bs * (L.mptr[t / 3 + 1] - m0), t % 3, L.bs, L.mem, L.P, res,
+                                j, kTailLanes);
 #pragma unroll
       for (int m = 1; m < kTailLanes; m <<= 1) sum += __shfl_xor(sum, m, 64);
       if (j == 0) rc[t] = sum;
@@ -1042,24 +1073,24 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars*
     __syncthreads();
   }
   {
-    const double* crhs = lds + T.lds_c;
-    double* ckx = lds + T.lds_c + T.nc;
-    for (int tt = tid; tt < T.nc * kTailLanes; tt += kTailThreads) {
+    const double* crhs = lds + TT.lds_c;
+    double* ckx = lds + TT.lds_c + TT.nc;
+    for (int tt = tid; tt < TT.nc * kTailLanes; tt += kTailThreads) {
       const int i = tt / kTailLanes, j = tt % kTailLanes;
-      const double* row = T.cinv + (size_t)i * T.nc;
+      const double* row = TT.cinv + (size_t)i * TT.nc;
       double sum = 0.0;
       // entries k = j, j + kTailLanes, ... in order, their loads issued 8 at a time
-      for (int k = j; k < T.nc; k += 8 * kTailLanes) {
+      for (int k = j; k < TT.nc; k += 8 * kTailLanes) {
         double a[8], b[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const int kk = k + u * kTailLanes < T.nc ? k + u * kTailLanes : k;
+          const int kk = k + u * kTailLanes < TT.nc ? k + u * kTailLanes : k;
           a[u] = row[kk];
           b[u] = crhs[kk];
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-          if (k + u * kTailLanes < T.nc) sum = sum + a[u] * b[u];
+          if (k + u * kTailLanes < TT.nc) sum = sum + a[u] * b[u];
       }
 #pragma unroll
       for (int m = 1; m < kTailLanes; m <<= 1) sum += __shfl_xor(sum, m, 64);
@@ -1067,16 +1098,16 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars*
     }
   }
   __syncthreads();
-  for (int l = T.nlev - 1; l >= 0; --l) {
-    const TailLevel& L = T.L[l];
+  for (int l = TT.nlev - 1; l >= 0; --l) {
+    const TailLevel<T>& L = TT.L[l];
     const int n = L.A.n;
     double* rhs = lds + L.lds;
     double *x = rhs + n, *x2 = x + n, *d = x2 + n, *res = d + n;
     // the level below left its result in its res vector (the coarsest: after its rhs)
-    const double* e = l + 1 < T.nlev ? lds + T.L[l + 1].lds + 4 * T.L[l + 1].A.n
-                                     : lds + T.lds_c + T.nc;
+    const double* e = l + 1 < TT.nlev ? lds + TT.L[l + 1].lds + 4 * TT.L[l + 1].A.n
+                                      : lds + TT.lds_c + TT.nc;
     for (int f = tid; f < n; f += kTailThreads) {
-      const int a = L.agg[f / L.bs];
+      const int a = (kReg && l == 0) ? top.agg : L.agg[f / L.bs];
       const double* p = L.P + (size_t)f * 3;
       const double* ea = e + 3 * a;
       x2[f] = x2[f] + (p[0] * ea[0] + p[1] * ea[1] + p[2] * ea[2]);
@@ -1084,15 +1115,16 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(Tail T, const PcgScalars*
     __syncthreads();
     for (int i = tid; i < n; i += kTailThreads) {
       const double y = row(l, L, x2, i);
-      const double dd = 0.0 + L.c1r * (L.dinv[i] * (rhs[i] - y));
+      const double dd = 0.0 + L.c1r * (dinv_of(l, L, i) * (rhs[i] - y));
       d[i] = dd;
       x[i] = x2[i] + dd;
     }
     __syncthreads();
-    double* out = l == 0 ? T.out : res;
+    double* out = l == 0 ? TT.out : res;
     for (int i = tid; i < n; i += kTailThreads) {
       const double y = row(l, L, x, i);
-      const double dd = (L.c2d != 0.0 ? L.c2d * d[i] : 0.0) + L.c2r * (L.dinv[i] * (rhs[i] - y));
+      const double dd =
+          (L.c2d != 0.0 ? L.c2d * d[i] : 0.0) + L.c2r * (dinv_of(l, L, i) * (rhs[i] - y));
       d[i] = dd;
       out[i] = x[i] + dd;
     }
@@ -2047,17 +2079,39 @@ int tail_start(const Multigrid& mg) {
   return t < last ? t : -1;
 }
 
-void launch_tail(Context& c, int t, const double* rhs, double* out, const PcgScalars* sc) {
+// Whether the fused tail reads FP32 row values: as the cycle's other levels do (the learned
+// smoother's cycle with FP32 copies on every tail level)
+bool tail_v32(const Multigrid& mg) {
+  if (mg.tail_from < 0 || mg.fine_smoother != 0) return false;
+  for (int l = mg.tail_from; l + 1 < (int)mg.lv.size(); ++l)
+    if (!mg.lv[l]->val32.n) return false;
+  return true;
+}
+
+// whether the top tail level keeps its rows in registers (one row per thread, <= kTailRegW
+// entries)
+bool tail_reg(const Multigrid& mg) {
+  const MgLevel& top = *mg.lv[mg.tail_from];
+  bool reg = top.n <= kTailThreads;
+  for (int s = 0; reg && s < top.nslices; ++s) reg = top.h_off[s + 1] - top.h_off[s] <= 64 * kTailRegW;
+  return reg;
+}
+
+template <class T>
+void launch_tail_t(Context& c, int t, const double* rhs, double* out, const PcgScalars* sc) {
   Multigrid& mg = c.mg;
   const int last = (int)mg.lv.size() - 1;
-  Tail T{};
-  T.nlev = last - t;
+  Tail<T> T_{};
+  T_.nlev = last - t;
   int off = 0;
-  for (int k = 0; k < T.nlev; ++k) {
+  for (int k = 0; k < T_.nlev; ++k) {
     MgLevel& L = *mg.lv[t + k];
     const Cheb ch = cheb(L.lam);
-    TailLevel& q = T.L[k];
-    q.A = mats(L);
+    TailLevel<T>& q = T_.L[k];
+    if constexpr (std::is_same<T, float>::value)
+      q.A = mats32(L);
+    else
+      q.A = mats(L);
     q.dinv = L.dinv.get();
     q.mptr = L.mptr.get();
     q.mem = L.mem.get();
@@ -2071,20 +2125,23 @@ void launch_tail(Context& c, int t, const double* rhs, double* out, const PcgSca
     q.c2d = ch.c2d;
     q.c2r = ch.c2r;
   }
-  T.cinv = mg.cinv.get();
-  T.nc = mg.lv[last]->n;
-  T.lds_c = off;
-  T.rhs = rhs;
-  T.out = out;
+  T_.cinv = mg.cinv.get();
+  T_.nc = mg.lv[last]->n;
+  T_.lds_c = off;
+  T_.rhs = rhs;
+  T_.out = out;
   const size_t bytes = sizeof(double) * (size_t)tail_lds_doubles(mg, t);
-  // the top level's rows in registers when they fit (one row per thread, <= kTailRegW entries)
-  const MgLevel& top = *mg.lv[t];
-  bool reg = top.n <= kTailThreads;
-  for (int s = 0; reg && s < top.nslices; ++s) reg = top.h_off[s + 1] - top.h_off[s] <= 64 * kTailRegW;
-  if (reg)
-    hipLaunchKernelGGL(k_tail<true>, dim3(1), dim3(kTailThreads), bytes, c.stream, T, sc);
+  if (tail_reg(mg))
+    hipLaunchKernelGGL((k_tail<true, T>), dim3(1), dim3(kTailThreads), bytes, c.stream, T_, sc);
   else
-    hipLaunchKernelGGL(k_tail<false>, dim3(1), dim3(kTailThreads), bytes, c.stream, T, sc);
+    hipLaunchKernelGGL((k_tail<false, T>), dim3(1), dim3(kTailThreads), bytes, c.stream, T_, sc);
+}
+
+void launch_tail(Context& c, int t, const double* rhs, double* out, const PcgScalars* sc) {
+  if (tail_v32(c.mg))
+    launch_tail_t<float>(c, t, rhs, out, sc);
+  else
+    launch_tail_t<double>(c, t, rhs, out, sc);
 }
 
 // where the restriction of level l's residual goes: straight into the K-cycle's residual when
