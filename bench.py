@@ -47,6 +47,23 @@ METRIC = "time-to-rtol 1e-10 (s) learned/mixed FCG vs FP64 Jacobi-CG; SpMV HBM G
 HBM_PEAK_GBS = 8000.0
 
 
+# arithmetic of each method: the outer iteration, its preconditioner, the network
+DTYPES = {"fp64": "f64", "mg_cheb": "f64", "ir32": "f32 + f64 refinement",
+          "learned1": "f64 outer + f32 M + bf16 network",
+          "learned32": "f32 + f64 refinement + f32 M + bf16 network",
+          "learned_mg": "f64 outer + f32 cycle + bf16 network"}
+PRECISION = {
+    "fp64": "FP64 matrix, vectors and sums",
+    "mg_cheb": "FP64 flexible CG; FP64 multilevel cycle (Chebyshev smoothing)",
+    "ir32": "FP32 matrix copy and vectors with FP64 dot products; FP64 x and true-residual refreshes",
+    "learned1": "FP64 flexible CG; M stored in FP32, applied with FP64 sums; network on BF16 MFMA",
+    "learned32": "FP32 flexible CG with FP64 refreshes; M in FP32; network on BF16 MFMA",
+    "learned_mg": ("FP64 flexible CG, SpMV and true residual; cycle row sums read FP32 copies of "
+                   "the level-0 and coarse matrices and M's FP32 values, accumulating in FP64; "
+                   "network forward on BF16 MFMA with FP32 accumulation"),
+}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -66,12 +83,31 @@ def cpu_threads():
     return max(1, min(16, os.cpu_count() or 1))
 
 
-def cpu_baseline(prob, iterations, sample_iters):
+def cpu_model():
+    """The host CPU's model name and logical CPU count (from /proc/cpuinfo)."""
+    name = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(model=name, logical_cpus=os.cpu_count())
+
+
+def cpu_baseline(prob, iterations, sample_iters, full_limit_s=40.0):
     """The oracle's restatement of the reference path on the host: pattern + assembly + loads
     + Dirichlet in full (sequential, as the reference), then Jacobi-PCG iterations of the same
-    recurrence, timed on one core (`sample_iters`) and on all threads (10x as many); the
-    time to rtol is the set-up time plus the per-iteration time times the GPU FP64 iteration
-    count."""
+    recurrence, timed on one core (`sample_iters`) and on all threads (10x as many).
+    - When the threaded solve to rtol 1e-10 is predicted to take at most `full_limit_s`, it
+      is run to convergence and `value` is the measured set-up + solve time (the prediction
+      is reported beside it); otherwise `value` is the set-up time plus the threaded time per
+      iteration times the GPU FP64 iteration count.
+    - Where the reference's own serial core is built (oracle/_ref), its pattern + assembly
+      and PCG iterations run on the same system on one core: the calibration of the
+      restatement against the reference on this host."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     oc = oracle_lib.Oracle(oracle_lib.ORACLE_PATH, "oc")
@@ -99,15 +135,53 @@ def cpu_baseline(prob, iterations, sample_iters):
     per_it = (t3 - t2) / max(1, st_mt["iterations"])
     est = (t1 - t0) + per_it * iterations
     log(f"cpu baseline: 1 core {per_it1 * 1e3:.1f} ms/it, {threads} threads "
-        f"{per_it * 1e3:.2f} ms/it")
-    return dict(value=est, unit="s", cores=threads, kind="port",
-                sample=(f"oracle restatement: pattern+assembly+loads+Dirichlet on 1 core "
-                        f"({t1 - t0:.2f} s) + {st_mt['iterations']} Jacobi-PCG iterations on "
-                        f"{threads} OpenMP threads ({per_it * 1e3:.2f} ms/it), extrapolated to "
-                        f"the GPU FP64 count of {iterations} iterations"),
-                assembly_s=t1 - t0, s_per_iteration=per_it,
-                one_core=dict(s_per_iteration=per_it1, iterations_timed=st["iterations"],
-                              value=(t1 - t0) + per_it1 * iterations))
+        f"{per_it * 1e3:.2f} ms/it, predicted {est:.1f} s")
+    out = dict(value=est, unit="s", cores=threads, kind="port", measured=False,
+               sample=(f"oracle restatement: pattern+assembly+loads+Dirichlet on 1 core "
+                       f"({t1 - t0:.2f} s) + {st_mt['iterations']} Jacobi-PCG iterations on "
+                       f"{threads} OpenMP threads ({per_it * 1e3:.2f} ms/it), extrapolated to "
+                       f"the GPU FP64 count of {iterations} iterations"),
+               host=cpu_model(), assembly_s=t1 - t0, s_per_iteration=per_it, predicted_s=est,
+               one_core=dict(s_per_iteration=per_it1, iterations_timed=st["iterations"],
+                             value=(t1 - t0) + per_it1 * iterations))
+    if per_it * iterations <= full_limit_s:
+        t4 = time.perf_counter()
+        _, st_full = oc.pcg_jacobi_mt(rp, c, K, b, 1e-10, 20 * iterations + 1000, threads=threads)
+        t5 = time.perf_counter()
+        out.update(value=(t1 - t0) + (t5 - t4), measured=True,
+                   sample=(f"oracle restatement: pattern+assembly+loads+Dirichlet on 1 core "
+                           f"({t1 - t0:.2f} s) + Jacobi-PCG to rtol 1e-10 on {threads} OpenMP "
+                           f"threads, run to convergence ({st_full['iterations']} iterations, "
+                           f"{t5 - t4:.2f} s)"),
+                   iterations=st_full["iterations"], converged=bool(st_full["converged"]))
+        log(f"cpu baseline: measured {out['value']:.2f} s to convergence "
+            f"({st_full['iterations']} iterations) against {est:.2f} s predicted")
+    if os.path.exists(oracle_lib.REF_PATH):
+        # the reference's own core on the same system: pattern + assembly (with the uniform
+        # modulus, as the reference's assembly takes one material; the element loop's cost is
+        # the same), and PCG iterations on the oracle's eliminated system
+        ref = oracle_lib.Oracle(oracle_lib.REF_PATH, "ref")
+        t_o = time.perf_counter()
+        rp_o, c_o = oc.build_pattern(m.num_nodes, m.tri)
+        oc.assemble(m.xy, m.tri, rp_o, c_o, E=prob.E, nu=prob.nu, strain=strain, t=prob.thickness)
+        t_asm_uniform = time.perf_counter() - t_o
+        t6 = time.perf_counter()
+        rp_r, c_r = ref.build_pattern(m.num_nodes, m.tri)
+        ref.assemble(m.xy, m.tri, rp_r, c_r, E=prob.E, nu=prob.nu, strain=strain,
+                     t=prob.thickness)
+        t7 = time.perf_counter()
+        ref_iters = max(5, min(sample_iters, int(3.0 / max(per_it1, 1e-6))))
+        _, st_r = ref.pcg_jacobi(rp, c, K, b, 1e-300, ref_iters)
+        t8 = time.perf_counter()
+        ref_it = (t8 - t7) / max(1, st_r["iterations"])
+        out["reference_one_core"] = dict(
+            pattern_assembly_s=t7 - t6, s_per_iteration=ref_it, iterations_timed=st_r["iterations"],
+            oracle_over_reference_assembly=t_asm_uniform / (t7 - t6),
+            oracle_over_reference_per_iteration=per_it1 / ref_it)
+        log(f"cpu baseline: reference core, 1 core: pattern+assembly {t7 - t6:.2f} s "
+            f"(oracle {t_asm_uniform:.2f} s), {ref_it * 1e3:.1f} ms/it (oracle "
+            f"{per_it1 * 1e3:.1f})")
+    return out
 
 
 def oracle_plate_solve(oc, p, rtol):
@@ -488,6 +562,7 @@ def main():
     t_dir = dev.profile_kernel(3, reps=50)
     copy_bytes = 1 << 30
     t_copy = dev.profile_kernel(4, reps=20, nbytes=copy_bytes)
+    t_read = dev.profile_kernel(10, reps=20, nbytes=copy_bytes)
     t_spmv32 = dev.profile_kernel(5, reps=50)
     mg_levels = None
     if "learned_mg" in chosen or "mg_cheb" in chosen:
@@ -512,12 +587,14 @@ def main():
     b_spmv = spmv_bytes(n, nnz)
     achieved = b_spmv / t_spmv / 1e9
     copy_gbs = 2 * copy_bytes / t_copy / 1e9
+    read_gbs = copy_bytes / t_read / 1e9
     vec_bytes_upd = 8 * n * 7     # reads x p r Ap invd, writes x r
     vec_bytes_dir = 8 * n * 4     # reads r invd p, writes p
     iter_bytes = b_spmv + 8 * n * 11
     log(f"spmv {t_spmv * 1e6:.1f} us ({achieved:.0f} GB/s), update {t_upd * 1e6:.1f} us "
         f"({vec_bytes_upd / t_upd / 1e9:.0f} GB/s), direction {t_dir * 1e6:.1f} us "
-        f"({vec_bytes_dir / t_dir / 1e9:.0f} GB/s), copy {copy_gbs:.0f} GB/s")
+        f"({vec_bytes_dir / t_dir / 1e9:.0f} GB/s), copy {copy_gbs:.0f} GB/s, "
+        f"read {read_gbs:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -535,7 +612,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if best in ("ir32", "learned32") else "f64",
+        "dtype": DTYPES[best],
+        "precision": PRECISION[best],
         "mg_levels": mg_levels,
         "train": train_info,
         "fcg_kernel_s": fcg_kernels,
@@ -553,10 +631,12 @@ def main():
         "spmv_gbs": achieved,
         "spmv_gbs_scalar_csr_model": (12 * nnz + 4 * (n + 1) + 16 * n) / t_spmv / 1e9,
         "copy_gbs": copy_gbs,
+        "read_gbs": read_gbs,
         "spmv32_gbs": (4 * nnz + nnz + 4 * (n // 2 + 1) + 8 * n) / t_spmv32 / 1e9,
         "fp64_iteration_gbs": (iter_bytes / fp64["s_per_iteration"] / 1e9) if fp64 else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "frac_of_read_stream": achieved / read_gbs,
+                     "traffic": traffic,
                      "traffic_source": traffic_src,
                      "kernel": "k_cg_spmv", "bytes_per_launch": b_spmv,
                      "avg_launch_s": t_spmv},

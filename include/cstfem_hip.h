@@ -331,7 +331,9 @@ int cstfem_mg_apply(cstfem_ctx* ctx, const double* r, double* z, int n);
  * stream, timed with HIP events; solver state is untouched.  kind: 0 fused SpMV + p'Ap,
  * 1 plain SpMV, 2 CG update (axpys + dots), 3 CG direction, 4 copy of `bytes` bytes,
  * (achievable HBM bandwidth), 5 FP32 SpMV + p'Ap; flexible CG in FP64: 6 SpMV + p'q, p'r,
- * 7 preconditioner z = M r + z'q (after cstfem_mpnn_build), 8 update, 9 direction. */
+ * 7 preconditioner z = M r + z'q (after cstfem_mpnn_build), 8 update, 9 direction;
+ * 10 read of `bytes` bytes (achievable read-only HBM bandwidth).  Kinds 4 and 10 need no
+ * matrix. */
 int cstfem_profile_kernel(cstfem_ctx* ctx, int kind, int reps, int64_t bytes,
                           double* avg_seconds);
 

@@ -894,9 +894,10 @@ int cstfem_mg_apply(cstfem_ctx* ctx, const double* r, double* z, int n) {
 int cstfem_profile_kernel(cstfem_ctx* ctx, int kind, int reps, int64_t bytes, double* avg_seconds) {
   return guarded([&] {
     need(ctx && avg_seconds, "cstfem_profile_kernel: NULL argument");
-    need(kind >= 0 && kind <= 9, "cstfem_profile_kernel: kind must be 0..9");
+    need(kind >= 0 && kind <= 10, "cstfem_profile_kernel: kind must be 0..10");
     need(reps > 0, "cstfem_profile_kernel: reps must be positive");
-    need(kind != 4 || bytes >= 32, "cstfem_profile_kernel: copy needs bytes >= 32");
+    need((kind != 4 && kind != 10) || bytes >= 32,
+         "cstfem_profile_kernel: copy and read need bytes >= 32");
     use_device(&ctx->c);
     *avg_seconds = profile_kernel(ctx->c, kind, reps, bytes);
     return CSTFEM_OK;

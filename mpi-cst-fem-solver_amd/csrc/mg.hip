@@ -968,9 +968,10 @@ struct Tail {
 // Row i of the tail's top level, loaded once into registers (kReg: that level has at most
 // kTailThreads rows of at most kTailRegW entries): its four row sums then read only LDS.  The
 // sums run over the entries in the order of row_entries, so they are bit-identical to rowS_f.
+// The values are held as FP64 whatever the stored type (the conversion is exact).
 template <class T>
 struct RegRow {
-  T v[kTailRegW];
+  double v[kTailRegW];
   int b[kTailRegW / 3];
   int w = 0;
   __device__ void load(const MatST<T>& A, int i) {
@@ -978,7 +979,8 @@ struct RegRow {
     const int64_t o = A.off[s];
     w = (int)((A.off[s + 1] - o) >> 6);
 #pragma unroll
-    for (int k = 0; k < kTailRegW; ++k) v[k] = k < w ? A.val[o + (int64_t)k * 64 + lane] : T(0);
+    for (int k = 0; k < kTailRegW; ++k)
+      v[k] = k < w ? (double)A.val[o + (int64_t)k * 64 + lane] : 0.0;
 #pragma unroll
     for (int k = 0; k < kTailRegW / 3; ++k)
       b[k] = 3 * k < w ? A.bcol[o / 3 + (int64_t)k * 64 + lane] : 0;
@@ -987,7 +989,7 @@ struct RegRow {
     double a = 0.0;
 #pragma unroll
     for (int k = 0; k < kTailRegW; ++k)
-      if (k < w) a = a + (double)v[k] * x[3 * b[k / 3] + k % 3];
+      if (k < w) a = a + v[k] * x[3 * b[k / 3] + k % 3];
     return a;
   }
 };
@@ -1997,13 +1999,14 @@ int tail_start(const Multigrid& mg) {
   return t < last ? t : -1;
 }
 
-// Whether the fused tail reads FP32 row values: as the cycle's other levels do (the learned
-// smoother's cycle with FP32 copies on every tail level)
+// Whether the fused tail reads FP32 row values.  It does not: the tail levels keep their FP64
+// values even in the learned smoother's cycle, whose other levels read FP32 copies, because
+// the tail ran 3 us per call slower on FP32 values (19.8 against 16.7 us on the 4M-DOF
+// plate's 972-DOF tail, same run, same results otherwise).  Kept as a switch for the two
+// instantiations of k_tail.
 bool tail_v32(const Multigrid& mg) {
-  if (mg.tail_from < 0 || mg.fine_smoother != 0) return false;
-  for (int l = mg.tail_from; l + 1 < (int)mg.lv.size(); ++l)
-    if (!mg.lv[l]->val32.n) return false;
-  return true;
+  (void)mg;
+  return false;
 }
 
 // whether the top tail level keeps its rows in registers (one row per thread, <= kTailRegW

@@ -518,6 +518,23 @@ __global__ __launch_bounds__(kThreads) void k_copy(int64_t n2, const double2* __
     reinterpret_cast<v2d*>(b)[i] = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(a) + i);
 }
 
+constexpr int kReadGrid = 2048;  // workgroups of k_read
+
+// Reads n2 double2 values once (non-temporal loads) and writes one sum per wavefront: the
+// read-only streaming rate, the yardstick of kernels that read far more than they write
+__global__ __launch_bounds__(kThreads) void k_read(int64_t n2, const double2* __restrict__ a,
+                                                   double* __restrict__ part) {
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n2;
+       i += (int64_t)gridDim.x * kThreads) {
+    const v2d v = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(a) + i);
+    s += v[0] + v[1];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) part[blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)] = s;
+}
+
 Mat<F64> mat64(Context& c) {
   return Mat<F64>{c.sell_off.get(), c.sell_col.get(), c.sell_val.get(), c.sell_row0.get(),
                   c.sell_nrow.get(), c.nslices, c.nb};
@@ -1559,10 +1576,11 @@ int solve_fcg_learned(Context& c, const cstfem_solver_options& o, cstfem_solver_
 // 8 update (k_fcg_update), 9 direction (k_fcg_direction).  Solver state is not modified
 // (private scalars and vectors).
 double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
-  if (kind != 4 && !c.have_pattern) throw Error(CSTFEM_ERR_ORDER, "profile: no matrix on the device");
+  const bool stream = kind == 4 || kind == 10;  // copy or read of `bytes`, no matrix needed
+  if (!stream && !c.have_pattern) throw Error(CSTFEM_ERR_ORDER, "profile: no matrix on the device");
   if (kind == 7 && !c.mpnn.built)
     throw Error(CSTFEM_ERR_ORDER, "profile: build the learned preconditioner first");
-  if (kind != 4) ensure_vectors(c);
+  if (!stream) ensure_vectors(c);
   DBuf<PcgScalars> sc;
   sc.alloc(1);
   PcgScalars init{};
@@ -1573,11 +1591,11 @@ double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
   DBuf<double> a, b, w, part;
   DBuf<float> a32, v32;
   const size_t nv = 2 * (size_t)c.nb;
-  part.alloc(6 * kMaxGrid);
-  if (kind == 4) {
+  part.alloc(std::max<size_t>(6 * kMaxGrid, 4 * kReadGrid));  // k_read: 4 sums per workgroup
+  if (stream) {
     const int64_t n2 = bytes / 16;  // double2 elements in `bytes`
     a.alloc(2 * n2);
-    b.alloc(2 * n2);
+    b.alloc(kind == 4 ? 2 * n2 : 1);
     HIP_CHECK(hipMemsetAsync(a.get(), 0, sizeof(double) * 2 * n2, c.stream));
   } else {
     a.alloc(nv);
@@ -1658,6 +1676,12 @@ double profile_kernel(Context& c, int kind, int reps, int64_t bytes) {
                            (const double2*)b.get(), (double2*)a.get(), (const double*)part.get(),
                            gs, (const double*)(part.get() + 3 * kMaxGrid), gs,
                            (const PcgScalars*)sc.get(), 0);
+        break;
+      case 10:
+        // the fastest shape of a sweep over grids (1024-65536) and loads per lane (1, 2, 4),
+        // non-temporal and plain: 2048 workgroups, one 16-byte non-temporal load per lane
+        hipLaunchKernelGGL(k_read, dim3(kReadGrid), dim3(kThreads), 0, c.stream, bytes / 16,
+                           (const double2*)a.get(), part.get());
         break;
       default: {
         const int64_t n2 = bytes / 16;

@@ -408,7 +408,8 @@ class Device:
 
     def profile_kernel(self, kind, reps=20, nbytes=0):
         """Average seconds per launch of a hot kernel (0 fused SpMV, 1 SpMV, 2 update,
-        3 direction, 4 copy of nbytes)."""
+        3 direction, 4 copy of nbytes, 5 FP32 SpMV, 6-9 flexible-CG kernels, 10 read of
+        nbytes)."""
         t = ctypes.c_double()
         _check(lib().cstfem_profile_kernel(self._h, int(kind), int(reps), int(nbytes),
                                            ctypes.byref(t)))

@@ -125,10 +125,23 @@ def rounded32(A):
                          shape=A.shape)
 
 
+def tail_start(sizes, kdepth):
+    """First level of the device's fused single-workgroup tail (csrc/mg.hip, tail_start):
+    levels below the K-cycle levels, at most 4 above the coarsest, whose vectors (5 per level,
+    2 on the coarsest) fit 64 KB of LDS; None when there is no tail."""
+    last = len(sizes) - 1
+    t = last
+    while (t - 1 >= 1 and t - 1 > kdepth and last - (t - 1) <= 4
+           and 8 * (2 * sizes[last] + 5 * sum(sizes[t - 1:last])) <= 64 * 1024):
+        t -= 1
+    return t if t < last else None
+
+
 def learned_hierarchy(dev, theta, monkeypatch, coarse32, tail, n=40, kdepth=3):
     """The device's learned-smoother hierarchy on a plate and the oracle's, with the oracle's
     row sums reading the values the device's cycle reads: FP32 copies of K (level-0
-    residuals) and of the coarse matrices when CSTFEM_MG_COARSE32 is on."""
+    residuals) and of the coarse matrices when CSTFEM_MG_COARSE32 is on, except on the levels
+    of the fused tail, which keep their FP64 values."""
     prob = hetero(n, e_seed=3, seed=4)
     A, xy, fixed, bptr, bcol = system(dev, prob)
     dev.mpnn_set_params(theta)
@@ -141,8 +154,10 @@ def learned_hierarchy(dev, theta, monkeypatch, coarse32, tail, n=40, kdepth=3):
     lams = [dev.mg_level_info(l)[3] for l in range(len(levels))]
     assert 0.5 < lams[0] <= 1.5  # the set-up kept the learned smoother
     if coarse32 == "1":
-        for L in levels[:-1]:
-            L.As = rounded32(L.A)
+        t = tail_start([L.n for L in levels], kdepth) if tail == "1" else None
+        for l, L in enumerate(levels[:-1]):
+            if t is None or l < t:
+                L.As = rounded32(L.A)
     return prob, A, levels, lams, device_m(dev, prob, A)
 
 
@@ -151,9 +166,10 @@ def learned_hierarchy(dev, theta, monkeypatch, coarse32, tail, n=40, kdepth=3):
 def test_learned_cycle_matches_oracle(dev, theta40, monkeypatch, coarse32, tail, kdepth):
     """One application of the headline cycle (learned fine smoother: k0_msmooth, k0_resid or
     k0_resid32, restriction, K-cycles or V-cycles, prolongation, the fused tail) against the
-    oracle's cycle with the same M and the same stored values (the tail reads the FP32 copies
-    as the other levels do): 1e-12 relative (summation orders differ; measured ~1e-14, while
-    M's values taken in FP64 instead move the cycle by ~3e-9)."""
+    oracle's cycle with the same M and the same stored values (FP32 copies where the device's
+    cycle reads them, FP64 values on the fused tail's levels): 1e-12 relative (summation
+    orders differ; measured ~1e-14, while M's values taken in FP64 instead move the cycle by
+    ~3e-9)."""
     prob, A, levels, lams, M = learned_hierarchy(dev, theta40, monkeypatch, coarse32, tail,
                                                  kdepth=kdepth)
     rng = np.random.default_rng(17)
