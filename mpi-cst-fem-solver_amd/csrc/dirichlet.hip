@@ -4,9 +4,10 @@
 // are visited: the rows of constrained DOFs and the rows holding a constrained column.  On a
 // structurally symmetric pattern those are the block rows listed in the constrained nodes'
 // own rows, so the work is proportional to the constraints, not to the matrix:
-//   k_dir_mark   one thread per constrained DOF sets its bit in a DOF bitmap and appends the
-//                block rows of its node's row (itself included) to a compact list, each once
-//                (a block-row bitmap decides who appends);
+//   k_dir_mark   one thread per constrained DOF sets its bit in a DOF bitmap (and its value in
+//                a dense array of prescribed values) and appends the block rows of its node's
+//                row (itself included) to a compact list, each once (a block-row bitmap
+//                decides who appends; one counter increment per wavefront and chunk);
 //   k_dir_rows   one thread per scalar row of the list visits its blocks in ascending column
 //                order, so the subtractions b_i -= K_ik u_k happen in the reference's column
 //                order and every value of K and b equals the reference's; the pattern is kept
@@ -66,37 +67,94 @@ __device__ inline bool test_bit(const unsigned* __restrict__ bits, int i) {
   return (bits[i >> 5] >> (i & 31)) & 1u;
 }
 
-// appends block row r to the list unless it is there already
-__device__ inline void list_row(int r, unsigned* __restrict__ row_bits, int* __restrict__ rows,
-                                int* __restrict__ ctr, int cap) {
-  const unsigned bit = 1u << (r & 31);
-  const unsigned old = atomicOr(&row_bits[r >> 5], bit);
-  if (!(old & bit)) {
-    const int idx = atomicAdd(&ctr[0], 1);
-    if (idx < cap)
-      rows[idx] = r;
-    else
-      ctr[1] = 1;
+constexpr int kMarkChunk = 16;  // rows of a node's row a lane marks at once
+
+// Sets the DOF bit of every constrained DOF (and its prescribed value in uval, where given),
+// and appends the block rows of its node's row (its own row first, even without a stored
+// diagonal) to the compact list, each row once: a flag word per block row (row_flag), or
+// where there is no room for one the block-row bitmap, decides which thread appends.  A lane
+// takes its rows kMarkChunk at a time: their column loads, then their flag updates, are
+// issued together, and the whole wavefront's appends of a chunk take a single
+// counter increment (ballots give each appending lane its slot), so a chunk costs three
+// memory round trips however many rows it appends.
+__global__ __launch_bounds__(256) void k_dir_mark(int m, const int* __restrict__ dofs,
+                                                  const double* __restrict__ vals,
+                                                  const int* __restrict__ brow_ptr,
+                                                  const int* __restrict__ bcol, int all_rows,
+                                                  unsigned* __restrict__ dof_bits,
+                                                  unsigned* __restrict__ row_bits,
+                                                  int* __restrict__ row_flag,
+                                                  int* __restrict__ rows, int* __restrict__ ctr,
+                                                  int cap, double* __restrict__ uval) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = k < m;
+  int v = 0, q0 = 0, len = 0;
+  if (live) {
+    const int dof = dofs[k];
+    atomicOr(&dof_bits[dof >> 5], 1u << (dof & 31));
+    if (uval) uval[dof] = vals[k];
+    v = dof >> 1;
+    if (!all_rows) {
+      q0 = brow_ptr[v];
+      len = 1 + brow_ptr[v + 1] - q0;
+    }
+  }
+  if (all_rows) return;
+  int steps = len;  // the wavefront's longest walk
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) steps = max(steps, __shfl_xor(steps, off, 64));
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int u0 = 0; u0 < steps; u0 += kMarkChunk) {
+    int r[kMarkChunk];
+#pragma unroll
+    for (int j = 0; j < kMarkChunk; ++j) {
+      const int u = u0 + j;
+      r[j] = u < len ? (u == 0 ? v : bcol[q0 + u - 1]) : -1;
+    }
+    bool fresh[kMarkChunk];
+#pragma unroll
+    for (int j = 0; j < kMarkChunk; ++j) {
+      fresh[j] = false;
+      if (r[j] >= 0) {
+        if (row_flag) {  // one word per block row: only threads naming the same row contend
+          fresh[j] = atomicExch(&row_flag[r[j]], 1) == 0;
+        } else {
+          const unsigned bit = 1u << (r[j] & 31);
+          fresh[j] = !(atomicOr(&row_bits[r[j] >> 5], bit) & bit);
+        }
+      }
+    }
+    unsigned long long mask[kMarkChunk];
+    int total = 0;
+#pragma unroll
+    for (int j = 0; j < kMarkChunk; ++j) {
+      mask[j] = __ballot(fresh[j]);
+      total += __popcll(mask[j]);
+    }
+    if (!total) continue;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&ctr[0], total);
+    base = __shfl(base, 0, 64);
+#pragma unroll
+    for (int j = 0; j < kMarkChunk; ++j) {
+      if (fresh[j]) {
+        const int idx = base + __popcll(mask[j] & below);
+        if (idx < cap)
+          rows[idx] = r[j];
+        else
+          ctr[1] = 1;
+      }
+      base += __popcll(mask[j]);
+    }
   }
 }
 
-__global__ void k_dir_mark(int m, const int* __restrict__ dofs, const int* __restrict__ brow_ptr,
-                           const int* __restrict__ bcol, int all_rows,
-                           unsigned* __restrict__ dof_bits, unsigned* __restrict__ row_bits,
-                           int* __restrict__ rows, int* __restrict__ ctr, int cap) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  const int dof = dofs[k];
-  atomicOr(&dof_bits[dof >> 5], 1u << (dof & 31));
-  if (all_rows) return;
-  const int v = dof >> 1;
-  list_row(v, row_bits, rows, ctr, cap);  // its own row, even without a stored diagonal
-  for (int q = brow_ptr[v]; q < brow_ptr[v + 1]; ++q) list_row(bcol[q], row_bits, rows, ctr, cap);
-}
-
-// prescribed value of constrained DOF j: binary search in the sorted list
-__device__ inline double ubar(int j, const int* __restrict__ dofs, const double* __restrict__ vals,
-                              int m) {
+// prescribed value of constrained DOF j: from the dense array uval where there is one, else by
+// binary search in the sorted list
+__device__ inline double ubar(int j, const double* __restrict__ uval, const int* __restrict__ dofs,
+                              const double* __restrict__ vals, int m) {
+  if (uval) return uval[j];
   int lo = 0, hi = m;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
@@ -115,7 +173,8 @@ __global__ void k_dir_rows(int nb, int n, int all_rows, const int* __restrict__ 
                            const int* __restrict__ bcol, const int* __restrict__ bdiag,
                            const int64_t* __restrict__ spos, double* __restrict__ val,
                            const unsigned* __restrict__ dof_bits, const int* __restrict__ dofs,
-                           const double* __restrict__ vals, int m, double* __restrict__ b,
+                           const double* __restrict__ vals, int m,
+                           const double* __restrict__ uval, double* __restrict__ b,
                            unsigned char* __restrict__ fixed_union, int* __restrict__ err) {
   if (!all_rows && ctr[1]) return;  // the list overflowed: change nothing
   const int64_t cnt = all_rows ? nb : min(ctr[0], cap);
@@ -141,7 +200,7 @@ __global__ void k_dir_rows(int nb, int n, int all_rows, const int* __restrict__ 
         if (2 * mc != i) v[0] = 0.0;
         if (2 * mc + 1 != i) v[1] = 0.0;
       }
-      b[i] = diag * ubar(i, dofs, vals, m);
+      b[i] = diag * ubar(i, uval, dofs, vals, m);
     } else {
       // free row: b_i -= K_ik u_k and K_ik = 0 for every constrained column, in column order
       double bi = b[i];
@@ -160,11 +219,11 @@ __global__ void k_dir_rows(int nb, int n, int all_rows, const int* __restrict__ 
           if (!f) continue;
           double* v = val + sell_vidx(spos[q + u], 2 * c);
           if (f & 1u) {
-            bi -= v[0] * ubar(j0, dofs, vals, m);
+            bi -= v[0] * ubar(j0, uval, dofs, vals, m);
             v[0] = 0.0;
           }
           if (f & 2u) {
-            bi -= v[1] * ubar(j0 + 1, dofs, vals, m);
+            bi -= v[1] * ubar(j0 + 1, uval, dofs, vals, m);
             v[1] = 0.0;
           }
         }
@@ -176,12 +235,18 @@ __global__ void k_dir_rows(int nb, int n, int all_rows, const int* __restrict__ 
 
 __global__ void k_dir_clear(int m, const int* __restrict__ dofs, const int* __restrict__ ctr,
                             int cap, const int* __restrict__ rows, int64_t row_words,
-                            unsigned* __restrict__ dof_bits, unsigned* __restrict__ row_bits) {
+                            unsigned* __restrict__ dof_bits, unsigned* __restrict__ row_bits,
+                            int* __restrict__ row_flag) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   for (int64_t k = t0; k < m; k += stride) dof_bits[dofs[k] >> 5] = 0u;
   const int listed = min(ctr[0], cap);
-  for (int64_t k = t0; k < listed; k += stride) row_bits[rows[k] >> 5] = 0u;
+  for (int64_t k = t0; k < listed; k += stride) {
+    if (row_flag)
+      row_flag[rows[k]] = 0;
+    else
+      row_bits[rows[k] >> 5] = 0u;
+  }
   if (ctr[1])  // rows beyond the list's capacity set bits it does not name
     for (int64_t k = t0; k < row_words; k += stride) row_bits[k] = 0u;
 }
@@ -196,6 +261,8 @@ struct DirWork {
   unsigned* row_bits;
   int* rows;
   int cap;
+  double* uval;   // prescribed values by DOF (n entries; only constrained ones are read), or null
+  int* row_flag;  // a flag word per block row (zero between calls) in place of row_bits, or null
 };
 
 // DirichletDofs::add (bc.cpp:107-116) over the caller's pairs: the first pair in input order
@@ -253,8 +320,8 @@ void run_elimination(Context& c, int m, const DirWork& w, int out_ctr[3]) {
   HIP_CHECK(hipGetLastError());
   if (m > 0) {
     hipLaunchKernelGGL(k_dir_mark, dim3(grid_for(m, 256, 0)), dim3(256), 0, c.stream, m, w.dofs,
-                       c.brow_ptr.get(), c.bcol.get(), all_rows, w.dof_bits, w.row_bits, w.rows,
-                       w.ctr, w.cap);
+                       (const double*)w.vals, c.brow_ptr.get(), c.bcol.get(), all_rows, w.dof_bits,
+                       w.row_bits, w.row_flag, w.rows, w.ctr, w.cap, w.uval);
     HIP_CHECK(hipGetLastError());
     // rows to visit: at most the distinct constrained nodes times (1 + the widest row)
     const int64_t bound = all_rows ? (int64_t)c.nb
@@ -263,12 +330,12 @@ void run_elimination(Context& c, int m, const DirWork& w, int out_ctr[3]) {
     hipLaunchKernelGGL(k_dir_rows, dim3(grid_for(2 * bound, 256, 4096)), dim3(256), 0, c.stream,
                        c.nb, n, all_rows, w.rows, w.ctr, w.cap, c.brow_ptr.get(), c.bcol.get(),
                        c.bdiag.get(), c.sell_pos.get(), c.sell_val.get(), w.dof_bits, w.dofs,
-                       w.vals, m, c.b.get(), c.fixed.get(), w.ctr + 2);
+                       w.vals, m, (const double*)w.uval, c.b.get(), c.fixed.get(), w.ctr + 2);
     HIP_CHECK(hipGetLastError());
     const int64_t row_words = (c.nb + 31) / 32;
     hipLaunchKernelGGL(k_dir_clear, dim3(grid_for(std::max<int64_t>(m, 1), 256, 1024)), dim3(256), 0,
                        c.stream, m, w.dofs, w.ctr, w.cap, w.rows, row_words, w.dof_bits,
-                       w.row_bits);
+                       w.row_bits, w.row_flag);
     HIP_CHECK(hipGetLastError());
   }
   HIP_CHECK(hipMemcpyAsync(out_ctr, w.ctr, sizeof(int) * 3, hipMemcpyDeviceToHost, c.stream));
@@ -315,8 +382,14 @@ void apply_dirichlet_device(Context& c, int m, const int* dofs, const double* va
   }
   w.dof_bits = c.dir_bits.get();
   w.row_bits = c.dir_bits.get() + dof_words;
+  if (c.dir_flag.n < (size_t)c.nb) {
+    grow(c.dir_flag, (size_t)c.nb);
+    HIP_CHECK(hipMemsetAsync(c.dir_flag.get(), 0, sizeof(int) * c.dir_flag.n, c.stream));
+  }
+  w.row_flag = c.dir_flag.get();
   w.rows = grow(c.dir_rows, (size_t)c.nb);
   w.cap = c.nb;  // every block row fits: the list never overflows
+  w.uval = grow(c.dir_uval, (size_t)c.n);  // read only where this call set the DOF's bit
   if (mu > 0) {
     HIP_CHECK(hipMemcpyAsync(w.dofs, d.data(), sizeof(int) * mu, hipMemcpyHostToDevice, c.stream));
     HIP_CHECK(hipMemcpyAsync(w.vals, v.data(), sizeof(double) * mu, hipMemcpyHostToDevice, c.stream));
@@ -379,6 +452,8 @@ void apply_dirichlet_noalloc(Context& c, int m, const int* dofs, const double* v
   w.row_bits = reinterpret_cast<unsigned*>(base + na::kRowBits);
   w.rows = reinterpret_cast<int*>(base + na::kRows);
   w.cap = na::kMaxRows;
+  w.uval = nullptr;      // no room for n values: binary search in the sorted list
+  w.row_flag = nullptr;  // nor for a word per block row: the block-row bitmap
   if (mu > 0) {
     HIP_CHECK(hipMemcpyAsync(w.dofs, d.data(), sizeof(int) * mu, hipMemcpyHostToDevice, c.stream));
     HIP_CHECK(hipMemcpyAsync(w.vals, v.data(), sizeof(double) * mu, hipMemcpyHostToDevice, c.stream));

@@ -335,8 +335,8 @@ struct Context {
   int max_row_blocks = 0;         // blocks of the longest block row
   // Dirichlet elimination workspace, grown on demand and kept: sorted constrained DOFs and
   // values, the DOF and block-row bitmaps (zero between calls) and the affected-row list
-  DBuf<int> dir_dofs, dir_rows;
-  DBuf<double> dir_vals;
+  DBuf<int> dir_dofs, dir_rows, dir_flag;  // dir_flag: a word per block row (zero between calls)
+  DBuf<double> dir_vals, dir_uval;  // sorted values; prescribed value by DOF (n entries)
   DBuf<unsigned> dir_bits;
   // device address of the allocation-free elimination's scratch array on this GPU
   unsigned char* na_scratch = nullptr;
