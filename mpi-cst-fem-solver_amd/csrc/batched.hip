@@ -21,6 +21,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <sstream>
 
@@ -520,30 +521,36 @@ double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batc
   }
   int grid = cfg.groups_per_cu * (c.cu_count > 0 ? c.cu_count : 256);
   if (grid > c.batch_count) grid = c.batch_count;
-  // one captured graph: queue reset (a kernel, not a memset node) + the persistent solve
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
-  HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, c.stream, c.batch_queue.get());
-  if (fp32)
-    hipLaunchKernelGGL(k_batch_cg<true>, dim3(grid), dim3(64 * waves), 0, c.stream, args);
-  else
-    hipLaunchKernelGGL(k_batch_cg<false>, dim3(grid), dim3(64 * waves), 0, c.stream, args);
-  HIP_CHECK(hipStreamEndCapture(c.stream, &g));
-  HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  // one captured graph: queue reset (a kernel, not a memset node) + the persistent solve.  It
+  // is kept in the context and replayed while the launch (precision, shape, every argument:
+  // buffers, counts, tolerances) is the same; anything else captures it anew.
+  std::vector<unsigned char> key(sizeof(BatchArgs) + 3 * sizeof(int));
+  const int head[3] = {fp32 ? 1 : 0, waves, grid};
+  std::memcpy(key.data(), head, sizeof head);
+  std::memcpy(key.data() + sizeof head, &args, sizeof args);
+  if (!c.batch_exec || key != c.batch_graph_key) {
+    c.drop_batch_graph();
+    HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, c.stream, c.batch_queue.get());
+    if (fp32)
+      hipLaunchKernelGGL(k_batch_cg<true>, dim3(grid), dim3(64 * waves), 0, c.stream, args);
+    else
+      hipLaunchKernelGGL(k_batch_cg<false>, dim3(grid), dim3(64 * waves), 0, c.stream, args);
+    HIP_CHECK(hipStreamEndCapture(c.stream, &c.batch_graph));
+    HIP_CHECK(hipGraphInstantiate(&c.batch_exec, c.batch_graph, nullptr, nullptr, 0));
+    c.batch_graph_key = key;
+  }
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
   HIP_CHECK(hipEventRecord(e0, c.stream));
-  HIP_CHECK(hipGraphLaunch(ge, c.stream));
+  HIP_CHECK(hipGraphLaunch(c.batch_exec, c.stream));
   HIP_CHECK(hipEventRecord(e1, c.stream));
   HIP_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
   HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipGraphExecDestroy(ge);
-  (void)hipGraphDestroy(g);
   if (results)
     HIP_CHECK(hipMemcpy(results, c.batch_results.get(), sizeof(cstfem_batch_result) * c.batch_count,
                         hipMemcpyDeviceToHost));

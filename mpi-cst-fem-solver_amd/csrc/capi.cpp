@@ -26,6 +26,8 @@ void hip_check(hipError_t e, const char* what, const char* file, int line) {
 
 Context::~Context() {
   mg.join_prepared();  // its download runs on the auxiliary stream
+  if (stream) (void)hipStreamSynchronize(stream);
+  drop_batch_graph();
   if (h_scal) (void)hipHostFree(h_scal);
   if (aux) (void)hipStreamDestroy(aux);
   if (stream) (void)hipStreamDestroy(stream);

@@ -325,6 +325,18 @@ struct Context {
   int batch_count = 0;
   DBuf<int> batch_seg_slice, batch_order, batch_queue;
   DBuf<cstfem_batch_result> batch_results;
+  // the batched solve's captured graph, replayed while its launch is unchanged (key: its
+  // precision, shape and kernel arguments)
+  hipGraph_t batch_graph = nullptr;
+  hipGraphExec_t batch_exec = nullptr;
+  std::vector<unsigned char> batch_graph_key;
+  void drop_batch_graph() {
+    if (batch_exec) (void)hipGraphExecDestroy(batch_exec);
+    if (batch_graph) (void)hipGraphDestroy(batch_graph);
+    batch_exec = nullptr;
+    batch_graph = nullptr;
+    batch_graph_key.clear();
+  }
 
   // constrained DOFs (union over apply_dirichlet calls since the last assembly): one byte
   // per DOF, allocated with the vectors; fixed_valid is false until the first elimination
