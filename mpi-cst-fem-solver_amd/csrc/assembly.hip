@@ -385,6 +385,16 @@ __device__ inline int local_index(const int* t, int node) {
   return t[0] == node ? 0 : (t[1] == node ? 1 : 2);
 }
 
+constexpr int kSlotElems = 16;   // elements of a node held in registers by k_slot_map
+constexpr int kSlotBlocks = 16;  // blocks of a row held in registers by k_slot_map
+
+// For block (n, m) the elements holding both nodes, ascending: the count of each block of row
+// n (kFill false), then the contribution codes e * 9 + a * 3 + b into their slots (kFill true).
+// Node n's element list and those elements' three nodes are loaded once, together, into
+// registers, and each block m takes n's elements that contain m, in their ascending order
+// (the intersection of n's and m's element lists, without walking m's list).  A node with
+// more than kSlotElems elements or a row of more than kSlotBlocks blocks takes the same
+// result by merging the two ascending element lists.
 template <bool kFill>
 __global__ void k_slot_map(int nb, const int* __restrict__ tri, const int* __restrict__ ptr,
                            const int* __restrict__ list, const int* __restrict__ brow_ptr,
@@ -393,7 +403,47 @@ __global__ void k_slot_map(int nb, const int* __restrict__ tri, const int* __res
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nb) return;
   const int pn0 = ptr[n], pn1 = ptr[n + 1];
-  for (int q = brow_ptr[n]; q < brow_ptr[n + 1]; ++q) {
+  const int q0 = brow_ptr[n], q1 = brow_ptr[n + 1];
+  const int ne = pn1 - pn0, d = q1 - q0;
+  if (ne <= kSlotElems && d <= kSlotBlocks) {
+    int el[kSlotElems], bc[kSlotBlocks], o[kSlotBlocks];
+#pragma unroll
+    for (int i = 0; i < kSlotElems; ++i) el[i] = i < ne ? list[pn0 + i] : 0;
+#pragma unroll
+    for (int k = 0; k < kSlotBlocks; ++k) {
+      bc[k] = k < d ? bcol[q0 + k] : -1;
+      o[k] = kFill && k < d ? cnt_or_ptr[q0 + k] : 0;
+    }
+    int t0[kSlotElems], t1[kSlotElems], t2[kSlotElems];
+#pragma unroll
+    for (int i = 0; i < kSlotElems; ++i) {
+      const int* t = tri + 3 * (int64_t)el[i];
+      t0[i] = i < ne ? t[0] : -1;
+      t1[i] = i < ne ? t[1] : -1;
+      t2[i] = i < ne ? t[2] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kSlotBlocks; ++k) {
+      if (k < d) {  // guards, not breaks: the loops unroll and every array stays in registers
+        const int m = bc[k];
+        int count = 0;
+#pragma unroll
+        for (int i = 0; i < kSlotElems; ++i) {
+          const int b = t0[i] == m ? 0 : (t1[i] == m ? 1 : (t2[i] == m ? 2 : -1));
+          if (i < ne && b >= 0) {
+            if (kFill) {
+              const int a = t0[i] == n ? 0 : (t1[i] == n ? 1 : 2);
+              src[o[k] + count] = el[i] * 9 + a * 3 + b;
+            }
+            ++count;
+          }
+        }
+        if (!kFill) cnt_or_ptr[q0 + k] = count;
+      }
+    }
+    return;
+  }
+  for (int q = q0; q < q1; ++q) {
     const int m = bcol[q];
     int i = pn0, j = ptr[m];
     const int j1 = ptr[m + 1];
@@ -512,6 +562,24 @@ __device__ inline void ke_block(const double xy[6], const double D[9], double t,
 typedef double v2d __attribute__((ext_vector_type(2)));
 
 // one wavefront per SELL slice, lane = block row; writes are coalesced
+constexpr int kAsmGroup = 2;    // contributions whose loads k_assemble issues together
+constexpr int kAsmBlocks = 16;  // blocks of a row whose slot ranges k_assemble keeps in registers
+
+// The element data of one contribution e * 9 + a * 3 + b: its block's local indices, the
+// element's coordinates and its modulus and Poisson ratio (D is formed at use)
+struct Contribution {
+  int la, lb;
+  double exy[6];
+  double E, nu;
+};
+
+// Values of the block row at `lane` of slice s: every 2x2 block sums its contributions in
+// slot order (ascending element) from 0.0, as the reference's element loop adds them.  The
+// row's contributions are consecutive in slot_src (block after block), so they are taken in
+// groups of kAsmGroup across block boundaries: a group's codes, then its triangles, then
+// their coordinates and moduli are loaded together (three memory round trips per group
+// instead of three per contribution); the block ends come from registers.  Rows of more than
+// kAsmBlocks blocks go block by block.
 __global__ void k_assemble(const int* __restrict__ row0, const int* __restrict__ nrow,
                            int nslices, const int64_t* __restrict__ off,
                            const int* __restrict__ brow_ptr, const int* __restrict__ slot_ptr,
@@ -526,33 +594,87 @@ __global__ void k_assemble(const int* __restrict__ row0, const int* __restrict__
   const int64_t o = off[s];
   const int b0 = brow_ptr[r], d = brow_ptr[r + 1] - b0;
   const double* Du = Dm.d;
-  for (int k = 0; k < d; ++k) {
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const int q = b0 + k;
-    for (int c = slot_ptr[q]; c < slot_ptr[q + 1]; ++c) {
-      const int code = slot_src[c];
-      const int e = code / 9, ab = code - 9 * e;
-      const int la = ab / 3, lb = ab - 3 * la;
-      const int n0 = tri[3 * e], n1 = tri[3 * e + 1], n2 = tri[3 * e + 2];
-      const double exy[6] = {xy[2 * n0], xy[2 * n0 + 1], xy[2 * n1],
-                             xy[2 * n1 + 1], xy[2 * n2], xy[2 * n2 + 1]};
-      double De[9];
-      if (E_elem) {
-        constitutive(E_elem[e], nu_elem ? nu_elem[e] : nu, strain, De);
-      } else {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) De[i] = Du[i];
-      }
-      double kb[4];
-      ke_block(exy, De, t, la, lb, kb);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc[i] += kb[i];
-    }
+  auto store = [&](int k, const double acc[4]) {
     // planar block (sell_vidx): the wavefront writes the 1 KB of row-0 pairs, then the
     // 1 KB of row-1 pairs, each as one coalesced 16-byte-per-lane store
     double* vk = val + 4 * (o + (int64_t)k * kSlice) + 2 * lane;
     *reinterpret_cast<v2d*>(vk) = v2d{acc[0], acc[1]};
     *reinterpret_cast<v2d*>(vk + 2 * kSlice) = v2d{acc[2], acc[3]};
+  };
+  auto load = [&](int code, Contribution& q) {
+    const int e = code / 9, ab = code - 9 * e;
+    q.la = ab / 3;
+    q.lb = ab - 3 * q.la;
+    const int n0 = tri[3 * e], n1 = tri[3 * e + 1], n2 = tri[3 * e + 2];
+    q.exy[0] = xy[2 * n0];
+    q.exy[1] = xy[2 * n0 + 1];
+    q.exy[2] = xy[2 * n1];
+    q.exy[3] = xy[2 * n1 + 1];
+    q.exy[4] = xy[2 * n2];
+    q.exy[5] = xy[2 * n2 + 1];
+    q.E = E_elem ? E_elem[e] : 0.0;
+    q.nu = nu_elem ? nu_elem[e] : nu;
+  };
+  // Ke block (la, lb) of a loaded contribution: D from its E and nu, or the uniform D
+  auto block = [&](const Contribution& q, double kb[4]) {
+    double De[9];
+    if (E_elem) {
+      constitutive(q.E, q.nu, strain, De);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) De[i] = Du[i];
+    }
+    ke_block(q.exy, De, t, q.la, q.lb, kb);
+  };
+  if (d <= kAsmBlocks) {
+    int se[kAsmBlocks + 1];  // slot range of block k: [se[k], se[k + 1])
+#pragma unroll
+    for (int k = 0; k <= kAsmBlocks; ++k) se[k] = k <= d ? slot_ptr[b0 + k] : 0;
+    const int c0 = se[0];
+    int c1 = 0;
+#pragma unroll
+    for (int k = 0; k <= kAsmBlocks; ++k) c1 = k == d ? se[k] : c1;
+    int k = 0;          // current block
+    int end = se[1];    // its end
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = c0; c < c1; c += kAsmGroup) {
+      int code[kAsmGroup];
+#pragma unroll
+      for (int u = 0; u < kAsmGroup; ++u) code[u] = c + u < c1 ? slot_src[c + u] : slot_src[c];
+      Contribution q[kAsmGroup];
+#pragma unroll
+      for (int u = 0; u < kAsmGroup; ++u) load(code[u], q[u]);
+#pragma unroll
+      for (int u = 0; u < kAsmGroup; ++u) {
+        if (c + u >= c1) break;
+        while (c + u >= end) {  // the block is complete (a block holds at least one)
+          store(k, acc);
+          acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
+          ++k;
+#pragma unroll
+          for (int j = 1; j <= kAsmBlocks; ++j) end = j == k + 1 ? se[j] : end;
+        }
+        double kb[4];
+        block(q[u], kb);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] += kb[i];
+      }
+    }
+    if (d > 0) store(k, acc);
+    return;
+  }
+  for (int k = 0; k < d; ++k) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int q = b0 + k;
+    for (int c = slot_ptr[q]; c < slot_ptr[q + 1]; ++c) {
+      Contribution cb;
+      load(slot_src[c], cb);
+      double kb[4];
+      block(cb, kb);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] += kb[i];
+    }
+    store(k, acc);
   }
 }
 
