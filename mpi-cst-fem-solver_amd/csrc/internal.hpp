@@ -181,7 +181,6 @@ struct MgLevel {
     nslices = 0;
     lam = 0.0;
     nagg = 0;
-    val32.release();
   }
 };
 
@@ -206,11 +205,15 @@ struct Multigrid {
   bool fuse = true;
   int fuse_n = 200000;
   int64_t fold_max = 200000;
+  // the set-up's lane-per-aggregate kernels on levels of many aggregates (CSTFEM_MG_SETUP_LANES;
+  // 0: the wavefront forms everywhere, same results)
+  bool setup_lanes = true;
   DBuf<double> cinv;
   // device workspaces of the set-up, grown on demand and kept, so that repeated set-ups
-  // allocate nothing: member-list scratch, coarse degrees and error flags (iwork), the coarse
-  // row of every coarse block (crow), power-iteration vectors (dwork)
-  DBuf<int> iwork, crow;
+  // allocate nothing: member-list scratch, coarse degrees and error flags (iwork), the
+  // aggregates the lane forms of the set-up kernels hand to the wavefront forms (big: a count,
+  // then the list), power-iteration vectors (dwork)
+  DBuf<int> iwork, big;
   DBuf<double> dwork;
   DBuf<double> dwork2;             // the learned smoother's check, on the auxiliary stream
   DBuf<double> dwork3;             // the levels' smoothing bounds, on the auxiliary stream
@@ -319,6 +322,7 @@ struct Context {
   // vectors
   DBuf<double> b, x, r, p, Ap, inv_diag, work;
   DBuf<float> r32, p32, Ap32, x32, invd32;
+  DBuf<double> z64, ax64;  // the FP64 flexible CG's z and A x (kept across solves)
   bool setup_done = false;
 
   // batch of independent plates (batched.hip)

@@ -1137,82 +1137,232 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(Tail<T> T_, const PcgScal
 
 // ---- setup kernels -------------------------------------------------------------------
 
-// Galerkin product from level 0 (2x2 blocks): thread t = coarse block (crow[t], ccol[t]) at
-// position t - cptr[crow[t]] of its row; sums P_i^T A_q P_j over the aggregate's member nodes i
-// (member order) and their blocks q = (i, j) (column order) with agg(j) = ccol[t].
-__global__ void k_rap0(int ncblk, const int* __restrict__ crow, const int* __restrict__ cptr,
-                       const int* __restrict__ ccol, const int* __restrict__ mptr,
-                       const int* __restrict__ mem, const int* __restrict__ agg,
-                       const int* __restrict__ brow_ptr, const int* __restrict__ bcol,
-                       const int64_t* __restrict__ spos, const double* __restrict__ fval,
-                       const double* __restrict__ P, const int64_t* __restrict__ coff,
-                       double* __restrict__ cval) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncblk) return;
-  const int a = crow[t], b = ccol[t], pos = t - cptr[a];
-  double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  for (int m = mptr[a]; m < mptr[a + 1]; ++m) {
-    const int i = mem[m];
-    const double* Pi = P + (size_t)(2 * i) * 3;
-    for (int q = brow_ptr[i]; q < brow_ptr[i + 1]; ++q) {
-      const int j = bcol[q];
-      if (agg[j] != b) continue;
-      const int64_t sq = spos[q];
-      const double Aq[4] = {fval[sell_vidx(sq, 0)], fval[sell_vidx(sq, 1)], fval[sell_vidx(sq, 2)],
-                            fval[sell_vidx(sq, 3)]};
-      const double* Pj = P + (size_t)(2 * j) * 3;
-      // T = A_q P_j (2x3), acc += P_i^T T
-      double T[2][3];
-      for (int k = 0; k < 2; ++k)
-        for (int c = 0; c < 3; ++c) T[k][c] = Aq[2 * k] * Pj[c] + Aq[2 * k + 1] * Pj[3 + c];
-      for (int c = 0; c < 3; ++c)
-        for (int cc = 0; cc < 3; ++cc) acc[c][cc] += Pi[c] * T[0][cc] + Pi[3 + c] * T[1][cc];
-    }
-  }
-  for (int c = 0; c < 3; ++c) {
-    const int row = 3 * a + c, s = row >> 6, lane = row & 63;
-    for (int cc = 0; cc < 3; ++cc) cval[coff[s] + (int64_t)(3 * pos + cc) * 64 + lane] = acc[c][cc];
-  }
+// ---- wavefront helpers of the set-up kernels -----------------------------------------
+
+constexpr int kMaxCoarseDeg = 128;  // block columns of one coarse block row
+constexpr int kSetupMembers = 128;  // members of an aggregate one wavefront takes (two per lane)
+constexpr int kPairSlots = 8;       // (member, block) pairs per lane held in registers
+
+// v of lane `src` (wave-uniform) in every lane
+__device__ inline double lane_value(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Galerkin product from a level >= 1 (3x3 blocks in scalar SELL; block q of fine row i sits at
-// position q - fptr[i] of that row)
-__global__ void k_rapS(int ncblk, const int* __restrict__ crow, const int* __restrict__ cptr,
-                       const int* __restrict__ ccol, const int* __restrict__ mptr,
-                       const int* __restrict__ mem, const int* __restrict__ agg,
-                       const int* __restrict__ fptr, const int* __restrict__ fcol,
-                       const int64_t* __restrict__ foff, const double* __restrict__ fval,
-                       const double* __restrict__ P, const int64_t* __restrict__ coff,
-                       double* __restrict__ cval) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncblk) return;
-  const int a = crow[t], b = ccol[t], pos = t - cptr[a];
-  double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  for (int m = mptr[a]; m < mptr[a + 1]; ++m) {
-    const int i = mem[m];
-    const double* Pi = P + (size_t)(3 * i) * 3;
-    for (int q = fptr[i]; q < fptr[i + 1]; ++q) {
-      const int j = fcol[q];
-      if (agg[j] != b) continue;
-      const int fp = q - fptr[i];
-      double Aq[3][3];
-      for (int k = 0; k < 3; ++k) {
-        const int row = 3 * i + k, s = row >> 6, lane = row & 63;
-        for (int kk = 0; kk < 3; ++kk) Aq[k][kk] = fval[foff[s] + (int64_t)(3 * fp + kk) * 64 + lane];
-      }
-      const double* Pj = P + (size_t)(3 * j) * 3;
-      double T[3][3];
-      for (int k = 0; k < 3; ++k)
-        for (int c = 0; c < 3; ++c)
-          T[k][c] = Aq[k][0] * Pj[c] + Aq[k][1] * Pj[3 + c] + Aq[k][2] * Pj[6 + c];
-      for (int c = 0; c < 3; ++c)
-        for (int cc = 0; cc < 3; ++cc)
-          acc[c][cc] += Pi[c] * T[0][cc] + Pi[3 + c] * T[1][cc] + Pi[6 + c] * T[2][cc];
-    }
+__device__ inline int wave_min(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ inline int wave_inclusive_sum(int v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
   }
-  for (int c = 0; c < 3; ++c) {
-    const int row = 3 * a + c, s = row >> 6, lane = row & 63;
-    for (int cc = 0; cc < 3; ++cc) cval[coff[s] + (int64_t)(3 * pos + cc) * 64 + lane] = acc[c][cc];
+  return v;
+}
+
+// The (member, block) pairs of an aggregate, in member order and then block order, as one
+// wavefront numbers them: member t (node node[t], blocks first[t] ..) owns the pairs
+// incl[t] - (its block count) .. incl[t] - 1, so pair p of member t is block
+// p + first[t] - (incl[t] - count).  LDS arrays of kSetupMembers entries (nm <= kSetupMembers);
+// returns the pair count.  The caller synchronises before reading the arrays.
+struct MemberPairs {
+  int incl[kSetupMembers], base[kSetupMembers], node[kSetupMembers], first[kSetupMembers];
+};
+
+__device__ inline int member_pairs(int m0, int nm, const int* __restrict__ mem,
+                                   const int* __restrict__ bptr, MemberPairs& mp, int lane) {
+  int total = 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int t = lane + 64 * h;
+    int d = 0, q0 = 0, i = 0;
+    if (t < nm) {
+      i = mem[m0 + t];
+      q0 = bptr[i];
+      d = bptr[i + 1] - q0;
+    }
+    const int in = wave_inclusive_sum(d, lane) + total;
+    if (t < nm) {
+      mp.incl[t] = in;
+      mp.base[t] = q0 - (in - d);
+      mp.node[t] = i;
+      mp.first[t] = q0;
+    }
+    total = __shfl(in, 63, 64);
+  }
+  return total;
+}
+
+// member of pair p: the first t with incl[t] > p
+__device__ inline int pair_member(const MemberPairs& mp, int nm, int p) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (mp.incl[mid] > p) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// Galerkin contributions P_i^T A_q P_j of one fine block q = (i, j), as 3x3 row-major C.
+// Level 0: 2x2 blocks in the planar SELL-64 layout (spos = each block's position).
+struct RapFine0 {
+  const int64_t* __restrict__ spos;
+  const double* __restrict__ fval;
+  __device__ void operator()(int i, int q, int j, int, const double* __restrict__ P,
+                             double (&C)[9]) const {
+    const int64_t sq = spos[q];
+    const double Aq[4] = {fval[sell_vidx(sq, 0)], fval[sell_vidx(sq, 1)], fval[sell_vidx(sq, 2)],
+                          fval[sell_vidx(sq, 3)]};
+    const double* Pi = P + (size_t)(2 * i) * 3;
+    const double* Pj = P + (size_t)(2 * j) * 3;
+    // T = A_q P_j (2x3), C = P_i^T T
+    double T[2][3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) T[k][c] = Aq[2 * k] * Pj[c] + Aq[2 * k + 1] * Pj[3 + c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) C[3 * c + cc] = Pi[c] * T[0][cc] + Pi[3 + c] * T[1][cc];
+  }
+};
+
+// Levels >= 1: 3x3 blocks in scalar SELL-64; block q of fine row i sits at position fp of
+// that row.
+struct RapFineS {
+  const int64_t* __restrict__ foff;
+  const double* __restrict__ fval;
+  __device__ void operator()(int i, int, int j, int fp, const double* __restrict__ P,
+                             double (&C)[9]) const {
+    double Aq[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int row = 3 * i + k, s = row >> 6, lane = row & 63;
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) Aq[k][kk] = fval[foff[s] + (int64_t)(3 * fp + kk) * 64 + lane];
+    }
+    const double* Pi = P + (size_t)(3 * i) * 3;
+    const double* Pj = P + (size_t)(3 * j) * 3;
+    double T[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        T[k][c] = Aq[k][0] * Pj[c] + Aq[k][1] * Pj[3 + c] + Aq[k][2] * Pj[6 + c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+        C[3 * c + cc] = Pi[c] * T[0][cc] + Pi[3 + c] * T[1][cc] + Pi[6 + c] * T[2][cc];
+  }
+};
+
+// Galerkin product A_{l+1} = P^T A_l P, one wavefront (one workgroup) per coarse block row a.
+// Coarse block (a, b) is the sum of the contributions of the fine blocks q = (i, j) with i a
+// member of a and agg(j) = b, added one at a time in member order and then block order.
+// Lanes compute the contributions of 64 pairs at a time; lane t owns the coarse blocks at
+// positions t and 64 + t (kSlots = 2: rows of more than 64 blocks) of the row and adds its
+// pairs in that order.
+template <class Fine, int kSlots>
+__global__ __launch_bounds__(64) void k_rap(const int* __restrict__ mptr,
+                                            const int* __restrict__ mem,
+                                            const int* __restrict__ agg,
+                                            const int* __restrict__ bptr,
+                                            const int* __restrict__ bcol, Fine fine,
+                                            const double* __restrict__ P,
+                                            const int* __restrict__ cptr,
+                                            const int* __restrict__ ccol,
+                                            const int64_t* __restrict__ coff,
+                                            double* __restrict__ cval) {
+  __shared__ MemberPairs mp;
+  __shared__ int cols[kMaxCoarseDeg];
+  __shared__ int order[64];
+  __shared__ double cl[64 * 9];
+  const int a = blockIdx.x, lane = threadIdx.x;
+  const int m0 = mptr[a], nm = mptr[a + 1] - m0;
+  const int c0 = cptr[a], deg = cptr[a + 1] - c0;
+  // the set-up has rejected larger aggregates and coarse rows before this runs
+  if (nm > kSetupMembers || deg > 64 * kSlots || deg <= 0) return;
+  for (int t = lane; t < deg; t += 64) cols[t] = ccol[c0 + t];
+  const int T = member_pairs(m0, nm, mem, bptr, mp, lane);
+  __syncthreads();
+  double acc[kSlots][9];
+#pragma unroll
+  for (int h = 0; h < kSlots; ++h)
+#pragma unroll
+    for (int e = 0; e < 9; ++e) acc[h][e] = 0.0;
+  const uint64_t below = (1ull << lane) - 1;
+  for (int p0 = 0; p0 < T; p0 += 64) {
+    const int p = p0 + lane;
+    const bool valid = p < T;
+    int pos = -1;
+    if (valid) {
+      const int t = pair_member(mp, nm, p);
+      const int i = mp.node[t];
+      const int q = mp.base[t] + p;
+      const int j = bcol[q];
+      const int b = agg[j];
+      int lo = 0, hi = deg - 1;  // position of b in the ascending row
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cols[mid] < b) lo = mid + 1;
+        else hi = mid;
+      }
+      pos = lo;
+      double C[9];
+      fine(i, q, j, q - mp.first[t], P, C);
+#pragma unroll
+      for (int e = 0; e < 9; ++e) cl[9 * lane + e] = C[e];
+    }
+    // the chunk's pairs grouped by position, each group in pair order
+    int run = 0, slot = 0, start0 = 0, cnt0 = 0, start1 = 0, cnt1 = 0;
+    for (int v = 0; v < deg; ++v) {
+      const uint64_t mask = __ballot(valid && pos == v);
+      const int c = __popcll(mask);
+      if (valid && pos == v) slot = run + __popcll(mask & below);
+      if (v == lane) {
+        start0 = run;
+        cnt0 = c;
+      }
+      if (v == lane + 64) {
+        start1 = run;
+        cnt1 = c;
+      }
+      run += c;
+    }
+    if (valid) order[slot] = lane;
+    __syncthreads();
+    for (int k = 0; k < cnt0; ++k) {
+      const double* src = cl + 9 * order[start0 + k];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) acc[0][e] += src[e];
+    }
+    if (kSlots == 2)
+      for (int k = 0; k < cnt1; ++k) {
+        const double* src = cl + 9 * order[start1 + k];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[kSlots - 1][e] += src[e];
+      }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int h = 0; h < kSlots; ++h) {
+    const int pos = lane + 64 * h;
+    if (pos >= deg) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int row = 3 * a + c, s = row >> 6, ln = row & 63;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) cval[coff[s] + (int64_t)(3 * pos + cc) * 64 + ln] = acc[h][3 * c + cc];
+    }
   }
 }
 
@@ -1246,48 +1396,150 @@ __device__ inline double nns_entry(int row, int c, const double* __restrict__ B,
   return c == 0 ? 0.0 : (c == 1 ? 1.0 : xy[2 * (size_t)i] - cx);
 }
 
-// Modified Gram-Schmidt of the near-null space on each aggregate (one thread per aggregate,
-// rows in member order, then DOF order): Q into P (n x 3), R (3x3 row-major) into Rn[9a..].
-// A column whose remaining norm is <= 1e-10 of its original norm (or zero) is dead: zero in Q
-// and a zero row in R.
-__global__ void k_mgs(int na, int bs, const int* __restrict__ mptr, const int* __restrict__ mem,
-                      const double* __restrict__ B, const double* __restrict__ xy,
-                      const unsigned char* __restrict__ fixed, double cx, double cy,
-                      double* __restrict__ P, double* __restrict__ Rn) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+// sum of x over the rows k = 0 .. m-1 of a wavefront (row k in lane k % 64, slot k / 64),
+// added one at a time in row order, as one thread would
+__device__ inline double row_order_sum(const double (&x)[3], int m) {
+  double s = 0.0;
+#pragma unroll
+  for (int sl = 0; sl < 3; ++sl) {
+    const int cnt = min(64, m - 64 * sl);
+    for (int k = 0; k < cnt; ++k) s += lane_value(x[sl], k);
+  }
+  return s;
+}
+
+// Modified Gram-Schmidt of the near-null space on aggregate a (rows in member order, then DOF
+// order): Q into P (n x 3), R (3x3 row-major) into Rn[9a..].  Every sum runs over the rows in
+// order, one addition at a time.  A column whose remaining norm is <= 1e-10 of its original
+// norm (or zero) is dead: zero in Q and a zero row in R.  Two forms with the same arithmetic:
+// one wavefront per aggregate (row k in lane k % 64), and one lane per aggregate with its rows
+// in registers (aggregates of up to kLaneRows rows; the larger ones go to a list for the
+// wavefront form).
+struct MgsArgs {
+  int bs;
+  const int* __restrict__ mptr;
+  const int* __restrict__ mem;
+  const double* __restrict__ B;
+  const double* __restrict__ xy;
+  const unsigned char* __restrict__ fixed;
+  double cx, cy;
+  double* __restrict__ P;
+  double* __restrict__ Rn;
+};
+
+__device__ inline void mgs_wave(int a, int lane, const MgsArgs& g) {
+  const int bs = g.bs;
+  const int m0 = g.mptr[a], m = bs * (g.mptr[a + 1] - m0);
+  if (m > 3 * 64) return;  // the set-up rejects aggregates above 192 DOFs
+  int row[3];
+  double v[3][3];
+#pragma unroll
+  for (int sl = 0; sl < 3; ++sl) {
+    const int k = lane + 64 * sl;
+    row[sl] = k < m ? bs * g.mem[m0 + k / bs] + k % bs : -1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      v[sl][c] = row[sl] >= 0 ? nns_entry(row[sl], c, g.B, g.xy, g.fixed, g.cx, g.cy) : 0.0;
+  }
+  double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  bool alive[3] = {false, false, false};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double x[3];
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) x[sl] = v[sl][c] * v[sl][c];
+    const double n0 = sqrt(row_order_sum(x, m));
+#pragma unroll
+    for (int cc = 0; cc < c; ++cc) {
+      if (!alive[cc]) continue;
+#pragma unroll
+      for (int sl = 0; sl < 3; ++sl) x[sl] = v[sl][cc] * v[sl][c];
+      const double dot = row_order_sum(x, m);
+      R[cc][c] = dot;
+#pragma unroll
+      for (int sl = 0; sl < 3; ++sl) v[sl][c] -= dot * v[sl][cc];
+    }
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) x[sl] = v[sl][c] * v[sl][c];
+    const double nr = sqrt(row_order_sum(x, m));
+    const bool ok = n0 > 0.0 && nr > 1e-10 * n0;
+    alive[c] = ok;
+    if (ok) R[c][c] = nr;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+      if (!ok) R[c][cc] = 0.0;
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) v[sl][c] = ok ? v[sl][c] / nr : 0.0;
+  }
+#pragma unroll
+  for (int sl = 0; sl < 3; ++sl)
+    if (row[sl] >= 0)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g.P[(size_t)row[sl] * 3 + c] = v[sl][c];
+  double r = 0.0;
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+    if (lane == t) r = alive[t / 3] ? R[t / 3][t % 3] : 0.0;
+  if (lane < 9) g.Rn[9 * (size_t)a + lane] = r;
+}
+
+// one wavefront per aggregate: every aggregate (list == nullptr), or the *count listed ones
+__global__ __launch_bounds__(kT) void k_mgs(int na, const int* __restrict__ list,
+                                            const int* __restrict__ count, MgsArgs g) {
+  const int nw = list ? *count : na;
+  const int lane = threadIdx.x & 63;
+  for (int w = (int)(((int64_t)blockIdx.x * kT + threadIdx.x) >> 6); w < nw;
+       w += gridDim.x * kW)
+    mgs_wave(list ? list[w] : w, lane, g);
+}
+
+constexpr int kLaneRows = 20;      // rows of an aggregate one lane holds (10 nodes of 2 DOFs)
+constexpr int kMgsLaneMin = 4096;  // aggregates from which the lane form runs (level 0)
+constexpr int kMgsListGrid = 256;  // workgroups of the wavefront form over a list
+
+template <int BS>
+__global__ __launch_bounds__(kT) void k_mgs_lanes(int na, MgsArgs g, int* __restrict__ big,
+                                                  int* __restrict__ nbig) {
+  const int a = blockIdx.x * kT + threadIdx.x;
   if (a >= na) return;
-  const int m0 = mptr[a], m = bs * (mptr[a + 1] - m0);
+  const int m0 = g.mptr[a], m = BS * (g.mptr[a + 1] - m0);
+  if (m > kLaneRows) {
+    big[atomicAdd(nbig, 1)] = a;
+    return;
+  }
+  int row[kLaneRows];
+  double v[kLaneRows][3];
+#pragma unroll
+  for (int k = 0; k < kLaneRows; ++k) {
+    row[k] = k < m ? BS * g.mem[m0 + k / BS] + k % BS : -1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      v[k][c] = row[k] >= 0 ? nns_entry(row[k], c, g.B, g.xy, g.fixed, g.cx, g.cy) : 0.0;
+  }
   double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   bool alive[3] = {false, false, false};
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     double n0 = 0.0;
-    for (int k = 0; k < m; ++k) {
-      const int row = bs * mem[m0 + k / bs] + k % bs;
-      const double v = nns_entry(row, c, B, xy, fixed, cx, cy);
-      P[(size_t)row * 3 + c] = v;
-      n0 += v * v;
-    }
+#pragma unroll
+    for (int k = 0; k < kLaneRows; ++k)
+      if (k < m) n0 += v[k][c] * v[k][c];
     n0 = sqrt(n0);
 #pragma unroll
     for (int cc = 0; cc < c; ++cc) {
       if (!alive[cc]) continue;
       double dot = 0.0;
-      for (int k = 0; k < m; ++k) {
-        const int row = bs * mem[m0 + k / bs] + k % bs;
-        dot += P[(size_t)row * 3 + cc] * P[(size_t)row * 3 + c];
-      }
+#pragma unroll
+      for (int k = 0; k < kLaneRows; ++k)
+        if (k < m) dot += v[k][cc] * v[k][c];
       R[cc][c] = dot;
-      for (int k = 0; k < m; ++k) {
-        const int row = bs * mem[m0 + k / bs] + k % bs;
-        P[(size_t)row * 3 + c] -= dot * P[(size_t)row * 3 + cc];
-      }
+#pragma unroll
+      for (int k = 0; k < kLaneRows; ++k) v[k][c] -= dot * v[k][cc];
     }
     double nr = 0.0;
-    for (int k = 0; k < m; ++k) {
-      const int row = bs * mem[m0 + k / bs] + k % bs;
-      nr += P[(size_t)row * 3 + c] * P[(size_t)row * 3 + c];
-    }
+#pragma unroll
+    for (int k = 0; k < kLaneRows; ++k)
+      if (k < m) nr += v[k][c] * v[k][c];
     nr = sqrt(nr);
     const bool ok = n0 > 0.0 && nr > 1e-10 * n0;
     alive[c] = ok;
@@ -1295,24 +1547,41 @@ __global__ void k_mgs(int na, int bs, const int* __restrict__ mptr, const int* _
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc)
       if (!ok) R[c][cc] = 0.0;
-    for (int k = 0; k < m; ++k) {
-      const int row = bs * mem[m0 + k / bs] + k % bs;
-      P[(size_t)row * 3 + c] = ok ? P[(size_t)row * 3 + c] / nr : 0.0;
-    }
+#pragma unroll
+    for (int k = 0; k < kLaneRows; ++k) v[k][c] = ok ? v[k][c] / nr : 0.0;
   }
+#pragma unroll
+  for (int k = 0; k < kLaneRows; ++k)
+    if (k < m)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g.P[(size_t)row[k] * 3 + c] = v[k][c];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int cc = 0; cc < 3; ++cc) Rn[9 * (size_t)a + 3 * c + cc] = alive[c] ? R[c][cc] : 0.0;
+    for (int cc = 0; cc < 3; ++cc) g.Rn[9 * (size_t)a + 3 * c + cc] = alive[c] ? R[c][cc] : 0.0;
 }
 
-// bandwidth of a node graph with ascending columns: max over rows of i - first and last - i
-__global__ void k_graph_bandwidth(int n, const int* __restrict__ ptr, const int* __restrict__ col,
-                                  int* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || ptr[i + 1] == ptr[i]) return;
-  const int b = max(i - col[ptr[i]], col[ptr[i + 1] - 1] - i);
-  atomicMax(out, b);
+// bandwidth of a node graph with ascending columns: max over rows of i - first and last - i.
+// A bounded grid strides over the rows and each workgroup adds one atomic: atomics on one
+// address serialise at the memory side, one per wavefront of 2M rows took 0.36 ms.
+constexpr int kBandwidthGrid = 512;
+__global__ __launch_bounds__(kT) void k_graph_bandwidth(int n, const int* __restrict__ ptr,
+                                                        const int* __restrict__ col,
+                                                        int* __restrict__ out) {
+  __shared__ int lds[kW];
+  int b = 0;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT)
+    if (ptr[i + 1] > ptr[i]) b = max(b, max(i - col[ptr[i]], col[ptr[i + 1] - 1] - i));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) b = max(b, __shfl_xor(b, off, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = lds[0];
+#pragma unroll
+    for (int w = 1; w < kW; ++w) m = max(m, lds[w]);
+    if (m > 0) atomicMax(out, m);
+  }
 }
 
 // member lists of the aggregates: counts, offsets (a scan), members in arrival order, then
@@ -1348,10 +1617,8 @@ __global__ void k_member_sort(int na, int bs, const int* __restrict__ mptr, int*
   }
 }
 
-constexpr int kMaxCoarseDeg = 128;  // block columns of one coarse block row
-
-// Sorted distinct aggregates adjacent to aggregate a (over its members' block rows); -1 if
-// there are more than kMaxCoarseDeg.
+// Sorted distinct aggregates adjacent to aggregate a (over its members' block rows), by one
+// thread into out[]; -1 if there are more than kMaxCoarseDeg.
 __device__ inline int coarse_row(int a, const int* __restrict__ mptr, const int* __restrict__ mem,
                                  const int* __restrict__ bptr, const int* __restrict__ bcol,
                                  const int* __restrict__ agg, int* out) {
@@ -1372,31 +1639,188 @@ __device__ inline int coarse_row(int a, const int* __restrict__ mptr, const int*
   return len;
 }
 
-__global__ void k_cpat_count(int na, const int* __restrict__ mptr, const int* __restrict__ mem,
-                             const int* __restrict__ bptr, const int* __restrict__ bcol,
-                             const int* __restrict__ agg, int* __restrict__ deg,
-                             int* __restrict__ err) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= na) return;
-  int buf[kMaxCoarseDeg];
-  const int len = coarse_row(a, mptr, mem, bptr, bcol, agg, buf);
-  if (len < 0) atomicMax(err, 1);
-  deg[a] = len < 0 ? 0 : len;
+// The same by one wavefront (one workgroup): the lanes take the (member, block) pairs, and the
+// distinct aggregates come out in ascending order, one per step (the smallest above the last).
+// Column t ends in lane t % 64 (col0 for t < 64, col1 above).  Aggregates with more pairs
+// than the lanes' register slots go through coarse_row in lane 0, with out[] in LDS.
+__device__ inline int coarse_row_wave(int a, const int* __restrict__ mptr,
+                                      const int* __restrict__ mem, const int* __restrict__ bptr,
+                                      const int* __restrict__ bcol, const int* __restrict__ agg,
+                                      MemberPairs& mp, int* spill, int lane, int& col0,
+                                      int& col1) {
+  const int m0 = mptr[a], nm = mptr[a + 1] - m0;
+  col0 = col1 = -1;
+  int T = -1;
+  if (nm <= kSetupMembers) T = member_pairs(m0, nm, mem, bptr, mp, lane);
+  __syncthreads();
+  if (T < 0 || T > 64 * kPairSlots) {
+    int len = 0;
+    if (lane == 0) len = coarse_row(a, mptr, mem, bptr, bcol, agg, spill);
+    len = __shfl(len, 0, 64);
+    __syncthreads();
+    if (lane < len) col0 = spill[lane];
+    if (lane + 64 < len) col1 = spill[lane + 64];
+    return len;
+  }
+  int b[kPairSlots];
+#pragma unroll
+  for (int s = 0; s < kPairSlots; ++s) {
+    const int p = lane + 64 * s;
+    b[s] = INT_MAX;
+    if (p < T) {
+      const int t = pair_member(mp, nm, p);
+      b[s] = agg[bcol[mp.base[t] + p]];
+    }
+  }
+  int last = -1, len = 0;
+  for (;;) {
+    int mn = INT_MAX;
+#pragma unroll
+    for (int s = 0; s < kPairSlots; ++s)
+      if (b[s] > last) mn = min(mn, b[s]);
+    mn = wave_min(mn);
+    if (mn == INT_MAX) break;
+    if (len == kMaxCoarseDeg) return -1;
+    if (lane == (len & 63)) {
+      if (len < 64) col0 = mn;
+      else col1 = mn;
+    }
+    last = mn;
+    ++len;
+  }
+  return len;
 }
 
-__global__ void k_cpat_fill(int na, const int* __restrict__ mptr, const int* __restrict__ mem,
-                            const int* __restrict__ bptr, const int* __restrict__ bcol,
-                            const int* __restrict__ agg, const int* __restrict__ cptr,
-                            int* __restrict__ ccol, int* __restrict__ crow) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= na) return;
-  int buf[kMaxCoarseDeg];
-  const int len = coarse_row(a, mptr, mem, bptr, bcol, agg, buf);
-  const int o = cptr[a];
-  for (int t = 0; t < len; ++t) {
-    ccol[o + t] = buf[t];
-    crow[o + t] = a;
+// Coarse block pattern, one wavefront per aggregate: every aggregate (list == nullptr) or the
+// *count listed ones.  k_cpat_count: the degrees (err = 1 above kMaxCoarseDeg);
+// k_cpat_fill: the columns at cptr.
+__global__ __launch_bounds__(64) void k_cpat_count(int na, const int* __restrict__ list,
+                                                   const int* __restrict__ count,
+                                                   const int* __restrict__ mptr,
+                                                   const int* __restrict__ mem,
+                                                   const int* __restrict__ bptr,
+                                                   const int* __restrict__ bcol,
+                                                   const int* __restrict__ agg,
+                                                   int* __restrict__ deg, int* __restrict__ err) {
+  __shared__ MemberPairs mp;
+  __shared__ int spill[kMaxCoarseDeg];
+  const int lane = threadIdx.x, nw = list ? *count : na;
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const int a = list ? list[w] : w;
+    int col0, col1;
+    const int len = coarse_row_wave(a, mptr, mem, bptr, bcol, agg, mp, spill, lane, col0, col1);
+    if (lane == 0) {
+      if (len < 0) atomicMax(err, 1);
+      deg[a] = len < 0 ? 0 : len;
+    }
   }
+}
+
+__global__ __launch_bounds__(64) void k_cpat_fill(int na, const int* __restrict__ list,
+                                                  const int* __restrict__ count,
+                                                  const int* __restrict__ mptr,
+                                                  const int* __restrict__ mem,
+                                                  const int* __restrict__ bptr,
+                                                  const int* __restrict__ bcol,
+                                                  const int* __restrict__ agg,
+                                                  const int* __restrict__ cptr,
+                                                  int* __restrict__ ccol) {
+  __shared__ MemberPairs mp;
+  __shared__ int spill[kMaxCoarseDeg];
+  const int lane = threadIdx.x, nw = list ? *count : na;
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const int a = list ? list[w] : w;
+    int col0, col1;
+    const int len = coarse_row_wave(a, mptr, mem, bptr, bcol, agg, mp, spill, lane, col0, col1);
+    const int o = cptr[a];
+    if (lane < len) ccol[o + lane] = col0;
+    if (lane + 64 < len) ccol[o + lane + 64] = col1;
+  }
+}
+
+// The same with one lane per aggregate, for levels of many small aggregates: the lane keeps
+// the row sorted in registers (up to kLaneDeg columns) and inserts each member's neighbour
+// aggregates, loaded kLaneBlocks at a time; a row with more columns is listed (count form)
+// for the wavefront form and skipped (fill form).
+constexpr int kLaneDeg = 16;
+constexpr int kLaneBlocks = 16;
+constexpr int kCpatListGrid = 1024;
+constexpr int kCpatLaneMin = 65536;  // aggregates from which the lane form runs
+
+__device__ inline int coarse_row_lane(int a, const int* __restrict__ mptr,
+                                      const int* __restrict__ mem, const int* __restrict__ bptr,
+                                      const int* __restrict__ bcol, const int* __restrict__ agg,
+                                      int (&r)[kLaneDeg]) {
+  int len = 0;
+#pragma unroll
+  for (int t = 0; t < kLaneDeg; ++t) r[t] = INT_MAX;
+  for (int m = mptr[a]; m < mptr[a + 1]; ++m) {
+    const int i = mem[m];
+    const int q1 = bptr[i + 1];
+    for (int q0 = bptr[i]; q0 < q1; q0 += kLaneBlocks) {
+      int bb[kLaneBlocks];
+#pragma unroll
+      for (int u = 0; u < kLaneBlocks; ++u) bb[u] = q0 + u < q1 ? bcol[q0 + u] : -1;
+#pragma unroll
+      for (int u = 0; u < kLaneBlocks; ++u) bb[u] = bb[u] >= 0 ? agg[bb[u]] : -1;
+#pragma unroll
+      for (int u = 0; u < kLaneBlocks; ++u) {
+        const int b = bb[u];
+        if (b < 0) continue;
+        int below = 0;
+        bool dup = false;
+#pragma unroll
+        for (int t = 0; t < kLaneDeg; ++t) {
+          below += r[t] < b ? 1 : 0;  // unused slots hold INT_MAX
+          dup = dup || r[t] == b;
+        }
+        if (dup) continue;
+        if (len == kLaneDeg) return -1;
+#pragma unroll
+        for (int t = kLaneDeg - 1; t > 0; --t)
+          if (t > below) r[t] = r[t - 1];
+#pragma unroll
+        for (int t = 0; t < kLaneDeg; ++t)
+          if (t == below) r[t] = b;
+        ++len;
+      }
+    }
+  }
+  return len;
+}
+
+__global__ __launch_bounds__(kT) void k_cpat_count_lanes(int na, const int* __restrict__ mptr,
+                                                         const int* __restrict__ mem,
+                                                         const int* __restrict__ bptr,
+                                                         const int* __restrict__ bcol,
+                                                         const int* __restrict__ agg,
+                                                         int* __restrict__ deg,
+                                                         int* __restrict__ big,
+                                                         int* __restrict__ nbig) {
+  const int a = blockIdx.x * kT + threadIdx.x;
+  if (a >= na) return;
+  int r[kLaneDeg];
+  const int len = coarse_row_lane(a, mptr, mem, bptr, bcol, agg, r);
+  if (len < 0) big[atomicAdd(nbig, 1)] = a;
+  else deg[a] = len;
+}
+
+__global__ __launch_bounds__(kT) void k_cpat_fill_lanes(int na, const int* __restrict__ mptr,
+                                                        const int* __restrict__ mem,
+                                                        const int* __restrict__ bptr,
+                                                        const int* __restrict__ bcol,
+                                                        const int* __restrict__ agg,
+                                                        const int* __restrict__ cptr,
+                                                        int* __restrict__ ccol) {
+  const int a = blockIdx.x * kT + threadIdx.x;
+  if (a >= na) return;
+  int r[kLaneDeg];
+  const int len = coarse_row_lane(a, mptr, mem, bptr, bcol, agg, r);
+  if (len < 0) return;
+  const int o = cptr[a];
+#pragma unroll
+  for (int t = 0; t < kLaneDeg; ++t)
+    if (t < len) ccol[o + t] = r[t];
 }
 
 // column indices of the scalar SELL-64 layout of a 3x3-block pattern: lane = row, block
@@ -2301,7 +2725,8 @@ void mg_prepare(Context& c) {
   int* pin = pinned_ints(mg, need);
   int* bw = grow(mg.iwork, 1);
   HIP_CHECK(hipMemsetAsync(bw, 0, sizeof(int), c.stream));
-  hipLaunchKernelGGL(k_graph_bandwidth, dim3(grid1(c.nb)), dim3(kT), 0, c.stream, c.nb,
+  hipLaunchKernelGGL(k_graph_bandwidth, dim3(std::min(grid1(c.nb), kBandwidthGrid)), dim3(kT), 0,
+                     c.stream, c.nb,
                      (const int*)c.brow_ptr.get(), (const int*)c.bcol.get(), bw);
   HIP_CHECK(hipGetLastError());
   // the download follows the work queued so far on the main stream (the pattern's kernels,
@@ -2371,6 +2796,8 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     mg.fuse_n = fuse_n ? std::atoi(fuse_n) : 200000;
     const char* fold = std::getenv("CSTFEM_MG_FOLD");
     mg.fold_max = fold ? std::atoll(fold) : 200000;
+    const char* lanes = std::getenv("CSTFEM_MG_SETUP_LANES");
+    mg.setup_lanes = !(lanes && lanes[0] == '0');
   }
   // level 0: the context's matrix; its node graph is read on the host (aggregation) from a
   // page-locked copy.  The pattern is complete since build_pattern returned, so the copy runs
@@ -2387,6 +2814,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     bound_dev = learned_smoother_bound_enqueue(c, kSmootherSteps, c.aux);
   }
   MgLevel* L0 = mg.take_level();
+  L0->val32.release();  // level 0's FP32 values are the context's
   L0->n = c.n;
   L0->nodes = c.nb;
   L0->bs = 2;
@@ -2486,17 +2914,39 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     // tentative prolongator and the next near-null space (device)
     L.P.alloc((size_t)L.n * 3);
     C->nns.alloc((size_t)na * 9);
-    hipLaunchKernelGGL(k_mgs, dim3(grid1(na)), dim3(kT), 0, c.stream, na, L.bs,
-                       (const int*)L.mptr.get(), (const int*)L.mem.get(),
-                       (const double*)(l == 0 ? nullptr : L.nns.get()), (const double*)c.xy.get(),
-                       fixed, cx, cy, L.P.get(), C->nns.get());
+    {
+      const MgsArgs ga{L.bs, L.mptr.get(), L.mem.get(), l == 0 ? nullptr : L.nns.get(), c.xy.get(),
+                       fixed, cx, cy, L.P.get(), C->nns.get()};
+      if (mg.setup_lanes && L.bs == 2 && na >= kMgsLaneMin) {
+        // a lane per aggregate; aggregates above kLaneRows rows, listed, one wavefront each
+        int* big = grow(mg.big, (size_t)na + 1);
+        HIP_CHECK(hipMemsetAsync(big, 0, sizeof(int), c.stream));
+        hipLaunchKernelGGL(k_mgs_lanes<2>, dim3(grid1(na)), dim3(kT), 0, c.stream, na, ga, big + 1,
+                           big);
+        hipLaunchKernelGGL(k_mgs, dim3(kMgsListGrid), dim3(kT), 0, c.stream, na,
+                           (const int*)(big + 1), (const int*)big, ga);
+      } else {
+        hipLaunchKernelGGL(k_mgs, dim3(grid1(64 * (int64_t)na)), dim3(kT), 0, c.stream, na,
+                           (const int*)nullptr, (const int*)nullptr, ga);
+      }
+    }
     HIP_CHECK(hipGetLastError());
     tr.mark(c, l, "gram-schmidt");
     // coarse block pattern (device): sorted distinct neighbour aggregates of each aggregate
     const int* dbp = l == 0 ? c.brow_ptr.get() : L.d_bptr.get();
     const int* dbc = l == 0 ? c.bcol.get() : L.d_bcol.get();
     int* deg = cnt;  // the counts were consumed by the scan
-    hipLaunchKernelGGL(k_cpat_count, dim3(grid1(na)), dim3(kT), 0, c.stream, na,
+    // many small aggregates: a lane each, the few larger rows listed for a wavefront each
+    const bool lanes = mg.setup_lanes && na >= kCpatLaneMin;
+    int* big = lanes ? grow(mg.big, (size_t)na + 1) : nullptr;
+    if (lanes) {
+      HIP_CHECK(hipMemsetAsync(big, 0, sizeof(int), c.stream));
+      hipLaunchKernelGGL(k_cpat_count_lanes, dim3(grid1(na)), dim3(kT), 0, c.stream, na,
+                         (const int*)L.mptr.get(), (const int*)L.mem.get(), dbp, dbc,
+                         (const int*)L.agg.get(), deg, big + 1, big);
+    }
+    hipLaunchKernelGGL(k_cpat_count, dim3(lanes ? kCpatListGrid : na), dim3(64), 0, c.stream, na,
+                       (const int*)(lanes ? big + 1 : nullptr), (const int*)big,
                        (const int*)L.mptr.get(), (const int*)L.mem.get(), dbp, dbc,
                        (const int*)L.agg.get(), deg, flags);
     HIP_CHECK(hipGetLastError());
@@ -2514,11 +2964,14 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     const int ncblk = C->bptr[na];
     upload(C->d_bptr, C->bptr);
     C->d_bcol.alloc(std::max(ncblk, 1));
-    int* d_crow = grow(mg.crow, (size_t)std::max(ncblk, 1));
-    hipLaunchKernelGGL(k_cpat_fill, dim3(grid1(na)), dim3(kT), 0, c.stream, na,
+    if (lanes)
+      hipLaunchKernelGGL(k_cpat_fill_lanes, dim3(grid1(na)), dim3(kT), 0, c.stream, na,
+                         (const int*)L.mptr.get(), (const int*)L.mem.get(), dbp, dbc,
+                         (const int*)L.agg.get(), (const int*)C->d_bptr.get(), C->d_bcol.get());
+    hipLaunchKernelGGL(k_cpat_fill, dim3(lanes ? kCpatListGrid : na), dim3(64), 0, c.stream, na,
+                       (const int*)(lanes ? big + 1 : nullptr), (const int*)big,
                        (const int*)L.mptr.get(), (const int*)L.mem.get(), dbp, dbc,
-                       (const int*)L.agg.get(), (const int*)C->d_bptr.get(), C->d_bcol.get(),
-                       d_crow);
+                       (const int*)L.agg.get(), (const int*)C->d_bptr.get(), C->d_bcol.get());
     HIP_CHECK(hipGetLastError());
     C->bcol.resize(ncblk);
     HIP_CHECK(hipMemcpyAsync(C->bcol.data(), C->d_bcol.get(), sizeof(int) * ncblk, hipMemcpyDeviceToHost, c.stream));
@@ -2537,32 +2990,37 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     tr.mark(c, l, "sell layout");
     // Galerkin product
     if (ncblk > 0) {
-      if (l == 0)
-        hipLaunchKernelGGL(k_rap0, dim3(grid1(ncblk)), dim3(kT), 0, c.stream, ncblk,
-                           (const int*)d_crow, (const int*)C->d_bptr.get(),
-                           (const int*)C->d_bcol.get(), (const int*)L.mptr.get(),
-                           (const int*)L.mem.get(), (const int*)L.agg.get(), (const int*)c.brow_ptr.get(),
-                           (const int*)c.bcol.get(), (const int64_t*)c.sell_pos.get(),
-                           (const double*)c.sell_val.get(), (const double*)L.P.get(),
-                           (const int64_t*)C->off.get(), C->val.get());
-      else
-        hipLaunchKernelGGL(k_rapS, dim3(grid1(ncblk)), dim3(kT), 0, c.stream, ncblk,
-                           (const int*)d_crow, (const int*)C->d_bptr.get(),
-                           (const int*)C->d_bcol.get(), (const int*)L.mptr.get(),
-                           (const int*)L.mem.get(), (const int*)L.agg.get(), (const int*)L.d_bptr.get(),
-                           (const int*)L.d_bcol.get(), (const int64_t*)L.off.get(),
-                           (const double*)L.val.get(), (const double*)L.P.get(),
-                           (const int64_t*)C->off.get(), C->val.get());
+      const bool wide = *std::max_element(hdeg.begin(), hdeg.end()) > 64;
+      const RapFine0 f0{c.sell_pos.get(), c.sell_val.get()};
+      const RapFineS fs{L.off.get(), L.val.get()};
+      const int* fp = l == 0 ? c.brow_ptr.get() : L.d_bptr.get();
+      const int* fc = l == 0 ? c.bcol.get() : L.d_bcol.get();
+#define CSTFEM_RAP(F, f, S)                                                                          \
+  hipLaunchKernelGGL((k_rap<F, S>), dim3(na), dim3(64), 0, c.stream, (const int*)L.mptr.get(),     \
+                     (const int*)L.mem.get(), (const int*)L.agg.get(), fp, fc, f,                  \
+                     (const double*)L.P.get(), (const int*)C->d_bptr.get(),                        \
+                     (const int*)C->d_bcol.get(), (const int64_t*)C->off.get(), C->val.get())
+      if (l == 0) {
+        if (wide) CSTFEM_RAP(RapFine0, f0, 2);
+        else CSTFEM_RAP(RapFine0, f0, 1);
+      } else {
+        if (wide) CSTFEM_RAP(RapFineS, fs, 2);
+        else CSTFEM_RAP(RapFineS, fs, 1);
+      }
+#undef CSTFEM_RAP
       HIP_CHECK(hipGetLastError());
     }
     C->dinv.alloc(C->n);
     hipLaunchKernelGGL(ks_dinv, dim3(grid1(C->n)), dim3(kT), 0, c.stream, mats(*C), C->dinv.get());
     HIP_CHECK(hipGetLastError());
-    if (mg.coarse_fp32) {  // FP32 copy of the values for the cycle's row sums
+    if (mg.coarse_fp32) {  // FP32 copy of the values for the cycle's row sums (kept across
+                           // set-ups of the same size: no free and allocation per step)
       C->val32.alloc(C->val.n);
       hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(kT), 0, c.stream, (int64_t)C->val.n,
                          (const double*)C->val.get(), C->val32.get());
       HIP_CHECK(hipGetLastError());
+    } else {
+      C->val32.release();  // the cycle reads FP32 values wherever a level holds them
     }
     HIP_CHECK(hipStreamSynchronize(c.stream));  // C->bcol on the host, C->val for the inverse
     alloc_work(c, *C);

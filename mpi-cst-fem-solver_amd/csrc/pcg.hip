@@ -1330,7 +1330,8 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
   double* pC = pA + 3 * kMaxGrid;
   Mat<P> A;
   V *r, *p, *q, *z;
-  DBuf<double> z64, ax64;
+  DBuf<double>& z64 = c.z64;
+  DBuf<double>& ax64 = c.ax64;
   if (fp32) {
     c.r32.alloc(nv);
     c.p32.alloc(nv);

@@ -15,7 +15,10 @@
   set-up; one that breaks down during the solve hands over to the same hierarchy with the
   Chebyshev smoother; either way the solve converges;
 - a good trained smoother passes the set-up check;
-- the fused cycle kernels give bit-identical cycles and solves to the separate kernels.
+- the fused cycle kernels give bit-identical cycles and solves to the separate kernels;
+- the set-up's wavefront kernels handle aggregates of many (member, block) pairs and coarse
+  rows of more than 64 blocks (oracle), and its lane-per-aggregate kernels build the same
+  hierarchy, bit for bit, as the wavefront forms alone.
 No reference counterpart exists for this component.
 """
 import dataclasses
@@ -341,6 +344,81 @@ def test_prepared_aggregation_on_other_graphs(dev, case):
         dev.mg_setup(10, 200, 1)
         hs.append(hierarchy(dev))
     assert_same_hierarchy(hs[1], hs[0])
+
+
+def with_hub(prob, nx, spread):
+    """The plate with 20 extra triangles (0, a, b) fanning from node 0 to pairs of nodes
+    elsewhere (overlapping elements keep the matrix SPD; node 0 ends with 22 triangles, under
+    the pattern builder's 32).  Node 0 comes first in the greedy pass, so its aggregate takes
+    all the fan's nodes.  spread=False: pairs of adjacent nodes along the middle row (an
+    aggregate of about 30 nodes, a coarse row of about 20 blocks); spread=True: pairs two
+    nodes apart on a coarse grid over the plate (an aggregate of about 70 nodes with about
+    600 (member, block) pairs, a coarse row of about 80 blocks)."""
+    m = prob.mesh
+    W, xy = nx + 1, prob.mesh.xy
+    if spread:
+        s = nx // 6
+        pairs = [(r * W + i, r * W + i + 2) for r in range(3, nx - 1, s)
+                 for i in range(3, nx - 2, s)][:20]
+    else:
+        j = nx // 2
+        pairs = [(j * W + i, j * W + i + 1) for i in range(nx // 2 - 10, nx // 2 + 10)]
+    extra = []
+    for a, b in pairs:
+        d = (xy[a, 0] - xy[0, 0]) * (xy[b, 1] - xy[0, 1]) - (xy[b, 0] - xy[0, 0]) * (xy[a, 1] - xy[0, 1])
+        extra.append([0, a, b] if d > 0 else [0, b, a])
+    tri = np.concatenate([m.tri, np.asarray(extra, np.int32)])
+    E = None if prob.E_elem is None else np.concatenate([prob.E_elem, np.ones(len(extra))])
+    return dataclasses.replace(prob, mesh=dataclasses.replace(m, tri=tri), E_elem=E)
+
+
+def coarse_degrees(dev, level):
+    rp, cols, _ = dev.mg_download_matrix(level)
+    return np.diff(rp)[0::3] // 3
+
+
+def test_hub_hierarchy_matches_oracle(dev):
+    """An aggregate of more (member, block) pairs than the wavefront coarse-pattern kernel's
+    register slots hold (one lane then inserts them serially) and a coarse row of more than
+    64 blocks (two blocks per lane in the Galerkin kernel), against the oracle: aggregates
+    exact, the coarse matrix to 1e-12, one cycle to 1e-9.  Two levels: the hub's aggregate
+    would be too large for a third."""
+    A, xy, fixed, bptr, bcol = system(dev, with_hub(hetero(40, e_seed=8, seed=3), 40, True))
+    dev.mg_setup(2, 1000, 1, 1)
+    levels = mgo.setup(A, xy, fixed, max_levels=2, coarse_max=1000, ptr0=bptr, col0=bcol)
+    assert dev.mg_levels() == len(levels) == 2
+    agg0 = dev.mg_download_aggregates(0)
+    np.testing.assert_array_equal(agg0, levels[0].agg)
+    assert np.bincount(agg0, weights=np.diff(bptr)).max() > 512
+    assert coarse_degrees(dev, 1).max() > 64
+    nl = dev.mg_level_info(1)[0]
+    rp, cols, vals = dev.mg_download_matrix(1)
+    Ad = sp.csr_matrix((vals, cols, rp), shape=(nl, nl))
+    assert abs(Ad - levels[1].A).max() <= 1e-12 * abs(levels[1].A).max()
+    lams = [dev.mg_level_info(l)[3] for l in range(2)]
+    r = np.random.default_rng(5).standard_normal(dev.n)
+    zo = mgo.cycle(levels, 0, r, lams, kdepth=1)
+    assert np.linalg.norm(dev.mg_apply(r) - zo) <= 1e-9 * np.linalg.norm(zo)
+
+
+def test_setup_lane_forms_match_wavefront_forms(dev, monkeypatch):
+    """On a level of many small aggregates the Gram-Schmidt and coarse-pattern kernels run one
+    lane per aggregate and hand the aggregates above a lane's registers (more than 10 nodes,
+    more than 16 coarse blocks) to the wavefront forms through a list.  The hierarchy must be
+    the one the wavefront forms alone build (CSTFEM_MG_SETUP_LANES=0), bit for bit."""
+    prob = with_hub(hetero(1100, e_seed=8, seed=3), 1100, False)
+    out = []
+    for lanes in ("1", "0"):
+        monkeypatch.setenv("CSTFEM_MG_SETUP_LANES", lanes)
+        pipeline.load_problem(dev, prob)
+        pipeline.assemble_and_constrain(dev, prob)
+        dev.setup()
+        dev.mg_setup(10, 200, 1)
+        out.append(hierarchy(dev))
+    assert dev.mg_level_info(0)[1] >= 65536  # the lane forms ran on level 0
+    assert np.bincount(dev.mg_download_aggregates(0)).max() > 10
+    assert coarse_degrees(dev, 1).max() > 16
+    assert_same_hierarchy(out[0], out[1])
 
 
 def test_coarse_level_above_one_k_cycle_grid(dev):
