@@ -318,6 +318,11 @@ int cstfem_mg_levels(cstfem_ctx* ctx, int* levels);
  * On level 0 with a learned smoother it is the set-up's estimate of lambda_max(M K). */
 int cstfem_mg_level_info(cstfem_ctx* ctx, int level, int* n, int* nagg, int64_t* nnz,
                          double* lam);
+/* The fused single-workgroup tail of the cycle: the first level it covers (-1: none) and
+ * whether the set-up packed its operands into one array (1: one cycle level over the dense
+ * coarsest, every load issued at the kernel's start) or it reads them level by level (0).
+ * Both forms give the same bits. */
+int cstfem_mg_tail(cstfem_ctx* ctx, int* first_level, int* packed);
 /* Aggregate of every node of `level` (count = nodes of the level). */
 int cstfem_mg_download_aggregates(cstfem_ctx* ctx, int level, int* agg, int count);
 /* Coarse matrix of a level >= 1 as scalar CSR: row_ptr[n+1], cols/vals[nnz]. */

@@ -859,6 +859,14 @@ int cstfem_mg_level_info(cstfem_ctx* ctx, int level, int* n, int* nagg, int64_t*
   });
 }
 
+int cstfem_mg_tail(cstfem_ctx* ctx, int* first_level, int* packed) {
+  return guarded([&] {
+    need(ctx != nullptr, "cstfem_mg_tail: NULL context");
+    mg_tail_info(ctx->c, first_level, packed);
+    return CSTFEM_OK;
+  });
+}
+
 int cstfem_mg_download_aggregates(cstfem_ctx* ctx, int level, int* agg, int count) {
   return guarded([&] {
     need(ctx && agg, "cstfem_mg_download_aggregates: NULL argument");

@@ -209,6 +209,11 @@ struct Multigrid {
   // 0: the wavefront forms everywhere, same results)
   bool setup_lanes = true;
   DBuf<double> cinv;
+  // operands of the packed one-level tail (k_tail1), gathered at set-up for level tail_packed
+  // (-1: none; the cycle then runs k_tail)
+  DBuf<double> tail_pd;
+  DBuf<int> tail_pi;
+  int tail_packed = -1;
   // device workspaces of the set-up, grown on demand and kept, so that repeated set-ups
   // allocate nothing: member-list scratch, coarse degrees and error flags (iwork), the
   // aggregates the lane forms of the set-up kernels hand to the wavefront forms (big: a count,
@@ -266,6 +271,7 @@ struct Multigrid {
     spare.insert(spare.begin(), lv.begin(), lv.end());
     lv.clear();
     tail_from = -1;
+    tail_packed = -1;
     m_bound = 0.0;
     ready = false;
   }
@@ -446,6 +452,7 @@ int mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* s
 void mg_enqueue_coarse(Context& c, const double* r, double* z);
 int64_t mg_level_info(Context& c, int level, int* n, int* nagg, double* lam);
 int mg_levels(Context& c);
+void mg_tail_info(Context& c, int* first_level, int* packed);
 // switches the level-0 smoother (0 learned M, 1 Chebyshev; its spectral bound is computed on
 // first use)
 void mg_set_fine_smoother(Context& c, int smoother);

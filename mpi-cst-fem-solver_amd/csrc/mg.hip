@@ -1135,6 +1135,201 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(Tail<T> T_, const PcgScal
   }
 }
 
+// ---- packed tail: one cycle level above the dense coarsest ----------------------------------
+//
+// k_tail spends most of its time in chains of dependent loads: a slice offset before the row,
+// a member list before the restriction's rows and P entries, the dense inverse in two rounds.
+// When the tail holds a single cycle level (rows in registers) over a small enough coarsest
+// level, every operand it reads -- the rows, their scaling, each fine DOF's aggregate and P
+// row, each restriction lane's (row, P entry) pairs and each dense lane's inverse entries --
+// is gathered once per set-up into one thread-major array (k_tail_pack), and k_tail1 issues
+// all its loads at its start, in one round trip.  k_tail1 keeps k_tail's arithmetic and order,
+// so its result is k_tail's to the bit.
+constexpr int kPackR = 6;   // restriction entries per lane (aggregates of <= 48 rows)
+constexpr int kPackD = 14;  // dense entries per lane (a coarsest level of <= 112 DOFs)
+// slot e of thread t is element e * kTailThreads + t of the double array d or the int array i
+constexpr int kPdRow = 0;                   // the row's values (kTailRegW, zero beyond w)
+constexpr int kPdP = kPdRow + kTailRegW;    // P row of fine DOF t (3)
+constexpr int kPdDense = kPdP + 3;          // dense lane t's inverse entries (kPackD)
+constexpr int kPdDinv = kPdDense + kPackD;  // diagonal scaling of row t
+constexpr int kPdSlots = kPdDinv + 1;
+constexpr int kPiCol = 0;                     // first column (3 b) of each 3x3 block of the row
+constexpr int kPiW = kPiCol + kTailRegW / 3;  // entries of row t
+constexpr int kPiAgg = kPiW + 1;              // aggregate of fine DOF t
+constexpr int kPiResN = kPiAgg + 1;           // entries of restriction lane t
+constexpr int kPiRes = kPiResN + 1;           // their products' LDS slots 3 f + c (kPackR)
+constexpr int kPiSlots = kPiRes + kPackR;
+
+// One workgroup of kTailThreads: fills the slots of every thread from the level's arrays and
+// the dense inverse (row-major nc x nc), in the orders k_tail reads them.
+__global__ __launch_bounds__(kTailThreads) void k_tail_pack(TailLevel<double> L,
+                                                            const double* __restrict__ cinv,
+                                                            int nc, double* __restrict__ pd,
+                                                            int* __restrict__ pi) {
+  const int t = threadIdx.x;
+  auto D = [&](int e) -> double& { return pd[(size_t)e * kTailThreads + t]; };
+  auto I = [&](int e) -> int& { return pi[(size_t)e * kTailThreads + t]; };
+  int w = 0, a = 0;
+  double dinv = 0.0, p[3] = {0.0, 0.0, 0.0};
+  if (t < L.A.n) {
+    const int s = t >> 6, lane = t & 63;
+    const int64_t o = L.A.off[s];
+    w = (int)((L.A.off[s + 1] - o) >> 6);
+    for (int k = 0; k < kTailRegW; ++k) D(kPdRow + k) = k < w ? L.A.val[o + (int64_t)k * 64 + lane] : 0.0;
+    for (int k = 0; k < kTailRegW / 3; ++k)
+      I(kPiCol + k) = 3 * k < w ? 3 * L.A.bcol[o / 3 + (int64_t)k * 64 + lane] : 0;
+    dinv = L.dinv[t];
+    a = L.agg[t / L.bs];
+    for (int c = 0; c < 3; ++c) p[c] = L.P[(size_t)t * 3 + c];
+  } else {
+    for (int k = 0; k < kTailRegW; ++k) D(kPdRow + k) = 0.0;
+    for (int k = 0; k < kTailRegW / 3; ++k) I(kPiCol + k) = 0;
+  }
+  I(kPiW) = w;
+  I(kPiAgg) = a;
+  D(kPdDinv) = dinv;
+  for (int c = 0; c < 3; ++c) D(kPdP + c) = p[c];
+  // restriction lane t: coarse DOF q = t / kTailLanes, its member rows f for k = j,
+  // j + kTailLanes, ... (restrict_sum's order), each as the slot 3 f + q % 3 of the product
+  // P[f][q % 3] res[f]; the host checked that they fit kPackR
+  int rn = 0;
+  if (t < 3 * L.nagg * kTailLanes) {
+    const int q = t / kTailLanes, j = t % kTailLanes;
+    const int m0 = L.mptr[q / 3], rows = L.bs * (L.mptr[q / 3 + 1] - m0);
+    for (int k = j; k < rows && rn < kPackR; k += kTailLanes, ++rn)
+      I(kPiRes + rn) = 3 * (L.bs * L.mem[m0 + k / L.bs] + k % L.bs) + q % 3;
+  }
+  I(kPiResN) = rn;
+  for (int e = rn; e < kPackR; ++e) I(kPiRes + e) = 0;
+  // dense lane t: row t / kTailLanes of the inverse, entries k = j, j + kTailLanes, ...
+  for (int e = 0; e < kPackD; ++e) {
+    const int i = t / kTailLanes, k = t % kTailLanes + e * kTailLanes;
+    D(kPdDense + e) = (i < nc && k < nc) ? cinv[(size_t)i * nc + k] : 0.0;
+  }
+}
+
+struct TailPack {
+  const double* d;
+  const int* i;
+  int n;   // DOFs of the cycle level
+  int nr;  // restriction lanes (3 nagg kTailLanes)
+  int nc;  // DOFs of the coarsest level
+  double c1r, c2d, c2r;
+  const double* rhs;
+  double* out;
+};
+
+// k_tail's cycle for one level in registers (nlev = 1, kReg), operands from the pack.  The
+// restriction sums products P[f][c] res[f] that thread f forms (the same rounded products
+// k_tail adds).  The dense inverse's entries are loaded after the first barrier: they arrive
+// during the smoothing steps, and the registers they take are not live beside the prologue's.
+// LDS: x, x2 (n each), the products (3 n), the coarsest level's right-hand side and solution
+// (nc each).
+__global__ __launch_bounds__(kTailThreads) void k_tail1(TailPack T, const PcgScalars* __restrict__ sc) {
+  extern __shared__ double lds[];
+  const int t = threadIdx.x, n = T.n;
+  double* x = lds;
+  double* x2 = x + n;
+  double* prod = x2 + n;
+  double* crhs = prod + 3 * n;
+  double* ckx = crhs + T.nc;
+  const bool live = t < n;
+  int st = status_of(sc);
+  double rhs = live ? T.rhs[t] : 0.0;
+  const double* __restrict__ pd = T.d + t;
+  const int* __restrict__ pi = T.i + t;
+  double v[kTailRegW];
+  int b[kTailRegW / 3], rf[kPackR];
+#pragma unroll
+  for (int k = 0; k < kTailRegW; ++k) v[k] = pd[(kPdRow + k) * kTailThreads];
+#pragma unroll
+  for (int k = 0; k < kTailRegW / 3; ++k) b[k] = pi[(kPiCol + k) * kTailThreads];
+  const int w = pi[kPiW * kTailThreads];
+  const int agg = pi[kPiAgg * kTailThreads];
+  const int rn = pi[kPiResN * kTailThreads];
+  const double dinv = pd[kPdDinv * kTailThreads];
+  const double p0 = pd[kPdP * kTailThreads], p1 = pd[(kPdP + 1) * kTailThreads],
+               p2 = pd[(kPdP + 2) * kTailThreads];
+#pragma unroll
+  for (int e = 0; e < kPackR; ++e) rf[e] = pi[(kPiRes + e) * kTailThreads];
+  hold(st, rhs);
+  if (halted(st)) return;
+  // row sum in RegRow::sum's order
+  auto row = [&](const double* xv) {
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < kTailRegW; ++k)
+      if (k < w) a = a + v[k] * xv[b[k / 3] + k % 3];
+    return a;
+  };
+  double d = 0.0;
+  // pre-smoothing: two Chebyshev steps from zero
+  if (live) {
+    const double dd = 0.0 + T.c1r * (dinv * (rhs - 0.0));
+    d = dd;
+    x[t] = 0.0 + dd;
+  }
+  __syncthreads();
+  double cd[kPackD];
+#pragma unroll
+  for (int e = 0; e < kPackD; ++e) cd[e] = pd[(kPdDense + e) * kTailThreads];
+  if (live) {
+    const double y = row(x);
+    const double dd = (T.c2d != 0.0 ? T.c2d * d : 0.0) + T.c2r * (dinv * (rhs - y));
+    d = dd;
+    x2[t] = x[t] + dd;
+  }
+  __syncthreads();
+  if (live) {
+    const double r = rhs - row(x2);
+    prod[3 * t] = p0 * r;
+    prod[3 * t + 1] = p1 * r;
+    prod[3 * t + 2] = p2 * r;
+  }
+  __syncthreads();
+  // restriction, kTailLanes lanes per coarse DOF
+  if (t < T.nr) {
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < kPackR; ++e)
+      if (e < rn) s = s + prod[rf[e]];
+#pragma unroll
+    for (int m = 1; m < kTailLanes; m <<= 1) s += __shfl_xor(s, m, 64);
+    if (t % kTailLanes == 0) crhs[t / kTailLanes] = s;
+  }
+  __syncthreads();
+  // dense solve on the coarsest level, kTailLanes lanes per row
+  if (t < T.nc * kTailLanes) {
+    const int j = t % kTailLanes;
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < kPackD; ++e)
+      if (j + e * kTailLanes < T.nc) s = s + cd[e] * crhs[j + e * kTailLanes];
+#pragma unroll
+    for (int m = 1; m < kTailLanes; m <<= 1) s += __shfl_xor(s, m, 64);
+    if (j == 0) ckx[t / kTailLanes] = s;
+  }
+  __syncthreads();
+  if (live) {
+    const double* ea = ckx + 3 * agg;
+    x2[t] = x2[t] + (p0 * ea[0] + p1 * ea[1] + p2 * ea[2]);
+  }
+  __syncthreads();
+  // post-smoothing
+  if (live) {
+    const double y = row(x2);
+    const double dd = 0.0 + T.c1r * (dinv * (rhs - y));
+    d = dd;
+    x[t] = x2[t] + dd;
+  }
+  __syncthreads();
+  if (live) {
+    const double y = row(x);
+    const double dd = (T.c2d != 0.0 ? T.c2d * d : 0.0) + T.c2r * (dinv * (rhs - y));
+    T.out[t] = x[t] + dd;
+  }
+}
+
 // ---- setup kernels -------------------------------------------------------------------
 
 // ---- wavefront helpers of the set-up kernels -----------------------------------------
@@ -2482,7 +2677,53 @@ void launch_tail_t(Context& c, int t, const double* rhs, double* out, const PcgS
     hipLaunchKernelGGL((k_tail<false, T>), dim3(1), dim3(kTailThreads), bytes, c.stream, T_, sc);
 }
 
+// Gathers the packed tail's operands (k_tail_pack) when the tail is one level with its rows
+// in registers over a coarsest level and aggregates small enough for the pack's slots; else
+// the cycle keeps k_tail.  Called wherever tail_from changes.  CSTFEM_MG_TAIL_PACK=0: k_tail.
+void tail_pack_build(Context& c) {
+  Multigrid& mg = c.mg;
+  mg.tail_packed = -1;
+  const char* env = std::getenv("CSTFEM_MG_TAIL_PACK");
+  if (env && env[0] == '0') return;
+  const int t = mg.tail_from, last = (int)mg.lv.size() - 1;
+  if (t < 0 || last - t != 1 || tail_v32(mg) || !tail_reg(mg)) return;
+  MgLevel& L = *mg.lv[t];
+  const int nc = mg.lv[last]->n;
+  if (L.bs != 3 || 3 * L.nagg * kTailLanes > kTailThreads || nc > kPackD * kTailLanes ||
+      nc * kTailLanes > kTailThreads || nc != mg.nc || 3 * L.nagg != nc)
+    return;
+  std::vector<int> cnt(L.nagg, 0);
+  for (int i = 0; i < L.nodes; ++i) ++cnt[L.h_agg[i]];
+  for (int a = 0; a < L.nagg; ++a)
+    if (L.bs * cnt[a] > kPackR * kTailLanes) return;
+  mg.tail_pd.alloc((size_t)kPdSlots * kTailThreads);
+  mg.tail_pi.alloc((size_t)kPiSlots * kTailThreads);
+  TailLevel<double> q{};
+  q.A = mats(L);
+  q.dinv = L.dinv.get();
+  q.mptr = L.mptr.get();
+  q.mem = L.mem.get();
+  q.agg = L.agg.get();
+  q.P = L.P.get();
+  q.nagg = L.nagg;
+  q.bs = L.bs;
+  hipLaunchKernelGGL(k_tail_pack, dim3(1), dim3(kTailThreads), 0, c.stream, q,
+                     (const double*)mg.cinv.get(), nc, mg.tail_pd.get(), mg.tail_pi.get());
+  HIP_CHECK(hipGetLastError());
+  mg.tail_packed = t;
+}
+
 void launch_tail(Context& c, int t, const double* rhs, double* out, const PcgScalars* sc) {
+  Multigrid& mg = c.mg;
+  if (mg.tail_packed == t) {
+    const MgLevel& L = *mg.lv[t];
+    const Cheb ch = cheb(L.lam);
+    const TailPack T{mg.tail_pd.get(), mg.tail_pi.get(), L.n, 3 * L.nagg * kTailLanes, mg.nc,
+                     ch.c1r, ch.c2d, ch.c2r, rhs, out};
+    const size_t bytes = sizeof(double) * (5 * (size_t)L.n + 2 * (size_t)mg.nc);
+    hipLaunchKernelGGL(k_tail1, dim3(1), dim3(kTailThreads), bytes, c.stream, T, sc);
+    return;
+  }
   if (tail_v32(c.mg))
     launch_tail_t<float>(c, t, rhs, out, sc);
   else
@@ -3046,6 +3287,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     for (int l = fine_smoother == 1 ? 0 : 1; l + 1 < nl; ++l) mg.lv[l]->lam = lam[l];
   }
   mg.tail_from = tail_start(mg);
+  tail_pack_build(c);
   if (mg.coarse_fp32) ensure_vals32(c);
   if (bound_dev) {
     // A good learned smoother has lambda_max(M A) close to 1 (1.14-1.16 on the plates it was
@@ -3110,6 +3352,12 @@ int64_t mg_level_info(Context& c, int level, int* n, int* nagg, double* lam) {
 
 int mg_levels(Context& c) { return c.mg.ready ? (int)c.mg.lv.size() : 0; }
 
+void mg_tail_info(Context& c, int* first_level, int* packed) {
+  if (!c.mg.ready) throw Error(CSTFEM_ERR_ORDER, "multilevel: not set up");
+  if (first_level) *first_level = c.mg.tail_from;
+  if (packed) *packed = c.mg.tail_packed >= 0 && c.mg.tail_packed == c.mg.tail_from ? 1 : 0;
+}
+
 // Switches the level-0 smoother of a hierarchy set up for the learned one.  The Chebyshev
 // smoother takes the K-cycle on every level with it: with V-cycles below level 3 its
 // iteration count on the heterogeneous plate depends on rounding (78-189 against 72).
@@ -3126,6 +3374,7 @@ void mg_set_fine_smoother(Context& c, int smoother) {
   }
   mg.fine_smoother = smoother;
   mg.tail_from = tail_start(mg);
+  tail_pack_build(c);
 }
 
 }  // namespace cstfem_hip

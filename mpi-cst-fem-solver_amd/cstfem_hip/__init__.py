@@ -164,6 +164,7 @@ SIGNATURES = {
     "cstfem_mg_prepare": (ctypes.c_int, [_P]),
     "cstfem_mg_levels": (ctypes.c_int, [_P, _I]),
     "cstfem_mg_level_info": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _I64, _D]),
+    "cstfem_mg_tail": (ctypes.c_int, [_P, _I, _I]),
     "cstfem_mg_download_aggregates": (ctypes.c_int, [_P, ctypes.c_int, _I, ctypes.c_int]),
     "cstfem_mg_download_matrix": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _D]),
     "cstfem_mg_apply": (ctypes.c_int, [_P, _D, _D, ctypes.c_int]),
@@ -520,6 +521,12 @@ class Device:
         _check(lib().cstfem_mg_level_info(self._h, int(level), ctypes.byref(n), ctypes.byref(na),
                                           ctypes.byref(nnz), ctypes.byref(lam)))
         return n.value, na.value, nnz.value, lam.value
+
+    def mg_tail(self):
+        """(first level of the fused single-workgroup tail or -1, packed operands 0/1)."""
+        first, packed = ctypes.c_int(), ctypes.c_int()
+        _check(lib().cstfem_mg_tail(self._h, ctypes.byref(first), ctypes.byref(packed)))
+        return first.value, packed.value
 
     def mg_download_aggregates(self, level):
         n = self.mg_level_info(level)[0]

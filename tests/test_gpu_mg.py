@@ -449,6 +449,44 @@ def test_fused_tail_matches_separate_kernels(dev, monkeypatch, kdepth):
     assert np.linalg.norm(z_kern) > 0
 
 
+@pytest.mark.parametrize("smoother", [1, 0])
+def test_packed_tail_is_bit_identical(dev, theta40, monkeypatch, smoother):
+    """A one-level tail over the dense coarsest (here 867 DOFs over 108, the shape of the
+    4M-DOF plate's 972 over 108) runs k_tail1 on operands packed at set-up; with the pack
+    switched off (CSTFEM_MG_TAIL_PACK=0) k_tail reads them level by level.  Same arithmetic in
+    the same order: one cycle and a whole flexible CG solve are bit-identical, and both match
+    the separate kernels (CSTFEM_MG_TAIL=0) up to summation order (FP64 values) or to the
+    rounding of the FP32 copies those kernels read with the learned smoother."""
+    system(dev, hetero(150))
+    if smoother == 0:
+        dev.mpnn_set_params(theta40)
+        dev.mpnn_build()
+    r = np.random.default_rng(12).standard_normal(dev.n)
+    out = {}
+    for pack, tail in (("1", "1"), ("0", "1"), ("1", "0")):
+        monkeypatch.setenv("CSTFEM_MG_TAIL_PACK", pack)
+        monkeypatch.setenv("CSTFEM_MG_TAIL", tail)
+        dev.mg_setup(8, 120, smoother, 1)
+        sizes = [dev.mg_level_info(l)[0] for l in range(dev.mg_levels())]
+        assert sizes[-2:] == [867, 108], sizes
+        first, packed = dev.mg_tail()
+        assert (first, packed) == ((2, 1) if pack + tail == "11" else (2, 0) if tail == "1" else (-1, 0))
+        z = dev.mg_apply(r)
+        dev.mg_apply(r)  # the second call reuses the pack
+        x, s = dev.solve(rtol=1e-10, max_iter=500, method=FCG)
+        assert s.converged and s.true_rel_residual < 1e-10 and not s.fallback
+        out[pack + tail] = (z, x, s.iterations)
+    np.testing.assert_array_equal(out["11"][0], out["01"][0])
+    np.testing.assert_array_equal(out["11"][1], out["01"][1])
+    assert out["11"][2] == out["01"][2]
+    # the separate kernels read the FP32 copies of the tail levels' values where the learned
+    # smoother's cycle keeps them (the fused tails read FP64): relative rounding 2^-24 per entry
+    z_sep = out["10"][0]
+    tol = 1e-12 if smoother == 1 else 1e-6
+    assert np.linalg.norm(out["11"][0] - z_sep) <= tol * np.linalg.norm(z_sep)
+    assert abs(out["11"][2] - out["10"][2]) <= 1
+
+
 @pytest.mark.parametrize("smoother,kdepth", [(0, 3), (0, 1), (1, 3)])
 def test_fused_cycle_kernels_are_bit_identical(dev, theta40, monkeypatch, smoother, kdepth):
     """The fused cycle kernels (K-cycle updates folded into their readers, CSTFEM_MG_FOLD;
