@@ -474,6 +474,13 @@ def main():
                     help="coarse levels on the K-cycle (Chebyshev fine smoother)")
     ap.add_argument("--train-n", type=int, default=64, help="training plate nodes per side")
     ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--train-lr", type=float, default=1e-3, help="Adam step size")
+    ap.add_argument("--train-lr-final", type=float, default=None,
+                    help="step size at the end of a half-cosine decay (constant when absent)")
+    ap.add_argument("--train-pairs", type=int, default=32, help="white-noise pairs per set")
+    ap.add_argument("--train-seed", type=int, default=0, help="network and pair seed")
+    ap.add_argument("--train-resample", type=int, default=0,
+                    help="draw a new set of pairs every this many steps (0: one set)")
     ap.add_argument("--h-plates", type=int, default=2000, help="--config 4: training plates")
     ap.add_argument("--h-chunk", type=int, default=250, help="--config 4: plates per batch")
     ap.add_argument("--h-cells", type=int, default=128, help="--config 4: cells per side")
@@ -539,9 +546,14 @@ def main():
         with cstfem_hip.Device(0) as tdev:
             theta, losses = learned.train(tdev, learned.training_problem(args.config,
                                                                          args.train_n),
-                                          steps=args.train_steps, log=log)
+                                          steps=args.train_steps, log=log,
+                                          lr=args.train_lr, lr_final=args.train_lr_final,
+                                          pairs=args.train_pairs, resample=args.train_resample,
+                                          seed=args.train_seed)
         train_info = dict(train_s=time.perf_counter() - t0, plate_nodes=args.train_n ** 2,
-                          steps=args.train_steps, loss_first=losses[0], loss_last=losses[-1])
+                          steps=args.train_steps, lr=args.train_lr, lr_final=args.train_lr_final,
+                          pairs=args.train_pairs, resample=args.train_resample,
+                          seed=args.train_seed, loss_first=losses[0], loss_last=losses[-1])
         log(f"trained the preconditioner in {train_info['train_s']:.2f} s "
             f"(loss {losses[0]:.4f} -> {losses[-1]:.4f})")
 

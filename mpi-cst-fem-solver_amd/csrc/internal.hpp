@@ -109,10 +109,19 @@ struct PcgScalars {
   int best_k[2];   // flexible CG: iteration of best_rel, per parity
   int stall_window;
   int pad1;
+  // flexible CG's rate test (slow_window > 0): every slow_window iterations the best
+  // ||r||/||b|| must have dropped to slow_factor times its value at the previous check
+  // (slow_rel, slow_k: that value and iteration, per parity)
+  double slow_rel[2];
+  int slow_k[2];
+  int slow_window;
+  int pad2;
+  double slow_factor;
 };
 
 // kRefine: the FP32 iteration asks the host for an FP64 true-residual refresh.
-// kStalled: flexible CG made no progress for stall_window iterations.
+// kStalled: flexible CG made no progress for stall_window iterations, or too little over
+// slow_window iterations (rate test).
 enum { kRunning = 0, kConverged = 1, kBreakdown = 2, kMaxIter = 3, kRefine = 4, kStalled = 5 };
 
 constexpr int kMpnnRounds = 3;  // message-passing rounds of the learned preconditioner

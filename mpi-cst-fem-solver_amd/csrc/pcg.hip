@@ -1143,6 +1143,44 @@ __global__ __launch_bounds__(kThreads) void k_fcg_update(
   if (threadIdx.x == 0) partB[blockIdx.x] = t;
 }
 
+// The tests of flexible CG iteration k (parity cur) on its recursive ||r||/||b|| rel:
+// refinement (rel below rtol, or below delta times the last refresh's), max_iter, and stall
+// (NaN; no new best for stall_window iterations; or, with slow_window > 0 and far from rtol,
+// the best not down to slow_factor times its value slow_window iterations before).  Reads
+// the previous iteration's records (other parity slot: stable during the kernel); `writer`
+// carries them forward into slot cur and posts a stop.  Returns the status.
+__device__ inline int fcg_test_status(PcgScalars* __restrict__ sc, int cur, double rel, int k,
+                                      bool writer) {
+  const bool better = rel < sc->best_rel[cur ^ 1];
+  const double best = better ? rel : sc->best_rel[cur ^ 1];
+  const int best_k = better ? k : sc->best_k[cur ^ 1];
+  double slow_rel = sc->slow_rel[cur ^ 1];
+  int slow_k = sc->slow_k[cur ^ 1];
+  bool slow = false;
+  if (sc->slow_window > 0 && k - slow_k >= sc->slow_window) {
+    slow = !(best <= sc->slow_factor * slow_rel) && best > 1e3 * sc->rtol;
+    slow_rel = best;
+    slow_k = k;
+  }
+  int status = kRunning;
+  if (rel < sc->rtol || rel < sc->delta * sc->rel_ref)
+    status = kRefine;
+  else if (k >= sc->max_iter)
+    status = kMaxIter;
+  else if (!(rel == rel) || k - best_k > sc->stall_window || slow)
+    status = kStalled;
+  if (writer) {
+    sc->k_in[cur] = k;
+    sc->rel = rel;
+    sc->best_rel[cur] = best;
+    sc->best_k[cur] = best_k;
+    sc->slow_rel[cur] = slow_rel;
+    sc->slow_k[cur] = slow_k;
+    if (status != kRunning) sc->status = status;
+  }
+  return status;
+}
+
 // test: 1 = convergence / refinement / stall test first; 0 = preconditioner application only
 template <class P>
 __global__ __launch_bounds__(kThreads) void k_fcg_precond(
@@ -1155,25 +1193,7 @@ __global__ __launch_bounds__(kThreads) void k_fcg_precond(
     const double rr = reduce_partials(partB, nB, 1, lds);
     const int k = sc->k_in[cur ^ 1] + 1;
     const double rel = sqrt(rr) / sc->bnorm;
-    // progress record of the previous iteration (other parity slot: stable during this kernel)
-    const bool better = rel < sc->best_rel[cur ^ 1];
-    const double best = better ? rel : sc->best_rel[cur ^ 1];
-    const int best_k = better ? k : sc->best_k[cur ^ 1];
-    int status = kRunning;
-    if (rel < sc->rtol || rel < sc->delta * sc->rel_ref)
-      status = kRefine;
-    else if (k >= sc->max_iter)
-      status = kMaxIter;
-    else if (!(rel == rel) || k - best_k > sc->stall_window)
-      status = kStalled;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      sc->k_in[cur] = k;
-      sc->rel = rel;
-      sc->best_rel[cur] = best;
-      sc->best_k[cur] = best_k;
-      if (status != kRunning) sc->status = status;
-    }
-    if (status != kRunning) return;
+    if (fcg_test_status(sc, cur, rel, k, blockIdx.x == 0 && threadIdx.x == 0) != kRunning) return;
   }
   const int lane = threadIdx.x & 63;
   const SliceRange sr = slice_range(G.nslices);
@@ -1201,23 +1221,7 @@ __global__ __launch_bounds__(kThreads) void k_fcg_test(const double* __restrict_
   const double rr = reduce_partials(partB, nB, 1, lds);
   const int k = sc->k_in[cur ^ 1] + 1;
   const double rel = sqrt(rr) / sc->bnorm;
-  const bool better = rel < sc->best_rel[cur ^ 1];
-  const double best = better ? rel : sc->best_rel[cur ^ 1];
-  const int best_k = better ? k : sc->best_k[cur ^ 1];
-  int status = kRunning;
-  if (rel < sc->rtol || rel < sc->delta * sc->rel_ref)
-    status = kRefine;
-  else if (k >= sc->max_iter)
-    status = kMaxIter;
-  else if (!(rel == rel) || k - best_k > sc->stall_window)
-    status = kStalled;
-  if (threadIdx.x == 0) {
-    sc->k_in[cur] = k;
-    sc->rel = rel;
-    sc->best_rel[cur] = best;
-    sc->best_k[cur] = best_k;
-    if (status != kRunning) sc->status = status;
-  }
+  fcg_test_status(sc, cur, rel, k, threadIdx.x == 0);
 }
 
 // partials of z'q
@@ -1411,6 +1415,12 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
   // the multilevel preconditioner gains an order of magnitude every 10-20 iterations: 200
   // iterations without a new best residual are a stall
   init.stall_window = multilevel ? 200 : std::max(1000, o.max_iter / 20);
+  // and must at least halve its best residual every 50 iterations: a learned smoother that
+  // passed the set-up check but leaves some modes alone shows as a plateau whose small
+  // fluctuations keep setting new bests
+  init.slow_window = multilevel ? 50 : 0;
+  init.slow_factor = 0.5;
+  init.slow_rel[0] = init.slow_rel[1] = 1.0;
   HIP_CHECK(hipMemcpyAsync(c.scal.get(), &init, sizeof init, hipMemcpyHostToDevice, c.stream));
   hipEvent_t ev0, ev1;
   HIP_CHECK(hipEventCreate(&ev0));
@@ -1476,6 +1486,8 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
       upd.rel_ref = true_rel;
       upd.best_rel[0] = upd.best_rel[1] = true_rel;  // progress is measured from the refresh
       upd.best_k[0] = upd.best_k[1] = k;
+      upd.slow_rel[0] = upd.slow_rel[1] = true_rel;
+      upd.slow_k[0] = upd.slow_k[1] = k;
       HIP_CHECK(hipMemcpyAsync(c.scal.get(), &upd, sizeof upd, hipMemcpyHostToDevice, c.stream));
       precond((k - 1) & 1, 0);  // z = M r with the refreshed residual, z'q against the last q
       direction(0);             // p = z + beta p: the direction is kept
@@ -1499,7 +1511,10 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
     HIP_CHECK(hipMemsetAsync(c.x.get(), 0, sizeof(double) * nv, c.stream));
     return false;
   };
-  if (!converged && !zero_rhs && multilevel && c.mg.fine_smoother == 0) {
+  // a stop at max_iter is not a failure of the preconditioner: no fallback then (the
+  // fallbacks are for stalls and breakdowns, with iterations left)
+  const bool spent = last_status == kMaxIter || iters >= o.max_iter;
+  if (!converged && !zero_rhs && !spent && multilevel && c.mg.fine_smoother == 0) {
     // the learned smoother failed on this matrix: the same hierarchy with the Chebyshev
     // smoother, from the current x (its own stall falls back to Jacobi-CG)
     cstfem_solver_options fo = o;
@@ -1521,7 +1536,7 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
     last_status = converged ? kConverged : kMaxIter;
     fallback = true;
     fb_kind = fs.fallback ? fs.fallback : 2;
-  } else if (!converged) {
+  } else if (!converged && !spent) {
     // breakdown or stall of the learned preconditioner: Jacobi-CG (FP32 + FP64 refinement)
     // from the current x; the FP64 true residual decides
     fallback = true;

@@ -49,11 +49,15 @@ def white_pairs(dev, count: int, seed: int):
 
 
 def train(dev, problem, steps: int = 300, lr: float = 1e-3, pairs: int = 32, seed: int = 0,
-          theta0: np.ndarray | None = None, log=None):
+          theta0: np.ndarray | None = None, log=None, lr_final: float | None = None,
+          resample: int = 0):
     """Trains on one assembled and constrained plate (problem: a pipeline.Problem).
 
     Loss (device): (1/K) sum_k |M r_k - e_k|_K^2 / |e_k|_K^2 over white-noise pairs; Adam.
+    lr_final: the step size decays from lr to lr_final along a half cosine (constant lr when
+    None); resample > 0: a new set of pairs every `resample` steps (Adam's state carries on).
     Returns (theta, losses)."""
+    import math
     from . import pipeline
     pipeline.load_problem(dev, problem)
     pipeline.assemble_and_constrain(dev, problem)
@@ -64,7 +68,12 @@ def train(dev, problem, steps: int = 300, lr: float = 1e-3, pairs: int = 32, see
     dev.mpnn_set_pairs(R, E)
     losses = []
     for it in range(steps):
-        losses.append(dev.mpnn_train_step(lr))
+        if resample > 0 and it > 0 and it % resample == 0:
+            R, E = white_pairs(dev, pairs, seed + 1 + it // resample)
+            dev.mpnn_set_pairs(R, E)
+        step_lr = lr if lr_final is None else (
+            lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * it / steps)))
+        losses.append(dev.mpnn_train_step(step_lr))
         if log is not None and (it % 100 == 0 or it == steps - 1):
             log(f"train step {it}: loss {losses[-1]:.5f}")
     return dev.mpnn_get_params(), losses

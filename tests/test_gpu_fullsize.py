@@ -102,3 +102,34 @@ def test_full_size_system_bitwise_and_solutions(dev, oracle, config):
     assert s2.converged and not s2.fallback and s2.true_rel_residual < 1e-10
     assert s2.iterations * 100 < s.iterations
     assert true_rel(r, x2) < 1e-10
+
+
+def test_learned_plateau_hands_over_early(dev):
+    """A network that passes the set-up check (lambda_max(M K) 1.15) and still fails: trained
+    on a 160x160-node plate for 2000 steps, it takes the 4M-DOF plate's residual to ~5e-5 in
+    100 iterations and then plateaus, each fluctuation setting a slightly new best.  The rate
+    test (the best residual must halve every 50 iterations) hands over to the Chebyshev cycle
+    of the same hierarchy: 182 iterations in all, where the 200-iteration stall window alone
+    took 1,148.  Training is deterministic, so the plateau reproduces."""
+    tdev = cstfem_hip.Device(0)
+    try:
+        theta, _ = learned.train(tdev, learned.training_problem(2, 160), steps=2000)
+    finally:
+        tdev.close()
+    prob = pipeline.config_problem(2)
+    pipeline.load_problem(dev, prob)
+    pipeline.assemble_and_constrain(dev, prob, multilevel=True)
+    dev.setup()
+    dev.mpnn_set_params(theta)
+    dev.mpnn_build()
+    dev.mg_setup(12, 500, 0, 3)
+    assert dev.mg_level_info(0)[3] <= 1.5  # passes the set-up check
+    x, s = dev.solve(rtol=1e-10, max_iter=3000, method=cstfem_hip.METHOD_FCG_LEARNED)
+    print(f"plateauing network: {s.iterations} iterations, fallback {s.fallback}", flush=True)
+    assert s.fallback == 2, "this network no longer plateaus: choose another for this test"
+    assert s.converged and s.true_rel_residual < 1e-10
+    assert s.iterations < 300, s.iterations
+    # a stop at max_iter is not a stall: no fallback, the reference's message
+    x, s = dev.solve(rtol=1e-10, max_iter=100, method=cstfem_hip.METHOD_FCG_LEARNED)
+    assert not s.converged and s.fallback == 0 and s.iterations == 100
+    assert s.message.startswith("NOT converged: reached max_iter = 100"), s.message
