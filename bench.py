@@ -366,7 +366,8 @@ def run_protocol(args):
     smoother the set-up check rejects); the network then runs as the fine smoother of the
     multilevel flexible CG (FP32 values in the cycle, FP64 outer iteration) on the 4M-DOF
     heterogeneous plate.  Beside it: the same hierarchy with the default network (white
-    noise on one 64x64-node plate of the family) and FP64 Jacobi-CG."""
+    noise on one plate of the family, --train-n nodes per side, --train-steps Adam steps)
+    and FP64 Jacobi-CG."""
     import cstfem_hip
     from cstfem_hip import learned, pipeline
     t0 = time.perf_counter()
@@ -472,8 +473,8 @@ def main():
                     help="coarse levels on the K-cycle (learned fine smoother)")
     ap.add_argument("--mg-kcycle-cheb", type=int, default=16,
                     help="coarse levels on the K-cycle (Chebyshev fine smoother)")
-    ap.add_argument("--train-n", type=int, default=64, help="training plate nodes per side")
-    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--train-n", type=int, default=128, help="training plate nodes per side")
+    ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--train-lr", type=float, default=1e-3, help="Adam step size")
     ap.add_argument("--train-lr-final", type=float, default=None,
                     help="step size at the end of a half-cosine decay (constant when absent)")

@@ -24,10 +24,26 @@ void hip_check(hipError_t e, const char* what, const char* file, int line) {
                                        what + " (" + file + ":" + std::to_string(line) + ")");
 }
 
+unsigned char* Context::stage(size_t bytes) {
+  if (bytes > h_stage_n) {
+    if (h_stage) {
+      HIP_CHECK(hipStreamSynchronize(stream));
+      (void)hipHostFree(h_stage);
+      h_stage = nullptr;
+      h_stage_n = 0;
+    }
+    const size_t n = std::max<size_t>(bytes, 64 * 1024);
+    HIP_CHECK(hipHostMalloc((void**)&h_stage, n, hipHostMallocDefault));
+    h_stage_n = n;
+  }
+  return h_stage;
+}
+
 Context::~Context() {
   mg.join_prepared();  // its download runs on the auxiliary stream
   if (stream) (void)hipStreamSynchronize(stream);
   drop_batch_graph();
+  if (h_stage) (void)hipHostFree(h_stage);
   if (h_scal) (void)hipHostFree(h_scal);
   if (aux) (void)hipStreamDestroy(aux);
   if (stream) (void)hipStreamDestroy(stream);

@@ -307,7 +307,7 @@ void collect_dofs(int n, int m, const int* dofs, const double* values, std::vect
 
 // enqueues the elimination of the sorted constrained DOFs d (values v), whose copies are
 // already in w.dofs / w.vals on the library stream, and returns the counters
-void run_elimination(Context& c, int m, const DirWork& w, int out_ctr[3]) {
+void run_elimination(Context& c, int m, const DirWork& w, int out_ctr[3], bool staged) {
   const int n = c.n;
   const int all_rows = c.pattern_symmetric ? 0 : 1;
   if (!c.fixed_valid) {  // union of the constrained DOFs since the last assembly
@@ -338,8 +338,26 @@ void run_elimination(Context& c, int m, const DirWork& w, int out_ctr[3]) {
                        w.row_bits, w.row_flag);
     HIP_CHECK(hipGetLastError());
   }
-  HIP_CHECK(hipMemcpyAsync(out_ctr, w.ctr, sizeof(int) * 3, hipMemcpyDeviceToHost, c.stream));
+  // the counters through the page-locked staging, or (staged = false: the allocation-free
+  // entry point, which may not create the staging) straight into out_ctr
+  int* h_ctr = staged ? reinterpret_cast<int*>(c.stage(sizeof(int) * 3)) : out_ctr;
+  HIP_CHECK(hipMemcpyAsync(h_ctr, w.ctr, sizeof(int) * 3, hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (staged) std::copy(h_ctr, h_ctr + 3, out_ctr);
+}
+
+// the sorted DOFs and their values into w.dofs / w.vals, through the context's page-locked
+// staging (values first: 8-byte aligned)
+void upload_list(Context& c, const std::vector<int>& d, const std::vector<double>& v,
+                 const DirWork& w) {
+  const size_t mu = d.size();
+  unsigned char* h = c.stage(mu * (sizeof(double) + sizeof(int)));
+  double* hv = reinterpret_cast<double*>(h);
+  int* hd = reinterpret_cast<int*>(h + mu * sizeof(double));
+  std::copy(v.begin(), v.end(), hv);
+  std::copy(d.begin(), d.end(), hd);
+  HIP_CHECK(hipMemcpyAsync(w.vals, hv, sizeof(double) * mu, hipMemcpyHostToDevice, c.stream));
+  HIP_CHECK(hipMemcpyAsync(w.dofs, hd, sizeof(int) * mu, hipMemcpyHostToDevice, c.stream));
 }
 
 void check_preconditions(Context& c) {
@@ -390,12 +408,9 @@ void apply_dirichlet_device(Context& c, int m, const int* dofs, const double* va
   w.rows = grow(c.dir_rows, (size_t)c.nb);
   w.cap = c.nb;  // every block row fits: the list never overflows
   w.uval = grow(c.dir_uval, (size_t)c.n);  // read only where this call set the DOF's bit
-  if (mu > 0) {
-    HIP_CHECK(hipMemcpyAsync(w.dofs, d.data(), sizeof(int) * mu, hipMemcpyHostToDevice, c.stream));
-    HIP_CHECK(hipMemcpyAsync(w.vals, v.data(), sizeof(double) * mu, hipMemcpyHostToDevice, c.stream));
-  }
+  if (mu > 0) upload_list(c, d, v, w);
   int ctr[3];
-  run_elimination(c, mu, w, ctr);
+  run_elimination(c, mu, w, ctr, true);
   finish(c, ctr);
 }
 
@@ -454,12 +469,12 @@ void apply_dirichlet_noalloc(Context& c, int m, const int* dofs, const double* v
   w.cap = na::kMaxRows;
   w.uval = nullptr;      // no room for n values: binary search in the sorted list
   w.row_flag = nullptr;  // nor for a word per block row: the block-row bitmap
-  if (mu > 0) {
+  if (mu > 0) {  // pageable copies: the page-locked staging would be an allocation
     HIP_CHECK(hipMemcpyAsync(w.dofs, d.data(), sizeof(int) * mu, hipMemcpyHostToDevice, c.stream));
     HIP_CHECK(hipMemcpyAsync(w.vals, v.data(), sizeof(double) * mu, hipMemcpyHostToDevice, c.stream));
   }
   int ctr[3];
-  run_elimination(c, mu, w, ctr);
+  run_elimination(c, mu, w, ctr, false);
   if (ctr[1])
     throw Error(CSTFEM_ERR_ARGUMENT,
                 "apply_dirichlet_noalloc: the elimination visits more than " +

@@ -385,6 +385,7 @@ struct Context {
   // tractions at the Gauss points, grown on demand and kept
   DBuf<int> trac_i;
   DBuf<double> trac_d;
+  DBuf<double> body_w;  // element weights t*|A|/3 of the body force
 
   // reduction scratch
   DBuf<double> partials;   // 4 * max_blocks
@@ -392,6 +393,13 @@ struct Context {
   PcgScalars* h_scal = nullptr;  // pinned
   DBuf<int> err_flag;            // [0] = min bad index (INT_MAX = none)
   int red_blocks = 0;            // grid used by reduction kernels
+  // page-locked staging of the small host<->device transfers of loads and Dirichlet (lists,
+  // counters): copies from pageable memory cost the host a staging copy and a wait each.
+  // Grown on demand; every user synchronises the stream before it returns, so the next
+  // user finds no copy in flight.
+  unsigned char* h_stage = nullptr;
+  size_t h_stage_n = 0;
+  unsigned char* stage(size_t bytes);
 
   ~Context();
 };

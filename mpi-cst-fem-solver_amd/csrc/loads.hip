@@ -97,22 +97,39 @@ __global__ void k_trac_sum(int nnodes, const int* __restrict__ node, const int* 
   f[2 * v + 1] = fy;
 }
 
-// t*|A|/3*(bx, by) to each node of every element, per node in ascending element order
+// The body force t*|A|/3*(bx, by) on each node of every element, per node in ascending
+// element order, in two passes: the weight t*|A|/3 of every element (coalesced over the
+// elements), then per node the sum over its element list (the same products in the same
+// order as one pass per node, which would walk a chain list -> triangle -> coordinates for
+// each of its elements)
+__global__ void k_body_weight(int64_t ne, const int* __restrict__ tri, const double* __restrict__ xy,
+                              double t, double* __restrict__ w) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int n0 = tri[3 * e], n1 = tri[3 * e + 1], n2 = tri[3 * e + 2];
+  const double x1 = xy[2 * n0], y1 = xy[2 * n0 + 1], x2 = xy[2 * n1], y2 = xy[2 * n1 + 1];
+  const double x3 = xy[2 * n2], y3 = xy[2 * n2 + 1];
+  const double area = 0.5 * ((x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1));
+  w[e] = t * fabs(area) / 3.0;
+}
+
 __global__ void k_body_force(int nn, const int* __restrict__ ptr, const int* __restrict__ list,
-                             const int* __restrict__ tri, const double* __restrict__ xy,
-                             double t, double bx, double by, double* __restrict__ f) {
+                             const double* __restrict__ w, double bx, double by,
+                             double* __restrict__ f) {
   const int nd = blockIdx.x * blockDim.x + threadIdx.x;
   if (nd >= nn) return;
+  const int k0 = ptr[nd], k1 = ptr[nd + 1];
   double fx = f[2 * nd], fy = f[2 * nd + 1];
-  for (int k = ptr[nd]; k < ptr[nd + 1]; ++k) {
-    const int e = list[k];
-    const int n0 = tri[3 * e], n1 = tri[3 * e + 1], n2 = tri[3 * e + 2];
-    const double x1 = xy[2 * n0], y1 = xy[2 * n0 + 1], x2 = xy[2 * n1], y2 = xy[2 * n1 + 1];
-    const double x3 = xy[2 * n2], y3 = xy[2 * n2 + 1];
-    const double area = 0.5 * ((x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1));
-    const double w = t * fabs(area) / 3.0;
-    fx += w * bx;
-    fy += w * by;
+  for (int k = k0; k < k1; k += 4) {
+    double we[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) we[u] = k + u < k1 ? w[list[k + u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k + u < k1) {
+        fx += we[u] * bx;
+        fy += we[u] * by;
+      }
   }
   f[2 * nd] = fx;
   f[2 * nd + 1] = fy;
@@ -167,24 +184,33 @@ void add_traction_device(Context& c, int nedge, const int* edges, const Traction
   // packed index data: edges[2E] | count[E] | node[U] | ptr[U+1] | ent[2E]
   const size_t o_cnt = 2 * (size_t)nedge, o_node = o_cnt + nedge, o_ptr = o_node + U,
                o_ent = o_ptr + U + 1, total = o_ent + 2 * (size_t)nedge;
-  std::vector<int> pack(total);
-  std::copy(edges, edges + 2 * (size_t)nedge, pack.begin());
-  std::copy(node.begin(), node.end(), pack.begin() + o_node);
-  int* ptr = pack.data() + o_ptr;
+  // host side in the context's page-locked staging: the function traction's Gauss points and
+  // values (doubles first), then the packed index data and two flag words
+  const bool host_fn = src.kind != TractionSource::kConstant;
+  const size_t nd_h = host_fn ? (size_t)(kGeo + 4) * nedge : 0;
+  unsigned char* stage = c.stage(sizeof(double) * nd_h + sizeof(int) * (total + 2));
+  double* h_geo = reinterpret_cast<double*>(stage);
+  double* h_t = h_geo + (host_fn ? (size_t)kGeo * nedge : 0);
+  int* pack = reinterpret_cast<int*>(stage + sizeof(double) * nd_h);
+  int* h_flag = pack + total;  // [0] = INT_MAX (upload), [1] = the flag read back
+  std::fill(pack, pack + total, 0);
+  std::copy(edges, edges + 2 * (size_t)nedge, pack);
+  std::copy(node.begin(), node.end(), pack + o_node);
+  int* ptr = pack + o_ptr;
   ptr[0] = 0;
   for (int u = 0; u < U; ++u) ptr[u + 1] = ptr[u] + cnt[u];
   std::vector<int> fill(ptr, ptr + U);
   for (int k = 0; k < 2 * nedge; ++k) pack[o_ent + fill[slot[edges[k]]]++] = k;  // 2k + role
   int* di = grow(c.trac_i, total);
-  const bool host_fn = src.kind != TractionSource::kConstant;
   double* dd = grow(c.trac_d, (size_t)nedge * (kGeo + (host_fn ? 4 : 0)));
   double* geo = dd;
   double* tv = host_fn ? dd + (size_t)kGeo * nedge : nullptr;
   int* err = c.err_flag.get();
   const int big = INT_MAX;
   const double g = 1.0 / std::sqrt(3.0);
-  HIP_CHECK(hipMemcpyAsync(di, pack.data(), sizeof(int) * total, hipMemcpyHostToDevice, c.stream));
-  HIP_CHECK(hipMemcpyAsync(err, &big, sizeof(int), hipMemcpyHostToDevice, c.stream));
+  h_flag[0] = big;
+  HIP_CHECK(hipMemcpyAsync(di, pack, sizeof(int) * total, hipMemcpyHostToDevice, c.stream));
+  HIP_CHECK(hipMemcpyAsync(err, h_flag, sizeof(int), hipMemcpyHostToDevice, c.stream));
   hipLaunchKernelGGL(k_trac_edges, dim3(grid_for(nedge, 128, 0)), dim3(128), 0, c.stream, nedge,
                      di, c.ne_ptr.get(), c.ne_list.get(), c.tri.get(), c.xy.get(), g, geo,
                      di + o_cnt, err);
@@ -201,12 +227,12 @@ void add_traction_device(Context& c, int nedge, const int* edges, const Traction
   };
   if (host_fn) {
     // the traction function runs on the host at the Gauss points the device computed
-    std::vector<double> h_geo((size_t)kGeo * nedge), pts(8 * (size_t)nedge), t(4 * (size_t)nedge);
-    int bad = INT_MAX;
-    HIP_CHECK(hipMemcpyAsync(h_geo.data(), geo, sizeof(double) * h_geo.size(), hipMemcpyDeviceToHost, c.stream));
-    HIP_CHECK(hipMemcpyAsync(&bad, err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    std::vector<double> pts(8 * (size_t)nedge);
+    double* t = h_t;
+    HIP_CHECK(hipMemcpyAsync(h_geo, geo, sizeof(double) * kGeo * nedge, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(h_flag + 1, err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     HIP_CHECK(hipStreamSynchronize(c.stream));
-    check_edges(bad);
+    check_edges(h_flag[1]);
     for (int k = 0; k < nedge; ++k) {
       const double* o = &h_geo[(size_t)kGeo * k];
       for (int qi = 0; qi < 2; ++qi) {
@@ -222,28 +248,29 @@ void add_traction_device(Context& c, int nedge, const int* edges, const Traction
         kirsch_traction(src.p0, src.p1, pts[4 * j], pts[4 * j + 1], pts[4 * j + 2],
                         pts[4 * j + 3], &t[2 * j]);
     } else {
-      src.fn(src.user, 2 * nedge, pts.data(), t.data());
+      src.fn(src.user, 2 * nedge, pts.data(), t);
     }
-    HIP_CHECK(hipMemcpyAsync(tv, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(tv, t, sizeof(double) * 4 * nedge, hipMemcpyHostToDevice, c.stream));
   }
   hipLaunchKernelGGL(k_trac_sum, dim3(grid_for(U, 128, 0)), dim3(128), 0, c.stream, U,
                      di + o_node, di + o_ptr, di + o_ent, geo, (const double*)tv, src.p0, src.p1,
                      c.thick, g, err, c.b.get());
   HIP_CHECK(hipGetLastError());
-  int bad = INT_MAX;
-  HIP_CHECK(hipMemcpyAsync(&bad, err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipMemcpyAsync(h_flag + 1, err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
-  check_edges(bad);
+  check_edges(h_flag[1]);
 }
 
 void add_body_force_device(Context& c, double bx, double by) {
   if (c.ne_ptr.n == 0) build_node_elements(c);
   if (c.b.n == 0) throw Error(CSTFEM_ERR_ORDER, "add_body_force: set the right-hand side first");
+  double* w = grow(c.body_w, (size_t)std::max<int64_t>(c.ne, 1));
+  if (c.ne > 0)
+    hipLaunchKernelGGL(k_body_weight, dim3(grid_for(c.ne, 256, 0)), dim3(256), 0, c.stream, c.ne,
+                       c.tri.get(), c.xy.get(), c.thick, w);
   hipLaunchKernelGGL(k_body_force, dim3(grid_for(c.nn, 256, 0)), dim3(256), 0, c.stream, c.nn,
-                     c.ne_ptr.get(), c.ne_list.get(), c.tri.get(), c.xy.get(), c.thick, bx, by,
-                     c.b.get());
-  HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+                     c.ne_ptr.get(), c.ne_list.get(), (const double*)w, bx, by, c.b.get());
+  HIP_CHECK(hipGetLastError());  // nothing left to report: no wait for the kernel
 }
 
 }  // namespace cstfem_hip
