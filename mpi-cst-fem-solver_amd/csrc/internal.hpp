@@ -236,6 +236,17 @@ struct Multigrid {
   hipEvent_t lam_ev = nullptr;     // a level's bound starts after its matrix
   int* pinned = nullptr;  // page-locked staging of the fine node graph, kept across set-ups
   size_t pinned_n = 0;
+  // page-locked copy of the coarse block pattern just built (the next level's host
+  // aggregation reads it once pat_ev has passed), and the event after each aggregate-map
+  // upload from `pinned` (the next level overwrites the staging after it)
+  int* pinned_pat = nullptr;
+  size_t pinned_pat_n = 0;
+  hipEvent_t pat_ev = nullptr;
+  hipEvent_t map_ev = nullptr;
+  // the prepared aggregation's column download in pieces (one event each): its root pass
+  // starts once the row pointers are in and waits for each piece when it reaches it
+  static constexpr int kPrePieces = 8;
+  hipEvent_t pre_piece[kPrePieces] = {};
   // level-0 aggregation started ahead of the set-up (mg_prepare): a worker thread waits for
   // the node graph's download (auxiliary stream, event pre_done), runs the greedy pass into
   // pre_agg and stages the map for upload; the set-up joins it.  Uses `pinned` until joined.
@@ -271,6 +282,11 @@ struct Multigrid {
     clear();
     drop_spare();
     if (pinned) (void)hipHostFree(pinned);
+    if (pinned_pat) (void)hipHostFree(pinned_pat);
+    if (pat_ev) (void)hipEventDestroy(pat_ev);
+    for (hipEvent_t& e : pre_piece)
+      if (e) (void)hipEventDestroy(e);
+    if (map_ev) (void)hipEventDestroy(map_ev);
     if (pre_queued) (void)hipEventDestroy(pre_queued);
     if (pre_done) (void)hipEventDestroy(pre_done);
     if (bound_ev) (void)hipEventDestroy(bound_ev);

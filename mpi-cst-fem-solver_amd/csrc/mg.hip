@@ -27,6 +27,7 @@
 //              flexible CG steps preconditioned by the cycle of level l+1) while l+1 <= kdepth,
 //              and one cycle of level l+1 (V-cycle) below that.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <climits>
@@ -34,6 +35,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <type_traits>
 
@@ -2245,6 +2247,19 @@ int* pinned_ints(Multigrid& mg, size_t count) {
   return mg.pinned;
 }
 
+// the page-locked buffer of the coarse pattern's download (no copy into it is in flight when
+// it grows: the set-up waits for each download before the next level's)
+int* pinned_pattern(Multigrid& mg, size_t count) {
+  if (count > mg.pinned_pat_n) {
+    if (mg.pinned_pat) HIP_CHECK(hipHostFree(mg.pinned_pat));
+    mg.pinned_pat = nullptr;
+    mg.pinned_pat_n = 0;
+    HIP_CHECK(hipHostMalloc((void**)&mg.pinned_pat, count * sizeof(int), hipHostMallocDefault));
+    mg.pinned_pat_n = count;
+  }
+  return mg.pinned_pat;
+}
+
 // a workspace of at least `count` elements (reallocated only to grow)
 template <class T>
 T* grow(DBuf<T>& b, size_t count) {
@@ -2974,23 +2989,45 @@ void mg_prepare(Context& c) {
   // the bandwidth, the uploads of an earlier set-up from the same staging)
   HIP_CHECK(hipEventRecord(mg.pre_queued, c.stream));
   HIP_CHECK(hipStreamWaitEvent(c.aux, mg.pre_queued, 0));
+  // row pointers and bandwidth first (the worker starts on them), then the columns in pieces
   HIP_CHECK(hipMemcpyAsync(pin, c.brow_ptr.get(), sizeof(int) * (c.nb + 1), hipMemcpyDeviceToHost, c.aux));
-  HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1, c.bcol.get(), sizeof(int) * c.nblk, hipMemcpyDeviceToHost, c.aux));
   HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1 + c.nblk, bw, sizeof(int), hipMemcpyDeviceToHost, c.aux));
   HIP_CHECK(hipEventRecord(mg.pre_done, c.aux));
+  constexpr int K = Multigrid::kPrePieces;
+  int64_t piece_end[K];
+  for (int p = 0; p < K; ++p) {
+    const int64_t lo = c.nblk * p / K, hi = c.nblk * (p + 1) / K;
+    piece_end[p] = hi;
+    if (!mg.pre_piece[p]) HIP_CHECK(hipEventCreateWithFlags(&mg.pre_piece[p], hipEventDisableTiming));
+    if (hi > lo)
+      HIP_CHECK(hipMemcpyAsync(pin + c.nb + 1 + lo, c.bcol.get() + lo, sizeof(int) * (hi - lo),
+                               hipMemcpyDeviceToHost, c.aux));
+    HIP_CHECK(hipEventRecord(mg.pre_piece[p], c.aux));
+  }
   const int64_t nblk = c.nblk;
   const int nb = c.nb;
   const hipEvent_t done = mg.pre_done;
+  std::array<hipEvent_t, K> pieces;
+  std::array<int64_t, K> ends;
+  for (int p = 0; p < K; ++p) {
+    pieces[p] = mg.pre_piece[p];
+    ends[p] = piece_end[p];
+  }
   mg.pre_error = nullptr;
   mg.pre_staged = false;
   const size_t room = mg.pinned_n;
-  mg.pre_worker = std::thread([&mg, pin, nb, nblk, done, room]() {
+  mg.pre_worker = std::thread([&mg, pin, nb, nblk, done, room, pieces, ends]() {
     try {
       const bool trace = std::getenv("CSTFEM_MG_TRACE") != nullptr;
       const auto t0 = std::chrono::steady_clock::now();
       HIP_CHECK(hipEventSynchronize(done));
       const auto t1 = std::chrono::steady_clock::now();
-      mg.pre_nagg = aggregate_pipelined(pin, pin + nb + 1, nb, pin[nb + 1 + nblk], mg.pre_agg);
+      int next = 0;  // the first piece not yet waited for
+      auto need = [&](int64_t m) -> int64_t {
+        while (next < K && (next == 0 ? 0 : ends[next - 1]) < m) HIP_CHECK(hipEventSynchronize(pieces[next++]));
+        return next == 0 ? 0 : ends[next - 1];
+      };
+      mg.pre_nagg = aggregate_pipelined(pin, pin + nb + 1, nb, pin[nb + 1 + nblk], mg.pre_agg, need);
       const auto t2 = std::chrono::steady_clock::now();
       // the map into the staging area for its upload, over the node graph (no longer read)
       if ((size_t)nb <= room) {
@@ -3000,7 +3037,9 @@ void mg_prepare(Context& c) {
       if (trace) {
         const auto t3 = std::chrono::steady_clock::now();
         auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
-        std::fprintf(stderr, "[mg prepare] download %.3f ms, aggregation %.3f ms, staging %.3f ms\n",
+        std::fprintf(stderr,
+                     "[mg prepare] row pointers %.3f ms, aggregation with the columns arriving "
+                     "%.3f ms, staging %.3f ms\n",
                      ms(t0, t1), ms(t1, t2), ms(t2, t3));
       }
     } catch (...) {
@@ -3106,6 +3145,12 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     MgLevel& L = *mg.lv[l];
     const bool last = l + 1 >= max_levels - 1;
     // aggregation (host, greedy in node order) and members (ascending node order)
+    if (l > 0) {
+      // this level's block pattern, downloaded while the level above was built: the host
+      // aggregation below overlaps that level's Galerkin product on the device
+      HIP_CHECK(hipEventSynchronize(mg.pat_ev));
+      L.bcol.assign(mg.pinned_pat, mg.pinned_pat + L.bptr[L.nodes]);
+    }
     if (!(l == 0 && prepared)) {
       const int* hp = l == 0 ? pin : L.bptr.data();
       const int* hc = l == 0 ? pin + c.nb + 1 : L.bcol.data();
@@ -3122,8 +3167,9 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
       stage = mg.pinned;  // the worker copied the map there
     } else {
       // the upload of the level above reads the staging area: let it finish (on level 0 only
-      // the node-graph download used it, and that was waited for)
-      if (l > 0 || (size_t)L.nodes > mg.pinned_n) HIP_CHECK(hipStreamSynchronize(c.stream));
+      // the node-graph download used it, and that was waited for); growing it frees it
+      if ((size_t)L.nodes > mg.pinned_n) HIP_CHECK(hipStreamSynchronize(c.stream));
+      else if (l > 0) HIP_CHECK(hipEventSynchronize(mg.map_ev));
       stage = pinned_ints(mg, std::max(mg.pinned_n, (size_t)L.nodes));
       std::copy(L.h_agg.begin(), L.h_agg.end(), stage);
     }
@@ -3131,6 +3177,8 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     L.mptr.alloc(na + 1);
     L.mem.alloc(L.nodes);
     HIP_CHECK(hipMemcpyAsync(L.agg.get(), stage, sizeof(int) * L.nodes, hipMemcpyHostToDevice, c.stream));
+    if (!mg.map_ev) HIP_CHECK(hipEventCreateWithFlags(&mg.map_ev, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(mg.map_ev, c.stream));
     // iwork: flags[2] (coarse degree overflow, aggregate too large), counts[na + 1] (later the
     // coarse degrees), cursors[na]
     const size_t niw = 2 + 2 * (size_t)na + 1;
@@ -3214,8 +3262,12 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
                        (const int*)L.mptr.get(), (const int*)L.mem.get(), dbp, dbc,
                        (const int*)L.agg.get(), (const int*)C->d_bptr.get(), C->d_bcol.get());
     HIP_CHECK(hipGetLastError());
-    C->bcol.resize(ncblk);
-    HIP_CHECK(hipMemcpyAsync(C->bcol.data(), C->d_bcol.get(), sizeof(int) * ncblk, hipMemcpyDeviceToHost, c.stream));
+    // the pattern goes to the host for the next level's aggregation without holding this
+    // thread: page-locked buffer, event
+    if (!mg.pat_ev) HIP_CHECK(hipEventCreateWithFlags(&mg.pat_ev, hipEventDisableTiming));
+    HIP_CHECK(hipMemcpyAsync(pinned_pattern(mg, std::max(ncblk, 1)), C->d_bcol.get(),
+                             sizeof(int) * ncblk, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipEventRecord(mg.pat_ev, c.stream));
     tr.mark(c, l, "coarse pattern");
     // scalar SELL-64 layout of the coarse matrix
     sell_offsets(*C, C->bptr);
@@ -3263,13 +3315,14 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     } else {
       C->val32.release();  // the cycle reads FP32 values wherever a level holds them
     }
-    HIP_CHECK(hipStreamSynchronize(c.stream));  // C->bcol on the host, C->val for the inverse
     alloc_work(c, *C);
     tr.mark(c, l, "galerkin, dinv, work");
     if (last || C->n <= coarse_max || na == L.nodes) {
       if (C->n > 4096)
         throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: coarsest level has " + std::to_string(C->n) +
                                              " DOFs (> 4096): allow more levels");
+      HIP_CHECK(hipStreamSynchronize(c.stream));  // its values for the inverse, its pattern
+      C->bcol.assign(mg.pinned_pat, mg.pinned_pat + ncblk);
       coarsest_inverse(c, *C);
       tr.mark(c, l + 1, "coarsest inverse");
       break;

@@ -1,9 +1,16 @@
 // The two-thread aggregation (aggregate_pipelined: the leftover pass on a second thread,
 // synchronised by a published progress counter) against the sequential pass, on node graphs
-// above the 100,000-node threshold where the second thread is used.  Built with
-// -fsanitize=thread by the sanitizer test: ThreadSanitizer reports any access pair the
-// progress counter does not order.  Exit status 0 when every graph matches.
+// above the 100,000-node threshold where the second thread is used; and the same with the
+// column array still arriving: a producer thread copies it in pieces, each published under
+// a mutex, and the aggregation's `need` callback waits for the pieces it reaches (the way the
+// device download's events are waited for).  Built with -fsanitize=thread by the sanitizer
+// test: ThreadSanitizer reports any access pair the progress counter, the mutex or the
+// callback does not order.  Exit status 0 when every graph matches.
 #include <algorithm>
+#include <condition_variable>
+#include <cstdint>
+#include <mutex>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <numeric>
@@ -42,11 +49,42 @@ Graph grid_graph(int nx, int ny, const std::vector<int>& perm) {
   return g;
 }
 
+// aggregate_pipelined while another thread fills col in `pieces` pieces
+int pipelined_arriving(const Graph& g, int pieces, std::vector<int>& out) {
+  const int64_t m = (int64_t)g.col.size();
+  std::vector<int> col(m, -1);
+  std::mutex mu;
+  std::condition_variable cv;
+  int64_t arrived = 0;  // leading entries of col published
+  std::thread producer([&]() {
+    for (int p = 0; p < pieces; ++p) {
+      const int64_t lo = m * p / pieces, hi = m * (p + 1) / pieces;
+      std::copy(g.col.begin() + lo, g.col.begin() + hi, col.begin() + lo);
+      std::this_thread::yield();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        arrived = hi;
+      }
+      cv.notify_all();
+    }
+  });
+  auto need = [&](int64_t want) -> int64_t {
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait(lk, [&]() { return arrived >= want; });
+    return arrived;
+  };
+  const int na = cstfem_hip::aggregate_pipelined(g.ptr.data(), col.data(), g.n, g.bandwidth, out,
+                                                 need);
+  producer.join();
+  return na;
+}
+
 int check(const char* name, const Graph& g) {
-  std::vector<int> a, b;
+  std::vector<int> a, b, c;
   const int na = cstfem_hip::aggregate(g.ptr.data(), g.col.data(), g.n, a);
   const int nb = cstfem_hip::aggregate_pipelined(g.ptr.data(), g.col.data(), g.n, g.bandwidth, b);
-  const bool ok = na == nb && a == b;
+  const int nc = pipelined_arriving(g, 8, c);
+  const bool ok = na == nb && a == b && na == nc && a == c;
   std::printf("%s: %d nodes, bandwidth %d, %d aggregates: %s\n", name, g.n, g.bandwidth, na,
               ok ? "identical" : "DIFFERENT");
   return ok ? 0 : 1;
