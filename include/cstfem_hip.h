@@ -219,11 +219,16 @@ int cstfem_set_material_fields(cstfem_ctx* ctx, const double* E_elem, const doub
  * cstfem_build_pattern and before cstfem_assemble (the matrix layout is re-aligned to the
  * plates). */
 int cstfem_batch_define(cstfem_ctx* ctx, int count, const int* node_off);
-/* Launch shape of the persistent batched solve: wavefronts per workgroup (4, 8, 16) and
- * resident workgroups per CU (1..4).  Results do not depend on it. */
+/* Launch shape of the persistent batched solve: wavefronts per workgroup (4, 8, 16),
+ * workgroups per CU (1..4, at most what fits), and the most workgroups one plate may take
+ * (max_team: 1, 2, 4, 8 or 16; 0 = 16).  With max_team > 1 the largest plates are each
+ * solved by a team of co-resident workgroups that split its slices and meet at barriers, so
+ * that one big plate no longer sets the makespan.  Results do not depend on the shape: the
+ * reductions sum per-slice partials in one fixed order whoever computed them. */
 typedef struct {
   int waves_per_group;
   int groups_per_cu;
+  int max_team;
 } cstfem_batch_config;
 typedef struct {
   int iterations;
@@ -235,7 +240,7 @@ typedef struct {
 } cstfem_batch_result;
 /* Solves every plate in one persistent launch (method CSTFEM_METHOD_CG_FP64[_REFINED] or
  * CSTFEM_METHOD_IR_FP32; every plate is verified by its FP64 true residual).  cfg NULL = the
- * default shape (8 waves, 1 group per CU).  results[count]; x_out[n] may be NULL; *seconds =
+ * default shape (8 waves, 1 group per CU, teams of up to 16).  results[count]; x_out[n] may be NULL; *seconds =
  * device time of the launch.  Returns 2 if any plate did not converge. */
 int cstfem_batch_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt,
                        const cstfem_batch_config* cfg, cstfem_batch_result* results,

@@ -65,6 +65,19 @@ struct BatchArgs {
   double* snap;
   int snap_first, snap_count;
   int64_t snap_stride;
+  // teams: workgroup g < team_wgs is member team[g].y of team[g].z solving plate team[g].x;
+  // the other plates (single[0 .. single_count), decreasing cost) go one workgroup each
+  // through the queue.  Team partials live in gpart (3 arrays of the batch's slice count),
+  // team_cnt[plate] counts barrier arrivals (zeroed with the queue), team_err flags a wait
+  // that exceeded its bound.
+  const int4* team;
+  int team_wgs;
+  const int* single;
+  int single_count;
+  double* gpart;
+  int64_t nslices;
+  int* team_cnt;
+  int* team_err;
 };
 
 __device__ inline double wave_sum(double v) {
@@ -77,9 +90,10 @@ __device__ inline double wave_sum(double v) {
 // 64-lane butterfly (zeros beyond n), then the group sums by one more butterfly.  The
 // arithmetic is the same for any workgroup size, which wave sums which group does not
 // matter, and it costs two barriers.  Every thread returns the total.
-__device__ inline double fixed_sum(const double* a, int n) {
+template <class Sync>
+__device__ inline double fixed_sum(const double* a, int n, Sync& sync) {
   __shared__ double grp[kMaxSlicesPerPlate / 64];
-  __syncthreads();  // the partials are written
+  sync();  // the partials are written (by this workgroup, or by every member of its team)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
   const int G = (n + 63) >> 6;
   for (int q = wave; q < G; q += W) {
@@ -176,11 +190,61 @@ struct Vec<true> {
   using S = float;
 };
 
-// One plate, whole solve, one workgroup.  Slices s0..s1-1; wave w owns slices s0+w, s0+w+W, ...
-// in every phase, so a row's values are produced and consumed by the same lane except across
-// the barriers that separate the SpMV (which reads neighbours) from the updates.
+constexpr int kTeamSpin = 1 << 24;  // polls of a team barrier before it gives up (about 1 s)
+
+// The workgroups solving one plate: member m of T.  sync() is a workgroup barrier, and for
+// T > 1 also a barrier of the team: thread 0 publishes the workgroup's stores (device-scope
+// fence) and arrives on the plate's counter, then waits until all T members have arrived as
+// often as it has; the fence after the wait makes their stores visible to this workgroup.
+// A wait beyond kTeamSpin polls sets *err and goes on (the host reports the launch as
+// failed), so no member can wait forever.  Every member passes the same barriers (the
+// control flow depends only on reductions every member computes alike).
+struct Team {
+  int m = 0, T = 1;
+  int* cnt = nullptr;
+  int* err = nullptr;
+  int passed = 0;
+  bool failed = false;  // a barrier gave up (this member's or another's): leave the plate
+  __device__ void operator()() {
+    __shared__ int gave_up;
+    __syncthreads();
+    if (T == 1) return;
+    ++passed;
+    if (threadIdx.x == 0) {
+      __threadfence();
+      atomicAdd(cnt, 1);
+      const int target = T * passed;
+      int spins = 0, stop = 0;
+      while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        ++spins;
+        if ((spins & 1023) == 0 &&
+            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          stop = 1;  // another member gave up
+          break;
+        }
+        if (spins > kTeamSpin) {
+          atomicExch(err, 1);
+          stop = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __threadfence();
+      gave_up = stop;
+    }
+    __syncthreads();
+    failed = failed || gave_up != 0;
+  }
+};
+
+// One plate, whole solve, by one workgroup or a team of them.  Slices s0..s1-1; wave w of
+// member m owns slices s0 + m W + w, then every T W slices, in every phase, so a row's values
+// are produced and consumed by the same lane except across the barriers that separate the
+// SpMV (which reads neighbours) from the updates.  The reductions read the per-slice partials
+// of the whole plate (LDS for one workgroup, gpart for a team) in fixed_sum's fixed order, so
+// the result does not depend on T.
 template <bool kF32>
-__device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* part1,
+__device__ void solve_plate(const BatchArgs& A, int k, Team& team, double* part0, double* part1,
                             double* part2) {
   using V = typename Vec<kF32>::V;
   using S = typename Vec<kF32>::S;
@@ -189,7 +253,13 @@ __device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* pa
   V* Ap = kF32 ? reinterpret_cast<V*>(A.Ap32) : reinterpret_cast<V*>(A.Ap);
   const V* invd = kF32 ? reinterpret_cast<const V*>(A.invd32) : reinterpret_cast<const V*>(A.invd);
   const int s0 = A.seg_slice[k], s1 = A.seg_slice[k + 1], ns = s1 - s0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63, W0 = blockDim.x >> 6;
+  const int wave = team.m * W0 + (threadIdx.x >> 6), W = team.T * W0;
+  if (team.T > 1) {  // the team's partials: the plate's stretch of the global arrays
+    part0 = A.gpart + s0;
+    part1 = A.gpart + A.nslices + s0;
+    part2 = A.gpart + 2 * A.nslices + s0;
+  }
 
   // x = 0, r = b, p = D^-1 r; b'b (FP64) and r'z
   for (int s = s0 + wave; s < s1; s += W) {
@@ -218,12 +288,12 @@ __device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* pa
       part2[s - s0] = rz;
     }
   }
-  const double bnorm = sqrt(fixed_sum(part1, ns));
-  double rz = fixed_sum(part2, ns);
-  int it = 0, refreshes = 0, status = kRunning;
+  const double bnorm = sqrt(fixed_sum(part1, ns, team));
+  double rz = fixed_sum(part2, ns, team);
+  int it = 0, refreshes = 0, status = team.failed ? kMaxIter : kRunning;
   double rel = 1.0, true_rel = 0.0, rel_ref = 1.0, prev_true = 1.0;
   int stagnant = 0;
-  if (bnorm == 0.0) {
+  if (bnorm == 0.0 && status == kRunning) {
     status = kConverged;
     rel = 0.0;
   }
@@ -245,7 +315,11 @@ __device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* pa
       t = wave_sum(t);
       if (lane == 0) part0[s - s0] = t;
     }
-    const double pAp = fixed_sum(part0, ns);
+    const double pAp = fixed_sum(part0, ns, team);
+    if (team.failed) {
+      status = kMaxIter;
+      break;
+    }
     if (!(pAp > 0.0)) {
       status = kBreakdown;
       break;
@@ -286,14 +360,18 @@ __device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* pa
         part2[s - s0] = rz_s;
       }
     }
-    const double rr = fixed_sum(part1, ns);
-    double rz_new = fixed_sum(part2, ns);
+    const double rr = fixed_sum(part1, ns, team);
+    double rz_new = fixed_sum(part2, ns, team);
+    if (team.failed) {
+      status = kMaxIter;
+      break;
+    }
     ++it;
     rel = sqrt(rr) / bnorm;
     bool restart = false;
     if (rel < A.rtol || rel < A.delta * rel_ref) {
       // refresh: FP64 true residual r64 = b - A x (x of every row is final after the barrier)
-      __syncthreads();
+      team();
       for (int s = s0 + wave; s < s1; s += W) {
         double y0, y1;
         row_spmv(A, (const double2*)A.x, s, lane, y0, y1);
@@ -318,8 +396,12 @@ __device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* pa
           part2[s - s0] = t_rz;
         }
       }
-      true_rel = sqrt(fixed_sum(part1, ns)) / bnorm;
-      rz_new = fixed_sum(part2, ns);
+      true_rel = sqrt(fixed_sum(part1, ns, team)) / bnorm;
+      rz_new = fixed_sum(part2, ns, team);
+      if (team.failed) {
+        status = kMaxIter;
+        break;
+      }
       ++refreshes;
       if (true_rel < A.rtol) {
         status = kConverged;
@@ -351,9 +433,13 @@ __device__ void solve_plate(const BatchArgs& A, int k, double* part0, double* pa
         p[row] = pi;
       }
     }
-    __syncthreads();  // p complete before the next SpMV reads neighbours
+    team();  // p complete before the next SpMV reads neighbours
+    if (team.failed) {
+      status = kMaxIter;
+      break;
+    }
   }
-  if (threadIdx.x == 0) {
+  if (team.m == 0 && threadIdx.x == 0) {
     cstfem_batch_result res{};
     res.iterations = it;
     res.converged = status == kConverged ? 1 : 0;
@@ -371,18 +457,30 @@ __global__ void k_batch_cg(BatchArgs A) {
   __shared__ double part0[kMaxSlicesPerPlate], part1[kMaxSlicesPerPlate],
       part2[kMaxSlicesPerPlate];
   __shared__ int next;
+  if ((int)blockIdx.x < A.team_wgs) {  // a member of a team: its plate first, then the queue
+    const int4 t = A.team[blockIdx.x];
+    Team team{t.y, t.z, A.team_cnt + t.x, A.team_err, 0};
+    solve_plate<kF32>(A, t.x, team, part0, part1, part2);
+  }
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(A.queue, 1);
     __syncthreads();
     const int q = next;
     __syncthreads();
-    if (q >= A.count) return;  // the queue only grows: every workgroup reaches this exit
-    solve_plate<kF32>(A, A.order[q], part0, part1, part2);
+    if (q >= A.single_count) return;  // the queue only grows: every workgroup reaches this exit
+    Team one{};
+    solve_plate<kF32>(A, A.single[q], one, part0, part1, part2);
   }
 }
 
-__global__ void k_reset_queue(int* q) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
+// the queue and the teams' arrival counters start at zero in every launch
+__global__ void k_reset_queue(int* q, int* cnt, int count, int* err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    *q = 0;
+    *err = 0;
+  }
+  for (int k = i; k < count; k += gridDim.x * blockDim.x) cnt[k] = 0;
 }
 
 // every snapshot row starts as x* (a plate that stops early leaves e = 0 in later rows)
@@ -451,6 +549,7 @@ void batch_define(Context& c, int count, const int* node_off) {
   });
   c.batch_order.alloc(count);
   HIP_CHECK(hipMemcpyAsync(c.batch_order.get(), order.data(), sizeof(int) * count, hipMemcpyHostToDevice, c.stream));
+  c.h_batch_order = order;
   c.batch_results.alloc(count);
   c.batch_queue.alloc(1);
   HIP_CHECK(hipStreamSynchronize(c.stream));
@@ -481,6 +580,7 @@ BatchArgs batch_args(Context& c, const cstfem_solver_options& o) {
   a.seg_slice = c.batch_seg_slice.get();
   a.count = c.batch_count;
   a.queue = c.batch_queue.get();
+  a.nslices = c.nslices;
   a.out = c.batch_results.get();
   a.rtol = o.rtol;
   const bool fp32 = o.method == CSTFEM_METHOD_IR_FP32;
@@ -488,6 +588,35 @@ BatchArgs batch_args(Context& c, const cstfem_solver_options& o) {
   a.max_iter = o.max_iter;
   a.max_refresh = o.max_outer > 0 ? o.max_outer : 200;
   return a;
+}
+
+// Teams of the persistent launch of `grid` workgroups (all resident): plates in decreasing
+// estimated cost n^1.5 take T = 2, 4, .. max_team workgroups while a member's share of the
+// plate stays above the average share of a workgroup (total cost / grid) and each member
+// keeps at least one slice per wavefront; teams take at most half the grid, the rest of the
+// plates go one workgroup each through the queue.  Fills the member map and the single list.
+void plan_teams(Context& c, int grid, int waves, int max_team, std::vector<int4>& team,
+                std::vector<int>& single) {
+  team.clear();
+  single.clear();
+  const int count = c.batch_count;
+  auto cost = [&](int k) {
+    const double n = (double)(c.h_segments[k + 1] - c.h_segments[k]);
+    return n * std::sqrt(n);
+  };
+  double total = 0.0;
+  for (int k = 0; k < count; ++k) total += cost(k);
+  const double share = total / std::max(grid, 1);
+  for (int k : c.h_batch_order) {
+    const int slices = c.h_seg_slice[k + 1] - c.h_seg_slice[k];
+    int T = 1;
+    while (2 * T <= max_team && cost(k) / T > share && slices >= 2 * T * waves) T *= 2;
+    if (T > 1 && (int)team.size() + T <= grid / 2) {
+      for (int m = 0; m < T; ++m) team.push_back(make_int4(k, m, T, 0));
+    } else {
+      single.push_back(k);
+    }
+  }
 }
 
 }  // namespace
@@ -504,6 +633,9 @@ double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batc
     throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_solve: waves_per_group must be 4, 8 or 16");
   if (cfg.groups_per_cu < 1 || cfg.groups_per_cu > 4)
     throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_solve: groups_per_cu must be 1..4");
+  const int max_team = cfg.max_team == 0 ? 16 : cfg.max_team;
+  if (max_team != 1 && max_team != 2 && max_team != 4 && max_team != 8 && max_team != 16)
+    throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_solve: max_team must be 0, 1, 2, 4, 8 or 16");
   const bool fp32 = o.method == CSTFEM_METHOD_IR_FP32;
   const size_t nv = 2 * (size_t)c.nb;
   if (fp32) {
@@ -519,19 +651,51 @@ double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batc
     args.snap_count = snaps->count;
     args.snap_stride = snaps->stride;
   }
-  int grid = cfg.groups_per_cu * (c.cu_count > 0 ? c.cu_count : 256);
-  if (grid > c.batch_count) grid = c.batch_count;
+  // every workgroup of the launch is resident at once (a team member waits for the others):
+  // at most what fits per CU
+  int fit = 0;
+  if (fp32)
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_batch_cg<true>, 64 * waves, 0));
+  else
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_batch_cg<false>, 64 * waves, 0));
+  if (fit < 1) throw Error(CSTFEM_ERR_DEVICE, "cstfem_batch_solve: the kernel fits no CU");
+  int grid = std::min(cfg.groups_per_cu, fit) * (c.cu_count > 0 ? c.cu_count : 256);
+  std::vector<int4> team;
+  std::vector<int> single;
+  plan_teams(c, grid, waves, max_team, team, single);
+  grid = std::min<int>(grid, (int)(team.size() + single.size()));
+  c.batch_team.alloc(std::max<size_t>(team.size(), 1));
+  c.batch_single.alloc(std::max<size_t>(single.size(), 1));
+  c.batch_team_cnt.alloc(c.batch_count);
+  c.batch_team_err.alloc(1);
+  c.batch_gpart.alloc(3 * (size_t)std::max(c.nslices, 1));
+  if (!team.empty())
+    HIP_CHECK(hipMemcpyAsync(c.batch_team.get(), team.data(), sizeof(int4) * team.size(),
+                             hipMemcpyHostToDevice, c.stream));
+  if (!single.empty())
+    HIP_CHECK(hipMemcpyAsync(c.batch_single.get(), single.data(), sizeof(int) * single.size(),
+                             hipMemcpyHostToDevice, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));  // the host vectors go out of scope
+  args.team = c.batch_team.get();
+  args.team_wgs = (int)team.size();
+  args.single = c.batch_single.get();
+  args.single_count = (int)single.size();
+  args.gpart = c.batch_gpart.get();
+  args.team_cnt = c.batch_team_cnt.get();
+  args.team_err = c.batch_team_err.get();
   // one captured graph: queue reset (a kernel, not a memset node) + the persistent solve.  It
   // is kept in the context and replayed while the launch (precision, shape, every argument:
   // buffers, counts, tolerances) is the same; anything else captures it anew.
   std::vector<unsigned char> key(sizeof(BatchArgs) + 3 * sizeof(int));
-  const int head[3] = {fp32 ? 1 : 0, waves, grid};
+  const int head[3] = {fp32 ? 1 : 0, waves, grid};  // (the team plan's content is uploaded anew)
   std::memcpy(key.data(), head, sizeof head);
   std::memcpy(key.data() + sizeof head, &args, sizeof args);
   if (!c.batch_exec || key != c.batch_graph_key) {
     c.drop_batch_graph();
     HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, c.stream, c.batch_queue.get());
+    hipLaunchKernelGGL(k_reset_queue, dim3(grid_for(c.batch_count, 256, 64)), dim3(256), 0,
+                       c.stream, c.batch_queue.get(), c.batch_team_cnt.get(), c.batch_count,
+                       c.batch_team_err.get());
     if (fp32)
       hipLaunchKernelGGL(k_batch_cg<true>, dim3(grid), dim3(64 * waves), 0, c.stream, args);
     else
@@ -551,6 +715,11 @@ double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batc
   HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  int team_err = 0;
+  HIP_CHECK(hipMemcpy(&team_err, c.batch_team_err.get(), sizeof(int), hipMemcpyDeviceToHost));
+  if (team_err)
+    throw Error(CSTFEM_ERR_DEVICE, "cstfem_batch_solve: a team barrier timed out (the launch's "
+                                   "results are invalid)");
   if (results)
     HIP_CHECK(hipMemcpy(results, c.batch_results.get(), sizeof(cstfem_batch_result) * c.batch_count,
                         hipMemcpyDeviceToHost));
@@ -568,7 +737,7 @@ cstfem_batch_config batch_autotune(Context& c, const cstfem_solver_options& o,
   std::snprintf(name, sizeof name, "%s", prop.gcnArchName);
   std::ostringstream key;
   key << name << " cus=" << c.cu_count << " plates=" << c.batch_count << " rows=" << c.nb
-      << " slices=" << c.nslices << " method=" << o.method;
+      << " slices=" << c.nslices << " method=" << o.method << " shape=waves,groups,team";
   if (cache_path && *cache_path) {
     std::ifstream in(cache_path);
     std::string line;
@@ -578,7 +747,7 @@ cstfem_batch_config batch_autotune(Context& c, const cstfem_solver_options& o,
       std::istringstream vals(line.substr(bar + 1));
       cstfem_batch_config cfg{};
       double t = 0.0;
-      if (vals >> cfg.waves_per_group >> cfg.groups_per_cu >> t) {
+      if (vals >> cfg.waves_per_group >> cfg.groups_per_cu >> cfg.max_team >> t) {
         if (best_seconds) *best_seconds = t;
         return cfg;
       }
@@ -586,14 +755,16 @@ cstfem_batch_config batch_autotune(Context& c, const cstfem_solver_options& o,
   }
   const int waves[] = {4, 8, 16};
   const int groups[] = {1, 2, 4};
-  cstfem_batch_config best{8, 1};
+  const int teams[] = {1, 16};
+  cstfem_batch_config best{8, 1, 16};
   double best_t = 1e300;
   std::vector<cstfem_batch_result> ref(c.batch_count), cur(c.batch_count);
   std::vector<double> x_ref(c.n), x_cur(c.n);
   bool first = true;
   for (int w : waves)
-    for (int gpc : groups) {
-      const cstfem_batch_config cfg{w, gpc};
+    for (int gpc : groups)
+      for (int tm : teams) {
+      const cstfem_batch_config cfg{w, gpc, tm};
       const double t = batch_solve(c, o, cfg, cur.data());
       HIP_CHECK(hipMemcpy(x_cur.data(), c.x.get(), sizeof(double) * c.n, hipMemcpyDeviceToHost));
       if (first) {
@@ -615,8 +786,8 @@ cstfem_batch_config batch_autotune(Context& c, const cstfem_solver_options& o,
     }
   if (cache_path && *cache_path) {
     std::ofstream out(cache_path, std::ios::app);
-    out << key.str() << "|" << best.waves_per_group << " " << best.groups_per_cu << " " << best_t
-        << "\n";
+    out << key.str() << "|" << best.waves_per_group << " " << best.groups_per_cu << " "
+        << best.max_team << " " << best_t << "\n";
   }
   if (best_seconds) *best_seconds = best_t;
   return best;
@@ -635,7 +806,7 @@ int batch_harvest(Context& c, const cstfem_solver_options& o, int k_first, int k
     throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_batch_harvest: x* needs an FP64 method");
   const int K = k_last - k_first + 1;
   const int64_t n = 2 * (int64_t)c.nb;
-  const cstfem_batch_config cfg{8, 1};
+  const cstfem_batch_config cfg{8, 1, 16};
   std::vector<cstfem_batch_result> res(c.batch_count);
   batch_solve(c, o, cfg, res.data());
   // x* needs to be far more accurate than the errors e_k of the first iterations, not to

@@ -675,7 +675,7 @@ int cstfem_batch_solve(cstfem_ctx* ctx, const cstfem_solver_options* opt,
     Context& c = ctx->c;
     use_device(&c);
     need(opt->rtol > 0 && opt->max_iter >= 0, "cstfem_batch_solve: bad rtol or max_iter");
-    const cstfem_batch_config def{8, 1};
+    const cstfem_batch_config def{8, 1, 16};
     std::vector<cstfem_batch_result> res(c.batch_count > 0 ? c.batch_count : 1);
     const double t = batch_solve(c, *opt, cfg ? *cfg : def, res.data());
     if (results) std::copy(res.begin(), res.begin() + c.batch_count, results);

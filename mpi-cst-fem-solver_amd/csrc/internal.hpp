@@ -359,6 +359,12 @@ struct Context {
   // batch of independent plates (batched.hip)
   int batch_count = 0;
   DBuf<int> batch_seg_slice, batch_order, batch_queue;
+  std::vector<int> h_batch_order;  // plates by decreasing estimated cost
+  // teams of the batched solve: member map, single-plate list, arrival counters, error flag,
+  // and the teams' per-slice partials (3 x slices)
+  DBuf<int4> batch_team;
+  DBuf<int> batch_single, batch_team_cnt, batch_team_err;
+  DBuf<double> batch_gpart;
   DBuf<cstfem_batch_result> batch_results;
   // the batched solve's captured graph, replayed while its launch is unchanged (key: its
   // precision, shape and kernel arguments)

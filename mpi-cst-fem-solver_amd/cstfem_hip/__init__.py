@@ -58,7 +58,8 @@ class SolverStatsC(ctypes.Structure):
 
 
 class BatchConfig(ctypes.Structure):
-    _fields_ = [("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int)]
+    _fields_ = [("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int),
+                ("max_team", ctypes.c_int)]
 
 
 class KirschMetricsC(ctypes.Structure):
@@ -405,7 +406,7 @@ class Device:
         path = cache_path.encode() if cache_path else None
         _check(lib().cstfem_batch_autotune(self._h, ctypes.byref(o), path, ctypes.byref(best),
                                            ctypes.byref(t)))
-        return (best.waves_per_group, best.groups_per_cu), t.value
+        return (best.waves_per_group, best.groups_per_cu, best.max_team), t.value
 
     def profile_kernel(self, kind, reps=20, nbytes=0):
         """Average seconds per launch of a hot kernel (0 fused SpMV, 1 SpMV, 2 update,

@@ -7,7 +7,9 @@
 - for a fixed sample (every 16th plate and the largest), the plate's block of the batched
   pattern, K and b equal the oracle's single-plate arrays bit for bit, and the solution
   matches the oracle's single-plate Jacobi-CG (src/solver.cpp:35-131): iterations +-2,
-  solution within 1e-8 relative.
+  solution within 1e-8 relative;
+- the team launch (the largest plates split over up to 16 co-resident workgroups, the
+  default) and one workgroup per plate give the same bits, in FP64 and in FP32.
 """
 import numpy as np
 import pytest
@@ -81,3 +83,15 @@ def test_batch512_fp32_refined_true_residuals(dev, batch512):
     assert all(r.converged for r in r32)
     tr = per_plate_true_residual(bp, A, b, x32)
     assert np.all(tr < 1.05e-10), (int(np.argmax(tr)), float(tr.max()))
+
+
+@pytest.mark.parametrize("method", [cstfem_hip.METHOD_CG_FP64_REFINED, cstfem_hip.METHOD_IR_FP32])
+def test_batch512_teams_are_bit_identical(dev, batch512, method):
+    bp = batch512
+    pipeline.load_batch(dev, bp)
+    x1, r1, t1 = dev.batch_solve(method=method, config=(16, 1, 1))
+    x16, r16, t16 = dev.batch_solve(method=method, config=(16, 1, 16))
+    print(f"method {method}: one workgroup per plate {t1:.3f} s, teams of up to 16 {t16:.3f} s")
+    assert np.array_equal(x1, x16)
+    assert [r.iterations for r in r1] == [r.iterations for r in r16]
+    assert all(r.converged for r in r16)

@@ -1,6 +1,7 @@
 """Batched persistent Jacobi-CG over independent plates: every plate equals its own
 single-plate solve (oracle), the assembled block of every plate is bitwise the oracle's, the
-launch shape never changes results, and the FP32 mode reaches the FP64-verified tolerance."""
+launch shape (workgroup size, workgroups per CU, teams of workgroups per plate) never changes
+results, and the FP32 mode reaches the FP64-verified tolerance."""
 import numpy as np
 import pytest
 
@@ -54,15 +55,20 @@ def test_batch_matches_single_plates(dev, oracle, small_batch):
 
 
 def test_batch_launch_shape_invariance_and_autotune(dev, small_batch, tmp_path):
+    """Launch shapes, one workgroup per plate (max_team 1) or teams of up to 2..16 (the
+    largest plates split over co-resident workgroups), give the same bits: every reduction
+    sums the same per-slice partials in one fixed order."""
     bp = small_batch
     pipeline.load_batch(dev, bp)
-    x0, r0, _ = dev.batch_solve(config=(4, 1))
-    x1, r1, _ = dev.batch_solve(config=(16, 2))
-    assert np.array_equal(x0, x1)
-    assert [r.iterations for r in r0] == [r.iterations for r in r1]
+    x0, r0, _ = dev.batch_solve(config=(4, 1, 1))
+    for cfg in ((16, 2), (4, 1, 16), (4, 2, 2), (8, 1, 4), (16, 1, 16)):
+        x1, r1, _ = dev.batch_solve(config=cfg)
+        assert np.array_equal(x0, x1), cfg
+        assert [r.iterations for r in r0] == [r.iterations for r in r1], cfg
+        assert [r.refinements for r in r0] == [r.refinements for r in r1], cfg
     cache = str(tmp_path / "autotune.txt")
     best, t = dev.batch_autotune(cache_path=cache)
-    assert best[0] in (4, 8, 16) and best[1] in (1, 2, 4) and t > 0
+    assert best[0] in (4, 8, 16) and best[1] in (1, 2, 4) and best[2] in (1, 16) and t > 0
     best2, _ = dev.batch_autotune(cache_path=cache)  # served from the cache
     assert best2 == best
     x2, r2, _ = dev.batch_solve(config=best)
