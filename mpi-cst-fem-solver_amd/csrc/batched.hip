@@ -21,6 +21,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -78,6 +79,7 @@ struct BatchArgs {
   int64_t nslices;
   int* team_cnt;
   int* team_err;
+  int team_spin;  // polls of a team barrier before it gives up
 };
 
 __device__ inline double wave_sum(double v) {
@@ -190,7 +192,7 @@ struct Vec<true> {
   using S = float;
 };
 
-constexpr int kTeamSpin = 1 << 24;  // polls of a team barrier before it gives up (about 1 s)
+constexpr int kTeamSpin = 1 << 24;  // polls of a team barrier before it gives up (about 0.5 s)
 
 // The workgroups solving one plate: member m of T.  sync() is a workgroup barrier, and for
 // T > 1 also a barrier of the team: thread 0 publishes the workgroup's stores (device-scope
@@ -203,6 +205,7 @@ struct Team {
   int m = 0, T = 1;
   int* cnt = nullptr;
   int* err = nullptr;
+  int spin_max = kTeamSpin;
   int passed = 0;
   bool failed = false;  // a barrier gave up (this member's or another's): leave the plate
   __device__ void operator()() {
@@ -222,7 +225,7 @@ struct Team {
           stop = 1;  // another member gave up
           break;
         }
-        if (spins > kTeamSpin) {
+        if (spins > spin_max) {
           atomicExch(err, 1);
           stop = 1;
           break;
@@ -459,7 +462,7 @@ __global__ void k_batch_cg(BatchArgs A) {
   __shared__ int next;
   if ((int)blockIdx.x < A.team_wgs) {  // a member of a team: its plate first, then the queue
     const int4 t = A.team[blockIdx.x];
-    Team team{t.y, t.z, A.team_cnt + t.x, A.team_err, 0};
+    Team team{t.y, t.z, A.team_cnt + t.x, A.team_err, A.team_spin, 0};
     solve_plate<kF32>(A, t.x, team, part0, part1, part2);
   }
   for (;;) {
@@ -683,6 +686,12 @@ double batch_solve(Context& c, const cstfem_solver_options& o, const cstfem_batc
   args.gpart = c.batch_gpart.get();
   args.team_cnt = c.batch_team_cnt.get();
   args.team_err = c.batch_team_err.get();
+  // CSTFEM_BATCH_TEAM_SPIN: a smaller bound on the team barriers' wait (the failure path's
+  // test sets it to a few polls, so that barriers give up without anything being wrong)
+  {
+    const char* env = std::getenv("CSTFEM_BATCH_TEAM_SPIN");
+    args.team_spin = env ? std::max(1, std::atoi(env)) : kTeamSpin;
+  }
   // one captured graph: queue reset (a kernel, not a memset node) + the persistent solve.  It
   // is kept in the context and replayed while the launch (precision, shape, every argument:
   // buffers, counts, tolerances) is the same; anything else captures it anew.

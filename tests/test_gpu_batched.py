@@ -75,6 +75,24 @@ def test_batch_launch_shape_invariance_and_autotune(dev, small_batch, tmp_path):
     assert np.array_equal(x0, x2)
 
 
+def test_team_barrier_gives_up_cleanly(dev, small_batch, monkeypatch):
+    """A team barrier whose wait exceeds its bound flags the launch and every member of that
+    team leaves its plate.  Forced from the host with a bound of one poll (nothing is wrong,
+    the members just do not arrive within one poll of each other), the solve reports the
+    failure instead of waiting; the next solve, with the default bound, gives the earlier
+    bits again (the counters and the flag start at zero in every launch)."""
+    bp = small_batch
+    pipeline.load_batch(dev, bp)
+    x0, r0, _ = dev.batch_solve(config=(4, 1, 16))
+    monkeypatch.setenv("CSTFEM_BATCH_TEAM_SPIN", "1")
+    with pytest.raises(cstfem_hip.CstfemError, match="team barrier timed out"):
+        dev.batch_solve(config=(4, 1, 16))
+    monkeypatch.delenv("CSTFEM_BATCH_TEAM_SPIN")
+    x1, r1, _ = dev.batch_solve(config=(4, 1, 16))
+    assert np.array_equal(x0, x1)
+    assert [r.iterations for r in r0] == [r.iterations for r in r1]
+
+
 def test_batch_fp32_refined(dev, small_batch):
     bp = small_batch
     pipeline.load_batch(dev, bp)
