@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
+#include <optional>
 #include <type_traits>
 
 #include "internal.hpp"
@@ -1440,6 +1441,15 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
   precond(0, 0);
   direction(1);
   HIP_CHECK(hipGetLastError());
+  // a multilevel iteration is ~1.4 ms of ~100 launches: two of them hide the host's status
+  // read, and a short chunk keeps the graph small and the launches queued past a stop few
+  const int chunk = multilevel ? 2 : (c.n > 200000 ? 16 : 64);  // even
+  // the iteration graph is captured while the device still runs the initialisation and the
+  // first preconditioner application queued above: the capture records launches without
+  // running them, and the work queued before it does not wait (a multilevel capture and its
+  // instantiation take about as long as one cycle)
+  std::optional<Chunk> gr;
+  if (o.max_iter > 0) gr.emplace(c, chunk, iteration);
   PcgScalars s = read_scalars(c);
   const double bnorm = s.bnorm;
   const bool zero_rhs = bnorm == 0.0;
@@ -1447,14 +1457,10 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
   double true_rel = zero_rhs ? 0.0 : 1.0;
   bool converged = zero_rhs, fallback = false;
   if (!zero_rhs && o.max_iter > 0) {
-    // a multilevel iteration is ~2 ms of ~160 launches: two of them hide the host's status
-    // read, and a short chunk keeps the graph small and the launches queued past a stop few
-    const int chunk = multilevel ? 2 : (c.n > 200000 ? 16 : 64);  // even
-    Chunk gr(c, chunk, iteration);
     double prev_true = 1.0;
     int stagnant = 0;
     for (;;) {
-      run_until_stopped(c, gr, chunk, o.max_iter);
+      run_until_stopped(c, *gr, chunk, o.max_iter);
       s = read_scalars(c);
       last_status = s.status;
       const int k = s.k_in[0] > s.k_in[1] ? s.k_in[0] : s.k_in[1];
