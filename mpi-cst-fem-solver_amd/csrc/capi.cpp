@@ -167,6 +167,7 @@ int cstfem_upload_mesh(cstfem_ctx* ctx, int nn, const double* xy, int64_t ne, co
     c.mpnn.npairs = 0;
     c.ne_ptr.release();
     c.ne_list.release();
+    c.trac_groups.clear();
     c.b.release();
     return CSTFEM_OK;
   });
@@ -443,9 +444,13 @@ int cstfem_set_rhs(cstfem_ctx* ctx, const double* b, int n) {
     ensure_vectors(c);
     HIP_CHECK(hipMemsetAsync(c.b.get(), 0, sizeof(double) * c.b.n, c.stream));
     // b == NULL: the zero vector the reference starts its load assembly from, set on the
-    // device instead of copied from the host
-    if (b) HIP_CHECK(hipMemcpyAsync(c.b.get(), b, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
-    HIP_CHECK(hipStreamSynchronize(c.stream));
+    // device instead of copied from the host, and nothing to wait for (the library's later
+    // work follows it on the stream); a host vector is waited for before it is handed back
+    if (b) {
+      HIP_CHECK(hipMemcpyAsync(c.b.get(), b, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+      HIP_CHECK(hipStreamSynchronize(c.stream));
+    }
+    HIP_CHECK(hipGetLastError());
     return CSTFEM_OK;
   });
 }

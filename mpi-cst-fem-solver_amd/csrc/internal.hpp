@@ -23,6 +23,7 @@
 
 #include <cstdint>
 #include <exception>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -408,6 +409,16 @@ struct Context {
   DBuf<int> trac_i;
   DBuf<double> trac_d;
   DBuf<double> body_w;  // element weights t*|A|/3 of the body force
+  // constant-traction groups already applied on this mesh: their packed index lists on the
+  // device, their edges checked (each on exactly one triangle).  A repeated group skips the
+  // host packing, the upload and the wait for the check.  Dropped when the mesh changes.
+  struct TractionGroup {
+    std::vector<int> edges;
+    DBuf<int> idx;
+    int U = 0;
+  };
+  std::vector<std::unique_ptr<TractionGroup>> trac_groups;
+  static constexpr size_t kTractionGroups = 16;
 
   // reduction scratch
   DBuf<double> partials;   // 4 * max_blocks

@@ -76,7 +76,7 @@ __global__ void k_trac_sum(int nnodes, const int* __restrict__ node, const int* 
                            const int* __restrict__ ent, const double* __restrict__ geo,
                            const double* __restrict__ tv, double t0, double t1, double thick,
                            double g, const int* __restrict__ err, double* __restrict__ f) {
-  if (*err != INT_MAX) return;  // a bad edge: f stays as it was
+  if (err && *err != INT_MAX) return;  // a bad edge: f stays as it was (err NULL: checked)
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= nnodes) return;
   const int v = node[u];
@@ -168,6 +168,31 @@ void add_traction_device(Context& c, int nedge, const int* edges, const Traction
     if (edges[k] < 0 || edges[k] >= c.nn)
       throw Error(CSTFEM_ERR_ARGUMENT, "cstfem_add_traction: edge node out of range");
   if (nedge == 0) return;
+  const bool host_fn = src.kind != TractionSource::kConstant;
+  const double g = 1.0 / std::sqrt(3.0);
+  if (!host_fn) {
+    // a constant traction on a group applied before on this mesh: its index lists are on the
+    // device and its edges passed the check, so the two kernels alone, with nothing to wait for
+    for (const auto& grp : c.trac_groups) {
+      if (grp->edges.size() != 2 * (size_t)nedge ||
+          !std::equal(grp->edges.begin(), grp->edges.end(), edges))
+        continue;
+      const int* di = grp->idx.get();
+      const int U = grp->U;
+      const size_t o_cnt = 2 * (size_t)nedge, o_node = o_cnt + nedge, o_ptr = o_node + U,
+                   o_ent = o_ptr + U + 1;
+      double* geo = grow(c.trac_d, (size_t)nedge * kGeo);
+      hipLaunchKernelGGL(k_trac_edges, dim3(grid_for(nedge, 128, 0)), dim3(128), 0, c.stream, nedge,
+                         di, c.ne_ptr.get(), c.ne_list.get(), c.tri.get(), c.xy.get(), g, geo,
+                         grp->idx.get() + o_cnt, c.err_flag.get());
+      hipLaunchKernelGGL(k_trac_sum, dim3(grid_for(U, 128, 0)), dim3(128), 0, c.stream, U,
+                         di + o_node, di + o_ptr, di + o_ent, (const double*)geo,
+                         (const double*)nullptr, src.p0, src.p1, c.thick, g, (const int*)nullptr,
+                         c.b.get());
+      HIP_CHECK(hipGetLastError());
+      return;
+    }
+  }
   // touched nodes in first-touch order, each with its (edge, role) entries in edge order
   std::unordered_map<int, int> slot;
   slot.reserve(2 * (size_t)nedge);
@@ -186,7 +211,6 @@ void add_traction_device(Context& c, int nedge, const int* edges, const Traction
                o_ent = o_ptr + U + 1, total = o_ent + 2 * (size_t)nedge;
   // host side in the context's page-locked staging: the function traction's Gauss points and
   // values (doubles first), then the packed index data and two flag words
-  const bool host_fn = src.kind != TractionSource::kConstant;
   const size_t nd_h = host_fn ? (size_t)(kGeo + 4) * nedge : 0;
   unsigned char* stage = c.stage(sizeof(double) * nd_h + sizeof(int) * (total + 2));
   double* h_geo = reinterpret_cast<double*>(stage);
@@ -201,13 +225,22 @@ void add_traction_device(Context& c, int nedge, const int* edges, const Traction
   for (int u = 0; u < U; ++u) ptr[u + 1] = ptr[u] + cnt[u];
   std::vector<int> fill(ptr, ptr + U);
   for (int k = 0; k < 2 * nedge; ++k) pack[o_ent + fill[slot[edges[k]]]++] = k;  // 2k + role
-  int* di = grow(c.trac_i, total);
+  // a constant traction's group keeps its index lists for later calls (once its check passed)
+  std::unique_ptr<Context::TractionGroup> keep;
+  int* di;
+  if (host_fn) {
+    di = grow(c.trac_i, total);
+  } else {
+    keep = std::make_unique<Context::TractionGroup>();
+    keep->idx.alloc(total);
+    keep->U = U;
+    di = keep->idx.get();
+  }
   double* dd = grow(c.trac_d, (size_t)nedge * (kGeo + (host_fn ? 4 : 0)));
   double* geo = dd;
   double* tv = host_fn ? dd + (size_t)kGeo * nedge : nullptr;
   int* err = c.err_flag.get();
   const int big = INT_MAX;
-  const double g = 1.0 / std::sqrt(3.0);
   h_flag[0] = big;
   HIP_CHECK(hipMemcpyAsync(di, pack, sizeof(int) * total, hipMemcpyHostToDevice, c.stream));
   HIP_CHECK(hipMemcpyAsync(err, h_flag, sizeof(int), hipMemcpyHostToDevice, c.stream));
@@ -259,6 +292,11 @@ void add_traction_device(Context& c, int nedge, const int* edges, const Traction
   HIP_CHECK(hipMemcpyAsync(h_flag + 1, err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   check_edges(h_flag[1]);
+  if (keep) {
+    keep->edges.assign(edges, edges + 2 * (size_t)nedge);
+    if (c.trac_groups.size() >= Context::kTractionGroups) c.trac_groups.erase(c.trac_groups.begin());
+    c.trac_groups.push_back(std::move(keep));
+  }
 }
 
 void add_body_force_device(Context& c, double bx, double by) {

@@ -409,6 +409,38 @@ __device__ inline double row_entries(const MatST<T>& A, const XF& xf, int64_t o,
   return a;
 }
 
+// One whole SELL row, its entries taken block by block (a row holds 3 entries per 3x3 block,
+// one block column each): the block columns and values of B blocks are loaded first, then
+// their 3 B x entries, so that a row of up to B blocks costs two memory round trips; the
+// products are added in ascending entry order, as row_entries adds them (the same sum, bit
+// for bit).
+template <int B, class XF, class T>
+__device__ inline double row_blocks(const MatST<T>& A, const XF& xf, int64_t o, int lane, int w,
+                                    double a) {
+  for (int k = 0; k < w; k += 3 * B) {
+    int bc[B];
+    T v[3 * B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int kb = k + 3 * u < w ? k + 3 * u : 0;  // past the row's end: block 0 again
+      bc[u] = A.bcol[o / 3 + (int64_t)(kb / 3) * 64 + lane];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[3 * u + c] = A.val[o + (int64_t)(kb + c) * 64 + lane];
+    }
+    double xv[3 * B];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xv[3 * u + c] = xf(3 * bc[u] + c);
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (k + 3 * u < w)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a = a + (double)v[3 * u + c] * xv[3 * u + c];
+  }
+  return a;
+}
+
 // Where row i of a scalar level sits in its slice: value index o of the slice's first entry,
 // entries per row w, lane
 struct RowAt {
@@ -423,9 +455,13 @@ __device__ inline RowAt row_at(const MatST<T>& A, int i) {
   return RowAt{A.off[s], A.off[s + 1], i & 63};
 }
 
+// one lane per row: 8 blocks (24 entries) per round, so that most rows (7 blocks in the
+// interior of a plate's level 1) take one round of two memory round trips.  ks_cheb on level
+// 1 of the 4M-DOF plate: 18.6 µs, against 19.6 µs with 4 blocks per round and 20.4 µs with
+// row_entries' groups of 8 entries
 template <class XF, class T>
 __device__ inline double rowS_f(const MatST<T>& A, const XF& xf, const RowAt& ra) {
-  return row_entries<8>(A, xf, ra.o, ra.lane, ra.w(), 0, 1, 0.0);
+  return row_blocks<8>(A, xf, ra.o, ra.lane, ra.w(), 0.0);
 }
 
 template <class XF, class T>

@@ -1560,6 +1560,17 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
     true_rel = fs.true_rel_residual;
     last_status = converged ? kConverged : kMaxIter;
   }
+  if (!converged && !zero_rhs && !fallback && o.max_iter > 0) {
+    // stopped without converging: the FP64 true residual of the x it stopped at (the last
+    // refresh's, if any, belongs to an earlier x)
+    spmv_fp64(c, c.x.get(), Ax);
+    hipLaunchKernelGGL(k_residual, dim3(gv), dim3(kThreads), 0, c.stream, c.nb,
+                       (const double2*)c.b.get(), (const double2*)Ax, (double2*)r64, partX(c));
+    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(kThreads), 0, c.stream, partX(c), gv,
+                       &c.scal.get()->rr);
+    HIP_CHECK(hipGetLastError());
+    true_rel = std::sqrt(read_scalars(c).rr) / bnorm;
+  }
   s = read_scalars(c);
   HIP_CHECK(hipEventRecord(ev1, c.stream));
   HIP_CHECK(hipEventSynchronize(ev1));

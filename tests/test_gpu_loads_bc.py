@@ -158,6 +158,41 @@ def test_traction_bad_edge_leaves_f(dev):
     assert np.array_equal(dev.download_rhs(), f0)
 
 
+def test_traction_repeated_group_bitwise(dev, oracle):
+    """A constant traction on a group applied before on the same mesh reuses its checked index
+    lists (no host packing, upload or wait): the same f bit for bit, other values and groups
+    too; a bad group still raises after good ones; a new mesh starts afresh."""
+    prob = pipeline.plate_problem(90, 70, Lx=1.3, jitter=0.2, seed=5)
+    m = prob.mesh
+    fresh_rhs(dev, prob)
+    for rep in range(3):
+        dev.set_rhs(None)
+        f = np.zeros(m.num_dofs)
+        for g, tx, ty in (("right", 1.0 + rep, -0.5), ("top", -0.25, 2.0 - rep)):
+            dev.add_traction(m.groups[g], tx=tx, ty=ty)
+            oracle.add_traction(m.xy, m.tri, m.groups[g], f, t=prob.thickness, tx=tx, ty=ty)
+        assert np.array_equal(dev.download_rhs(), f), rep
+    # the reversed orientation is another group (its own check and lists)
+    dev.set_rhs(None)
+    f = np.zeros(m.num_dofs)
+    dev.add_traction(m.groups["right"][:, ::-1], tx=0.5, ty=0.5)
+    oracle.add_traction(m.xy, m.tri, m.groups["right"][:, ::-1], f, t=prob.thickness, tx=0.5,
+                        ty=0.5)
+    assert np.array_equal(dev.download_rhs(), f)
+    with pytest.raises(cstfem_hip.CstfemError, match="is adjacent to 0 triangles"):
+        dev.add_traction(np.array([[0, 90 * 71 - 1]], np.int32), tx=1.0)
+    with pytest.raises(cstfem_hip.CstfemError, match="is adjacent to 0 triangles"):
+        dev.add_traction(np.array([[0, 90 * 71 - 1]], np.int32), tx=1.0)
+    # another mesh: the same node numbers are other edges, checked again
+    prob2 = pipeline.plate_problem(40, 30, seed=2)
+    fresh_rhs(dev, prob2)
+    f = np.zeros(prob2.mesh.num_dofs)
+    dev.add_traction(prob2.mesh.groups["right"], tx=1.0, ty=0.0)
+    oracle.add_traction(prob2.mesh.xy, prob2.mesh.tri, prob2.mesh.groups["right"], f,
+                        t=prob2.thickness, tx=1.0, ty=0.0)
+    assert np.array_equal(dev.download_rhs(), f)
+
+
 # ---- Dirichlet elimination (bc.cpp:137-166)
 
 def test_dirichlet_repeated_calls_vs_oracle(dev, oracle):
