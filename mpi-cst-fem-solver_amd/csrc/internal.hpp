@@ -175,6 +175,10 @@ struct MgLevel {
   std::vector<int> h_agg;
   DBuf<int> agg, mptr, mem;  // aggregate of every node; member nodes of every aggregate
   DBuf<double> P;            // n x 3: rows of the tentative prolongator
+  // level 0 of the learned smoother's cycle (coarse_fp32): P rounded to FP32, which that
+  // cycle's restriction and prolongation read (the Galerkin product reads P)
+  DBuf<float> P32;
+  bool p32 = false;
   DBuf<double> nns;          // n x 3 near-null space of levels >= 1 (R factors of the level above)
   // work vectors
   DBuf<double> rhs, x, x2, d, res, kx, kres, kres2, kp, kq, kz;

@@ -800,8 +800,9 @@ __global__ __launch_bounds__(kT) void ks_dot(int n, const double* __restrict__ a
 // sum over the member rows k = k0, k0 + step, ... (k < rows) of the aggregate whose member
 // list starts at mem[m0] (rows in member order, then DOF order) of P[f * 3 + c] res[f], in
 // that order; loads in groups of 8
+template <class PT>
 __device__ inline double restrict_sum(int m0, int rows, int c, int bs,
-                                      const int* __restrict__ mem, const double* __restrict__ P,
+                                      const int* __restrict__ mem, const PT* __restrict__ P,
                                       const double* __restrict__ res, int k0, int step) {
   double s = 0.0;
   for (int k = k0; k < rows; k += 8 * step) {
@@ -814,7 +815,7 @@ __device__ inline double restrict_sum(int m0, int rows, int c, int bs,
     double p[8], r[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      p[u] = P[(size_t)f[u] * 3 + c];
+      p[u] = (double)P[(size_t)f[u] * 3 + c];
       r[u] = res[f[u]];
     }
 #pragma unroll
@@ -826,10 +827,10 @@ __device__ inline double restrict_sum(int m0, int rows, int c, int bs,
 
 // rc[3a + c] = sum over member nodes i of a, k < bs:  P[(bs i + k) * 3 + c] * res[bs i + k];
 // LPR lanes per coarse DOF split the member rows (butterfly sum, as in rowS_lanes)
-template <int LPR>
+template <int LPR, class PT>
 __global__ __launch_bounds__(kT) void k_restrict(int nagg, int bs, const int* __restrict__ mptr,
                                                  const int* __restrict__ mem,
-                                                 const double* __restrict__ P,
+                                                 const PT* __restrict__ P,
                                                  const double* __restrict__ res,
                                                  double* __restrict__ rc,
                                                  const PcgScalars* __restrict__ sc) {
@@ -905,8 +906,9 @@ __global__ __launch_bounds__(kT) void ks_resid_restrict(MatST<T> A, const double
 
 // x[f] += sum_c P[f * 3 + c] * e[3 agg(node of f) + c] (XProlong with xp.x2 = x); with xp.kp,
 // e = kx + alpha kp, alpha from the coarse K-cycle's last partials
-__global__ __launch_bounds__(kT) void k_prolong(int n, XProlong xp, const double* __restrict__ part,
-                                                int np, double* __restrict__ x,
+__global__ __launch_bounds__(kT) void k_prolong(int n, XProlong xp, const float* __restrict__ P32,
+                                                const double* __restrict__ part, int np,
+                                                double* __restrict__ x,
                                                 const PcgScalars* __restrict__ sc) {
   __shared__ double lds[kW];
   const int f = blockIdx.x * kT + threadIdx.x;
@@ -916,9 +918,15 @@ __global__ __launch_bounds__(kT) void k_prolong(int n, XProlong xp, const double
   if (f < n) {
     x2f = xp.x2[f];
     a = xp.agg[f / xp.bs];
-    p0 = xp.P[(size_t)f * 3];
-    p1 = xp.P[(size_t)f * 3 + 1];
-    p2 = xp.P[(size_t)f * 3 + 2];
+    if (P32) {  // P32 non-null: P's rows rounded to FP32 instead of xp.P
+      p0 = (double)P32[(size_t)f * 3];
+      p1 = (double)P32[(size_t)f * 3 + 1];
+      p2 = (double)P32[(size_t)f * 3 + 2];
+    } else {
+      p0 = xp.P[(size_t)f * 3];
+      p1 = xp.P[(size_t)f * 3 + 1];
+      p2 = xp.P[(size_t)f * 3 + 2];
+    }
   }
   StepParts pq, pr;
   if (xp.kp) {
@@ -1817,6 +1825,12 @@ __global__ __launch_bounds__(kT) void k_graph_bandwidth(int n, const int* __rest
   }
 }
 
+// y = x rounded to FP32
+__global__ void k_round32(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+
 // member lists of the aggregates: counts, offsets (a scan), members in arrival order, then
 // every list sorted ascending (the order of the Gram-Schmidt step, the restriction and the
 // Galerkin sums); err = 1 if an aggregate holds more than 192 DOFs
@@ -2599,11 +2613,28 @@ int restrict_lanes(const MgLevel& L) {
 }
 
 // restriction of level L's residual res into rc (3 per aggregate)
-void launch_restrict(Context& c, MgLevel& L, const double* res, double* rc, const PcgScalars* sc) {
+template <class PT>
+void launch_restrict_t(Context& c, MgLevel& L, const PT* P, const double* res, double* rc,
+                       const PcgScalars* sc) {
   const int lpr = restrict_lanes(L);
-  CSTFEM_LPR_SWITCH(lpr, k_restrict, grid1((int64_t)3 * L.nagg * lpr), L.nagg, L.bs,
-                    (const int*)L.mptr.get(), (const int*)L.mem.get(), (const double*)L.P.get(),
-                    res, rc, sc);
+  const int grid = grid1((int64_t)3 * L.nagg * lpr);
+  const int* mptr = L.mptr.get();
+  const int* mem = L.mem.get();
+  switch (lpr) {
+    case 1: hipLaunchKernelGGL((k_restrict<1, PT>), dim3(grid), dim3(kT), 0, c.stream, L.nagg, L.bs, mptr, mem, P, res, rc, sc); break;
+    case 2: hipLaunchKernelGGL((k_restrict<2, PT>), dim3(grid), dim3(kT), 0, c.stream, L.nagg, L.bs, mptr, mem, P, res, rc, sc); break;
+    case 4: hipLaunchKernelGGL((k_restrict<4, PT>), dim3(grid), dim3(kT), 0, c.stream, L.nagg, L.bs, mptr, mem, P, res, rc, sc); break;
+    default: hipLaunchKernelGGL((k_restrict<8, PT>), dim3(grid), dim3(kT), 0, c.stream, L.nagg, L.bs, mptr, mem, P, res, rc, sc); break;
+  }
+}
+
+// p32: with level 0's FP32 copy of P
+void launch_restrict(Context& c, MgLevel& L, const double* res, double* rc, const PcgScalars* sc,
+                     bool p32 = false) {
+  if (p32)
+    launch_restrict_t(c, L, (const float*)L.P32.get(), res, rc, sc);
+  else
+    launch_restrict_t(c, L, (const double*)L.P.get(), res, rc, sc);
 }
 
 // levels >= 1 of fewer than fuse_n DOFs run the fused residual + restriction
@@ -2834,10 +2865,12 @@ XProlong prolong_of(const MgLevel& L, const MgLevel& C, double* x2, const Fold& 
 }
 
 // x2 += P e (k_prolong) on level l after its coarse correction
-void launch_prolong(Context& c, int l, double* x2, const Fold& f, const PcgScalars* sc) {
+void launch_prolong(Context& c, int l, double* x2, const Fold& f, const PcgScalars* sc,
+                    bool p32 = false) {
   MgLevel& L = *c.mg.lv[l];
   hipLaunchKernelGGL(k_prolong, dim3(grid1(L.n)), dim3(kT), 0, c.stream, L.n,
-                     prolong_of(L, *c.mg.lv[l + 1], x2, f), f.part, f.np, x2, sc);
+                     prolong_of(L, *c.mg.lv[l + 1], x2, f), p32 ? (const float*)L.P32.get() : nullptr,
+                     f.part, f.np, x2, sc);
 }
 
 // cycle on level l >= 1: out = B_l(rhs)
@@ -2967,8 +3000,10 @@ int mg_enqueue_apply(Context& c, const double* r, double* z, const PcgScalars* s
                          (double2*)L.res.get(), sc);
   };
   resid0();
-  launch_restrict(c, L, L.res.get(), coarse_rhs(mg, 0), sc);
-  launch_prolong(c, 0, z, coarse_correction(c, 0, sc), sc);
+  // ... and reads P in FP32 for its restriction and prolongation
+  const bool p32 = v32 && L.p32;
+  launch_restrict(c, L, L.res.get(), coarse_rhs(mg, 0), sc, p32);
+  launch_prolong(c, 0, z, coarse_correction(c, 0, sc), sc, p32);
   // post-smoothing
   if (learned) {
     resid0();
@@ -3256,6 +3291,13 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
       }
     }
     HIP_CHECK(hipGetLastError());
+    L.p32 = l == 0 && mg.coarse_fp32;
+    if (L.p32) {  // the learned cycle's FP32 copy of level 0's P (4M DOFs: 48 MB less per pass)
+      L.P32.alloc(3 * (size_t)L.n);
+      hipLaunchKernelGGL(k_round32, dim3(grid1(3 * (int64_t)L.n)), dim3(kT), 0, c.stream,
+                         3 * (int64_t)L.n, (const double*)L.P.get(), L.P32.get());
+      HIP_CHECK(hipGetLastError());
+    }
     tr.mark(c, l, "gram-schmidt");
     // coarse block pattern (device): sorted distinct neighbour aggregates of each aggregate
     const int* dbp = l == 0 ? c.brow_ptr.get() : L.d_bptr.get();

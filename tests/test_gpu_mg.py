@@ -143,8 +143,8 @@ def tail_start(sizes, kdepth):
 def learned_hierarchy(dev, theta, monkeypatch, coarse32, tail, n=40, kdepth=3):
     """The device's learned-smoother hierarchy on a plate and the oracle's, with the oracle's
     row sums reading the values the device's cycle reads: FP32 copies of K (level-0
-    residuals) and of the coarse matrices when CSTFEM_MG_COARSE32 is on, except on the levels
-    of the fused tail, which keep their FP64 values."""
+    residuals), of the coarse matrices and of level 0's P when CSTFEM_MG_COARSE32 is on,
+    except on the levels of the fused tail, which keep their FP64 values."""
     prob = hetero(n, e_seed=3, seed=4)
     A, xy, fixed, bptr, bcol = system(dev, prob)
     dev.mpnn_set_params(theta)
@@ -161,6 +161,9 @@ def learned_hierarchy(dev, theta, monkeypatch, coarse32, tail, n=40, kdepth=3):
         for l, L in enumerate(levels[:-1]):
             if t is None or l < t:
                 L.As = rounded32(L.A)
+        # level 0's restriction and prolongation read P rounded to FP32 (the Galerkin
+        # products, already formed, read it in FP64)
+        levels[0].P = rounded32(levels[0].P)
     return prob, A, levels, lams, device_m(dev, prob, A)
 
 
