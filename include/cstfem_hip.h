@@ -315,7 +315,10 @@ int cstfem_mg_setup(cstfem_ctx* ctx, int max_levels, int coarse_max, int fine_sm
  * aggregated on a worker thread, while the caller goes on with assembly, loads, Dirichlet,
  * cstfem_setup and cstfem_mpnn_build; the next cstfem_mg_setup waits for it and uses it.
  * Call after cstfem_build_stiffness_pattern; a new pattern (or mesh, or CSR upload) drops a
- * started aggregation.  The result is the same as without it. */
+ * started aggregation.  From the first call on, every later cstfem_build_stiffness_pattern of
+ * the context starts it by itself as soon as the node graph exists (before the pattern's SELL
+ * layout and slot map), and a call for a pattern whose aggregation is under way does nothing.
+ * The result is the same as without it. */
 int cstfem_mg_prepare(cstfem_ctx* ctx);
 int cstfem_mg_levels(cstfem_ctx* ctx, int* levels);
 /* DOFs, aggregates (0 on the coarsest), stored entries and the upper end of the smoothing

@@ -272,6 +272,10 @@ void build_pattern(Context& c) {
   c.generic = false;
   c.setup_done = false;
   c.vals32_valid = false;
+  // the node graph is complete on the stream: a context that asked for the multilevel
+  // set-up's prepared aggregation starts it here, so that its download and its host pass
+  // overlap the rest of the pattern (the SELL layout and the slot map)
+  if (c.mg.auto_prepare) mg_prepare_start(c);
   build_sell(c);
   build_slot_map(c);
   HIP_CHECK(hipStreamSynchronize(c.stream));

@@ -257,6 +257,7 @@ struct Multigrid {
   // pre_agg and stages the map for upload; the set-up joins it.  Uses `pinned` until joined.
   std::thread pre_worker;
   bool pre_started = false;
+  bool auto_prepare = false;  // mg_prepare was called: pattern builds start the aggregation
   std::vector<int> pre_agg;
   int pre_nagg = 0;
   bool pre_staged = false;  // the worker also copied the aggregate map into `pinned`
@@ -494,6 +495,9 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
 // starts the level-0 aggregation of the current mesh pattern (node-graph download on the
 // auxiliary stream, greedy pass on a host thread); the next mg_setup takes its result
 void mg_prepare(Context& c);
+// starts the level-0 aggregation of the current mesh pattern now (mg_prepare's work; the
+// pattern build calls it right after the node graph once mg_prepare has been called)
+void mg_prepare_start(Context& c);
 // waits for a started level-0 aggregation and drops it (before the pattern changes)
 void mg_drop_prepared(Context& c);
 // z = B r on the library stream (level-0 vectors of 2*nb doubles); kernels return at once

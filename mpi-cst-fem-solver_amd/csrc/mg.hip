@@ -3040,6 +3040,13 @@ void mg_drop_prepared(Context& c) {
 void mg_prepare(Context& c) {
   if (!c.have_pattern || !c.mesh_pattern || c.n != 2 * c.nb)
     throw Error(CSTFEM_ERR_ORDER, "multilevel: prepare needs a mesh pattern (build_stiffness_pattern first)");
+  // from now on every mesh pattern build starts it as soon as the node graph exists
+  c.mg.auto_prepare = true;
+  if (c.mg.pre_started) return;  // started for this pattern (a new pattern drops it)
+  mg_prepare_start(c);
+}
+
+void mg_prepare_start(Context& c) {
   Multigrid& mg = c.mg;
   mg_drop_prepared(c);
   if (!c.aux) HIP_CHECK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
