@@ -223,6 +223,7 @@ struct Multigrid {
   // 0: the wavefront forms everywhere, same results)
   bool setup_lanes = true;
   DBuf<double> cinv;
+  DBuf<int> inv_flag;  // the device inverse's "not positive definite" flag
   // operands of the packed one-level tail (k_tail1), gathered at set-up for level tail_packed
   // (-1: none; the cycle then runs k_tail)
   DBuf<double> tail_pd;

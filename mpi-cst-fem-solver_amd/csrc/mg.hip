@@ -2430,7 +2430,84 @@ Cheb cheb(double lam) {
   return Cheb{1.0 / theta, rho1 * rho0, 2.0 * rho1 / delta};
 }
 
-// dense inverse of the coarsest matrix (dead DOFs: identity rows, zero solution)
+// Dense inverse of the coarsest matrix on the device, one workgroup, the matrix in LDS: rows
+// assembled from the SELL level in entry order (the host form's order), dead DOFs (diagonal
+// not positive) replaced by identity rows and columns, then Gauss-Jordan elimination with the
+// pivots in order (positive for a symmetric positive definite matrix; *flag = 1 and the
+// kernel stops at the first that is not).  cinv is row-major n x n, zero on dead rows and
+// columns, as the host form leaves it.
+constexpr int kInvThreads = 1024;
+__global__ __launch_bounds__(kInvThreads) void k_coarsest_inverse(MatS A, int n,
+                                                                  double* __restrict__ cinv,
+                                                                  int* __restrict__ flag) {
+  extern __shared__ double M[];  // n x n, then n dead flags (as doubles)
+  double* dead = M + (size_t)n * n;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int e = t; e < n * n; e += nt) M[e] = 0.0;
+  __syncthreads();
+  for (int r = t; r < n; r += nt) {
+    const int s = r >> 6, lane = r & 63;
+    const int64_t o = A.off[s];
+    const int w = (int)((A.off[s + 1] - o) >> 6);
+    for (int k = 0; k < w; ++k) M[(size_t)r * n + A.col(o, k, lane)] += A.val[o + (int64_t)k * 64 + lane];
+  }
+  __syncthreads();
+  for (int i = t; i < n; i += nt) dead[i] = M[(size_t)i * n + i] > 0.0 ? 0.0 : 1.0;
+  __syncthreads();
+  for (int e = t; e < n * n; e += nt) {
+    const int i = e / n, j = e % n;
+    if (dead[i] != 0.0 || dead[j] != 0.0) M[e] = i == j ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const double p = M[(size_t)k * n + k];  // the same value in every thread
+    if (!(p > 0.0)) {
+      if (t == 0) *flag = 1;
+      return;
+    }
+    const double ip = 1.0 / p;
+    // the entries off row and column k, from the old row and column k
+    for (int e = t; e < n * n; e += nt) {
+      const int i = e / n, j = e % n;
+      if (i != k && j != k) M[e] = M[e] - M[(size_t)i * n + k] * (M[(size_t)k * n + j] * ip);
+    }
+    __syncthreads();
+    // then row and column k
+    for (int j = t; j < n; j += nt) {
+      if (j != k) {
+        M[(size_t)k * n + j] = M[(size_t)k * n + j] * ip;
+        M[(size_t)j * n + k] = -M[(size_t)j * n + k] * ip;
+      }
+    }
+    if (t == 0) M[(size_t)k * n + k] = ip;
+    __syncthreads();
+  }
+  for (int e = t; e < n * n; e += nt) {
+    const int i = e / n, j = e % n;
+    cinv[e] = dead[i] != 0.0 || dead[j] != 0.0 ? 0.0 : M[e];
+  }
+}
+
+// LDS bytes k_coarsest_inverse needs for n DOFs
+size_t device_inverse_bytes(int n) { return sizeof(double) * ((size_t)n * n + n); }
+
+// enqueues the device form when its matrix fits one workgroup's LDS (false: use the host's)
+bool coarsest_inverse_device(Context& c, MgLevel& L) {
+  const int n = L.n;
+  int lds = 0;
+  HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device));
+  if (n <= 0 || device_inverse_bytes(n) > (size_t)lds) return false;
+  c.mg.nc = n;
+  c.mg.cinv.alloc((size_t)n * n);
+  int* flag = grow(c.mg.inv_flag, 1);
+  HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), c.stream));
+  hipLaunchKernelGGL(k_coarsest_inverse, dim3(1), dim3(kInvThreads), device_inverse_bytes(n),
+                     c.stream, mats(L), n, c.mg.cinv.get(), flag);
+  HIP_CHECK(hipGetLastError());
+  return true;
+}
+
+// dense inverse of the coarsest matrix (dead DOFs: identity rows, zero solution), on the host
 void coarsest_inverse(Context& c, MgLevel& L) {
   const int n = L.n;
   std::vector<double> val(L.val.n);
@@ -3219,6 +3296,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
   }
   const double cx = 0.5 * (xlo + xhi), cy = 0.5 * (ylo + yhi);
   const unsigned char* fixed = c.fixed_valid && c.fixed.n >= (size_t)c.n ? c.fixed.get() : nullptr;
+  bool inv_on_device = false;
   for (int l = 0;; ++l) {
     MgLevel& L = *mg.lv[l];
     const bool last = l + 1 >= max_levels - 1;
@@ -3406,16 +3484,28 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
       if (C->n > 4096)
         throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: coarsest level has " + std::to_string(C->n) +
                                              " DOFs (> 4096): allow more levels");
-      HIP_CHECK(hipStreamSynchronize(c.stream));  // its values for the inverse, its pattern
-      C->bcol.assign(mg.pinned_pat, mg.pinned_pat + ncblk);
-      coarsest_inverse(c, *C);
+      if (coarsest_inverse_device(c, *C)) {
+        inv_on_device = true;
+        HIP_CHECK(hipEventSynchronize(mg.pat_ev));  // its pattern
+        C->bcol.assign(mg.pinned_pat, mg.pinned_pat + ncblk);
+      } else {
+        HIP_CHECK(hipStreamSynchronize(c.stream));  // its values for the inverse, its pattern
+        C->bcol.assign(mg.pinned_pat, mg.pinned_pat + ncblk);
+        coarsest_inverse(c, *C);
+      }
       tr.mark(c, l + 1, "coarsest inverse");
       break;
     }
     bound_async(l + 1);
     tr.mark(c, l + 1, "smoothing bound");
   }
+  int inv_flag = 0;
+  if (inv_on_device)
+    HIP_CHECK(hipMemcpyAsync(&inv_flag, mg.inv_flag.get(), sizeof(int), hipMemcpyDeviceToHost,
+                             c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (inv_flag)
+    throw Error(CSTFEM_ERR_BAD_DIAGONAL, "multilevel: coarsest matrix is not positive definite");
   {
     // the smoothing bounds (levels below the coarsest; level 0 with the Chebyshev smoother)
     const int nl = (int)mg.lv.size();
