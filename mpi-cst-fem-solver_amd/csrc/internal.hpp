@@ -240,6 +240,7 @@ struct Multigrid {
   DBuf<double> lam_dev;            // those bounds on the device, one per level
   hipEvent_t bound_ev = nullptr;   // the check starts after this point of the main stream
   hipEvent_t lam_ev = nullptr;     // a level's bound starts after its matrix
+  std::vector<hipEvent_t> lvl_ev;  // the same, one per level (the set-up's bound helper)
   int* pinned = nullptr;  // page-locked staging of the fine node graph, kept across set-ups
   size_t pinned_n = 0;
   // page-locked copy of the coarse block pattern just built (the next level's host
@@ -298,6 +299,7 @@ struct Multigrid {
     if (pre_done) (void)hipEventDestroy(pre_done);
     if (bound_ev) (void)hipEventDestroy(bound_ev);
     if (lam_ev) (void)hipEventDestroy(lam_ev);
+    for (hipEvent_t e : lvl_ev) (void)hipEventDestroy(e);
   }
   void clear() {
     spare.insert(spare.begin(), lv.begin(), lv.end());

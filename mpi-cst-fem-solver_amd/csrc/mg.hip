@@ -30,6 +30,10 @@
 #include <array>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -3204,6 +3208,75 @@ void mg_prepare_start(Context& c) {
   mg.pre_started = true;
 }
 
+// The helper thread of mg_setup that enqueues the levels' spectral bounds (see there).
+class BoundHelper {
+ public:
+  BoundHelper(Context& c, double* lam_dev) : c_(c), lam_dev_(lam_dev) {}
+  ~BoundHelper() {
+    try {
+      finish();
+    } catch (...) {  // an error already on its way out of the set-up
+    }
+  }
+  // the bound of `level` after event `ev` of the main stream
+  void push(int level, hipEvent_t ev) {
+    if (!th_.joinable()) th_ = std::thread([this]() { serve(); });
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      q_.emplace_back(level, ev);
+    }
+    cv_.notify_one();
+  }
+  // waits until every pushed bound is enqueued; rethrows the helper's error
+  void finish() {
+    if (th_.joinable()) {
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        done_ = true;
+      }
+      cv_.notify_one();
+      th_.join();
+    }
+    if (err_) {
+      const std::exception_ptr e = err_;
+      err_ = nullptr;
+      std::rethrow_exception(e);
+    }
+  }
+
+ private:
+  void serve() {
+    if (hipSetDevice(c_.device) != hipSuccess) err_ = std::make_exception_ptr(
+        Error(CSTFEM_ERR_DEVICE, "multilevel: hipSetDevice failed on the bound helper"));
+    for (;;) {
+      std::pair<int, hipEvent_t> job;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&]() { return done_ || !q_.empty(); });
+        if (q_.empty()) return;
+        job = q_.front();
+        q_.pop_front();
+      }
+      const int level = job.first;
+      if (err_) continue;  // after an error: drain the queue, enqueue nothing more
+      try {
+        HIP_CHECK(hipStreamWaitEvent(c_.aux, job.second, 0));
+        smoothing_max_enqueue(c_, level, c_.aux, c_.mg.dwork3, lam_dev_ + level);
+      } catch (...) {
+        err_ = std::current_exception();
+      }
+    }
+  }
+  Context& c_;
+  double* lam_dev_;
+  std::thread th_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<std::pair<int, hipEvent_t>> q_;
+  bool done_ = false;
+  std::exception_ptr err_;
+};
+
 void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int kdepth) {
   if (!c.setup_done) throw Error(CSTFEM_ERR_ORDER, "multilevel: call cstfem_setup first");
   if (!c.mesh_pattern || c.n != 2 * c.nb)
@@ -3278,11 +3351,20 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
   // smoothing bounds of the levels run on the auxiliary stream, each after its level's matrix
   // is built on the main stream; the set-up collects them at the end
   double* lam_dev = grow(mg.lam_dev, kLamSlots);
+  // A level's bound (about 45 launches) is enqueued on the auxiliary stream by a helper thread,
+  // so that this thread goes on with the next level meanwhile: each level gets an event on
+  // the main stream after its matrix, and the helper makes the auxiliary stream wait for it.
+  // The helper takes the levels in order, so the auxiliary stream's work is the same as when
+  // this thread enqueued it.
+  BoundHelper bounds(c, lam_dev);
   auto bound_async = [&](int level) {
-    if (!mg.lam_ev) HIP_CHECK(hipEventCreateWithFlags(&mg.lam_ev, hipEventDisableTiming));
-    HIP_CHECK(hipEventRecord(mg.lam_ev, c.stream));
-    HIP_CHECK(hipStreamWaitEvent(c.aux, mg.lam_ev, 0));
-    smoothing_max_enqueue(c, level, c.aux, mg.dwork3, lam_dev + level);
+    while ((int)mg.lvl_ev.size() <= level) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      mg.lvl_ev.push_back(e);
+    }
+    HIP_CHECK(hipEventRecord(mg.lvl_ev[level], c.stream));
+    bounds.push(level, mg.lvl_ev[level]);
   };
   if (fine_smoother == 1) bound_async(0);
   tr.mark(c, 0, "work, smoothing");
@@ -3499,6 +3581,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     bound_async(l + 1);
     tr.mark(c, l + 1, "smoothing bound");
   }
+  bounds.finish();  // every bound enqueued (rethrows the helper's error)
   int inv_flag = 0;
   if (inv_on_device)
     HIP_CHECK(hipMemcpyAsync(&inv_flag, mg.inv_flag.get(), sizeof(int), hipMemcpyDeviceToHost,
