@@ -3484,17 +3484,23 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
                        (const int*)L.mptr.get(), (const int*)L.mem.get(), dbp, dbc,
                        (const int*)L.agg.get(), deg, flags);
     HIP_CHECK(hipGetLastError());
-    std::vector<int> hdeg(na);
-    int hflag[2] = {0, 0};
-    HIP_CHECK(hipMemcpyAsync(hdeg.data(), deg, sizeof(int) * na, hipMemcpyDeviceToHost, c.stream));
-    HIP_CHECK(hipMemcpyAsync(hflag, flags, sizeof(hflag), hipMemcpyDeviceToHost, c.stream));
+    // the degrees and the two flags through the context's page-locked staging (a pageable
+    // destination takes the runtime's slower staged path)
+    int* hdeg = reinterpret_cast<int*>(c.stage(sizeof(int) * ((size_t)na + 2)));
+    HIP_CHECK(hipMemcpyAsync(hdeg, deg, sizeof(int) * na, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(hdeg + na, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, c.stream));
     HIP_CHECK(hipStreamSynchronize(c.stream));
+    const int hflag[2] = {hdeg[na], hdeg[na + 1]};
     if (hflag[1]) throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: aggregate too large");
     if (hflag[0])
       throw Error(CSTFEM_ERR_ARGUMENT, "multilevel: an aggregate touches more than " +
                                            std::to_string(kMaxCoarseDeg) + " aggregates");
     C->bptr.assign(na + 1, 0);
-    for (int a = 0; a < na; ++a) C->bptr[a + 1] = C->bptr[a] + hdeg[a];
+    int max_deg = 0;
+    for (int a = 0; a < na; ++a) {
+      C->bptr[a + 1] = C->bptr[a] + hdeg[a];
+      max_deg = std::max(max_deg, hdeg[a]);
+    }
     const int ncblk = C->bptr[na];
     upload(C->d_bptr, C->bptr);
     C->d_bcol.alloc(std::max(ncblk, 1));
@@ -3528,7 +3534,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
     tr.mark(c, l, "sell layout");
     // Galerkin product
     if (ncblk > 0) {
-      const bool wide = *std::max_element(hdeg.begin(), hdeg.end()) > 64;
+      const bool wide = max_deg > 64;
       const RapFine0 f0{c.sell_pos.get(), c.sell_val.get()};
       const RapFineS fs{L.off.get(), L.val.get()};
       const int* fp = l == 0 ? c.brow_ptr.get() : L.d_bptr.get();
