@@ -3502,7 +3502,22 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
       max_deg = std::max(max_deg, hdeg[a]);
     }
     const int ncblk = C->bptr[na];
-    upload(C->d_bptr, C->bptr);
+    // the coarse row pointers and the slice offsets of its SELL layout go up through the
+    // page-locked staging, queued on the stream (the next level's use of the staging follows
+    // them in stream order, after this level's last wait)
+    sell_offsets(*C, C->bptr);
+    {
+      const size_t nb1 = (size_t)na + 1, ns1 = (size_t)C->nslices + 1;
+      unsigned char* st = c.stage(sizeof(int64_t) * ns1 + sizeof(int) * nb1);
+      int64_t* s_off = reinterpret_cast<int64_t*>(st);
+      int* s_bptr = reinterpret_cast<int*>(st + sizeof(int64_t) * ns1);
+      std::copy(C->h_off.begin(), C->h_off.end(), s_off);
+      std::copy(C->bptr.begin(), C->bptr.end(), s_bptr);
+      C->d_bptr.alloc(nb1);
+      C->off.alloc(ns1);
+      HIP_CHECK(hipMemcpyAsync(C->d_bptr.get(), s_bptr, sizeof(int) * nb1, hipMemcpyHostToDevice, c.stream));
+      HIP_CHECK(hipMemcpyAsync(C->off.get(), s_off, sizeof(int64_t) * ns1, hipMemcpyHostToDevice, c.stream));
+    }
     C->d_bcol.alloc(std::max(ncblk, 1));
     if (lanes)
       hipLaunchKernelGGL(k_cpat_fill_lanes, dim3(grid1(na)), dim3(kT), 0, c.stream, na,
@@ -3520,9 +3535,7 @@ void mg_setup(Context& c, int max_levels, int coarse_max, int fine_smoother, int
                              sizeof(int) * ncblk, hipMemcpyDeviceToHost, c.stream));
     HIP_CHECK(hipEventRecord(mg.pat_ev, c.stream));
     tr.mark(c, l, "coarse pattern");
-    // scalar SELL-64 layout of the coarse matrix
-    sell_offsets(*C, C->bptr);
-    upload(C->off, C->h_off);
+    // scalar SELL-64 layout of the coarse matrix (offsets uploaded above)
     const int64_t nent = C->h_off[C->nslices];
     C->scol.alloc(std::max<int64_t>(nent / 3, 1));
     C->val.alloc(std::max<int64_t>(nent, 1));
