@@ -703,25 +703,35 @@ void enqueue_iteration32(Context& c, int cur) {
 // `chunk` iterations as one unit of work: a captured hipGraph, or -- when the environment
 // sets CSTFEM_GRAPH=0, for tools that cannot follow graph launches -- the same kernels
 // launched directly.
+// A one-iteration chunk holds two graphs, one per parity, launched alternately from parity
+// 0 (restart()): a stop then leaves one iteration's launches in flight instead of two.
 struct Chunk {
-  Graph gr;
+  Graph gr, gr1;
   std::function<void(int)> iteration;
   int n = 0;
+  int next = 0;  // parity of the next one-iteration chunk
   bool graph = true;
   Chunk(Context& c, int count, std::function<void(int)> it) : iteration(std::move(it)), n(count) {
     const char* env = std::getenv("CSTFEM_GRAPH");
     graph = !(env && env[0] == '0');
     if (!graph) return;
-    HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < n; ++i) iteration(i & 1);
-    HIP_CHECK(hipStreamEndCapture(c.stream, &gr.g));
-    HIP_CHECK(hipGraphInstantiate(&gr.e, gr.g, nullptr, nullptr, 0));
+    capture(c, gr, 0);
+    if (n == 1) capture(c, gr1, 1);
   }
+  void capture(Context& c, Graph& g, int parity) {
+    HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < n; ++i) iteration((i + parity) & 1);
+    HIP_CHECK(hipStreamEndCapture(c.stream, &g.g));
+    HIP_CHECK(hipGraphInstantiate(&g.e, g.g, nullptr, nullptr, 0));
+  }
+  void restart() { next = 0; }
   void launch(hipStream_t s) {
+    const int p = n == 1 ? next : 0;
+    next ^= 1;
     if (graph) {
-      HIP_CHECK(hipGraphLaunch(gr.e, s));
+      HIP_CHECK(hipGraphLaunch(p ? gr1.e : gr.e, s));
     } else {
-      for (int i = 0; i < n; ++i) iteration(i & 1);
+      for (int i = 0; i < n; ++i) iteration((i + p) & 1);
       HIP_CHECK(hipGetLastError());
     }
   }
@@ -734,6 +744,7 @@ void run_until_stopped(Context& c, Chunk& graph, int chunk, int64_t max_iter) {
   HIP_CHECK(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
   const int64_t max_chunks = (max_iter + chunk - 1) / chunk + 2;
   int cur = 0;
+  graph.restart();  // every run starts at parity 0
   graph.launch(c.stream);
   HIP_CHECK(hipMemcpyAsync(slot[cur], c.scal.get(), sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipEventRecord(done[cur], c.stream));
@@ -1441,9 +1452,14 @@ int fcg_run(Context& c, const cstfem_solver_options& o, cstfem_solver_stats& st,
   precond(0, 0);
   direction(1);
   HIP_CHECK(hipGetLastError());
-  // a multilevel iteration is ~1.4 ms of ~100 launches: two of them hide the host's status
-  // read, and a short chunk keeps the graph small and the launches queued past a stop few
-  const int chunk = multilevel ? 2 : (c.n > 200000 ? 16 : 64);  // even
+  // a multilevel iteration is ~1.3 ms of ~100 launches: one of them hides the host's status
+  // read (a graph per parity), and a stop leaves one iteration's launches in flight (0.7%
+  // less time per iteration than chunks of two on the 4M-DOF plate; CSTFEM_FCG_CHUNK sets 1
+  // or an even count)
+  const char* env_chunk = std::getenv("CSTFEM_FCG_CHUNK");
+  const int want = env_chunk ? std::atoi(env_chunk) : 1;
+  const int chunk = multilevel ? (want == 1 || (want >= 2 && want % 2 == 0) ? want : 2)
+                               : (c.n > 200000 ? 16 : 64);
   // the iteration graph is captured while the device still runs the initialisation and the
   // first preconditioner application queued above: the capture records launches without
   // running them, and the work queued before it does not wait (a multilevel capture and its
